@@ -1,0 +1,253 @@
+"""Benchmark: ANI energy/force + BAOAB Langevin MD on the 23,328-atom periodic water box.
+
+One step = BAOAB integrate -> cell-list neighbour lists (rebuilt every step) ->
+AEV -> species MLP (forward + input gradient) -> AEV adjoint (forces, virial)
+-> FP64 per-system reductions, all on one GPU with inputs resident in HBM.
+
+Prints ONE JSON line.  ``value`` is the BF16-MLP form's atom-steps/s; the
+plain FP32 form, their ratio, the per-step p50/p99 latency, the energy and
+force errors of both forms against an FP64 evaluation, the roofline of the
+dominant kernel and a CPU baseline (the NumPy oracle on the host cores) are
+reported beside it.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp32": 157.3}
+SPECIES7 = ("H", "C", "N", "O", "S", "F", "Cl")
+MODEL_SEED = 11
+MD_SEED = 2024
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload(name: str) -> dict:
+    from fennol_amd import systems
+
+    if name == "water23k":
+        return systems.water_config2()
+    if name == "water7k":
+        return systems.water_config1()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def model_params(cfg):
+    """Seeded random-init weights (truncated lecun normal, zero biases) for H, C, N, O networks."""
+    from fennol_amd.params import init_species_params
+
+    return init_species_params(MODEL_SEED, cfg.aev_length)
+
+
+def algorithmic_units(sb, cfg, precision: str) -> dict:
+    """Per-launch algorithmic bytes / flops of each stage, from the measured neighbour counts."""
+    nr = sb.n_nbr.cpu().numpy().astype(np.float64)
+    na = sb.n_nbr_ang.cpu().numpy().astype(np.float64)
+    N = sb.n_atoms
+    A = cfg.aev_length
+    out_b = 2 if precision == "bf16" else 4
+    per_atom_mlp = 2.0 * (A * 256 + 256 * 192 + 192 * 160 + 160)   # forward
+    per_atom_mlp += 2.0 * (160 * 192 + 192 * 256 + 256 * A)         # input gradient
+    return {
+        # xyz + species in, neighbour rows (radial + angular) and counts out
+        "nblist": ("hbm", float(N * (12 + 4 + 8) + 4 * nr.sum() + 4 * na.sum())),
+        # xyz, rows in; AEV row out
+        "aev_fwd": ("hbm", float(N * (12 + 8) + 4 * nr.sum() + 4 * na.sum() + N * A * out_b)),
+        "mlp": ("mfma", float(N * per_atom_mlp)),
+        # dE/dAEV in (fp32), rows in; forces (self + neighbour) and virial out
+        "aev_bwd": ("hbm", float(N * (12 + 8) + 4 * nr.sum() + 4 * na.sum() + N * A * 4 + N * (24 + 36))),
+        "reduce": ("hbm", float(N * (4 + 36))),
+        "baoab": ("hbm", float(N * (48 + 24 + 8 + 8 + 48 + 12))),
+    }
+
+
+def run_md(system, cfg, params, precision, steps, warmup, timed_stages=True):
+    import torch
+
+    from fennol_amd.md import LangevinMD
+    from fennol_amd.potential import ANIPotential, StageTimer
+
+    pot = ANIPotential(cfg, params, precision=precision)
+    md = LangevinMD(pot, system, dt_fs=0.5, temperature=300.0, gamma_thz=1.0, seed=MD_SEED, check_every=0)
+    for _ in range(warmup):
+        md.step()
+    torch.cuda.synchronize()
+    md.verify()
+    timer = StageTimer() if timed_stages else None
+    pot.timer = timer
+    step_ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step_ev[0].record()
+    for k in range(steps):
+        md.step()
+        step_ev[k + 1].record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    pot.timer = None
+    md.verify()
+    lat = np.array([step_ev[k].elapsed_time(step_ev[k + 1]) for k in range(steps)])
+    stages = timer.times_ms() if timer else {}
+    return pot, md, (t1 - t0), lat, stages
+
+
+def accuracy(system, cfg, params, precision, fp64):
+    """Energy (meV/atom) and force-RMS (eV/A) errors against the FP64 evaluation."""
+    from fennol_amd.potential import ANIPotential
+
+    pot = ANIPotential(cfg, params, precision=precision)
+    E, F, V = pot.energy_forces_virial(system["species"], system["coordinates"], system["natoms"],
+                                       system.get("cells"))
+    Er, Fr = fp64
+    n = len(system["species"])
+    return {"energy_err_mev_per_atom": float(1000.0 * abs(E[0] - Er[0]) / n),
+            "force_rms_err_ev_per_a": float(np.sqrt(np.mean((F - Fr) ** 2)))}
+
+
+def cpu_reference(system, cfg, params):
+    """The cpu_baseline leg: the FP64 NumPy oracle runs one MD step (energy, forces, BAOAB) on
+    the host cores and is timed; its energy and forces are also the FP64 accuracy reference."""
+    import oracle as O
+
+    p = O.AEVParams(species_order=cfg.species_order)
+    model = O.Model(p, params)
+    x = np.asarray(system["coordinates"], np.float64)
+    t0 = time.perf_counter()
+    g = O.build_graphs(x, system["natoms"], p, system.get("cells"), system.get("reciprocal_cells"))
+    E, F, V, _ = O.energy_forces_virial(model, x, system["species"], system["natoms"], system.get("cells"),
+                                        system.get("reciprocal_cells"), graphs=g)
+    mass = O.ATOMIC_MASSES[system["species"]] / O.md_units.DA
+    a1, a2 = O.langevin_coefficients(5e-4, 1.0, O.md_units.K_B * 300.0, mass)
+    noise = O.philox_normals(MD_SEED, 0, x.shape[0])
+    O.baoab_step(x, np.zeros_like(x), F / O.md_units.energy_multiplier("eV"), mass, 5e-4, a1, a2, noise)
+    t1 = time.perf_counter()
+    return (E, F), t1 - t0
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="water23k", choices=["water23k", "water7k"])
+    ap.add_argument("--skip-cpu", action="store_true", help="skip the FP64 oracle (accuracy and CPU baseline)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (single-GPU build)",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    from fennol_amd.potential import ANIConfig
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+    system = workload(args.workload)
+    N = len(system["species"])
+    cfg = ANIConfig(species_order=SPECIES7)
+    params = model_params(cfg)
+    log(f"[bench] workload {args.workload}: {N} atoms, AEV {cfg.aev_length}")
+
+    results = {}
+    for precision in ("fp32", "bf16"):
+        log(f"[bench] MD {precision}: warmup {args.warmup}, steps {args.steps}")
+        pot, md, wall, lat, stages = run_md(system, cfg, params, precision, args.steps, args.warmup)
+        units = algorithmic_units(md.sb, cfg, precision)
+        results[precision] = dict(wall=wall, lat=lat, stages=stages, units=units,
+                                  max_nbr=md.sb.max_nbr, max_nbr_ang=md.sb.max_nbr_ang)
+        log(f"[bench]   {precision}: {1000 * wall / args.steps:.3f} ms/step; stages (ms): "
+            + ", ".join(f"{k} {v.mean():.3f}" for k, v in stages.items()))
+        del pot, md
+        torch.cuda.empty_cache()
+
+    bf, fp = results["bf16"], results["fp32"]
+    ms_bf = 1000.0 * bf["wall"] / args.steps
+    ms_fp = 1000.0 * fp["wall"] / args.steps
+    value = N * args.steps / bf["wall"]
+    value_fp32 = N * args.steps / fp["wall"]
+
+    # roofline of the dominant kernel of the headline (BF16) form
+    st = bf["stages"]
+    dom = max((k for k in st if k in bf["units"]), key=lambda k: st[k].mean())
+    bound, work = bf["units"][dom]
+    t_s = st[dom].mean() / 1000.0
+    if bound == "hbm":
+        achieved, peak, unit = work / t_s / 1e9, HBM_PEAK_GBS, "GB/s"
+    else:
+        achieved, peak, unit = work / t_s / 1e12, MFMA_PEAK_TFLOPS["bf16"], "TFLOP/s"
+    roofline = {"kernel": dom, "bound": bound, "achieved": round(achieved, 2), "peak": peak, "unit": unit,
+                "frac": round(achieved / peak, 4), "traffic": None}
+    stage_table = {}
+    for prec, r in results.items():
+        stage_table[prec] = {}
+        for k, v in r["stages"].items():
+            ent = {"ms": round(float(v.mean()), 4)}
+            if k in r["units"]:
+                b, w = r["units"][k]
+                if b == "hbm":
+                    ent["GB/s"] = round(w / (v.mean() / 1000.0) / 1e9, 1)
+                else:
+                    ent["TFLOP/s"] = round(w / (v.mean() / 1000.0) / 1e12, 2)
+            stage_table[prec][k] = ent
+
+    acc = {}
+    cpu = None
+    if not args.skip_cpu:
+        log("[bench] FP64 oracle on the host (accuracy reference + CPU baseline)")
+        fp64, t_cpu = cpu_reference(system, cfg, params)
+        for precision in ("fp32", "bf16"):
+            acc[precision] = accuracy(system, cfg, params, precision, fp64)
+        cores = len(os.sched_getaffinity(0))
+        cpu = {"value": round(N / t_cpu, 1), "unit": "atom-steps/s", "cores": cores, "kind": "port",
+               "sample": f"1 MD step (all-pairs nblist, AEV, MLP, forces, BAOAB) of the same {N}-atom box, "
+                         f"FP64 NumPy oracle, {t_cpu:.1f} s"}
+    out = {
+        "metric": "atom-steps/s and p50/p99 latency, batched BF16 vs plain FP32, at <=1 meV/atom err",
+        "value": round(value, 1),
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_bf, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded water box, seeded random-init weights)",
+        "config": {"workload": f"periodic water box, {N // 3} waters ({N} atoms), min-image, neighbour list "
+                               f"rebuilt every step, BAOAB Langevin dt=0.5 fs 300 K gamma=1/ps",
+                   "aev_length": cfg.aev_length, "mlp": "256-192-160-1 CELU(0.1), 4 species networks",
+                   "precision": "BF16 MLP GEMMs (FP32 accumulate), FP32 AEV, FP64 reductions and integrator"},
+        "fp32": {"value": round(value_fp32, 1), "ms_per_step": round(ms_fp, 4),
+                 "p50_ms": round(float(np.percentile(fp["lat"], 50)), 4),
+                 "p99_ms": round(float(np.percentile(fp["lat"], 99)), 4)},
+        "bf16_over_fp32": round(value / value_fp32, 3),
+        "p50_ms": round(float(np.percentile(bf["lat"], 50)), 4),
+        "p99_ms": round(float(np.percentile(bf["lat"], 99)), 4),
+        "accuracy_vs_fp64": acc,
+        "stages": stage_table,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,144 @@
+// Shared device helpers for the gfx950 kernels (wave64 CDNA4).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fnl_hip.h"
+
+#define FNL_WAVE 64
+
+#define FNL_LAUNCH_CHECK()                                  \
+  do {                                                      \
+    if (hipGetLastError() != hipSuccess) return FNL_ERR_HIP; \
+  } while (0)
+
+static inline int fnl_blocks(long long n, int per_block) {
+  return (int)((n + per_block - 1) / per_block);
+}
+
+static inline size_t fnl_align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// -------------------------------------------------------------------------
+// wave-level helpers
+// -------------------------------------------------------------------------
+__device__ __forceinline__ int fnl_lane() { return (int)__lane_id(); }
+
+// number of set bits of m below this lane
+__device__ __forceinline__ int fnl_rank_below(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+__device__ __forceinline__ float fnl_wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, FNL_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ int fnl_wave_max_i(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, FNL_WAVE));
+  return v;
+}
+
+// inclusive prefix sum over the wave
+__device__ __forceinline__ int fnl_wave_incl_scan(int v) {
+  const int lane = fnl_lane();
+#pragma unroll
+  for (int o = 1; o < FNL_WAVE; o <<= 1) {
+    int t = __shfl_up(v, o, FNL_WAVE);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// -------------------------------------------------------------------------
+// Minimum-image pair vector, float32, the reference's operation order
+// (models/preprocessing.py:127-143): vec = xj - xi; s = -rint(vec . R);
+// vec += s . C; no FMA contraction so that d12 matches bit for bit.
+// -------------------------------------------------------------------------
+struct FnlCell {
+  float c[9];  // rows = lattice vectors
+  float r[9];  // reciprocal (inverse)
+};
+
+__device__ __forceinline__ void fnl_load_cell(const float* __restrict__ cells,
+                                              const float* __restrict__ rec, int b,
+                                              FnlCell& cl) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    cl.c[k] = cells[b * 9 + k];
+    cl.r[k] = rec[b * 9 + k];
+  }
+}
+
+__device__ __forceinline__ void fnl_min_image(float& vx, float& vy, float& vz, const FnlCell& cl,
+                                              float* shift_out = nullptr) {
+#pragma clang fp contract(off)
+  const float f0 = (vx * cl.r[0] + vy * cl.r[3]) + vz * cl.r[6];
+  const float f1 = (vx * cl.r[1] + vy * cl.r[4]) + vz * cl.r[7];
+  const float f2 = (vx * cl.r[2] + vy * cl.r[5]) + vz * cl.r[8];
+  const float s0 = -rintf(f0), s1 = -rintf(f1), s2 = -rintf(f2);
+  vx = vx + ((s0 * cl.c[0] + s1 * cl.c[3]) + s2 * cl.c[6]);
+  vy = vy + ((s0 * cl.c[1] + s1 * cl.c[4]) + s2 * cl.c[7]);
+  vz = vz + ((s0 * cl.c[2] + s1 * cl.c[5]) + s2 * cl.c[8]);
+  if (shift_out) {
+    shift_out[0] = s0;
+    shift_out[1] = s1;
+    shift_out[2] = s2;
+  }
+}
+
+__device__ __forceinline__ float fnl_d12(float vx, float vy, float vz) {
+#pragma clang fp contract(off)
+  return (vx * vx + vy * vy) + vz * vz;
+}
+
+// -------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11) and Box-Muller normals.
+// -------------------------------------------------------------------------
+struct FnlU4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ FnlU4 fnl_philox4x32_10(FnlU4 c, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)M0 * c.x;
+    const uint64_t p1 = (uint64_t)M1 * c.z;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    FnlU4 n;
+    n.x = hi1 ^ c.y ^ k0;
+    n.y = lo1;
+    n.z = hi0 ^ c.w ^ k1;
+    n.w = lo0;
+    c = n;
+    k0 += W0;
+    k1 += W1;
+  }
+  return c;
+}
+
+// three N(0,1) doubles for (atom, step, stream) under key seed
+__device__ __forceinline__ void fnl_normals3(uint64_t seed, uint64_t step, uint32_t stream,
+                                             uint32_t atom, double out[3]) {
+  FnlU4 c{atom, (uint32_t)step, (uint32_t)(step >> 32), stream};
+  FnlU4 r = fnl_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const double s = 2.3283064365386963e-10;  // 2^-32
+  const double u0 = ((double)r.x + 0.5) * s, u1 = ((double)r.y + 0.5) * s;
+  const double u2 = ((double)r.z + 0.5) * s, u3 = ((double)r.w + 0.5) * s;
+  const double two_pi = 6.283185307179586;
+  const double rad0 = sqrt(-2.0 * log(u0));
+  const double rad1 = sqrt(-2.0 * log(u2));
+  double s0, c0, s1, c1;
+  sincos(two_pi * u1, &s0, &c0);
+  sincos(two_pi * u3, &s1, &c1);
+  out[0] = rad0 * c0;
+  out[1] = rad0 * s0;
+  out[2] = rad1 * c1;
+}

@@ -1,0 +1,151 @@
+// Per-system FP64 reductions, force assembly and the BAOAB Langevin step.
+//
+// References: total_energy segment_sum (models/fennix.py:240-246), virial
+// (fennix.py:320-347), kinetic tensor (md/integrate.py:320-339), BAOAB
+// integrate (md/integrate.py:286-294) with the LGV thermostat O-step
+// (md/thermostats.py:108-120).
+
+#include "fnl_common.h"
+
+namespace {
+
+// One workgroup per system, fixed-order tree: bitwise reproducible.
+__global__ void __launch_bounds__(256)
+k_system_reduce(const int32_t* __restrict__ sys_offsets, const float* __restrict__ energy_atom,
+                double* __restrict__ energy_sys, const float* __restrict__ virial_atom,
+                double* __restrict__ virial_sys, const double* __restrict__ vel,
+                const double* __restrict__ mass, double* __restrict__ kinetic_sys) {
+  constexpr int NV = 1 + 9 + 9;
+  __shared__ double red[NV][256];
+  const int b = blockIdx.x;
+  const int t = threadIdx.x;
+  const int a0 = sys_offsets[b], a1 = sys_offsets[b + 1];
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int i = a0 + t; i < a1; i += 256) {
+    if (energy_atom) acc[0] += (double)energy_atom[i];
+    if (virial_atom) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) acc[1 + k] += (double)virial_atom[9 * i + k];
+    }
+    if (vel) {
+      const double m = 0.5 * mass[i];
+      const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+      acc[10] += m * vx * vx; acc[11] += m * vx * vy; acc[12] += m * vx * vz;
+      acc[13] += m * vy * vx; acc[14] += m * vy * vy; acc[15] += m * vy * vz;
+      acc[16] += m * vz * vx; acc[17] += m * vz * vy; acc[18] += m * vz * vz;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) red[k][t] = acc[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) red[k][t] += red[k][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (energy_atom) energy_sys[b] = red[0][0];
+    if (virial_atom)
+      for (int k = 0; k < 9; ++k) virial_sys[9 * b + k] = red[1 + k][0];
+    if (vel)
+      for (int k = 0; k < 9; ++k) kinetic_sys[9 * b + k] = red[10 + k][0];
+  }
+}
+
+__global__ void k_forces(const float* __restrict__ gs, const float* __restrict__ gn, int n3, float scale,
+                         float* __restrict__ f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) f[i] = -(gs[i] + gn[i]) * scale;
+}
+
+__global__ void k_baoab(double* __restrict__ x, double* __restrict__ v, float* __restrict__ x32,
+                        const float* __restrict__ gs, const float* __restrict__ gn, float force_scale,
+                        const double* __restrict__ dtm, double half_dt, double a1,
+                        const double* __restrict__ a2, const double* __restrict__ noise, uint64_t seed,
+                        uint64_t step, int n_atoms) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_atoms) return;
+  double xi[3];
+  if (a2 && !noise) fnl_normals3(seed, step, 0u, (uint32_t)i, xi);
+  const double m = dtm[i];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double f = -((double)gs[3 * i + k] + (double)gn[3 * i + k]) * (double)force_scale;
+    double vv = v[3 * i + k] + m * f;           // B (full dt: merged half kicks)
+    double xx = x[3 * i + k] + half_dt * vv;    // A
+    if (a2) {                                   // O
+      const double nz = noise ? noise[3 * i + k] : xi[k];
+      vv = a1 * vv + a2[i] * nz;
+    }
+    xx = xx + half_dt * vv;                     // A
+    v[3 * i + k] = vv;
+    x[3 * i + k] = xx;
+    x32[3 * i + k] = (float)xx;
+  }
+}
+
+__global__ void k_normals(double* __restrict__ out, int n_atoms, uint64_t seed, uint64_t step, uint32_t sid) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_atoms) return;
+  double z[3];
+  fnl_normals3(seed, step, sid, (uint32_t)i, z);
+  out[3 * i] = z[0];
+  out[3 * i + 1] = z[1];
+  out[3 * i + 2] = z[2];
+}
+
+}  // namespace
+
+extern "C" int fnl_system_reduce(const int32_t* sys_offsets, int32_t n_sys, const float* energy_atom,
+                                 double* energy_sys, const float* virial_atom, double* virial_sys,
+                                 const double* vel, const double* mass, double* kinetic_sys,
+                                 fnl_stream_t stream) {
+  if (n_sys < 0) return FNL_ERR_INVALID;
+  if (n_sys == 0) return FNL_OK;
+  if (!sys_offsets) return FNL_ERR_INVALID;
+  if ((energy_atom && !energy_sys) || (virial_atom && !virial_sys) || (vel && (!mass || !kinetic_sys)))
+    return FNL_ERR_INVALID;
+  hipLaunchKernelGGL(k_system_reduce, dim3(n_sys), dim3(256), 0, (hipStream_t)stream, sys_offsets, energy_atom,
+                     energy_sys, virial_atom, virial_sys, vel, mass, kinetic_sys);
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
+}
+
+extern "C" int fnl_forces(const float* grad_self, const float* grad_nbr, int32_t n_atoms, float scale,
+                          float* forces, fnl_stream_t stream) {
+  if (n_atoms < 0) return FNL_ERR_INVALID;
+  if (n_atoms == 0) return FNL_OK;
+  if (!grad_self || !grad_nbr || !forces) return FNL_ERR_INVALID;
+  hipLaunchKernelGGL(k_forces, dim3(fnl_blocks(3LL * n_atoms, 256)), dim3(256), 0, (hipStream_t)stream,
+                     grad_self, grad_nbr, 3 * n_atoms, scale, forces);
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
+}
+
+extern "C" int fnl_baoab_step(double* x, double* v, float* x32, const float* grad_self, const float* grad_nbr,
+                              float force_scale, const double* dtm, double half_dt, double a1, const double* a2,
+                              const double* noise, uint64_t seed, uint64_t step, int32_t n_atoms,
+                              fnl_stream_t stream) {
+  if (n_atoms < 0) return FNL_ERR_INVALID;
+  if (n_atoms == 0) return FNL_OK;
+  if (!x || !v || !x32 || !grad_self || !grad_nbr || !dtm) return FNL_ERR_INVALID;
+  hipLaunchKernelGGL(k_baoab, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, (hipStream_t)stream, x, v, x32,
+                     grad_self, grad_nbr, force_scale, dtm, half_dt, a1, a2, noise, seed, step, n_atoms);
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
+}
+
+extern "C" int fnl_philox_normals(double* out, int32_t n_atoms, uint64_t seed, uint64_t step, uint32_t stream_id,
+                                  fnl_stream_t stream) {
+  if (n_atoms < 0) return FNL_ERR_INVALID;
+  if (n_atoms == 0) return FNL_OK;
+  if (!out) return FNL_ERR_INVALID;
+  hipLaunchKernelGGL(k_normals, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, (hipStream_t)stream, out, n_atoms,
+                     seed, step, stream_id);
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
+}

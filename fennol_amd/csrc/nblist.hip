@@ -1,0 +1,532 @@
+// Cell-list neighbour lists for gfx950.
+//
+// Replaces FeNNol's all-pairs GraphGenerator (models/preprocessing.py:74-638)
+// and the angular GraphFilter (:780-960) with an O(N) cell list that keeps the
+// reference's pair set bit for bit: every candidate pair is tested with the
+// reference's float32 arithmetic (fnl_min_image / fnl_d12), and the candidate
+// set (3x3x3 neighbouring cells, deduplicated for grids narrower than 3) is a
+// superset of the pairs within the cutoff because each cell is at least one
+// cutoff wide.
+//
+// Pipeline (all on the caller's stream):
+//   k_grid      one workgroup per system: grid dims (+ bounding box if open)
+//   k_bin       one thread per atom: cell id, population count
+//   k_scan      one workgroup: exclusive scan of cell populations
+//   k_scatter   one thread per atom: place atom in its cell segment
+//   k_cell_sort one wave per cell: order each segment by atom index
+//               (deterministic rows) and write cell-sorted float4 positions
+//   k_pairs     one wave per centre, centres in cell order: candidates of the
+//               neighbouring cells tested 64 at a time, kept pairs compacted
+//               with a wave ballot + prefix popcount into the centre's row.
+
+#include "fnl_common.h"
+
+namespace {
+
+constexpr int kGridWords = 12;  // per system: nx ny nz first | ox oy oz | iwx iwy iwz | pbc pad
+
+struct NblistWs {
+  int32_t* grid;       // [n_sys][kGridWords] (floats stored as bits)
+  int32_t* atom_cell;  // [N]
+  int32_t* cell_count; // [n_cells]
+  int32_t* cell_fill;  // [n_cells]
+  int32_t* cell_start; // [n_cells + 1]
+  int32_t* tmp_sorted; // [N]
+  int32_t* sorted;     // [N]
+  float4* pos;         // [N] cell-sorted x, y, z, atom index bits
+};
+
+size_t ws_layout(int n_atoms, int n_sys, int n_cells, char* base, NblistWs* ws) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = fnl_align256(off + bytes);
+    return o;
+  };
+  size_t o_grid = take((size_t)n_sys * kGridWords * 4);
+  size_t o_ac = take((size_t)n_atoms * 4);
+  size_t o_cc = take((size_t)n_cells * 4);
+  size_t o_cf = take((size_t)n_cells * 4);
+  size_t o_cs = take((size_t)(n_cells + 1) * 4);
+  size_t o_ts = take((size_t)n_atoms * 4);
+  size_t o_s = take((size_t)n_atoms * 4);
+  size_t o_p = take((size_t)n_atoms * 16);
+  if (base && ws) {
+    ws->grid = (int32_t*)(base + o_grid);
+    ws->atom_cell = (int32_t*)(base + o_ac);
+    ws->cell_count = (int32_t*)(base + o_cc);
+    ws->cell_fill = (int32_t*)(base + o_cf);
+    ws->cell_start = (int32_t*)(base + o_cs);
+    ws->tmp_sorted = (int32_t*)(base + o_ts);
+    ws->sorted = (int32_t*)(base + o_s);
+    ws->pos = (float4*)(base + o_p);
+  }
+  return off;
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int32_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, float v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// Cell widths must exceed the cutoff with a margin so that rounding in the
+// bin coordinates cannot push two atoms within the cutoff two cells apart.
+__device__ __forceinline__ int cells_along(float width, float rc, int cap) {
+  const float rbin = rc * 1.0001f + 1.0e-4f;
+  int n = (int)floorf(width / rbin);
+  n = max(n, 1);
+  return min(n, max(cap, 1));
+}
+
+__global__ void k_grid(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
+                       const int32_t* __restrict__ sys_offsets, int pbc_mode,
+                       const float* __restrict__ rec, const int32_t* __restrict__ grid_cap,
+                       float cutoff, int32_t* __restrict__ grid, int32_t* __restrict__ status) {
+  const int b = blockIdx.x;
+  const int t = threadIdx.x;
+  if (b == 0 && t < FNL_STATUS_STICKY) status[t] = 0;  // the sticky word persists
+  const int capx = grid_cap[b * 4 + 0], capy = grid_cap[b * 4 + 1], capz = grid_cap[b * 4 + 2];
+  const int first = grid_cap[b * 4 + 3];
+  int32_t* g = grid + b * kGridWords;
+  if (pbc_mode == FNL_PBC_MINIMUM_IMAGE) {
+    if (t == 0) {
+      const float* R = rec + b * 9;
+      // lattice-plane spacing along a = 1 / |column a of R|
+      int n[3];
+      for (int a = 0; a < 3; ++a) {
+        const float nrm = sqrtf(R[a] * R[a] + R[3 + a] * R[3 + a] + R[6 + a] * R[6 + a]);
+        const int cap = a == 0 ? capx : (a == 1 ? capy : capz);
+        n[a] = cells_along(1.0f / nrm, cutoff, cap);
+      }
+      g[0] = n[0]; g[1] = n[1]; g[2] = n[2]; g[3] = first;
+      g[4] = g[5] = g[6] = 0;
+      g[7] = __float_as_int((float)n[0]);
+      g[8] = __float_as_int((float)n[1]);
+      g[9] = __float_as_int((float)n[2]);
+      g[10] = 1;
+      g[11] = 0;
+    }
+    return;
+  }
+  // open boundaries: bounding box of the system's real atoms
+  __shared__ float smin[3][256];
+  __shared__ float smax[3][256];
+  const int a0 = sys_offsets[b], a1 = sys_offsets[b + 1];
+  float mn[3] = {3.0e38f, 3.0e38f, 3.0e38f}, mx[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+  for (int i = a0 + t; i < a1; i += blockDim.x) {
+    if (species_idx[i] < 0) continue;
+    for (int a = 0; a < 3; ++a) {
+      const float x = coords[3 * i + a];
+      mn[a] = fminf(mn[a], x);
+      mx[a] = fmaxf(mx[a], x);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    smin[a][t] = mn[a];
+    smax[a][t] = mx[a];
+  }
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      for (int a = 0; a < 3; ++a) {
+        smin[a][t] = fminf(smin[a][t], smin[a][t + s]);
+        smax[a][t] = fmaxf(smax[a][t], smax[a][t + s]);
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    int n[3];
+    float inv_w[3];
+    for (int a = 0; a < 3; ++a) {
+      const float ext = smax[a][0] - smin[a][0];
+      const int cap = a == 0 ? capx : (a == 1 ? capy : capz);
+      n[a] = (ext > 0.0f) ? cells_along(ext, cutoff, cap) : 1;
+      inv_w[a] = (n[a] > 1) ? (float)n[a] / ext : 0.0f;
+    }
+    g[0] = n[0]; g[1] = n[1]; g[2] = n[2]; g[3] = first;
+    g[4] = __float_as_int(smin[0][0] < 1.0e38f ? smin[0][0] : 0.0f);
+    g[5] = __float_as_int(smin[1][0] < 1.0e38f ? smin[1][0] : 0.0f);
+    g[6] = __float_as_int(smin[2][0] < 1.0e38f ? smin[2][0] : 0.0f);
+    g[7] = __float_as_int(inv_w[0]);
+    g[8] = __float_as_int(inv_w[1]);
+    g[9] = __float_as_int(inv_w[2]);
+    g[10] = 0;
+    g[11] = 0;
+  }
+}
+
+__device__ __forceinline__ int3 atom_cell3(const float* __restrict__ coords, int i,
+                                           const int32_t* __restrict__ g, const float* __restrict__ rec,
+                                           int b) {
+  const float x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
+  const int nx = g[0], ny = g[1], nz = g[2];
+  float f[3];
+  if (g[10]) {
+    const float* R = rec + b * 9;
+    f[0] = (x * R[0] + y * R[3]) + z * R[6];
+    f[1] = (x * R[1] + y * R[4]) + z * R[7];
+    f[2] = (x * R[2] + y * R[5]) + z * R[8];
+    for (int a = 0; a < 3; ++a) f[a] = (f[a] - floorf(f[a])) * __int_as_float(g[7 + a]);
+  } else {
+    f[0] = (x - __int_as_float(g[4])) * __int_as_float(g[7]);
+    f[1] = (y - __int_as_float(g[5])) * __int_as_float(g[8]);
+    f[2] = (z - __int_as_float(g[6])) * __int_as_float(g[9]);
+  }
+  int3 c;
+  c.x = min(max((int)floorf(f[0]), 0), nx - 1);
+  c.y = min(max((int)floorf(f[1]), 0), ny - 1);
+  c.z = min(max((int)floorf(f[2]), 0), nz - 1);
+  return c;
+}
+
+__global__ void k_bin(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
+                      const int32_t* __restrict__ batch_index, int n_atoms,
+                      const float* __restrict__ rec, const int32_t* __restrict__ grid,
+                      int32_t* __restrict__ atom_cell, int32_t* __restrict__ cell_count,
+                      int32_t* __restrict__ n_nbr, int32_t* __restrict__ n_nbr_ang, int n_cells,
+                      int32_t* __restrict__ status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_atoms) return;
+  if (species_idx[i] < 0) {
+    atom_cell[i] = -1;
+    n_nbr[i] = 0;
+    n_nbr_ang[i] = 0;
+    return;
+  }
+  const int b = batch_index[i];
+  const int32_t* g = grid + b * kGridWords;
+  const int3 c = atom_cell3(coords, i, g, rec, b);
+  const int cell = g[3] + (c.x * g[1] + c.y) * g[2] + c.z;
+  if (cell < 0 || cell >= n_cells) {  // inconsistent grid capacities: drop and flag
+    atom_cell[i] = -1;
+    n_nbr[i] = 0;
+    n_nbr_ang[i] = 0;
+    atomicOr(&status[FNL_STATUS_OVERFLOW], 2);
+    atomicOr(&status[FNL_STATUS_STICKY], 2);
+    return;
+  }
+  atom_cell[i] = cell;
+  atomicAdd(&cell_count[cell], 1);
+}
+
+// single-workgroup exclusive scan (1024 threads, chunked)
+__global__ void __launch_bounds__(1024) k_scan(const int32_t* __restrict__ count, int n,
+                                              int32_t* __restrict__ start, int32_t* __restrict__ status) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  const int t = threadIdx.x;
+  const int lane = t & 63, w = t >> 6;
+  if (t == 0) carry = 0;
+  int cmax = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += 1024) {
+    const int idx = base + t;
+    const int v = idx < n ? count[idx] : 0;
+    cmax = max(cmax, v);
+    int incl = fnl_wave_incl_scan(v);
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (w == 0) {
+      int s = lane < 16 ? wsum[lane] : 0;
+      int si = fnl_wave_incl_scan(s);
+      if (lane < 16) wsum[lane] = si - s;
+    }
+    __syncthreads();
+    const int excl = carry + wsum[w] + incl - v;
+    if (idx < n) start[idx] = excl;
+    __syncthreads();
+    if (t == 1023) carry = excl + v;
+    __syncthreads();
+  }
+  if (t == 0) start[n] = carry;
+  cmax = fnl_wave_max_i(cmax);
+  if (lane == 0 && cmax > 0) atomicMax(&status[FNL_STATUS_MAX_CELL], cmax);
+}
+
+__global__ void k_scatter(int n_atoms, const int32_t* __restrict__ atom_cell,
+                          const int32_t* __restrict__ cell_start, int32_t* __restrict__ cell_fill,
+                          int32_t* __restrict__ tmp_sorted) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_atoms) return;
+  const int c = atom_cell[i];
+  if (c < 0) return;
+  const int p = cell_start[c] + atomicAdd(&cell_fill[c], 1);
+  tmp_sorted[p] = i;
+}
+
+// one wave per cell: rank sort of the segment (atom indices are distinct)
+__global__ void k_cell_sort(int n_cells, const int32_t* __restrict__ cell_start,
+                            const int32_t* __restrict__ tmp_sorted, const float* __restrict__ coords,
+                            int32_t* __restrict__ sorted, float4* __restrict__ pos) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = fnl_lane();
+  if (wave >= n_cells) return;
+  const int s = cell_start[wave], e = cell_start[wave + 1];
+  const int cnt = e - s;
+  for (int a = lane; a < cnt; a += 64) {
+    const int v = tmp_sorted[s + a];
+    int rank = 0;
+    for (int q = 0; q < cnt; ++q) rank += (tmp_sorted[s + q] < v) ? 1 : 0;
+    sorted[s + rank] = v;
+    pos[s + rank] = make_float4(coords[3 * v], coords[3 * v + 1], coords[3 * v + 2], __int_as_float(v));
+  }
+}
+
+// offsets along one axis of the 3x3x3 neighbourhood, deduplicated
+__device__ __forceinline__ int axis_offsets(int c, int n, bool pbc, int* out) {
+  if (pbc) {
+    if (n == 1) { out[0] = 0; return 1; }
+    if (n == 2) { out[0] = 0; out[1] = 1 - 2 * c; return 2; }  // the two cells {0, 1}
+    out[0] = (c + n - 1) % n - c; out[1] = 0; out[2] = (c + 1) % n - c;
+    return 3;
+  }
+  int k = 0;
+  if (c > 0) out[k++] = -1;
+  out[k++] = 0;
+  if (c < n - 1) out[k++] = 1;
+  return k;
+}
+
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
+k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_index,
+        int n_binned_cap, const float* __restrict__ cells, const float* __restrict__ rec,
+        const int32_t* __restrict__ grid, const int32_t* __restrict__ atom_cell,
+        const int32_t* __restrict__ cell_start, const int32_t* __restrict__ sorted,
+        const float4* __restrict__ pos, float rc2, float rca2, int max_nbr, int max_nbr_ang,
+        int32_t* __restrict__ nbr, int32_t* __restrict__ n_nbr, int32_t* __restrict__ nbr_ang,
+        int32_t* __restrict__ n_nbr_ang, int32_t* __restrict__ status, int n_cells) {
+  __shared__ int s_cstart[WAVES][27];
+  __shared__ int s_cpre[WAVES][28];
+  const int w = threadIdx.x >> 6;
+  const int lane = fnl_lane();
+  const int slot = blockIdx.x * WAVES + w;
+  const int n_binned = cell_start[n_cells];
+  const bool active = slot < n_binned && slot < n_binned_cap;
+  int i = 0, b = 0, ncell = 0, total = 0, first = 0, nx = 1, ny = 1, nz = 1;
+  bool pbc = false;
+  if (active) {
+    i = sorted[slot];
+    b = batch_index[i];
+    const int32_t* g = grid + b * kGridWords;
+    nx = g[0]; ny = g[1]; nz = g[2]; first = g[3];
+    pbc = g[10] != 0;
+    const int cell = atom_cell[i] - first;
+    const int cz = cell % nz, cy = (cell / nz) % ny, cx = cell / (ny * nz);
+    int ox[3], oy[3], oz[3];
+    const int nox = axis_offsets(cx, nx, pbc, ox);
+    const int noy = axis_offsets(cy, ny, pbc, oy);
+    const int noz = axis_offsets(cz, nz, pbc, oz);
+    ncell = nox * noy * noz;
+    int cnt = 0;
+    if (lane < ncell) {
+      const int a = lane / (noy * noz), r = lane % (noy * noz);
+      const int bb = r / noz, c = r % noz;
+      const int nc = first + ((cx + ox[a]) * ny + (cy + oy[bb])) * nz + (cz + oz[c]);
+      s_cstart[w][lane] = cell_start[nc];
+      cnt = cell_start[nc + 1] - cell_start[nc];
+    }
+    const int incl = fnl_wave_incl_scan(lane < ncell ? cnt : 0);
+    if (lane < ncell) s_cpre[w][lane + 1] = incl;
+    if (lane == 0) s_cpre[w][0] = 0;
+    total = __shfl(incl, ncell - 1, 64);
+  }
+  __syncthreads();
+  if (!active) return;
+
+  FnlCell cl;
+  if (pbc) fnl_load_cell(cells, rec, b, cl);
+  const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
+  int count = 0, count_a = 0;
+  int32_t* row = nbr + (size_t)i * max_nbr;
+  int32_t* row_a = nbr_ang + (size_t)i * max_nbr_ang;
+  for (int base = 0; base < total; base += 64) {
+    const int t = base + lane;
+    bool keep = false, keep_a = false;
+    int j = -1;
+    if (t < total) {
+      int c = 0;
+      while (s_cpre[w][c + 1] <= t) ++c;
+      const float4 pj = pos[s_cstart[w][c] + (t - s_cpre[w][c])];
+      j = __float_as_int(pj.w);
+      if (j != i) {
+        float vx = pj.x - xi, vy = pj.y - yi, vz = pj.z - zi;
+        if (pbc) fnl_min_image(vx, vy, vz, cl);
+        const float d12 = fnl_d12(vx, vy, vz);
+        keep = d12 < rc2;
+        keep_a = d12 < rca2;
+      }
+    }
+    const unsigned long long m = __ballot(keep);
+    const unsigned long long ma = __ballot(keep_a);
+    if (keep) {
+      const int e = count + fnl_rank_below(m);
+      if (e < max_nbr) row[e] = j;
+      if (keep_a) {
+        const int ea = count_a + fnl_rank_below(ma);
+        if (ea < max_nbr_ang && e < max_nbr) row_a[ea] = e;
+      }
+    }
+    count += __popcll(m);
+    count_a += __popcll(ma);
+  }
+  if (lane == 0) {
+    n_nbr[i] = count;
+    n_nbr_ang[i] = count_a;
+    if (count > max_nbr || count_a > max_nbr_ang) {
+      atomicOr(&status[FNL_STATUS_OVERFLOW], 1);
+      atomicOr(&status[FNL_STATUS_STICKY], 1);
+    }
+    atomicMax(&status[FNL_STATUS_MAX_NBR], count);
+    atomicMax(&status[FNL_STATUS_MAX_NBR_ANG], count_a);
+  }
+}
+
+__global__ void k_export(const float* __restrict__ coords, const int32_t* __restrict__ batch_index,
+                         int n_atoms, int pbc_mode, const float* __restrict__ cells,
+                         const float* __restrict__ rec, const int32_t* __restrict__ nbr,
+                         const int32_t* __restrict__ n_nbr, int max_nbr,
+                         const int32_t* __restrict__ row_start, int32_t* __restrict__ edge_src,
+                         int32_t* __restrict__ edge_dst, float* __restrict__ d12,
+                         float* __restrict__ shifts) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = fnl_lane();
+  if (wave >= n_atoms) return;
+  const int i = wave;
+  const int n = min(n_nbr[i], max_nbr);
+  const int b = batch_index[i];
+  FnlCell cl;
+  const bool pbc = pbc_mode == FNL_PBC_MINIMUM_IMAGE;
+  if (pbc) fnl_load_cell(cells, rec, b, cl);
+  const int o = row_start[i];
+  for (int e = lane; e < n; e += 64) {
+    const int j = nbr[(size_t)i * max_nbr + e];
+    float vx = coords[3 * j] - coords[3 * i];
+    float vy = coords[3 * j + 1] - coords[3 * i + 1];
+    float vz = coords[3 * j + 2] - coords[3 * i + 2];
+    float s[3] = {0.f, 0.f, 0.f};
+    if (pbc) fnl_min_image(vx, vy, vz, cl, s);
+    edge_src[o + e] = i;
+    edge_dst[o + e] = j;
+    d12[o + e] = fnl_d12(vx, vy, vz);
+    if (shifts) {
+      shifts[3 * (o + e)] = s[0];
+      shifts[3 * (o + e) + 1] = s[1];
+      shifts[3 * (o + e) + 2] = s[2];
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int fnl_version(void) { return 10000; }
+
+extern "C" const char* fnl_strerror(int code) {
+  switch (code) {
+    case FNL_OK: return "ok";
+    case FNL_ERR_INVALID: return "invalid argument";
+    case FNL_ERR_HIP: return "HIP launch error";
+    case FNL_ERR_UNSUPPORTED: return "unsupported configuration";
+    default: return "unknown error";
+  }
+}
+
+extern "C" int fnl_fill_i32(int32_t* p, int32_t value, size_t count, fnl_stream_t stream) {
+  if (count == 0) return FNL_OK;
+  if (!p) return FNL_ERR_INVALID;
+  const int blocks = (int)std::min<size_t>((count + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks), dim3(256), 0, (hipStream_t)stream, p, value, count);
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
+}
+
+extern "C" int fnl_fill_f32(float* p, float value, size_t count, fnl_stream_t stream) {
+  if (count == 0) return FNL_OK;
+  if (!p) return FNL_ERR_INVALID;
+  const int blocks = (int)std::min<size_t>((count + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_f32, dim3(blocks), dim3(256), 0, (hipStream_t)stream, p, value, count);
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
+}
+
+extern "C" size_t fnl_nblist_workspace(int32_t n_atoms, int32_t n_sys, int32_t n_cells) {
+  if (n_atoms < 0 || n_sys < 1 || n_cells < 1) return 0;
+  return ws_layout(n_atoms, n_sys, n_cells, nullptr, nullptr);
+}
+
+extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
+                                const int32_t* batch_index, const int32_t* sys_offsets,
+                                int32_t n_atoms, int32_t n_sys, int32_t pbc_mode,
+                                const float* cells, const float* reciprocal_cells,
+                                const int32_t* grid, int32_t n_cells, float cutoff,
+                                float cutoff_ang, int32_t max_nbr, int32_t max_nbr_ang,
+                                void* workspace, size_t workspace_bytes, int32_t* nbr,
+                                int32_t* n_nbr, int32_t* nbr_ang, int32_t* n_nbr_ang,
+                                int32_t* status, fnl_stream_t stream) {
+  if (n_atoms < 0 || n_sys < 1 || n_cells < 1 || max_nbr < 1 || max_nbr_ang < 1) return FNL_ERR_INVALID;
+  if (!(cutoff > 0.0f) || cutoff_ang > cutoff) return FNL_ERR_INVALID;
+  if (pbc_mode != FNL_PBC_NONE && pbc_mode != FNL_PBC_MINIMUM_IMAGE) return FNL_ERR_UNSUPPORTED;
+  if (pbc_mode == FNL_PBC_MINIMUM_IMAGE && (!cells || !reciprocal_cells)) return FNL_ERR_INVALID;
+  if (!status || !grid || !sys_offsets) return FNL_ERR_INVALID;
+  if (n_atoms > 0 && (!coords || !species_idx || !batch_index || !nbr || !n_nbr || !nbr_ang || !n_nbr_ang))
+    return FNL_ERR_INVALID;
+  if (!workspace || workspace_bytes < fnl_nblist_workspace(n_atoms, n_sys, n_cells)) return FNL_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  NblistWs ws;
+  ws_layout(n_atoms, n_sys, n_cells, (char*)workspace, &ws);
+  const float rc2 = (float)((double)cutoff * (double)cutoff);
+  const float rca2 = (float)((double)cutoff_ang * (double)cutoff_ang);
+
+  hipLaunchKernelGGL(k_grid, dim3(n_sys), dim3(256), 0, s, coords, species_idx, sys_offsets, pbc_mode,
+                     reciprocal_cells, grid, cutoff, ws.grid, status);
+  FNL_LAUNCH_CHECK();
+  if (n_atoms == 0) return FNL_OK;
+  int rc = fnl_fill_i32(ws.cell_count, 0, (size_t)n_cells, stream);
+  if (rc) return rc;
+  rc = fnl_fill_i32(ws.cell_fill, 0, (size_t)n_cells, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_bin, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, s, coords, species_idx,
+                     batch_index, n_atoms, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_count,
+                     n_nbr, n_nbr_ang, n_cells, status);
+  FNL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, ws.cell_count, n_cells, ws.cell_start, status);
+  FNL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scatter, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, s, n_atoms, ws.atom_cell,
+                     ws.cell_start, ws.cell_fill, ws.tmp_sorted);
+  FNL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_cell_sort, dim3(fnl_blocks((long long)n_cells * 64, 256)), dim3(256), 0, s, n_cells,
+                     ws.cell_start, ws.tmp_sorted, coords, ws.sorted, ws.pos);
+  FNL_LAUNCH_CHECK();
+  constexpr int W = 4;
+  hipLaunchKernelGGL(k_pairs<W>, dim3(fnl_blocks(n_atoms, W)), dim3(64 * W), 0, s, coords, batch_index,
+                     n_atoms, cells, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_start, ws.sorted,
+                     ws.pos, rc2, rca2, max_nbr, max_nbr_ang, nbr, n_nbr, nbr_ang, n_nbr_ang, status,
+                     n_cells);
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
+}
+
+extern "C" int fnl_nblist_export(const float* coords, const int32_t* batch_index, int32_t n_atoms,
+                                 int32_t pbc_mode, const float* cells, const float* reciprocal_cells,
+                                 const int32_t* nbr, const int32_t* n_nbr, int32_t max_nbr,
+                                 const int32_t* row_start, int32_t* edge_src, int32_t* edge_dst,
+                                 float* d12, float* pbc_shifts, fnl_stream_t stream) {
+  if (n_atoms < 0 || max_nbr < 1) return FNL_ERR_INVALID;
+  if (n_atoms == 0) return FNL_OK;
+  if (!coords || !batch_index || !nbr || !n_nbr || !row_start || !edge_src || !edge_dst || !d12)
+    return FNL_ERR_INVALID;
+  if (pbc_mode == FNL_PBC_MINIMUM_IMAGE && (!cells || !reciprocal_cells)) return FNL_ERR_INVALID;
+  hipLaunchKernelGGL(k_export, dim3(fnl_blocks((long long)n_atoms * 64, 256)), dim3(256), 0,
+                     (hipStream_t)stream, coords, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells,
+                     nbr, n_nbr, max_nbr, row_start, edge_src, edge_dst, d12, pbc_shifts);
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
+}

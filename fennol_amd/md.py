@@ -1,0 +1,135 @@
+"""Langevin (BAOAB) and NVE molecular dynamics on the device.
+
+Mirrors fennol_md's step (md/integrate.py:570-600): ``integrate`` (BAOAB with
+merged half kicks, :286-294, LGV O-step md/thermostats.py:108-120), then the
+neighbour-list rebuild (``update_graphs``, :515-566, every step without a
+skin), then ``update_observables`` (energy and forces, :298-339).  Positions
+and velocities are FP64 on the device; the model reads a float32 copy written
+by the integrator kernel.  Units are fennol_md's (Angstrom, ps, kcal/mol);
+the model's energies are converted with its energy unit (integrate.py:303-305).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _abi, units
+from .potential import ANIPotential, SystemBuffers
+
+
+class LangevinMD:
+    """Device-resident BAOAB Langevin (``thermostat="LGV"``) or NVE dynamics.
+
+    ``system``: FeNNol input dict (species, coordinates [A], natoms, optional
+    cells).  ``seed`` keys the Philox streams: stream 1 draws the initial
+    velocities v ~ N(0, kT/m) (thermostats.py:64-67), stream 0 the O-step
+    noise of step n.  For NVE the initial velocities are rescaled to exactly
+    kT per degree of freedom (thermostats.py:172-189).
+    """
+
+    def __init__(self, potential: ANIPotential, system: dict, dt_fs: float = 0.5, temperature: float = 300.0,
+                 gamma_thz: float = 1.0, seed: int = 0, thermostat: str = "LGV", energy_unit: str = "eV",
+                 check_every: int = 1, sb: SystemBuffers | None = None):
+        self.pot = potential
+        self.lib = potential.lib
+        dev = potential.device
+        species = np.asarray(system["species"], np.int32)
+        natoms = system.get("natoms")
+        cells = system.get("cells")
+        self.sb = sb if sb is not None else potential.setup(species, natoms, cells)
+        self.n = species.shape[0]
+        mass_da = units.ATOMIC_MASSES[np.clip(species, 0, None)]
+        self.mass = np.where(species > 0, mass_da / units.DA, 1.0)       # md/initial.py:79-108
+        self.dt = dt_fs / units.FS_PER_PS                                  # ps
+        self.gamma = gamma_thz / units.THZ                                 # ps^-1
+        self.kT = units.K_B * temperature                                  # kcal/mol
+        self.thermostat = thermostat.upper()
+        self.seed = int(seed)
+        self.istep = 0
+        # 1: read the overflow flag after every rebuild (the reference's protocol);
+        # 0: no per-step host read -- capacities are sized with headroom up front
+        # and the sticky overflow word is checked by verify()
+        self.check_every = int(check_every)
+        self.model_energy_unit = units.energy_multiplier(energy_unit)
+        self.force_scale = 1.0 / self.model_energy_unit                    # model unit -> kcal/mol
+        self.t_mass = torch.tensor(self.mass, dtype=torch.float64, device=dev)
+        self.t_dtm = torch.tensor(self.dt / self.mass, dtype=torch.float64, device=dev)
+        if self.thermostat in ("LGV", "LANGEVIN"):
+            self.a1 = math.exp(-self.gamma * self.dt)                      # thermostats.py:62-63
+            a2 = np.sqrt((1.0 - self.a1 * self.a1) * self.kT / self.mass)
+            self.t_a2 = torch.tensor(np.where(species > 0, a2, 0.0), dtype=torch.float64, device=dev)
+        elif self.thermostat in ("NVE", "NONE"):
+            self.a1 = 1.0
+            self.t_a2 = None
+        else:
+            raise NotImplementedError(f"thermostat {thermostat}")
+        self.x = torch.tensor(np.asarray(system["coordinates"], np.float64), device=dev).contiguous()
+        self.x32 = self.x.float().contiguous()
+        self.v = torch.empty((self.n, 3), dtype=torch.float64, device=dev)
+        _abi.check(self.lib.fnl_philox_normals(self.v.data_ptr(), self.n, self.seed, 0, 1, _abi.stream_handle()),
+                   "fnl_philox_normals")
+        self.v *= torch.sqrt(self.kT / self.t_mass)[:, None]
+        self.v[torch.tensor(species <= 0, device=dev)] = 0.0
+        if self.t_a2 is None:
+            nreal = int((species > 0).sum())
+            kt_sys = float((self.t_mass[:, None] * self.v ** 2).sum()) / (3 * nreal)
+            self.v *= math.sqrt(self.kT / kt_sys)
+        if self.check_every == 0:
+            self.pot.fit_capacity(self.sb, self.x32)
+        # initial forces (integrate.py:604-616)
+        self.pot.compute(self.sb, self.x32)
+
+    def verify(self) -> None:
+        """Raise if a neighbour-list capacity overflowed since the last verify()."""
+        self.pot.check_sticky(self.sb)
+
+    # --------------------------------------------------------------------
+    def _integrate(self, stream=None) -> None:
+        s = _abi.stream_handle(stream)
+        sb = self.sb
+        with self.pot._stage("baoab"):
+            self._baoab(s, sb)
+
+    def _baoab(self, s, sb) -> None:
+        rc = self.lib.fnl_baoab_step(
+            self.x.data_ptr(), self.v.data_ptr(), self.x32.data_ptr(), sb.grad_self.data_ptr(),
+            sb.grad_nbr.data_ptr(), self.force_scale, self.t_dtm.data_ptr(), 0.5 * self.dt, self.a1,
+            _abi.ptr(self.t_a2), None, self.seed, self.istep, self.n, s)
+        _abi.check(rc, "fnl_baoab_step")
+
+    def step(self, check: bool | None = None, stream=None) -> None:
+        """integrate -> rebuild neighbour lists -> energy and forces at the new positions."""
+        self._integrate(stream)
+        self.istep += 1
+        if check is None:
+            check = self.check_every > 0 and (self.istep % self.check_every) == 0
+        self.pot.compute(self.sb, self.x32, forces=True, rebuild=True, check=check, stream=stream)
+
+    def run(self, nsteps: int) -> None:
+        for _ in range(nsteps):
+            self.step()
+
+    # -------------------------------------------------------------- observables
+    def energies(self) -> np.ndarray:
+        """Potential energy per system in the model's energy unit."""
+        return self.sb.energy.cpu().numpy()
+
+    def kinetic_tensor(self) -> np.ndarray:
+        """0.5 sum m v v^T per system in kcal/mol (integrate.py:320-339)."""
+        sb = self.sb
+        out = torch.zeros((sb.n_sys, 9), dtype=torch.float64, device=self.pot.device)
+        _abi.check(self.lib.fnl_system_reduce(sb.sys_offsets.data_ptr(), sb.n_sys, None, None, None, None,
+                                              self.v.data_ptr(), self.t_mass.data_ptr(), out.data_ptr(),
+                                              _abi.stream_handle()), "fnl_system_reduce")
+        return out.cpu().numpy().reshape(-1, 3, 3)
+
+    def temperature(self) -> np.ndarray:
+        """2 K / (3 N k_B) per system (md/dynamic.py printing)."""
+        ek = np.trace(self.kinetic_tensor(), axis1=1, axis2=2)
+        return 2.0 * ek / (3.0 * np.asarray(self.sb.natoms) * units.K_B)
+
+    def total_energy_kcal(self) -> np.ndarray:
+        return self.energies() / self.model_energy_unit + np.trace(self.kinetic_tensor(), axis1=1, axis2=2)

@@ -1,0 +1,460 @@
+"""Device-resident ANI potential: neighbour lists -> AEV -> species MLP -> forces.
+
+This is the engine behind the FeNNol-shaped modules of :mod:`fennol_amd.fennix`.
+It owns the device buffers of one batch of systems (:class:`SystemBuffers`) and
+launches the HIP kernels of ``libfnl_hip.so`` on the current torch stream.
+
+Reference path (FeNNol src/fennol): GraphGenerator / GraphFilter /
+GraphAngularExtension / SpeciesIndexer (models/preprocessing.py), ANIAEV
+(models/embeddings/ani.py:47-142), SpeciesIndexNet (models/misc/nets.py:
+301-426), total energy and gradients (models/fennix.py:216-347).
+"""
+
+from __future__ import annotations
+
+import contextlib
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _abi
+from .units import PERIODIC_TABLE, SYMBOL_TO_Z
+
+CELU_ALPHA = 0.1
+
+
+@dataclass(frozen=True)
+class ANIConfig:
+    """ANI_AEV hyper-parameters (models/embeddings/ani.py:18-41) plus the graph cutoffs."""
+
+    species_order: tuple = ("H", "C", "N", "O")
+    cutoff: float = 5.2
+    cutoff_angular: float = 3.5
+    radial_eta: float = 19.7
+    angular_eta: float = 12.5
+    radial_dist_divisions: int = 16
+    angular_dist_divisions: int = 4
+    zeta: float = 14.1
+    angle_sections: int = 8
+    radial_start: float = 0.8
+    angular_start: float = 0.8
+
+    @property
+    def n_species(self) -> int:
+        return len(self.species_order)
+
+    @property
+    def n_pairs(self) -> int:
+        return self.n_species * (self.n_species + 1) // 2
+
+    @property
+    def aev_length(self) -> int:
+        return (self.n_species * self.radial_dist_divisions
+                + self.n_pairs * self.angular_dist_divisions * self.angle_sections)
+
+    @property
+    def k_pad(self) -> int:
+        return ((self.aev_length + 63) // 64) * 64
+
+    def c_params(self) -> _abi.AevParams:
+        """Kernel parameters; shifts computed in float64 and rounded to float32 (ani.py:73-110)."""
+        p = _abi.AevParams()
+        p.n_species = self.n_species
+        p.n_radial = self.radial_dist_divisions
+        p.n_ang_dist = self.angular_dist_divisions
+        p.n_ang_sect = self.angle_sections
+        p.cutoff = self.cutoff
+        p.cutoff_ang = self.cutoff_angular
+        p.radial_eta = self.radial_eta
+        p.angular_eta = self.angular_eta
+        p.zeta = self.zeta
+        rs = np.linspace(self.radial_start, self.cutoff, self.radial_dist_divisions + 1)[:-1]
+        asf = np.linspace(self.angular_start, self.cutoff_angular, self.angular_dist_divisions + 1)[:-1]
+        tz = (np.linspace(0, np.pi, self.angle_sections + 1) + np.pi / (2 * self.angle_sections))[:-1]
+        for k, v in enumerate(rs):
+            p.radial_shifts[k] = float(v)
+        for k, v in enumerate(asf):
+            p.ang_shifts[k] = float(v)
+        for k, v in enumerate(tz):
+            p.ang_cos[k] = math.cos(v)
+            p.ang_sin[k] = math.sin(v)
+        return p
+
+    def species_map(self) -> np.ndarray:
+        """Atomic number -> AEV species index, 0 for elements not listed (ani.py:52-60)."""
+        conv = np.zeros(len(PERIODIC_TABLE) + 1, dtype=np.int32)
+        for i, s in enumerate(self.species_order):
+            conv[SYMBOL_TO_Z[s]] = i
+        return conv
+
+
+def _next_mult(x: int, m: int) -> int:
+    return ((x + m - 1) // m) * m
+
+
+class SpeciesNetworks:
+    """Per-species MLP weights on the device in both GEMM layouts.
+
+    ``params``: {symbol: [(W [in, out], b [out]) x 4]} with widths
+    A -> 256 -> 192 -> 160 -> 1 (FullyConnectedNet layers, nets.py:81-94).
+    """
+
+    def __init__(self, params: dict, k_in: int, k_pad: int, dtype: str = "fp32", device="cuda",
+                 celu_alpha: float = CELU_ALPHA):
+        if dtype not in ("fp32", "bf16"):
+            raise ValueError("dtype must be 'fp32' or 'bf16'")
+        if len(params) > _abi.FNL_MLP_MAX_NETS:
+            raise ValueError("too many networks")
+        self.species = list(params.keys())
+        self.dtype = dtype
+        self.k_pad = k_pad
+        tdt = torch.float32 if dtype == "fp32" else torch.bfloat16
+        self._keep = []
+        self.cw = _abi.MlpWeights()
+        self.cw.n_nets = len(self.species)
+        self.cw.k_pad = k_pad
+        self.cw.dtype = _abi.FNL_F32 if dtype == "fp32" else _abi.FNL_BF16
+        self.cw.celu_alpha = celu_alpha
+        dims = [k_in, _abi.FNL_MLP_H1, _abi.FNL_MLP_H2, _abi.FNL_MLP_H3, 1]
+        for n, s in enumerate(self.species):
+            layers = params[s]
+            if len(layers) != 4:
+                raise ValueError("networks must have 3 hidden layers (256, 192, 160) and a scalar output")
+            for l, (W, b) in enumerate(layers):
+                if tuple(np.shape(W)) != (dims[l], dims[l + 1]) or np.shape(b) != (dims[l + 1],):
+                    raise ValueError(f"layer {l + 1} of network {s} has shape {np.shape(W)}, expected "
+                                     f"{(dims[l], dims[l + 1])}")
+            W1 = np.zeros((k_pad, dims[1]))
+            W1[:k_in] = np.asarray(layers[0][0])
+            mats = [W1, np.asarray(layers[1][0]), np.asarray(layers[2][0])]
+            net = self.cw.nets[n]
+            for l, Wm in enumerate(mats):
+                w = torch.tensor(Wm, dtype=tdt, device=device).contiguous()
+                wt = torch.tensor(Wm.T, dtype=tdt, device=device).contiguous()
+                self._keep += [w, wt]
+                setattr(net, f"w{l + 1}", w.data_ptr())
+                setattr(net, f"w{l + 1}t", wt.data_ptr())
+            w4 = torch.tensor(np.asarray(layers[3][0])[:, 0], dtype=torch.float32, device=device)
+            bs = [torch.tensor(np.asarray(layers[l][1]), dtype=torch.float32, device=device) for l in range(4)]
+            self._keep += [w4, *bs]
+            net.w4 = w4.data_ptr()
+            net.b1, net.b2, net.b3, net.b4 = (x.data_ptr() for x in bs)
+        self.tile_rows = _abi.load().fnl_mlp_tile_rows(self.cw.dtype)
+
+
+@dataclass
+class SystemBuffers:
+    """Device state of one batch of systems with fixed composition."""
+
+    n_atoms: int
+    n_sys: int
+    pbc: int
+    species: np.ndarray                 # atomic numbers (host copy)
+    natoms: np.ndarray
+    sys_offsets: torch.Tensor           # int32 [B+1]
+    batch_index: torch.Tensor           # int32 [N]
+    species_idx: torch.Tensor           # int32 [N] AEV species index, -1 = padding
+    cells: torch.Tensor | None          # float32 [B,3,3]
+    rcells: torch.Tensor | None
+    grid: torch.Tensor                  # int32 [B,4]
+    n_cells: int
+    max_nbr: int
+    max_nbr_ang: int
+    nbr: torch.Tensor = None
+    n_nbr: torch.Tensor = None
+    nbr_ang: torch.Tensor = None
+    n_nbr_ang: torch.Tensor = None
+    status: torch.Tensor = None
+    workspace: torch.Tensor = None
+    perm: torch.Tensor = None
+    tile_start: torch.Tensor = None
+    tile_rows: torch.Tensor = None
+    tile_net: torch.Tensor = None
+    n_tiles: int = 0
+    aev: torch.Tensor = None
+    g_aev: torch.Tensor = None
+    e_atom: torch.Tensor = None
+    grad_self: torch.Tensor = None
+    grad_nbr: torch.Tensor = None
+    vir_atom: torch.Tensor = None
+    energy: torch.Tensor = None         # float64 [B]
+    virial: torch.Tensor = None         # float64 [B,9]
+    extra: dict = field(default_factory=dict)
+
+
+class StageTimer:
+    """HIP-event timing of the kernel stages on the current stream (for measurement only)."""
+
+    def __init__(self):
+        self.events: dict[str, list] = {}
+
+    @contextlib.contextmanager
+    def stage(self, name: str):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        yield
+        b.record()
+        self.events.setdefault(name, []).append((a, b))
+
+    def times_ms(self) -> dict[str, np.ndarray]:
+        torch.cuda.synchronize()
+        return {k: np.array([a.elapsed_time(b) for a, b in v]) for k, v in self.events.items()}
+
+
+class ANIPotential:
+    """ANI AEV + species MLPs on the GPU with an analytic backward.
+
+    ``precision``: "fp32" (AEV, GEMMs and activations in FP32) or "bf16" (AEV
+    computed in FP32 and stored in BF16, MLP GEMMs in BF16 on MFMA with FP32
+    accumulation).  Per-system energies and virials are reduced in FP64.
+    """
+
+    def __init__(self, config: ANIConfig, params: dict, precision: str = "fp32", device="cuda",
+                 max_nbr: int = 64, max_nbr_ang: int = 32):
+        self.lib = _abi.load()
+        self.config = config
+        self.precision = precision
+        self.device = torch.device(device)
+        self.cparams = config.c_params()
+        self.nets = SpeciesNetworks(params, config.aev_length, config.k_pad, precision, self.device)
+        self.net_of_z = {SYMBOL_TO_Z[s]: n for n, s in enumerate(self.nets.species)}
+        self.init_max_nbr = max_nbr
+        self.init_max_nbr_ang = max_nbr_ang
+        self.timer: StageTimer | None = None
+
+    def _stage(self, name: str):
+        return self.timer.stage(name) if self.timer is not None else contextlib.nullcontext()
+
+    # ------------------------------------------------------------------ setup
+    def grid_caps(self, natoms, cells) -> tuple[np.ndarray, int]:
+        """Per-system cell-grid capacities (cap_x, cap_y, cap_z, first_cell)."""
+        B = len(natoms)
+        g = np.zeros((B, 4), np.int32)
+        first = 0
+        rbin = self.config.cutoff * 1.0001 + 1.0e-4
+        for b in range(B):
+            if cells is not None:
+                R = np.linalg.inv(np.asarray(cells[b], np.float64))
+                spacing = 1.0 / np.sqrt((R ** 2).sum(axis=0))
+                caps = np.maximum(np.floor(spacing / rbin).astype(int), 1) + 1
+            else:
+                k = max(1, int(math.ceil(max(int(natoms[b]), 1) ** (1.0 / 3.0))))
+                caps = np.array([k, k, k])
+            g[b, :3] = caps
+            g[b, 3] = first
+            first += int(np.prod(caps))
+        return g, max(first, 1)
+
+    def setup(self, species, natoms=None, cells=None) -> SystemBuffers:
+        species = np.asarray(species, np.int32)
+        N = species.shape[0]
+        natoms = np.array([N], np.int32) if natoms is None else np.asarray(natoms, np.int32)
+        B = len(natoms)
+        if natoms.sum() > N:
+            raise ValueError("natoms exceeds the number of atoms")
+        dev = self.device
+        offs = np.concatenate([[0], np.cumsum(natoms)]).astype(np.int32)
+        batch_index = np.full(N, max(B - 1, 0), np.int32)
+        batch_index[: offs[-1]] = np.repeat(np.arange(B, dtype=np.int32), natoms)
+        conv = self.config.species_map()
+        sidx = np.where(species > 0, conv[np.clip(species, 0, conv.shape[0] - 1)], -1).astype(np.int32)
+        sidx[offs[-1]:] = -1
+        real = species[: offs[-1]]
+        real = real[real > 0]
+        missing = sorted({PERIODIC_TABLE[z] for z in np.unique(real)} - set(self.nets.species))
+        if missing:
+            raise ValueError(f"Species {missing} not found in networks. Handled species are "
+                             f"{self.nets.species} (nets.py:394-397)")
+        pbc = _abi.FNL_PBC_NONE
+        tc = tr = None
+        if cells is not None:
+            cells = np.asarray(cells, np.float64).reshape(-1, 3, 3)
+            if cells.shape[0] == 1 and B > 1:
+                cells = np.repeat(cells, B, axis=0)
+            pbc = _abi.FNL_PBC_MINIMUM_IMAGE
+            tc = torch.tensor(cells.astype(np.float32), device=dev).contiguous()
+            tr = torch.tensor(np.linalg.inv(cells).astype(np.float32), device=dev).contiguous()
+        grid, n_cells = self.grid_caps(natoms, cells)
+        sb = SystemBuffers(
+            n_atoms=N, n_sys=B, pbc=pbc, species=species, natoms=natoms,
+            sys_offsets=torch.tensor(offs, device=dev),
+            batch_index=torch.tensor(batch_index, device=dev),
+            species_idx=torch.tensor(sidx, device=dev),
+            cells=tc, rcells=tr, grid=torch.tensor(grid, device=dev), n_cells=n_cells,
+            max_nbr=self.init_max_nbr, max_nbr_ang=self.init_max_nbr_ang,
+        )
+        self._alloc_nblist(sb)
+        self._setup_tiles(sb)
+        kp = self.config.k_pad
+        adt = torch.float32 if self.precision == "fp32" else torch.bfloat16
+        sb.aev = torch.zeros((N, kp), dtype=adt, device=dev)
+        sb.g_aev = torch.zeros((N, kp), dtype=torch.float32, device=dev)
+        sb.e_atom = torch.zeros(N, dtype=torch.float32, device=dev)
+        sb.grad_self = torch.zeros((N, 3), dtype=torch.float32, device=dev)
+        sb.grad_nbr = torch.zeros((N, 3), dtype=torch.float32, device=dev)
+        sb.vir_atom = torch.zeros((N, 9), dtype=torch.float32, device=dev)
+        sb.energy = torch.zeros(B, dtype=torch.float64, device=dev)
+        sb.virial = torch.zeros((B, 9), dtype=torch.float64, device=dev)
+        return sb
+
+    def _alloc_nblist(self, sb: SystemBuffers) -> None:
+        dev = self.device
+        N = sb.n_atoms
+        sb.nbr = torch.zeros((max(N, 1), sb.max_nbr), dtype=torch.int32, device=dev)
+        sb.n_nbr = torch.zeros(max(N, 1), dtype=torch.int32, device=dev)
+        sb.nbr_ang = torch.zeros((max(N, 1), sb.max_nbr_ang), dtype=torch.int32, device=dev)
+        sb.n_nbr_ang = torch.zeros(max(N, 1), dtype=torch.int32, device=dev)
+        sb.status = torch.zeros(_abi.FNL_STATUS_WORDS, dtype=torch.int32, device=dev)
+        wsb = self.lib.fnl_nblist_workspace(N, sb.n_sys, sb.n_cells)
+        sb.workspace = torch.empty(max(int(wsb), 256), dtype=torch.uint8, device=dev)
+
+    def _setup_tiles(self, sb: SystemBuffers) -> None:
+        """Atoms grouped by network, cut into tiles of at most tile_rows rows (SpeciesIndexer role)."""
+        M = self.nets.tile_rows
+        net = np.array([self.net_of_z.get(int(z), -1) if z > 0 else -1 for z in sb.species], np.int64)
+        net[int(np.sum(sb.natoms)):] = -1
+        perm, ts, tr, tn = [], [], [], []
+        for n in range(len(self.nets.species)):
+            idx = np.nonzero(net == n)[0]
+            base = len(perm)
+            perm.extend(idx.tolist())
+            for s in range(0, idx.size, M):
+                ts.append(base + s)
+                tr.append(min(M, idx.size - s))
+                tn.append(n)
+        dev = self.device
+        sb.perm = torch.tensor(np.array(perm if perm else [0], np.int32), device=dev)
+        sb.tile_start = torch.tensor(np.array(ts if ts else [0], np.int32), device=dev)
+        sb.tile_rows = torch.tensor(np.array(tr if tr else [0], np.int32), device=dev)
+        sb.tile_net = torch.tensor(np.array(tn if tn else [0], np.int32), device=dev)
+        sb.n_tiles = len(ts)
+
+    # --------------------------------------------------------------- kernels
+    def build_nblist(self, sb: SystemBuffers, coords32: torch.Tensor, stream=None) -> None:
+        s = _abi.stream_handle(stream)
+        P = _abi.ptr
+        rc = self.lib.fnl_nblist_build(
+            P(coords32), P(sb.species_idx), P(sb.batch_index), P(sb.sys_offsets), sb.n_atoms, sb.n_sys, sb.pbc,
+            P(sb.cells), P(sb.rcells), P(sb.grid), sb.n_cells, self.config.cutoff, self.config.cutoff_angular,
+            sb.max_nbr, sb.max_nbr_ang, P(sb.workspace), sb.workspace.numel(), P(sb.nbr), P(sb.n_nbr),
+            P(sb.nbr_ang), P(sb.n_nbr_ang), P(sb.status), s)
+        _abi.check(rc, "fnl_nblist_build")
+
+    def check_reallocate(self, sb: SystemBuffers, coords32: torch.Tensor, mult: float = 1.05,
+                         add: int = 4) -> bool:
+        """Read the status words; on overflow grow the rows and rebuild (preprocessing.py:357-367).
+
+        Returns True if the lists were reallocated.  This synchronises the stream.
+        """
+        st = sb.status.cpu().numpy()
+        if st[_abi.FNL_STATUS_OVERFLOW] & 2:
+            raise RuntimeError("cell grid capacities inconsistent with the system")
+        if st[_abi.FNL_STATUS_OVERFLOW] == 0:
+            return False
+        sb.max_nbr = _next_mult(int(mult * max(st[_abi.FNL_STATUS_MAX_NBR], sb.max_nbr)) + add, 8)
+        sb.max_nbr_ang = _next_mult(int(mult * max(st[_abi.FNL_STATUS_MAX_NBR_ANG], sb.max_nbr_ang)) + add, 8)
+        self._alloc_nblist(sb)
+        self.build_nblist(sb, coords32)
+        st = sb.status.cpu().numpy()
+        if st[_abi.FNL_STATUS_OVERFLOW]:
+            raise RuntimeError("neighbour list still overflows after reallocation")
+        return True
+
+    def aev_forward(self, sb: SystemBuffers, coords32: torch.Tensor, stream=None) -> None:
+        s = _abi.stream_handle(stream)
+        P = _abi.ptr
+        dt = _abi.FNL_F32 if self.precision == "fp32" else _abi.FNL_BF16
+        rc = self.lib.fnl_aev_forward(
+            P(coords32), P(sb.species_idx), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
+            P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(sb.nbr_ang), P(sb.n_nbr_ang), sb.max_nbr_ang,
+            C.byref(self.cparams), P(sb.aev), sb.aev.shape[1], dt, s)
+        _abi.check(rc, "fnl_aev_forward")
+
+    def mlp(self, sb: SystemBuffers, with_grad: bool = True, stream=None) -> None:
+        s = _abi.stream_handle(stream)
+        P = _abi.ptr
+        rc = self.lib.fnl_species_mlp(
+            P(sb.aev), sb.aev.shape[1], C.byref(self.nets.cw), P(sb.perm), P(sb.tile_start), P(sb.tile_rows),
+            P(sb.tile_net), sb.n_tiles, P(sb.e_atom), P(sb.g_aev) if with_grad else None, sb.g_aev.shape[1], s)
+        _abi.check(rc, "fnl_species_mlp")
+
+    def aev_backward(self, sb: SystemBuffers, coords32: torch.Tensor, stream=None) -> None:
+        s = _abi.stream_handle(stream)
+        P = _abi.ptr
+        _abi.check(self.lib.fnl_fill_f32(P(sb.grad_nbr), 0.0, sb.grad_nbr.numel(), s), "fnl_fill_f32")
+        rc = self.lib.fnl_aev_backward(
+            P(coords32), P(sb.species_idx), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
+            P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(sb.nbr_ang), P(sb.n_nbr_ang), sb.max_nbr_ang,
+            C.byref(self.cparams), P(sb.g_aev), sb.g_aev.shape[1], P(sb.grad_self), P(sb.grad_nbr),
+            P(sb.vir_atom), s)
+        _abi.check(rc, "fnl_aev_backward")
+
+    def reduce(self, sb: SystemBuffers, with_virial: bool = True, stream=None) -> None:
+        s = _abi.stream_handle(stream)
+        P = _abi.ptr
+        rc = self.lib.fnl_system_reduce(
+            P(sb.sys_offsets), sb.n_sys, P(sb.e_atom), P(sb.energy), P(sb.vir_atom) if with_virial else None,
+            P(sb.virial), None, None, None, s)
+        _abi.check(rc, "fnl_system_reduce")
+
+    def compute(self, sb: SystemBuffers, coords32: torch.Tensor, forces: bool = True,
+                rebuild: bool = True, check: bool = True, stream=None) -> None:
+        """Energy (and gradients) of the batch at coords32 (float32 [N,3] device tensor)."""
+        if rebuild:
+            with self._stage("nblist"):
+                self.build_nblist(sb, coords32, stream)
+            if check:
+                self.check_reallocate(sb, coords32)
+        with self._stage("aev_fwd"):
+            self.aev_forward(sb, coords32, stream)
+        with self._stage("mlp"):
+            self.mlp(sb, with_grad=forces, stream=stream)
+        if forces:
+            with self._stage("aev_bwd"):
+                self.aev_backward(sb, coords32, stream)
+        with self._stage("reduce"):
+            self.reduce(sb, with_virial=forces, stream=stream)
+
+    def check_sticky(self, sb: SystemBuffers, clear: bool = True) -> None:
+        """Raise if any build since the last clear overflowed a capacity (synchronises)."""
+        v = int(sb.status[_abi.FNL_STATUS_STICKY].item())
+        if clear:
+            sb.status[_abi.FNL_STATUS_STICKY:_abi.FNL_STATUS_STICKY + 1].zero_()
+        if v:
+            raise RuntimeError("neighbour-list capacity overflowed during the run")
+
+    def fit_capacity(self, sb: SystemBuffers, coords32: torch.Tensor, margin: float = 1.25, add: int = 8) -> None:
+        """Size the rows from one build with headroom for an MD run without per-step host checks."""
+        self.build_nblist(sb, coords32)
+        self.check_reallocate(sb, coords32)
+        st = sb.status.cpu().numpy()
+        want = _next_mult(int(margin * st[_abi.FNL_STATUS_MAX_NBR]) + add, 8)
+        want_a = _next_mult(int(margin * st[_abi.FNL_STATUS_MAX_NBR_ANG]) + add, 8)
+        if want > sb.max_nbr or want_a > sb.max_nbr_ang:
+            sb.max_nbr = max(sb.max_nbr, want)
+            sb.max_nbr_ang = max(sb.max_nbr_ang, want_a)
+            self._alloc_nblist(sb)
+            self.build_nblist(sb, coords32)
+        sb.status[_abi.FNL_STATUS_STICKY:_abi.FNL_STATUS_STICKY + 1].zero_()
+
+    def forces(self, sb: SystemBuffers, scale: float = 1.0, stream=None) -> torch.Tensor:
+        out = torch.empty((sb.n_atoms, 3), dtype=torch.float32, device=self.device)
+        s = _abi.stream_handle(stream)
+        P = _abi.ptr
+        _abi.check(self.lib.fnl_forces(P(sb.grad_self), P(sb.grad_nbr), sb.n_atoms, scale, P(out), s),
+                   "fnl_forces")
+        return out
+
+    def energy_forces_virial(self, species, coords, natoms=None, cells=None, sb: SystemBuffers | None = None):
+        """Convenience: (E [B] f64, F [N,3] f32, virial [B,3,3] f64) as CPU NumPy arrays."""
+        if sb is None:
+            sb = self.setup(species, natoms, cells)
+        x = torch.as_tensor(np.asarray(coords, np.float32), device=self.device).contiguous()
+        self.compute(sb, x)
+        F = self.forces(sb)
+        torch.cuda.synchronize()
+        return (sb.energy.cpu().numpy(), F.cpu().numpy(), sb.virial.cpu().numpy().reshape(-1, 3, 3))

@@ -1,0 +1,256 @@
+/*
+ * fnl_hip.h -- C ABI of the MI355X (gfx950) kernels for FeNNol's ANI
+ * energy / force / MD path.
+ *
+ * Conventions
+ *   - Every buffer is device memory owned by the caller; the library keeps no
+ *     global state and never allocates.  Temporaries live in a caller-provided
+ *     workspace whose size is returned by the matching *_workspace() call.
+ *   - All work is enqueued on the caller's stream; no call synchronises.
+ *   - Return value: 0 on success, <0 on invalid arguments or a HIP launch error
+ *     (fnl_strerror() names it).  Capacity overflows are reported on the device
+ *     in the caller's status buffer, read by the host only when it chooses to
+ *     (the reference reads its overflow flag once per rebuild:
+ *     models/preprocessing.py:357-367, :1859-1878).
+ *   - Atoms of one system are contiguous; sys_offsets[b] .. sys_offsets[b+1].
+ *     Atoms with species index < 0 are padding: they get no edges, zero AEV,
+ *     zero energy and zero force.
+ *   - Cells are row-major [3][3] with rows = lattice vectors; reciprocal cells
+ *     are their inverses, so fractional coordinates are x . R.
+ *
+ * Reference citations are relative to FeNNol's src/fennol.
+ */
+#ifndef FNL_HIP_H
+#define FNL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* fnl_stream_t; /* == hipStream_t */
+
+#define FNL_OK 0
+#define FNL_ERR_INVALID (-1)
+#define FNL_ERR_HIP (-2)
+#define FNL_ERR_UNSUPPORTED (-3)
+
+/* periodic boundary modes */
+#define FNL_PBC_NONE 0
+#define FNL_PBC_MINIMUM_IMAGE 1
+
+/* element types */
+#define FNL_F32 0
+#define FNL_BF16 1
+
+/* status words written by fnl_nblist_build (int32[FNL_STATUS_WORDS]) */
+#define FNL_STATUS_OVERFLOW 0       /* nonzero if any capacity was exceeded       */
+#define FNL_STATUS_MAX_NBR 1        /* largest radial neighbour count seen          */
+#define FNL_STATUS_MAX_NBR_ANG 2    /* largest angular neighbour count seen         */
+#define FNL_STATUS_MAX_CELL 3       /* largest cell population seen                 */
+#define FNL_STATUS_STICKY 4         /* overflow flags OR-ed over every build since the
+                                       caller last cleared this word                */
+#define FNL_STATUS_WORDS 8
+
+int fnl_version(void);
+const char* fnl_strerror(int code);
+
+/* ------------------------------------------------------------------------
+ * Neighbour lists: cell-list radial graph + angular sub-graph.
+ *
+ * Replaces GraphGenerator.process (models/preprocessing.py:369-638, same pair
+ * set as the host path :74-355), GraphFilter.process (:909-956) and the
+ * neighbour-slot table of GraphAngularExtension.process (:1196-1276).
+ *
+ * Output is one row per atom ("ELL" layout):
+ *   nbr[i*max_nbr + e]          neighbour atom j of centre i, e < n_nbr[i]
+ *   nbr_ang[i*max_nbr_ang + a]  radial slot e whose d12 < cutoff_ang^2
+ * A pair (i, j) is kept iff d12 < cutoff^2 with d12 computed as the reference
+ * does in float32: vec = x_j - x_i; s = -rint(vec . R); vec += s . C;
+ * d12 = (vx*vx + vy*vy) + vz*vz (no contraction).  The triplets of centre i are
+ * all slot pairs a < b of its angular row (preprocessing.py:1237-1245).
+ *
+ * grid: int32[n_sys][4] = (cap_x, cap_y, cap_z, first_cell) -- the caller's
+ * upper bound on the cell grid of each system and the offset of its cells in
+ * the global cell arrays; n_cells = sum of cap_x*cap_y*cap_z.  The grid
+ * actually used has floor(width / cutoff) cells per direction, clamped to
+ * [1, cap]; width is the lattice-plane spacing (periodic) or the bounding box
+ * extent (open).
+ * ---------------------------------------------------------------------- */
+size_t fnl_nblist_workspace(int32_t n_atoms, int32_t n_sys, int32_t n_cells);
+
+int fnl_nblist_build(const float* coords, const int32_t* species_idx,
+                     const int32_t* batch_index, const int32_t* sys_offsets,
+                     int32_t n_atoms, int32_t n_sys, int32_t pbc_mode,
+                     const float* cells, const float* reciprocal_cells,
+                     const int32_t* grid, int32_t n_cells,
+                     float cutoff, float cutoff_ang,
+                     int32_t max_nbr, int32_t max_nbr_ang,
+                     void* workspace, size_t workspace_bytes,
+                     int32_t* nbr, int32_t* n_nbr,
+                     int32_t* nbr_ang, int32_t* n_nbr_ang,
+                     int32_t* status, fnl_stream_t stream);
+
+/* Flat directed edge list in the reference's dictionary layout
+ * (edge_src / edge_dst / d12 / pbc_shifts, preprocessing.py:328-341) from the
+ * rows above: edges of centre i occupy [row_start[i], row_start[i] + n_nbr[i]);
+ * row_start is an exclusive prefix sum of n_nbr computed by the caller. */
+int fnl_nblist_export(const float* coords, const int32_t* batch_index,
+                      int32_t n_atoms, int32_t pbc_mode,
+                      const float* cells, const float* reciprocal_cells,
+                      const int32_t* nbr, const int32_t* n_nbr, int32_t max_nbr,
+                      const int32_t* row_start,
+                      int32_t* edge_src, int32_t* edge_dst, float* d12,
+                      float* pbc_shifts, fnl_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * ANI AEV (models/embeddings/ani.py:47-142, with GraphProcessor
+ * preprocessing.py:699-777, GraphFilterProcessor :963-1043,
+ * GraphAngleProcessor :1283-1317 and the cosine SwitchFunction
+ * models/misc/misc.py:617-716 folded in).
+ * ---------------------------------------------------------------------- */
+#define FNL_MAX_RADIAL 32
+#define FNL_MAX_ANG_DIST 16
+#define FNL_MAX_ANG_SECT 16
+#define FNL_MAX_SPECIES 8
+
+typedef struct fnl_aev_params {
+  int32_t n_species;          /* S <= 8                                    */
+  int32_t n_radial;           /* radial_dist_divisions <= 16 (kernel tile) */
+  int32_t n_ang_dist;         /* angular_dist_divisions                    */
+  int32_t n_ang_sect;         /* angle_sections; n_ang_dist*n_ang_sect <= 32 */
+  float cutoff;               /* radial graph cutoff                       */
+  float cutoff_ang;           /* angular graph cutoff                      */
+  float radial_eta;
+  float angular_eta;
+  float zeta;
+  float radial_shifts[FNL_MAX_RADIAL];   /* linspace(radial_start, rc, nR+1)[:-1]   */
+  float ang_shifts[FNL_MAX_ANG_DIST];    /* linspace(angular_start, rca, nA+1)[:-1] */
+  float ang_cos[FNL_MAX_ANG_SECT];       /* cos(theta_z)                            */
+  float ang_sin[FNL_MAX_ANG_SECT];       /* sin(theta_z)                            */
+} fnl_aev_params;
+
+/* AEV length: S*nR + S(S+1)/2 * nA*nZ.  Row i of aev (stride aev_stride
+ * elements, element type out_dtype) gets the channels of centre i in the
+ * reference order; columns [length, aev_stride) are written as zero. */
+int fnl_aev_forward(const float* coords, const int32_t* species_idx,
+                    const int32_t* batch_index, int32_t n_atoms, int32_t pbc_mode,
+                    const float* cells, const float* reciprocal_cells,
+                    const int32_t* nbr, const int32_t* n_nbr, int32_t max_nbr,
+                    const int32_t* nbr_ang, const int32_t* n_nbr_ang, int32_t max_nbr_ang,
+                    const fnl_aev_params* params,
+                    void* aev, int32_t aev_stride, int32_t out_dtype,
+                    fnl_stream_t stream);
+
+/* Adjoint of fnl_aev_forward (the reverse pass jax.grad runs through
+ * ani.py:47-142 and the graph processors).  AEV terms are recomputed rather
+ * than stored.  Given g = dE/dAEV (float32 rows of stride g_stride):
+ *   grad_self[i] = -sum_e dE/dvec_ie        (written, float32[N][3])
+ *   grad_nbr[j] += dE/dvec_ij  for every edge (atomic, float32[N][3]; the
+ *                                          caller zeroes it first)
+ *   virial_atom[i] = sum_e vec_ie (x) dE/dvec_ie   (float32[N][9], row-major a,b)
+ * so that dE/dx = grad_self + grad_nbr and the per-system virial
+ * dE/dstrain (fennix.py:320-347) is the per-system sum of virial_atom. */
+int fnl_aev_backward(const float* coords, const int32_t* species_idx,
+                     const int32_t* batch_index, int32_t n_atoms, int32_t pbc_mode,
+                     const float* cells, const float* reciprocal_cells,
+                     const int32_t* nbr, const int32_t* n_nbr, int32_t max_nbr,
+                     const int32_t* nbr_ang, const int32_t* n_nbr_ang, int32_t max_nbr_ang,
+                     const fnl_aev_params* params,
+                     const float* g, int32_t g_stride,
+                     float* grad_self, float* grad_nbr, float* virial_atom,
+                     fnl_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Species MLPs as grouped GEMMs on MFMA (SpeciesIndexNet models/misc/nets.py:
+ * 301-426 over FullyConnectedNet :13-102, activation CELU(alpha)).
+ *
+ * Network shape: K -> H1 -> H2 -> H3 -> 1 with H1 = 256, H2 = 192, H3 = 160
+ * (the ANI-2x widths, examples/training/ani.yaml:78).  Weights per network
+ * (element type = compute dtype), K padded to k_pad (multiple of 64, zero
+ * rows):
+ *   w1t [H1][k_pad]  w2t [H2][H1]  w3t [H3][H2]   forward, K-contiguous
+ *   w1  [k_pad][H1]  w2  [H1][H2]  w3  [H2][H3]   backward, N-contiguous
+ *   w4 [H3], b1 [H1], b2 [H2], b3 [H3], b4 [1]   float32
+ * Atoms are processed in tiles of rows of one network: tile t covers
+ * perm[tile_start[t] .. tile_start[t] + tile_rows[t]) with network
+ * tile_net[t]; rows per tile <= fnl_mlp_tile_rows(dtype).
+ * Outputs: energy[perm[r]] and, if grad != NULL, dE/dAEV rows grad[perm[r]]
+ * (float32, stride grad_stride >= k_pad; columns >= k_pad untouched).
+ * ---------------------------------------------------------------------- */
+#define FNL_MLP_H1 256
+#define FNL_MLP_H2 192
+#define FNL_MLP_H3 160
+#define FNL_MLP_MAX_NETS 8
+
+typedef struct fnl_mlp_net {
+  const void* w1t; const void* w2t; const void* w3t;
+  const void* w1;  const void* w2;  const void* w3;
+  const float* w4; const float* b1; const float* b2; const float* b3; const float* b4;
+} fnl_mlp_net;
+
+typedef struct fnl_mlp_weights {
+  int32_t n_nets;
+  int32_t k_pad;
+  int32_t dtype;              /* FNL_F32 or FNL_BF16 (weights and GEMM inputs) */
+  float celu_alpha;
+  fnl_mlp_net nets[FNL_MLP_MAX_NETS];
+} fnl_mlp_weights;
+
+int fnl_mlp_tile_rows(int32_t dtype);
+
+int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_mlp_weights* w,
+                    const int32_t* perm, const int32_t* tile_start,
+                    const int32_t* tile_rows, const int32_t* tile_net, int32_t n_tiles,
+                    float* energy, float* grad, int32_t grad_stride,
+                    fnl_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Per-system reductions in FP64 (total_energy segment_sum, fennix.py:240-246;
+ * virial tensor, fennix.py:320-347; kinetic tensor, md/integrate.py:320-339).
+ * Deterministic: one workgroup per system, fixed summation tree.
+ * energy_atom may be NULL (energy_sys untouched), likewise virial_atom /
+ * virial_sys, and vel (kinetic_sys untouched; mass is then ignored).
+ * ---------------------------------------------------------------------- */
+int fnl_system_reduce(const int32_t* sys_offsets, int32_t n_sys,
+                      const float* energy_atom, double* energy_sys,
+                      const float* virial_atom, double* virial_sys,
+                      const double* vel, const double* mass, double* kinetic_sys,
+                      fnl_stream_t stream);
+
+/* forces = -(grad_self + grad_nbr) * scale  (float32[N][3]) */
+int fnl_forces(const float* grad_self, const float* grad_nbr, int32_t n_atoms,
+               float scale, float* forces, fnl_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * BAOAB Langevin step (md/integrate.py:286-294 with the LGV O-step of
+ * md/thermostats.py:108-120), FP64 positions and velocities:
+ *   F  = -(grad_self + grad_nbr) * force_scale
+ *   v += dtm * F;  x += dt/2 v;  v = a1 v + a2 xi;  x += dt/2 v
+ * xi ~ N(0,1) from Philox4x32-10 with counter (atom, step_lo, step_hi, 0) and
+ * key (seed_lo, seed_hi), Box-Muller on (r0,r1), (r2,r3); or, if noise != NULL,
+ * xi = noise (float64[N][3]).  a2 == NULL means no O-step (NVE).
+ * Also writes the float32 copy of x the model reads (x32).
+ * ---------------------------------------------------------------------- */
+int fnl_baoab_step(double* x, double* v, float* x32,
+                   const float* grad_self, const float* grad_nbr, float force_scale,
+                   const double* dtm, double half_dt, double a1, const double* a2,
+                   const double* noise, uint64_t seed, uint64_t step,
+                   int32_t n_atoms, fnl_stream_t stream);
+
+/* Standard normals float64[N][3] from the same generator (initial velocities
+ * use stream = 1; the O-step uses stream = 0). */
+int fnl_philox_normals(double* out, int32_t n_atoms, uint64_t seed, uint64_t step,
+                       uint32_t stream_id, fnl_stream_t stream);
+
+/* Device fill helpers (graph-capture safe replacements for memset). */
+int fnl_fill_i32(int32_t* p, int32_t value, size_t count, fnl_stream_t stream);
+int fnl_fill_f32(float* p, float value, size_t count, fnl_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FNL_HIP_H */

@@ -1,0 +1,245 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+Bars: neighbour pair sets and their float32 d12 bit-exact; FP32 AEV, energies
+(1e-5 relative) and forces (RMS 1e-4 eV/A) against the FP64 oracle; BF16 MLP
+within 1 meV/atom and 0.02 eV/A force RMS of FP64.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from fennol_amd import _abi, systems
+from fennol_amd.potential import ANIPotential
+from tests.helpers import canonical_pairs_oracle, configs, rows_to_pairs, triclinic_box, water_box_small
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _cases():
+    return {
+        "cluster64": systems.cluster_config0(0),
+        "water64": water_box_small(64, 1),
+        "water216": water_box_small(216, 2),
+        "triclinic": triclinic_box(150, 3),
+    }
+
+
+def _potential(cfg, precision="fp32", seed=11, max_nbr=64, max_nbr_ang=32):
+    p, c = cfg
+    params = O.init_mlp_params(seed, O.aev_length(p))
+    return params, ANIPotential(c, params, precision=precision, max_nbr=max_nbr, max_nbr_ang=max_nbr_ang)
+
+
+def _nblist(pot, s):
+    sb = pot.setup(s["species"], s["natoms"], s.get("cells"))
+    x = torch.tensor(np.asarray(s["coordinates"], np.float32), device="cuda")
+    pot.build_nblist(sb, x)
+    pot.check_reallocate(sb, x)
+    torch.cuda.synchronize()
+    return sb, x
+
+
+@pytest.mark.parametrize("name", ["cluster64", "water64", "water216", "triclinic"])
+def test_nblist_pair_set(gpu, name):
+    _flush(f"[nblist] {name}")
+    s = _cases()[name]
+    cfg = configs()
+    _, pot = _potential(cfg, max_nbr=16, max_nbr_ang=8)  # small rows: exercises regrowth
+    sb, x = _nblist(pot, s)
+    nbr = sb.nbr.cpu().numpy()
+    n_nbr = sb.n_nbr.cpu().numpy()
+    i, j = rows_to_pairs(nbr, n_nbr, sb.max_nbr)
+    # symmetric directed list
+    fwd = set(zip(i.tolist(), j.tolist()))
+    assert fwd == set(zip(j.tolist(), i.tolist()))
+    assert len(fwd) == len(i)  # no duplicate edge
+    half = np.array(sorted((a, b) for a, b in fwd if a < b), dtype=np.int64).reshape(-1, 2)
+    ref, ref_d12 = canonical_pairs_oracle(s["coordinates"], s["natoms"], cfg[1].cutoff, s.get("cells"))
+    np.testing.assert_array_equal(half, ref)
+    # d12 bit-exact via the reference-layout export
+    rs = np.concatenate([[0], np.cumsum(np.minimum(n_nbr, sb.max_nbr))]).astype(np.int32)
+    E = int(rs[-1])
+    dev = torch.device("cuda")
+    es = torch.empty(E, dtype=torch.int32, device=dev)
+    ed = torch.empty(E, dtype=torch.int32, device=dev)
+    d12 = torch.empty(E, dtype=torch.float32, device=dev)
+    sh = torch.empty((E, 3), dtype=torch.float32, device=dev)
+    rst = torch.tensor(rs[:-1], device=dev)
+    P = _abi.ptr
+    _abi.check(pot.lib.fnl_nblist_export(P(x), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
+                                         P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(rst), P(es), P(ed), P(d12), P(sh),
+                                         _abi.stream_handle()), "export")
+    es, ed, d12 = es.cpu().numpy(), ed.cpu().numpy(), d12.cpu().numpy()
+    m = es < ed
+    order = np.lexsort((ed[m], es[m]))
+    np.testing.assert_array_equal(d12[m][order].view(np.uint32), ref_d12.view(np.uint32))
+    # angular rows: the radial slots with d12 < rca^2
+    na = sb.n_nbr_ang.cpu().numpy()
+    nba = sb.nbr_ang.cpu().numpy()
+    rca2 = np.float32(cfg[1].cutoff_angular ** 2)
+    for a in range(sb.n_atoms):
+        row = nbr[a, : n_nbr[a]]
+        slots = nba[a, : na[a]]
+        dd = d12[rs[a]: rs[a + 1]]
+        expect = np.nonzero(dd < rca2)[0]
+        np.testing.assert_array_equal(np.sort(slots), expect)
+        assert np.all(row[slots] == ed[rs[a] + slots])
+
+
+def test_nblist_batched_clusters(gpu):
+    _flush("[nblist] batched clusters")
+    rng = np.random.Generator(np.random.PCG64(5))
+    sysl = [systems.lattice_cluster(int(n), rng) for n in (8, 33, 120, 300, 512)]
+    s = systems.batch_systems(sysl)
+    cfg = configs()
+    _, pot = _potential(cfg)
+    sb, x = _nblist(pot, s)
+    i, j = rows_to_pairs(sb.nbr.cpu().numpy(), sb.n_nbr.cpu().numpy(), sb.max_nbr)
+    half = np.array(sorted((a, b) for a, b in zip(i.tolist(), j.tolist()) if a < b), dtype=np.int64)
+    ref, _ = canonical_pairs_oracle(s["coordinates"], s["natoms"], cfg[1].cutoff)
+    np.testing.assert_array_equal(half, ref)
+
+
+def test_nblist_near_cutoff(gpu):
+    """Pairs placed within a few ulp of the cutoff keep the reference's decision."""
+    _flush("[nblist] near cutoff")
+    cfg = configs()
+    rc = np.float32(cfg[1].cutoff)
+    xs = []
+    for k in range(40):
+        base = np.array([10.0 * k, 0.0, 0.0], np.float32)
+        d = np.nextafter(rc, np.float32(np.inf) if k % 2 else np.float32(0), dtype=np.float32)
+        for _ in range(k // 2):
+            d = np.nextafter(d, np.float32(np.inf) if k % 2 else np.float32(0), dtype=np.float32)
+        xs += [base, base + np.array([d, 0, 0], np.float32)]
+    coords = np.array(xs, np.float32)
+    s = {"species": np.ones(len(xs), np.int32), "coordinates": coords, "natoms": np.array([len(xs)], np.int32)}
+    _, pot = _potential(cfg)
+    sb, x = _nblist(pot, s)
+    i, j = rows_to_pairs(sb.nbr.cpu().numpy(), sb.n_nbr.cpu().numpy(), sb.max_nbr)
+    half = np.array(sorted((a, b) for a, b in zip(i.tolist(), j.tolist()) if a < b), dtype=np.int64).reshape(-1, 2)
+    ref, _ = canonical_pairs_oracle(coords, s["natoms"], cfg[1].cutoff)
+    np.testing.assert_array_equal(half, ref.reshape(-1, 2))
+
+
+@pytest.mark.parametrize("name", ["cluster64", "water64", "water216"])
+def test_aev_forward(gpu, name):
+    _flush(f"[aev] {name}")
+    s = _cases()[name]
+    order = ("H", "C", "N", "O", "S", "F", "Cl") if name.startswith("water") else ("H", "C", "N", "O")
+    cfg = configs(order)
+    _, pot = _potential(cfg)
+    sb, x = _nblist(pot, s)
+    pot.aev_forward(sb, x)
+    aev = sb.aev.float().cpu().numpy()[:, : cfg[1].aev_length]
+    g = O.build_graphs(s["coordinates"], s["natoms"], cfg[0], s.get("cells"), s.get("reciprocal_cells"))
+    ref = O.ani_aev(np.asarray(s["coordinates"], np.float32).astype(np.float64), s["species"], g, cfg[0],
+                    s.get("cells"))
+    err = np.abs(aev - ref)
+    tol = 2e-5 * np.maximum(1.0, np.abs(ref))
+    assert np.all(err <= tol), f"max err {err.max()} at {np.unravel_index(err.argmax(), err.shape)}"
+    assert np.all(sb.aev.float().cpu().numpy()[:, cfg[1].aev_length:] == 0)
+
+
+@pytest.mark.parametrize("name", ["cluster64", "water64", "water216", "triclinic"])
+def test_energy_forces_virial_fp32(gpu, name):
+    _flush(f"[efv fp32] {name}")
+    s = _cases()[name]
+    if name == "triclinic":  # keep atoms apart for a smooth energy
+        s = water_box_small(27, 7)
+        cell = np.array([[12.0, 0.0, 0.0], [1.5, 11.0, 0.0], [-1.0, 2.0, 11.5]])
+        frac = np.asarray(s["coordinates"]) @ np.linalg.inv(s["cells"][0])
+        s = {**s, "coordinates": frac @ cell, "cells": cell[None], "reciprocal_cells": np.linalg.inv(cell)[None]}
+    order = ("H", "C", "N", "O", "S", "F", "Cl") if name.startswith("water") else ("H", "C", "N", "O")
+    cfg = configs(order)
+    params, pot = _potential(cfg)
+    E, F, V = pot.energy_forces_virial(s["species"], s["coordinates"], s["natoms"], s.get("cells"))
+    model = O.Model(cfg[0], params)
+    x32 = np.asarray(s["coordinates"], np.float32).astype(np.float64)
+    Er, Fr, Vr, _ = O.energy_forces_virial(model, x32, s["species"], s["natoms"], s.get("cells"),
+                                           s.get("reciprocal_cells"))
+    np.testing.assert_allclose(E, Er, rtol=1e-5, atol=1e-5 * np.abs(Er).max())
+    rms = np.sqrt(np.mean((F - Fr) ** 2))
+    assert rms < 1e-4, rms
+    np.testing.assert_allclose(V, Vr, rtol=1e-4, atol=1e-4 * np.abs(Vr).max())
+
+
+@pytest.mark.parametrize("name", ["cluster64", "water216"])
+def test_energy_forces_bf16(gpu, name):
+    _flush(f"[efv bf16] {name}")
+    s = _cases()[name]
+    order = ("H", "C", "N", "O", "S", "F", "Cl") if name.startswith("water") else ("H", "C", "N", "O")
+    cfg = configs(order)
+    params, pot = _potential(cfg, precision="bf16")
+    E, F, V = pot.energy_forces_virial(s["species"], s["coordinates"], s["natoms"], s.get("cells"))
+    model = O.Model(cfg[0], params)
+    Er, Fr, Vr, _ = O.energy_forces_virial(model, np.asarray(s["coordinates"], np.float64), s["species"],
+                                           s["natoms"], s.get("cells"), s.get("reciprocal_cells"))
+    n = len(s["species"])
+    e_err_mev = 1000.0 * np.abs(E - Er).max() / n
+    f_rms = np.sqrt(np.mean((F - Fr) ** 2))
+    _flush(f"  bf16 energy err {e_err_mev:.4f} meV/atom, force rms {f_rms:.5f} eV/A")
+    assert e_err_mev <= 1.0
+    assert f_rms <= 0.02
+
+
+def test_batched_systems_match_single(gpu):
+    _flush("[efv] batched vs single")
+    rng = np.random.Generator(np.random.PCG64(9))
+    sysl = [systems.lattice_cluster(int(n), rng) for n in (12, 64, 150)]
+    cfg = configs()
+    params, pot = _potential(cfg)
+    s = systems.batch_systems(sysl)
+    E, F, V = pot.energy_forces_virial(s["species"], s["coordinates"], s["natoms"])
+    off = 0
+    for b, sysd in enumerate(sysl):
+        e1, f1, v1 = pot.energy_forces_virial(sysd["species"], sysd["coordinates"])
+        n = len(sysd["species"])
+        np.testing.assert_allclose(E[b], e1[0], rtol=1e-6, atol=1e-6)
+        np.testing.assert_allclose(F[off:off + n], f1, rtol=1e-5, atol=1e-5)
+        off += n
+
+
+def test_philox_and_baoab_injected_noise(gpu):
+    _flush("[md] philox + baoab")
+    n = 257
+    lib = _abi.load()
+    out = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+    _abi.check(lib.fnl_philox_normals(out.data_ptr(), n, 1234567, 42, 0, _abi.stream_handle()), "normals")
+    ref = O.philox_normals(1234567, 42, n, 0)
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-12, atol=1e-12)
+    rng = np.random.Generator(np.random.PCG64(1))
+    x = rng.normal(size=(n, 3))
+    v = rng.normal(size=(n, 3))
+    gs = rng.normal(size=(n, 3)).astype(np.float32)
+    gn = rng.normal(size=(n, 3)).astype(np.float32)
+    mass = rng.uniform(1, 16, size=n)
+    noise = rng.normal(size=(n, 3))
+    dt = 5e-4
+    a1, a2 = O.langevin_coefficients(dt, 1.0, 0.596, mass)
+    scale = 1.0 / 0.04336409463078789
+    F = -(gs.astype(np.float64) + gn.astype(np.float64)) * np.float64(np.float32(scale))
+    xr, vr = O.baoab_step(x, v, F, mass, dt, a1, a2, noise)
+    dev = "cuda"
+    tx = torch.tensor(x, device=dev)
+    tv = torch.tensor(v, device=dev)
+    x32 = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    tdtm = torch.tensor(dt / mass, device=dev)
+    ta2 = torch.tensor(a2, device=dev)
+    tno = torch.tensor(noise, device=dev)
+    tgs = torch.tensor(gs, device=dev)
+    tgn = torch.tensor(gn, device=dev)
+    _abi.check(lib.fnl_baoab_step(tx.data_ptr(), tv.data_ptr(), x32.data_ptr(), tgs.data_ptr(), tgn.data_ptr(),
+                                  scale, tdtm.data_ptr(), 0.5 * dt, a1, ta2.data_ptr(), tno.data_ptr(), 0, 0, n,
+                                  _abi.stream_handle()), "baoab")
+    np.testing.assert_allclose(tx.cpu().numpy(), xr, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(tv.cpu().numpy(), vr, rtol=1e-13, atol=1e-13)
+    np.testing.assert_array_equal(x32.cpu().numpy(), xr.astype(np.float32))

@@ -1,0 +1,397 @@
+"""Pins of the oracle (CPU only): closed forms, brute force, finite differences, invariances.
+
+Each test is chosen so that a plausible slip in the oracle (a dropped term, a
+wrong sign, a swapped channel index) fails it; see the module docstring of
+oracle/ani_oracle.py for the mapping to reference files.
+"""
+
+from __future__ import annotations
+
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fennol_amd import systems
+from tests.helpers import triclinic_box, water_box_small
+
+
+# ----------------------------------------------------------------- units / RNG
+def test_units_match_reference_constants():
+    u = O.md_units
+    # fennol_md: 1 Da = 1/415.377... internal mass units; kT(300 K) in kcal/mol
+    assert abs(u.DA - 415.3774385247939) < 1e-9
+    assert abs(u.K_B * 300.0 - 0.5961614047814636) < 1e-12
+    assert abs(u.energy_multiplier("eV") - 27.211386024367243 / 627.5096080305927) < 1e-15
+    assert u.fs == pytest.approx(1000.0, rel=1e-12)
+
+
+def test_product_units_equal_oracle_units():
+    from fennol_amd import units
+
+    assert units.DA == O.md_units.DA
+    assert units.K_B == O.md_units.K_B
+    assert units.energy_multiplier("eV") == O.md_units.energy_multiplier("eV")
+    assert units.FS_PER_PS == pytest.approx(1000.0, rel=1e-12)
+
+
+def test_philox_known_answers():
+    """Random123 known-answer vectors for Philox4x32-10."""
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ]
+    for ctr, key, expect in kat:
+        got = O.philox4x32(np.array([ctr], np.uint32), np.array([key], np.uint32))[0]
+        assert tuple(int(v) for v in got) == expect
+
+
+def test_philox_normals_statistics():
+    z = O.philox_normals(7, 3, 200000).ravel()
+    assert abs(z.mean()) < 0.01
+    assert abs(z.std() - 1.0) < 0.01
+    assert abs(np.mean(z ** 4) - 3.0) < 0.05
+
+
+# --------------------------------------------------------------- neighbour list
+def test_simple_cubic_shell_counts():
+    """Perfect sc lattice: 6/12/8/6/24 neighbours at a, sqrt2 a, sqrt3 a, 2a, sqrt5 a."""
+    a, n = 1.0, 10
+    g = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3) * a
+    L = n * a
+    cells = (np.eye(3) * L)[None]
+    for rc, expect in [(1.2, 6), (1.5, 18), (1.8, 26), (2.1, 32), (2.3, 56)]:
+        s, d, d12, sh = O.half_pairs(g, [g.shape[0]], rc, cells, np.linalg.inv(cells))
+        counts = np.bincount(np.concatenate([s, d]), minlength=g.shape[0])
+        assert np.all(counts == expect), (rc, counts.min(), counts.max())
+
+
+def _brute_force_images(x, cell, rc):
+    """Nearest image of every pair by explicit enumeration of the 27 images (float64)."""
+    out = set()
+    n = x.shape[0]
+    shifts = np.array(list(itertools.product([-1, 0, 1], repeat=3)), float) @ cell
+    for i in range(n):
+        for j in range(i + 1, n):
+            v = x[j] - x[i] + shifts
+            d = np.sqrt((v ** 2).sum(-1)).min()
+            if d < rc:
+                out.add((i, j))
+    return out
+
+
+@pytest.mark.parametrize("maker", [lambda: water_box_small(27, 3), lambda: triclinic_box(60, 4)])
+def test_minimum_image_equals_nearest_image(maker):
+    s = maker()
+    cell = np.asarray(s["cells"][0])
+    rc = 5.2
+    spacing = 1.0 / np.sqrt((np.linalg.inv(cell) ** 2).sum(0))
+    src, dst, d12, sh = O.half_pairs(s["coordinates"], s["natoms"], rc, s["cells"], np.linalg.inv(s["cells"]))
+    got = set(zip(src.tolist(), dst.tolist()))
+    ref = _brute_force_images(np.asarray(s["coordinates"], np.float32).astype(np.float64), cell, rc)
+    if rc < 0.5 * spacing.min():
+        # away from exact-cutoff ties (float32 vs float64), the sets agree
+        assert got == ref
+    else:
+        # skewed small cell: rounding the fractional vector may not give the nearest image;
+        # every kept pair is still within rc of some image
+        assert got <= ref
+
+
+def test_general_pbc_contains_all_images():
+    s = water_box_small(8, 5)
+    cell = np.asarray(s["cells"][0])
+    rc = 5.2  # larger than half the box: several images per pair
+    src, dst, d12, sh = O.half_pairs(s["coordinates"], s["natoms"], rc, s["cells"], np.linalg.inv(s["cells"]),
+                                     minimum_image=False)
+    x = np.asarray(s["coordinates"], np.float64)
+    vec = x[dst] - x[src] + sh @ cell
+    np.testing.assert_allclose((vec ** 2).sum(-1), d12, rtol=1e-5)
+    assert np.all(d12 < rc * rc)
+    # count of images within rc by enumeration
+    n = x.shape[0]
+    count = 0
+    rng = range(-2, 3)
+    for i in range(n):
+        for j in range(i + 1, n):
+            for k in itertools.product(rng, rng, rng):
+                v = x[j] - x[i] + np.array(k) @ cell
+                count += (v @ v) < rc * rc
+    assert count == src.shape[0]
+
+
+def test_symmetrized_list_and_filter():
+    s = systems.cluster_config0(0)
+    s_, d_, d12, _ = O.half_pairs(s["coordinates"], s["natoms"], 5.2)
+    src, dst, dd, _ = O.symmetrize(s_, d_, d12, None)
+    assert set(zip(src.tolist(), dst.tolist())) == set(zip(dst.tolist(), src.tolist()))
+    ang = O.filter_graph(dd, 3.5)
+    assert np.all(dd[ang] < np.float32(3.5 ** 2))
+    assert np.all(np.delete(dd, ang) >= np.float32(3.5 ** 2))
+
+
+def test_triplet_count_and_brute_force():
+    s = systems.cluster_config0(1)
+    p = O.AEVParams()
+    g = O.build_graphs(s["coordinates"], s["natoms"], p)
+    nat = s["species"].shape[0]
+    na = np.bincount(g.src[g.ang_edges], minlength=nat)
+    assert g.angle_src.shape[0] == int((na * (na - 1) // 2).sum())
+    ae = g.ang_edges
+    trip = {(int(g.central[t]), *sorted((int(g.dst[ae[g.angle_src[t]]]), int(g.dst[ae[g.angle_dst[t]]]))))
+            for t in range(g.angle_src.shape[0])}
+    x = np.asarray(s["coordinates"], np.float32)
+    brute = set()
+    for i in range(nat):
+        nb = [j for j in range(nat) if j != i and ((x[j] - x[i]) ** 2).sum(dtype=np.float32) < np.float32(3.5 ** 2)]
+        for a, b in itertools.combinations(sorted(nb), 2):
+            brute.add((i, a, b))
+    assert trip == brute
+
+
+# ----------------------------------------------------------------------- switch
+def test_cosine_switch_closed_form():
+    rc = 5.2
+    assert O.cosine_switch(0.0, rc) == 1.0
+    assert abs(O.cosine_switch(rc - 1e-12, rc)) < 1e-20
+    assert O.cosine_switch(rc, rc) == 0.0
+    assert O.cosine_switch(rc / 2, rc) == pytest.approx(0.5)
+    h = 1e-6
+    dnum = (O.cosine_switch(rc - h, rc) - O.cosine_switch(rc - 3 * h, rc)) / (2 * h)
+    assert abs(dnum) < 1e-5
+
+
+# -------------------------------------------------------------------------- AEV
+def _dimer(d, z1=1, z2=8):
+    return {"species": np.array([z1, z2], np.int32), "coordinates": np.array([[0, 0, 0], [d, 0, 0]], float),
+            "natoms": np.array([2], np.int32)}
+
+
+def test_radial_aev_dimer_closed_form():
+    p = O.AEVParams()
+    d = 1.37
+    s = _dimer(d)
+    g = O.build_graphs(s["coordinates"], s["natoms"], p)
+    aev = O.ani_aev(s["coordinates"], s["species"], g, p)
+    Rk = 0.8 + (5.2 - 0.8) / 16 * np.arange(16)
+    term = 0.25 * (0.5 * np.cos(np.pi * d / 5.2) + 0.5) * np.exp(-19.7 * (d - Rk) ** 2)
+    nR = 16
+    # atom 0 (H) sees O (species index 3) ; atom 1 (O) sees H (index 0)
+    np.testing.assert_allclose(aev[0, 3 * nR:4 * nR], term, rtol=1e-12)
+    np.testing.assert_allclose(aev[1, 0:nR], term, rtol=1e-12)
+    mask = np.ones(aev.shape[1], bool)
+    mask[3 * nR:4 * nR] = False
+    assert np.all(aev[0, mask] == 0)  # no angular terms for a dimer
+
+
+def test_angular_aev_trimer_closed_form():
+    p = O.AEVParams()
+    # H at origin, C and N at distances 1.1 and 1.4 with a 100 degree angle
+    th = math.radians(100.0)
+    x = np.array([[0, 0, 0], [1.1, 0, 0], [1.4 * math.cos(th), 1.4 * math.sin(th), 0]], float)
+    s = {"species": np.array([1, 6, 7], np.int32), "coordinates": x, "natoms": np.array([3], np.int32)}
+    g = O.build_graphs(x, s["natoms"], p)
+    aev = O.ani_aev(x, s["species"], g, p)
+    rca, zeta, etaA = 3.5, 14.1, 12.5
+    fc = lambda r: 0.5 * np.cos(np.pi * r / rca) + 0.5
+    theta = np.arccos(0.95 * math.cos(th))
+    tz = np.pi / 16 + np.arange(8) * np.pi / 8
+    Ad = 0.8 + (3.5 - 0.8) / 4 * np.arange(4)
+    F = (1 + np.cos(theta - tz)) ** zeta
+    R = np.exp(-etaA * ((1.1 + 1.4) / 2 - Ad) ** 2)
+    expect = 2 * 0.5 ** zeta * fc(1.1) * fc(1.4) * (R[:, None] * F[None, :]).ravel()
+    # centre H (atom 0), pair (C, N) = species (1, 2) -> triu index of (1,2) in 4 species = 5
+    off = 4 * 16 + 5 * 32
+    np.testing.assert_allclose(aev[0, off:off + 32], expect, rtol=1e-12)
+    assert np.count_nonzero(aev[0, 4 * 16:]) == 32
+
+
+def test_aev_invariances():
+    p = O.AEVParams()
+    s = systems.cluster_config0(2)
+    x = np.asarray(s["coordinates"])
+    g = O.build_graphs(x, s["natoms"], p)
+    a0 = O.ani_aev(x, s["species"], g, p)
+    rng = np.random.Generator(np.random.PCG64(0))
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    x2 = x @ Q.T + np.array([3.0, -1.0, 2.0])
+    g2 = O.build_graphs(x2, s["natoms"], p)
+    a2 = O.ani_aev(x2, s["species"], g2, p)
+    np.testing.assert_allclose(a0, a2, rtol=1e-9, atol=1e-12)
+    perm = rng.permutation(x.shape[0])
+    g3 = O.build_graphs(x[perm], s["natoms"], p)
+    a3 = O.ani_aev(x[perm], s["species"][perm], g3, p)
+    np.testing.assert_allclose(a0[perm], a3, rtol=1e-9, atol=1e-12)
+
+
+# -------------------------------------------------------------------------- MLP
+def test_celu_closed_form():
+    x = np.array([-2.0, -0.1, 0.0, 0.3])
+    np.testing.assert_allclose(O.celu(x), [0.1 * (np.exp(-20) - 1), 0.1 * (np.exp(-1) - 1), 0.0, 0.3])
+
+
+def test_species_mlp_explicit_layers_and_grad():
+    rng = np.random.Generator(np.random.PCG64(3))
+    params = O.init_mlp_params(5, 12, hidden=(256, 192, 160), species=("H", "O"), zero_bias=False)
+    x = rng.normal(size=(6, 12))
+    species = np.array([1, 8, 1, 8, 8, 1])
+    e, g = O.species_mlp(x, species, params, return_grad=True)
+    for i in range(6):
+        layers = params["H" if species[i] == 1 else "O"]
+        h = x[i]
+        for l, (W, b) in enumerate(layers):
+            h = h @ W + b
+            if l < 3:
+                h = np.where(h > 0, h, 0.1 * (np.exp(h / 0.1) - 1))
+        assert e[i] == pytest.approx(h[0], rel=1e-12)
+    h = 1e-6
+    for i, k in [(0, 3), (4, 7)]:
+        xp, xm = x.copy(), x.copy()
+        xp[i, k] += h
+        xm[i, k] -= h
+        num = (O.species_mlp(xp, species, params)[i] - O.species_mlp(xm, species, params)[i]) / (2 * h)
+        assert g[i, k] == pytest.approx(num, rel=1e-6, abs=1e-9)
+
+
+def test_lecun_init_statistics():
+    rng = np.random.Generator(np.random.PCG64(0))
+    W = O.truncated_lecun_normal(rng, 1000, 400)
+    assert np.abs(W).max() <= 2.0 * math.sqrt(1 / 1000) / 0.87962566103423978 + 1e-12
+    assert W.std() == pytest.approx(math.sqrt(1 / 1000), rel=0.02)
+
+
+def test_unhandled_species_raises():
+    params = O.init_mlp_params(5, 4, species=("H",))
+    with pytest.raises(ValueError):
+        O.species_mlp(np.zeros((2, 4)), np.array([1, 8]), params)
+
+
+# ------------------------------------------------------- forces and the virial
+def _model(order=("H", "C", "N", "O"), seed=3):
+    p = O.AEVParams(species_order=order)
+    return O.Model(p, O.init_mlp_params(seed, O.aev_length(p)))
+
+
+@pytest.mark.parametrize("case", ["cluster", "water"])
+def test_forces_finite_differences(case):
+    if case == "cluster":
+        s = systems.cluster_config0(4)
+        m = _model()
+    else:
+        s = water_box_small(27, 6)
+        m = _model(("H", "C", "N", "O", "S", "F", "Cl"))
+    x = np.asarray(s["coordinates"], np.float64)
+    cells = s.get("cells")
+    rec = s.get("reciprocal_cells")
+    E, F, V, _ = O.energy_forces_virial(m, x, s["species"], s["natoms"], cells, rec)
+    g = O.build_graphs(x, s["natoms"], m.aev, cells, rec)
+    def central(i, a, h):
+        xp, xm = x.copy(), x.copy()
+        xp[i, a] += h
+        xm[i, a] -= h
+        ep, _ = O.model_energy(m, xp, s["species"], s["natoms"], cells, rec, graphs=g)
+        em, _ = O.model_energy(m, xm, s["species"], s["natoms"], cells, rec, graphs=g)
+        return -(ep[0] - em[0]) / (2 * h)
+
+    # central differences in FP64; the narrow radial Gaussians (eta = 19.7)
+    # make the O(h^2) error 1e-6 relative at h = 1e-4, so h = 1e-5 A
+    h = 1e-5
+    rng = np.random.Generator(np.random.PCG64(1))
+    for i in rng.choice(x.shape[0], 6, replace=False):
+        for a in range(3):
+            assert F[i, a] == pytest.approx(central(i, a, h), rel=2e-6, abs=1e-7)
+    np.testing.assert_allclose(F.sum(0), 0.0, atol=1e-10)
+
+
+def test_virial_is_strain_derivative():
+    s = water_box_small(27, 8)
+    m = _model(("H", "C", "N", "O", "S", "F", "Cl"))
+    x = np.asarray(s["coordinates"], np.float64)
+    cell = np.asarray(s["cells"][0], np.float64)
+    E, F, V, _ = O.energy_forces_virial(m, x, s["species"], s["natoms"], cell[None], np.linalg.inv(cell)[None])
+    g = O.build_graphs(x, s["natoms"], m.aev, cell[None], np.linalg.inv(cell)[None])
+    h = 1e-5
+    num = np.zeros((3, 3))
+    for a in range(3):
+        for b in range(3):
+            Ep = np.eye(3)
+            Ep[a, b] += h
+            Em = np.eye(3)
+            Em[a, b] -= h
+            ep, _ = O.model_energy(m, x @ Ep, s["species"], s["natoms"], (cell @ Ep)[None], None, graphs=g)
+            em, _ = O.model_energy(m, x @ Em, s["species"], s["natoms"], (cell @ Em)[None], None, graphs=g)
+            num[a, b] = (ep[0] - em[0]) / (2 * h)
+    np.testing.assert_allclose(V[0], num, rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(V[0], V[0].T, atol=1e-9)
+
+
+# ------------------------------------------------------------------------ BAOAB
+def test_baoab_free_particle_and_verlet_limit():
+    m = np.array([2.0])
+    x = np.array([[0.0, 1.0, 2.0]])
+    v = np.array([[1.0, -0.5, 0.25]])
+    dt = 0.1
+    x1, v1 = O.baoab_step(x, v, np.zeros((1, 3)), m, dt, 1.0, np.zeros(1), np.zeros((1, 3)))
+    np.testing.assert_allclose(x1, x + dt * v)
+    np.testing.assert_allclose(v1, v)
+    # gamma = 0, harmonic oscillator: bounded energy, time reversible
+    k = 3.0
+    x, v = np.array([[1.0, 0, 0]]), np.array([[0.0, 0, 0]])
+    F = -k * x
+    e0 = 0.5 * k * 1.0
+    traj = [(x.copy(), v.copy())]
+    for _ in range(2000):
+        x, v = O.baoab_step(x, v, F, m, dt, 1.0, np.zeros(1), np.zeros((1, 3)))
+        F = -k * x
+        traj.append((x.copy(), v.copy()))
+        vs = v + 0.5 * dt * F / m[:, None]
+        e = 0.5 * m[0] * (vs ** 2).sum() + 0.5 * k * (x ** 2).sum()
+        assert abs(e - e0) < 0.02 * e0
+    # reverse: negate the synchronised velocity and integrate back
+    xT, vT = traj[-1]
+    F = -k * xT
+    vrev = -(vT + dt * F / m[:, None])
+    x, v = xT.copy(), vrev
+    for _ in range(2000):
+        x, v = O.baoab_step(x, v, F, m, dt, 1.0, np.zeros(1), np.zeros((1, 3)))
+        F = -k * x
+    np.testing.assert_allclose(x, traj[0][0], atol=1e-9)
+
+
+def test_langevin_equipartition_and_ou_decay():
+    n = 20000
+    m = np.full(n, 3.0)
+    kT, gamma, dt = 0.6, 2.0, 0.05
+    a1, a2 = O.langevin_coefficients(dt, gamma, kT, m)
+    x = np.zeros((n, 3))
+    v = np.zeros((n, 3))
+    v0 = None
+    acf = []
+    for step in range(200):
+        noise = O.philox_normals(11, step, n)
+        x, v = O.baoab_step(x, v, np.zeros((n, 3)), m, dt, a1, a2, noise)
+        if step == 99:
+            v0 = v.copy()
+        if step >= 99:
+            acf.append((v * v0).mean() / (v0 * v0).mean())
+    ke = 0.5 * (m[:, None] * v ** 2).mean()
+    assert ke == pytest.approx(0.5 * kT, rel=0.03)
+    t = dt * np.arange(len(acf))
+    np.testing.assert_allclose(acf[:40], np.exp(-gamma * t[:40]), atol=0.03)
+
+
+def test_product_initialiser_matches_oracle():
+    from fennol_amd.params import from_variables, init_species_params, to_variables
+
+    a = init_species_params(7, 40)
+    b = O.init_mlp_params(7, 40)
+    for s in a:
+        for (wa, ba), (wb, bb) in zip(a[s], b[s]):
+            np.testing.assert_array_equal(wa, wb)
+            np.testing.assert_array_equal(ba, bb)
+    c = from_variables(to_variables(a))
+    assert all(np.array_equal(x[0], y[0]) for s in a for x, y in zip(a[s], c[s]))
