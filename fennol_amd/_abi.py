@@ -31,6 +31,7 @@ FNL_MAX_ANG_SECT = 16
 FNL_MAX_SPECIES = 8
 FNL_MLP_H1, FNL_MLP_H2, FNL_MLP_H3 = 256, 192, 160
 FNL_MLP_MAX_NETS = 8
+FNL_REDUCE_VALUES = 19
 
 vp = C.c_void_p
 i32 = C.c_int32
@@ -87,7 +88,7 @@ _SIGNATURES = {
                                    C.POINTER(AevParams), vp, i32, vp, vp, vp, vp]),
     "fnl_mlp_tile_rows": (C.c_int, [i32]),
     "fnl_species_mlp": (C.c_int, [vp, i32, C.POINTER(MlpWeights), vp, vp, vp, vp, i32, vp, vp, i32, vp]),
-    "fnl_system_reduce": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "fnl_system_reduce": (C.c_int, [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "fnl_forces": (C.c_int, [vp, vp, i32, f32, vp, vp]),
     "fnl_baoab_step": (C.c_int, [vp, vp, vp, vp, vp, f32, vp, f64, f64, vp, vp, u64, u64, i32, vp]),
     "fnl_philox_normals": (C.c_int, [vp, i32, u64, u64, u32, vp]),

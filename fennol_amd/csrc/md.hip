@@ -9,20 +9,31 @@
 
 namespace {
 
-// One workgroup per system, fixed-order tree: bitwise reproducible.
+constexpr int kRedVals = 1 + 9 + 9;  // energy, virial[9], kinetic[9]
+
+// Stage 1: one workgroup per segment (a run of at most seg_atoms atoms of one
+// system) -> FP64 partial sums.  Stage 2: one thread per system adds its
+// segments' partials in segment order.  Both use fixed summation orders, so
+// the result is bitwise reproducible.
 __global__ void __launch_bounds__(256)
-k_system_reduce(const int32_t* __restrict__ sys_offsets, const float* __restrict__ energy_atom,
-                double* __restrict__ energy_sys, const float* __restrict__ virial_atom,
-                double* __restrict__ virial_sys, const double* __restrict__ vel,
-                const double* __restrict__ mass, double* __restrict__ kinetic_sys) {
-  constexpr int NV = 1 + 9 + 9;
-  __shared__ double red[NV][256];
-  const int b = blockIdx.x;
+k_reduce_segments(const int32_t* __restrict__ sys_offsets, const int32_t* __restrict__ seg_offsets, int n_sys,
+                  int seg_atoms, const float* __restrict__ energy_atom, const float* __restrict__ virial_atom,
+                  const double* __restrict__ vel, const double* __restrict__ mass, double* __restrict__ partials) {
+  __shared__ double red[kRedVals][256];
+  const int sgi = blockIdx.x;
   const int t = threadIdx.x;
-  const int a0 = sys_offsets[b], a1 = sys_offsets[b + 1];
-  double acc[NV];
+  // system of this segment: largest b with seg_offsets[b] <= sgi
+  int lo = 0, hi = n_sys - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (seg_offsets[mid] <= sgi) lo = mid; else hi = mid - 1;
+  }
+  const int b = lo;
+  const int a0 = sys_offsets[b] + (sgi - seg_offsets[b]) * seg_atoms;
+  const int a1 = min(a0 + seg_atoms, sys_offsets[b + 1]);
+  double acc[kRedVals];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int k = 0; k < kRedVals; ++k) acc[k] = 0.0;
   for (int i = a0 + t; i < a1; i += 256) {
     if (energy_atom) acc[0] += (double)energy_atom[i];
     if (virial_atom) {
@@ -38,22 +49,34 @@ k_system_reduce(const int32_t* __restrict__ sys_offsets, const float* __restrict
     }
   }
 #pragma unroll
-  for (int k = 0; k < NV; ++k) red[k][t] = acc[k];
+  for (int k = 0; k < kRedVals; ++k) red[k][t] = acc[k];
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (t < s) {
 #pragma unroll
-      for (int k = 0; k < NV; ++k) red[k][t] += red[k][t + s];
+      for (int k = 0; k < kRedVals; ++k) red[k][t] += red[k][t + s];
     }
     __syncthreads();
   }
-  if (t == 0) {
-    if (energy_atom) energy_sys[b] = red[0][0];
-    if (virial_atom)
-      for (int k = 0; k < 9; ++k) virial_sys[9 * b + k] = red[1 + k][0];
-    if (vel)
-      for (int k = 0; k < 9; ++k) kinetic_sys[9 * b + k] = red[10 + k][0];
-  }
+  if (t < kRedVals) partials[(size_t)sgi * kRedVals + t] = red[t][0];
+}
+
+__global__ void k_reduce_systems(const int32_t* __restrict__ seg_offsets, int n_sys,
+                                 const double* __restrict__ partials, double* __restrict__ energy_sys,
+                                 double* __restrict__ virial_sys, double* __restrict__ kinetic_sys) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_sys) return;
+  double acc[kRedVals];
+#pragma unroll
+  for (int k = 0; k < kRedVals; ++k) acc[k] = 0.0;
+  for (int s = seg_offsets[b]; s < seg_offsets[b + 1]; ++s)
+#pragma unroll
+    for (int k = 0; k < kRedVals; ++k) acc[k] += partials[(size_t)s * kRedVals + k];
+  if (energy_sys) energy_sys[b] = acc[0];
+  if (virial_sys)
+    for (int k = 0; k < 9; ++k) virial_sys[9 * b + k] = acc[1 + k];
+  if (kinetic_sys)
+    for (int k = 0; k < 9; ++k) kinetic_sys[9 * b + k] = acc[10 + k];
 }
 
 __global__ void k_forces(const float* __restrict__ gs, const float* __restrict__ gn, int n3, float scale,
@@ -100,17 +123,25 @@ __global__ void k_normals(double* __restrict__ out, int n_atoms, uint64_t seed, 
 
 }  // namespace
 
-extern "C" int fnl_system_reduce(const int32_t* sys_offsets, int32_t n_sys, const float* energy_atom,
-                                 double* energy_sys, const float* virial_atom, double* virial_sys,
-                                 const double* vel, const double* mass, double* kinetic_sys,
+extern "C" int fnl_system_reduce(const int32_t* sys_offsets, const int32_t* seg_offsets, int32_t n_sys,
+                                 int32_t n_seg, int32_t seg_atoms, const float* energy_atom, double* energy_sys,
+                                 const float* virial_atom, double* virial_sys, const double* vel,
+                                 const double* mass, double* kinetic_sys, double* partials,
                                  fnl_stream_t stream) {
-  if (n_sys < 0) return FNL_ERR_INVALID;
+  if (n_sys < 0 || n_seg < 0 || seg_atoms < 1) return FNL_ERR_INVALID;
   if (n_sys == 0) return FNL_OK;
-  if (!sys_offsets) return FNL_ERR_INVALID;
+  if (!sys_offsets || !seg_offsets || (n_seg > 0 && !partials)) return FNL_ERR_INVALID;
   if ((energy_atom && !energy_sys) || (virial_atom && !virial_sys) || (vel && (!mass || !kinetic_sys)))
     return FNL_ERR_INVALID;
-  hipLaunchKernelGGL(k_system_reduce, dim3(n_sys), dim3(256), 0, (hipStream_t)stream, sys_offsets, energy_atom,
-                     energy_sys, virial_atom, virial_sys, vel, mass, kinetic_sys);
+  hipStream_t st = (hipStream_t)stream;
+  if (n_seg > 0) {
+    hipLaunchKernelGGL(k_reduce_segments, dim3(n_seg), dim3(256), 0, st, sys_offsets, seg_offsets, n_sys,
+                       seg_atoms, energy_atom, virial_atom, vel, mass, partials);
+    FNL_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_reduce_systems, dim3(fnl_blocks(n_sys, 64)), dim3(64), 0, st, seg_offsets, n_sys, partials,
+                     energy_atom ? energy_sys : nullptr, virial_atom ? virial_sys : nullptr,
+                     vel ? kinetic_sys : nullptr);
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }

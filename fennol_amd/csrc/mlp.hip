@@ -1,25 +1,26 @@
 // Species MLPs (SpeciesIndexNet over FullyConnectedNet, models/misc/nets.py:
 // 13-102, 301-426) as one fused forward + input-gradient kernel on MFMA.
 //
-// One workgroup (4 waves) per tile of M atoms that share a network.  The
-// tile walks K -> 256 -> 192 -> 160 -> 1 and back:
-//   forward   Z_l = H_{l-1} W_l + b_l, H_l = celu(Z_l) -> LDS (compute dtype),
-//             D_l = celu'(Z_l) kept in registers in the accumulator layout
-//   energy    E = H_3 . w4 + b4 (fp32, from the accumulators)
-//   backward  G_3 = w4 * D_3;  G_2 = (G_3 W_3^T) * D_2;  G_1 = (G_2 W_2^T) * D_1;
-//             dE/dAEV = G_1 W_1^T -> global (fp32)
-// Activations H_l are overwritten by G_l in the same LDS buffers.
+// One workgroup (4 waves) per tile of M atoms that share a network.  Every
+// GEMM is computed transposed, out^T = W^T act^T: the weights are the MFMA A
+// operand (streamed from global/L2, prefetched PF k-steps ahead in registers),
+// the activations the B operand (read from LDS), so that each lane's result
+// registers hold 4 consecutive output features of one atom and store as one
+// 8/16-byte vector into the row-major activation buffers.
 //
-// MFMA shapes (wave64):
-//   FP32: v_mfma_f32_16x16x4_f32    A: lane (row l%16, k l/16)
-//   BF16: v_mfma_f32_16x16x32_bf16  A: lane (row l%16, k 8*(l/16) .. +7)
-//   result: lane l, reg r -> (row 4*(l/16) + r, col l%16)
-// A fragments come from LDS, B fragments straight from global memory (the
-// weights stay resident in L2 across tiles) in the layout that makes each
-// lane's load contiguous: K-contiguous (transposed) matrices for BF16 (8 K
-// values per lane), N-contiguous ones for FP32 (16 lanes read 16 columns).
-
-#include <hip/hip_bfloat16.h>
+//   forward   H_l = celu(W_l^T H_{l-1} + b_l)  -> LDS [M][N_l] (compute dtype)
+//   energy    E = H_3 . w4 + b4                 (fp32 accumulators)
+//   backward  G_3 = w4 * D_3 ; G_2 = (W_3 G_3) * D_2 ; G_1 = (W_2 G_2) * D_1
+//             dE/dAEV = W_1 G_1 -> global (fp32)
+// with D_l = celu'(Z_l) recovered from the stored activation:
+//   celu'(z) = 1 (z > 0), exp(z/alpha) = 1 + h/alpha (z <= 0).
+// G_l overwrites H_l in place.  Layer-1 inputs (AEV rows) are staged through
+// a double-buffered LDS area that aliases the H_2/H_3 buffers, which are free
+// until layer 2.  LDS stays within 80 KB so two workgroups share a CU.
+//
+// MFMA (wave64): FP32 v_mfma_f32_16x16x4_f32, BF16 v_mfma_f32_16x16x32_bf16.
+// A operand lane l: row (l%16), k-group (l/16); B operand lane l: column
+// (l%16), k-group (l/16); result lane l, register r: row 4*(l/16)+r, col l%16.
 
 #include "fnl_common.h"
 
@@ -27,14 +28,14 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short us4 __attribute__((ext_vector_type(4)));
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
 constexpr int H1 = FNL_MLP_H1, H2 = FNL_MLP_H2, H3 = FNL_MLP_H3;
-constexpr int KC = 64;  // layer-1 K chunk staged through LDS
 
 struct NetPtrs {
-  const void* fw[3];   // B operand of the forward GEMMs
-  const void* bw[3];   // B operand of the backward GEMMs
+  const void* fw[3];  // A operand of the forward GEMMs
+  const void* bw[3];  // A operand of the backward GEMMs
   const float* w4;
   const float* b[4];
 };
@@ -45,161 +46,196 @@ struct MlpArgs {
   float alpha;
 };
 
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  return __builtin_bit_cast(unsigned short, static_cast<__bf16>(f));
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
+
 template <typename T>
 struct Tr;
 
 template <>
 struct Tr<float> {
-  static constexpr int KS = 4;  // K per MFMA
-  static constexpr int M = 32;  // rows per tile
-  static constexpr int PAD = 4; // LDS row padding (elements)
+  static constexpr int KS = 4;    // K per MFMA
+  static constexpr int M = 32;    // atoms per tile
+  static constexpr int PAD = 4;   // activation row padding (elements)
+  static constexpr int KC = 64;   // layer-1 K chunk
+  static constexpr int PF = 4;    // weight prefetch depth (k-steps)
   typedef float frag;
   static __device__ __forceinline__ f4 mma(frag a, frag b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
-  // A fragment from an LDS matrix (row-major, ld elements)
-  static __device__ __forceinline__ frag a_lds(const float* buf, int ld, int row0, int k0) {
-    const int l = fnl_lane();
-    return buf[(row0 + (l & 15)) * ld + k0 + (l >> 4)];
-  }
-  // B fragment: element (k, n) of an N-contiguous matrix p[k*ld + n]
-  static __device__ __forceinline__ frag b_glb(const float* p, int ld, int k0, int n0) {
+  // weights: element (n, k) of an N-contiguous matrix p[k*ld + n]
+  static __device__ __forceinline__ frag wload(const float* p, int ld, int n0, int k0) {
     const int l = fnl_lane();
     return p[(size_t)(k0 + (l >> 4)) * ld + n0 + (l & 15)];
   }
-  static __device__ __forceinline__ float to_f(float v) { return v; }
-  static __device__ __forceinline__ float from_f(float v) { return v; }
+  // activations: element (k, m) = buf[m*ld + k]
+  static __device__ __forceinline__ frag aload(const float* buf, int ld, int m0, int k0) {
+    const int l = fnl_lane();
+    return buf[(m0 + (l & 15)) * ld + k0 + (l >> 4)];
+  }
+  static __device__ __forceinline__ void store4(float* p, f4 v) { *(f4*)p = v; }
+  static __device__ __forceinline__ f4 load4(const float* p) { return *(const f4*)p; }
 };
-
-__device__ __forceinline__ unsigned short f2bf(float f) {
-  // round to nearest even
-  unsigned int u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16 | ((u & 0xffff) ? 0x40 : 0));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
-
-__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
 template <>
 struct Tr<unsigned short> {  // bf16 stored as raw bits
   static constexpr int KS = 32;
   static constexpr int M = 64;
   static constexpr int PAD = 8;
+  static constexpr int KC = 128;
+  static constexpr int PF = 2;
   typedef bf8 frag;
   static __device__ __forceinline__ f4 mma(frag a, frag b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
-  static __device__ __forceinline__ frag a_lds(const unsigned short* buf, int ld, int row0, int k0) {
+  // weights: element (n, k) of a K-contiguous matrix p[n*ld + k]
+  static __device__ __forceinline__ frag wload(const unsigned short* p, int ld, int n0, int k0) {
     const int l = fnl_lane();
-    const us8 v = *(const us8*)(buf + (row0 + (l & 15)) * ld + k0 + 8 * (l >> 4));
-    return __builtin_bit_cast(bf8, v);
+    return __builtin_bit_cast(bf8, *(const us8*)(p + (size_t)(n0 + (l & 15)) * ld + k0 + 8 * (l >> 4)));
   }
-  // B fragment: element (k, n) of a K-contiguous matrix p[n*ld + k]
-  static __device__ __forceinline__ frag b_glb(const unsigned short* p, int ld, int k0, int n0) {
+  static __device__ __forceinline__ frag aload(const unsigned short* buf, int ld, int m0, int k0) {
     const int l = fnl_lane();
-    const us8 v = *(const us8*)(p + (size_t)(n0 + (l & 15)) * ld + k0 + 8 * (l >> 4));
-    return __builtin_bit_cast(bf8, v);
+    return __builtin_bit_cast(bf8, *(const us8*)(buf + (m0 + (l & 15)) * ld + k0 + 8 * (l >> 4)));
   }
-  static __device__ __forceinline__ float to_f(unsigned short v) { return bf2f(v); }
-  static __device__ __forceinline__ unsigned short from_f(float v) { return f2bf(v); }
+  static __device__ __forceinline__ void store4(unsigned short* p, f4 v) {
+    us4 o;
+    o.x = f2bf(v.x);
+    o.y = f2bf(v.y);
+    o.z = f2bf(v.z);
+    o.w = f2bf(v.w);
+    *(us4*)p = o;
+  }
+  static __device__ __forceinline__ f4 load4(const unsigned short* p) {
+    const us4 v = *(const us4*)p;
+    return f4{bf2f(v.x), bf2f(v.y), bf2f(v.z), bf2f(v.w)};
+  }
 };
 
-__device__ __forceinline__ float celu(float z, float a) { return z > 0.0f ? z : a * expm1f(z / a); }
-__device__ __forceinline__ float celu_d(float z, float a) { return z > 0.0f ? 1.0f : __expf(z / a); }
+template <typename T>
+struct Lds {
+  static constexpr int M = Tr<T>::M;
+  static constexpr int LD1 = H1 + Tr<T>::PAD, LD2 = H2 + Tr<T>::PAD, LD3 = H3 + Tr<T>::PAD;
+  static constexpr int LDX = Tr<T>::KC + Tr<T>::PAD;
+  static constexpr int OFF1 = 0;
+  static constexpr int OFF2 = OFF1 + M * LD1 * (int)sizeof(T);
+  static constexpr int OFF3 = OFF2 + M * LD2 * (int)sizeof(T);
+  static constexpr int OFFE = OFF3 + M * LD3 * (int)sizeof(T);
+  static constexpr int BYTES = OFFE + 4 * M * 4;
+  static constexpr int XBUF = M * LDX * (int)sizeof(T);  // one staging buffer, at OFF2
+  static_assert(2 * XBUF <= OFFE - OFF2, "layer-1 staging must fit in the H2/H3 buffers");
+  static_assert(BYTES <= 80 * 1024, "two workgroups per CU");
+};
 
-// C[M x N] += A[M x K] (LDS) * B[K x N] (global) for the col-blocks of this wave
-// (cb = wave + 4 j).  K is a multiple of KS.
-template <typename T, int RB, int CBW>
-__device__ __forceinline__ void gemm_lds(f4 (&acc)[RB][CBW], const T* A, int lda, int K, const T* B,
-                                         int ldb, int ncb, int wave) {
+template <int NB, int MB>
+__device__ __forceinline__ void zero_acc(f4 (&acc)[NB][MB]) {
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+#pragma unroll
+    for (int j = 0; j < MB; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+}
+
+// acc[nb][mb] += sum_k W(n, k) act(k, m) over k in [0, K) for the n-blocks
+// nb0 + 4*nb (< ncb) of this wave.  W prefetched PF steps ahead.
+template <typename T, int NB, int MB>
+__device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ W, int ldw, int nb0, int ncb,
+                                       const T* act, int lda, int K) {
   typedef Tr<T> tr;
-  for (int k0 = 0; k0 < K; k0 += tr::KS) {
-    typename tr::frag a[RB];
+  constexpr int PF = tr::PF;
+  const int nsteps = K / tr::KS;
+  typename tr::frag wq[PF][NB];
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) a[rb] = tr::a_lds(A, lda, rb * 16, k0);
+  for (int u = 0; u < PF; ++u)
 #pragma unroll
-    for (int j = 0; j < CBW; ++j) {
-      const int cb = wave + 4 * j;
-      if (cb < ncb) {
-        const typename tr::frag b = tr::b_glb(B, ldb, k0, cb * 16);
+    for (int nb = 0; nb < NB; ++nb)
+      if (u < nsteps && nb0 + 4 * nb < ncb) wq[u][nb] = tr::wload(W, ldw, (nb0 + 4 * nb) * 16, u * tr::KS);
+  for (int s0 = 0; s0 < nsteps; s0 += PF) {
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) acc[rb][j] = tr::mma(a[rb], b, acc[rb][j]);
+    for (int u = 0; u < PF; ++u) {
+      const int s = s0 + u;
+      if (s < nsteps) {
+        typename tr::frag b[MB];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(act, lda, mb * 16, s * tr::KS);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          if (nb0 + 4 * nb < ncb) {
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) acc[nb][mb] = tr::mma(wq[u][nb], b[mb], acc[nb][mb]);
+            if (s + PF < nsteps) wq[u][nb] = tr::wload(W, ldw, (nb0 + 4 * nb) * 16, (s + PF) * tr::KS);
+          }
+        }
       }
     }
   }
 }
 
-template <int RB, int CBW>
-__device__ __forceinline__ void zero_acc(f4 (&acc)[RB][CBW]) {
-#pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-    for (int j = 0; j < CBW; ++j) acc[rb][j] = f4{0.f, 0.f, 0.f, 0.f};
-}
-
-// forward epilogue: z = acc + bias; H -> LDS (dtype T); D = celu'(z) -> regs
-template <typename T, int RB, int CBW>
-__device__ __forceinline__ void fwd_epilogue(f4 (&acc)[RB][CBW], f4 (&D)[RB][CBW], const float* bias, T* H,
-                                             int ldh, int ncb, int wave, float alpha) {
+// forward epilogue: h = celu(acc + b) -> acc and LDS [m][n]
+template <typename T, int NB, int MB>
+__device__ __forceinline__ void fwd_epilogue(f4 (&acc)[NB][MB], const float* __restrict__ bias, T* H, int ldh,
+                                             int nb0, int ncb, float alpha) {
   const int l = fnl_lane();
+  const float ia = 1.0f / alpha;
 #pragma unroll
-  for (int j = 0; j < CBW; ++j) {
-    const int cb = wave + 4 * j;
+  for (int nb = 0; nb < NB; ++nb) {
+    const int cb = nb0 + 4 * nb;
     if (cb >= ncb) continue;
-    const int col = cb * 16 + (l & 15);
-    const float bv = bias[col];
+    const int n = cb * 16 + 4 * (l >> 4);
+    const f4 bv = *(const f4*)(bias + n);
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
+    for (int mb = 0; mb < MB; ++mb) {
+      f4 v = acc[nb][mb] + bv;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rb * 16 + 4 * (l >> 4) + r;
-        const float z = acc[rb][j][r] + bv;
-        D[rb][j][r] = celu_d(z, alpha);
-        acc[rb][j][r] = celu(z, alpha);
-        H[row * ldh + col] = Tr<T>::from_f(acc[rb][j][r]);
-      }
+      for (int r = 0; r < 4; ++r) v[r] = v[r] > 0.0f ? v[r] : alpha * (__expf(v[r] * ia) - 1.0f);
+      acc[nb][mb] = v;
+      Tr<T>::store4(H + (mb * 16 + (l & 15)) * ldh + n, v);
     }
   }
 }
 
-// backward epilogue: G = acc * D -> LDS
-template <typename T, int RB, int CBW>
-__device__ __forceinline__ void bwd_epilogue(const f4 (&acc)[RB][CBW], const f4 (&D)[RB][CBW], T* G, int ldg,
-                                             int ncb, int wave) {
+// backward epilogue: G = acc * celu'(z) with celu'(z) = h > 0 ? 1 : 1 + h/alpha,
+// h read from the LDS buffer that G then overwrites
+template <typename T, int NB, int MB>
+__device__ __forceinline__ void bwd_epilogue(const f4 (&acc)[NB][MB], T* HG, int ld, int nb0, int ncb,
+                                             float alpha) {
   const int l = fnl_lane();
+  const float ia = 1.0f / alpha;
 #pragma unroll
-  for (int j = 0; j < CBW; ++j) {
-    const int cb = wave + 4 * j;
+  for (int nb = 0; nb < NB; ++nb) {
+    const int cb = nb0 + 4 * nb;
     if (cb >= ncb) continue;
-    const int col = cb * 16 + (l & 15);
+    const int n = cb * 16 + 4 * (l >> 4);
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+    for (int mb = 0; mb < MB; ++mb) {
+      T* p = HG + (mb * 16 + (l & 15)) * ld + n;
+      const f4 h = Tr<T>::load4(p);
+      f4 g;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rb * 16 + 4 * (l >> 4) + r;
-        G[row * ldg + col] = Tr<T>::from_f(acc[rb][j][r] * D[rb][j][r]);
-      }
+      for (int r = 0; r < 4; ++r) g[r] = acc[nb][mb][r] * (h[r] > 0.0f ? 1.0f : 1.0f + h[r] * ia);
+      Tr<T>::store4(p, g);
+    }
   }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256, 1)
+__global__ void __launch_bounds__(256, 2)
 k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int32_t* __restrict__ perm,
               const int32_t* __restrict__ tile_start, const int32_t* __restrict__ tile_rows,
               const int32_t* __restrict__ tile_net, float* __restrict__ energy, float* __restrict__ grad,
               int grad_stride) {
   typedef Tr<T> tr;
+  typedef Lds<T> L;
   constexpr int M = tr::M;
-  constexpr int RB = M / 16;
-  constexpr int LDX = KC + tr::PAD;
-  constexpr int LD1 = H1 + tr::PAD, LD2 = H2 + tr::PAD, LD3 = H3 + tr::PAD;
-  __shared__ __attribute__((aligned(16))) T Xs[M * LDX];
-  __shared__ __attribute__((aligned(16))) T B1[M * LD1];
-  __shared__ __attribute__((aligned(16))) T B2[M * LD2];
-  __shared__ __attribute__((aligned(16))) T B3[M * LD3];
-  __shared__ float Epart[4][M];
-  __shared__ int rows_atom[M];
+  constexpr int MB = M / 16;
+  constexpr int KC = tr::KC;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[L::BYTES];
+  T* B1 = (T*)(smem + L::OFF1);
+  T* B2 = (T*)(smem + L::OFF2);
+  T* B3 = (T*)(smem + L::OFF3);
+  float* Epart = (float*)(smem + L::OFFE);  // [4][M]
+  T* Xs0 = B2;
+  T* Xs1 = (T*)(smem + L::OFF2 + L::XBUF);
 
   const int t = threadIdx.x;
   const int wave = t >> 6;
@@ -211,146 +247,175 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   const NetPtrs& P = args.nets[net];
   const int K = args.k_pad;
   const float alpha = args.alpha;
-  if (t < M) rows_atom[t] = t < nrows ? perm[r0 + t] : -1;
-  __syncthreads();
 
-  // ---------------- layer 1: X (global, staged) x W1 ----------------
-  f4 acc1[RB][4];
-  f4 D1[RB][4];
-  zero_acc(acc1);
+  // ---------------- layer 1: X (global, staged) -> H1 ----------------
   constexpr int VEC = 16 / sizeof(T);          // elements per 16-byte load
-  constexpr int TPR = KC / VEC;                // threads per row
-  constexpr int ROWS_PER_PASS = 256 / TPR;
-  for (int kc = 0; kc < K; kc += KC) {
+  constexpr int TPR = KC / VEC;                // threads per row of a chunk
+  constexpr int RPP = 256 / TPR;               // rows per pass
+  constexpr int NPASS = M / RPP;
+  static_assert(M % RPP == 0, "staging passes");
+  int srow[NPASS];
+  const int scol = (t % TPR) * VEC;
 #pragma unroll
-    for (int rp = 0; rp < M; rp += ROWS_PER_PASS) {
-      const int row = rp + t / TPR;
-      const int c = (t % TPR) * VEC;
-      if (row < M) {
-        const int atom = rows_atom[row];
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (atom >= 0) v = *(const uint4*)(aev + (size_t)atom * aev_stride + kc + c);
-        *(uint4*)(Xs + row * LDX + c) = v;
-      }
-    }
-    __syncthreads();
-    gemm_lds<T, RB, 4>(acc1, Xs, LDX, KC, (const T*)P.fw[0] + (sizeof(T) == 2 ? kc : (size_t)kc * H1),
-                       sizeof(T) == 2 ? K : H1, H1 / 16, wave);
-    __syncthreads();
+  for (int p = 0; p < NPASS; ++p) {
+    const int row = p * RPP + t / TPR;
+    srow[p] = row < nrows ? perm[r0 + row] : -1;
   }
-  fwd_epilogue<T, RB, 4>(acc1, D1, P.b[0], B1, LD1, H1 / 16, wave, alpha);
+  uint4 xr[NPASS];
+  auto stage_load = [&](int kc) {
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p)
+      xr[p] = srow[p] >= 0 ? *(const uint4*)(aev + (size_t)srow[p] * aev_stride + kc + scol)
+                           : make_uint4(0u, 0u, 0u, 0u);
+  };
+  auto stage_store = [&](T* buf) {
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p) *(uint4*)(buf + (p * RPP + t / TPR) * L::LDX + scol) = xr[p];
+  };
+
+  f4 acc1[4][MB];
+  zero_acc(acc1);
+  {
+    constexpr int PF = tr::PF;
+    constexpr int SPC = KC / tr::KS;  // k-steps per chunk
+    static_assert(SPC % PF == 0, "prefetch ring aligned to chunks");
+    const T* W = (const T*)P.fw[0];
+    const int ldw = sizeof(T) == 2 ? K : H1;
+    const int nsteps = K / tr::KS;
+    typename tr::frag wq[PF][4];
+#pragma unroll
+    for (int u = 0; u < PF; ++u)
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) wq[u][nb] = tr::wload(W, ldw, (wave + 4 * nb) * 16, u * tr::KS);
+    stage_load(0);
+    stage_store(Xs0);
+    __syncthreads();
+    const int nchunks = K / KC;
+    for (int c = 0; c < nchunks; ++c) {
+      const T* X = (c & 1) ? Xs1 : Xs0;
+      if (c + 1 < nchunks) stage_load((c + 1) * KC);
+      for (int s0 = 0; s0 < SPC; s0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+          const int s = c * SPC + s0 + u;  // global k-step
+          typename tr::frag b[MB];
+#pragma unroll
+          for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(X, L::LDX, mb * 16, (s0 + u) * tr::KS);
+#pragma unroll
+          for (int nb = 0; nb < 4; ++nb) {
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) acc1[nb][mb] = tr::mma(wq[u][nb], b[mb], acc1[nb][mb]);
+            if (s + PF < nsteps) wq[u][nb] = tr::wload(W, ldw, (wave + 4 * nb) * 16, (s + PF) * tr::KS);
+          }
+        }
+      }
+      if (c + 1 < nchunks) stage_store((c & 1) ? Xs0 : Xs1);
+      __syncthreads();
+    }
+  }
+  fwd_epilogue<T, 4, MB>(acc1, P.b[0], B1, L::LD1, wave, H1 / 16, alpha);
   __syncthreads();
 
   // ---------------- layer 2 ----------------
-  f4 acc2[RB][3];
-  f4 D2[RB][3];
-  zero_acc(acc2);
-  gemm_lds<T, RB, 3>(acc2, B1, LD1, H1, (const T*)P.fw[1], sizeof(T) == 2 ? H1 : H2, H2 / 16, wave);
-  fwd_epilogue<T, RB, 3>(acc2, D2, P.b[1], B2, LD2, H2 / 16, wave, alpha);
+  {
+    f4 acc2[3][MB];
+    zero_acc(acc2);
+    gemm_t<T, 3, MB>(acc2, (const T*)P.fw[1], sizeof(T) == 2 ? H1 : H2, wave, H2 / 16, B1, L::LD1, H1);
+    fwd_epilogue<T, 3, MB>(acc2, P.b[1], B2, L::LD2, wave, H2 / 16, alpha);
+  }
   __syncthreads();
 
   // ---------------- layer 3 + energy ----------------
-  f4 acc3[RB][3];
-  f4 D3[RB][3];
+  f4 acc3[3][MB];
   zero_acc(acc3);
-  gemm_lds<T, RB, 3>(acc3, B2, LD2, H2, (const T*)P.fw[2], sizeof(T) == 2 ? H2 : H3, H3 / 16, wave);
-  fwd_epilogue<T, RB, 3>(acc3, D3, P.b[2], B3, LD3, H3 / 16, wave, alpha);
+  gemm_t<T, 3, MB>(acc3, (const T*)P.fw[2], sizeof(T) == 2 ? H2 : H3, wave, H3 / 16, B2, L::LD2, H2);
+  fwd_epilogue<T, 3, MB>(acc3, P.b[2], B3, L::LD3, wave, H3 / 16, alpha);
   {
-    // E_row = sum_c H3[row][c] w4[c]: lanes hold 4 rows x 1 column per block
-    float part[RB][4];
+    float part[MB];
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+    for (int mb = 0; mb < MB; ++mb) part[mb] = 0.0f;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) part[rb][r] = 0.0f;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const int cb = wave + 4 * j;
+    for (int nb = 0; nb < 3; ++nb) {
+      const int cb = wave + 4 * nb;
       if (cb >= H3 / 16) continue;
-      const float wv = P.w4[cb * 16 + (l & 15)];
+      const f4 wv = *(const f4*)(P.w4 + cb * 16 + 4 * (l >> 4));
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) part[rb][r] += acc3[rb][j][r] * wv;
+      for (int mb = 0; mb < MB; ++mb)
+        part[mb] += acc3[nb][mb][0] * wv[0] + acc3[nb][mb][1] * wv[1] + acc3[nb][mb][2] * wv[2] +
+                    acc3[nb][mb][3] * wv[3];
     }
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        float v = part[rb][r];
-        v += __shfl_xor(v, 8, 64);
-        v += __shfl_xor(v, 4, 64);
-        v += __shfl_xor(v, 2, 64);
-        v += __shfl_xor(v, 1, 64);
-        if ((l & 15) == 0) Epart[wave][rb * 16 + 4 * (l >> 4) + r] = v;
-      }
+    for (int mb = 0; mb < MB; ++mb) {
+      float v = part[mb];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      if (l < 16) Epart[wave * M + mb * 16 + l] = v;
+    }
   }
   __syncthreads();
-  if (t < M && rows_atom[t] >= 0) {
-    energy[rows_atom[t]] = ((Epart[0][t] + Epart[1][t]) + (Epart[2][t] + Epart[3][t])) + P.b[3][0];
+  if (t < nrows) {
+    energy[perm[r0 + t]] =
+        ((Epart[t] + Epart[M + t]) + (Epart[2 * M + t] + Epart[3 * M + t])) + P.b[3][0];
   }
   if (grad == nullptr) return;
 
   // ---------------- backward ----------------
-  // G3 = w4 * D3 -> B3
+  {
+    // G3 = w4 * celu'(z3) -> B3 (h3 is in acc3)
+    const float ia = 1.0f / alpha;
 #pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    const int cb = wave + 4 * j;
-    if (cb >= H3 / 16) continue;
-    const int col = cb * 16 + (l & 15);
-    const float wv = P.w4[col];
+    for (int nb = 0; nb < 3; ++nb) {
+      const int cb = wave + 4 * nb;
+      if (cb >= H3 / 16) continue;
+      const int n = cb * 16 + 4 * (l >> 4);
+      const f4 wv = *(const f4*)(P.w4 + n);
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rb * 16 + 4 * (l >> 4) + r;
-        B3[row * LD3 + col] = tr::from_f(wv * D3[rb][j][r]);
-      }
-  }
-  __syncthreads();
-  // G2 = (G3 W3^T) * D2 -> B2
-  zero_acc(acc2);
-  gemm_lds<T, RB, 3>(acc2, B3, LD3, H3, (const T*)P.bw[2], sizeof(T) == 2 ? H3 : H2, H2 / 16, wave);
-  bwd_epilogue<T, RB, 3>(acc2, D2, B2, LD2, H2 / 16, wave);
-  __syncthreads();
-  // G1 = (G2 W2^T) * D1 -> B1
-  zero_acc(acc1);
-  gemm_lds<T, RB, 4>(acc1, B2, LD2, H2, (const T*)P.bw[1], sizeof(T) == 2 ? H2 : H1, H1 / 16, wave);
-  bwd_epilogue<T, RB, 4>(acc1, D1, B1, LD1, H1 / 16, wave);
-  __syncthreads();
-  // dE/dX = G1 W1^T, 64 output columns per wave per pass
-  const int ncb_x = K / 16;
-  for (int cbase = 0; cbase < ncb_x; cbase += 16) {
-    f4 accx[RB][4];
-    zero_acc(accx);
-    // col-blocks of this pass: cbase + wave + 4 j
-    typedef Tr<T> trr;
-    for (int k0 = 0; k0 < H1; k0 += trr::KS) {
-      typename trr::frag a[RB];
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb) a[rb] = trr::a_lds(B1, LD1, rb * 16, k0);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int cb = cbase + wave + 4 * j;
-        if (cb < ncb_x) {
-          const typename trr::frag b = trr::b_glb((const T*)P.bw[0], sizeof(T) == 2 ? H1 : K, k0, cb * 16);
-#pragma unroll
-          for (int rb = 0; rb < RB; ++rb) accx[rb][j] = trr::mma(a[rb], b, accx[rb][j]);
-        }
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int cb = cbase + wave + 4 * j;
-      if (cb >= ncb_x) continue;
-      const int col = cb * 16 + (l & 15);
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
+      for (int mb = 0; mb < MB; ++mb) {
+        f4 g;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = rb * 16 + 4 * (l >> 4) + r;
-          const int atom = rows_atom[row];
-          if (atom >= 0) grad[(size_t)atom * grad_stride + col] = accx[rb][j][r];
+          const float h = acc3[nb][mb][r];
+          g[r] = wv[r] * (h > 0.0f ? 1.0f : 1.0f + h * ia);
         }
+        tr::store4(B3 + (mb * 16 + (l & 15)) * L::LD3 + n, g);
+      }
+    }
+  }
+  __syncthreads();
+  {
+    f4 acc2[3][MB];
+    zero_acc(acc2);
+    gemm_t<T, 3, MB>(acc2, (const T*)P.bw[2], sizeof(T) == 2 ? H3 : H2, wave, H2 / 16, B3, L::LD3, H3);
+    bwd_epilogue<T, 3, MB>(acc2, B2, L::LD2, wave, H2 / 16, alpha);
+  }
+  __syncthreads();
+  {
+    f4 accg[4][MB];
+    zero_acc(accg);
+    gemm_t<T, 4, MB>(accg, (const T*)P.bw[1], sizeof(T) == 2 ? H2 : H1, wave, H1 / 16, B2, L::LD2, H2);
+    bwd_epilogue<T, 4, MB>(accg, B1, L::LD1, wave, H1 / 16, alpha);
+  }
+  __syncthreads();
+  // dE/dX = W1 G1: input features in passes of 16 n-blocks (4 per wave)
+  int atom[MB];
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) {
+    const int m = mb * 16 + (l & 15);
+    atom[mb] = m < nrows ? perm[r0 + m] : -1;
+  }
+  const int ncb_x = K / 16;
+  for (int base = 0; base < ncb_x; base += 16) {
+    f4 accx[4][MB];
+    zero_acc(accx);
+    gemm_t<T, 4, MB>(accx, (const T*)P.bw[0], sizeof(T) == 2 ? H1 : K, base + wave, ncb_x, B1, L::LD1, H1);
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+      const int cb = base + wave + 4 * nb;
+      if (cb >= ncb_x) continue;
+      const int n = cb * 16 + 4 * (l >> 4);
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+        if (atom[mb] >= 0) *(f4*)(grad + (size_t)atom[mb] * grad_stride + n) = accx[nb][mb];
     }
   }
 }
@@ -371,13 +436,16 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
   if (n_tiles == 0) return FNL_OK;
   if (!aev || !perm || !tile_start || !tile_rows || !tile_net || !energy) return FNL_ERR_INVALID;
   if (w->n_nets < 1 || w->n_nets > FNL_MLP_MAX_NETS) return FNL_ERR_INVALID;
-  if (w->k_pad < KC || w->k_pad % KC != 0 || aev_stride < w->k_pad) return FNL_ERR_INVALID;
-  if (grad && grad_stride < w->k_pad) return FNL_ERR_INVALID;
+  const bool bf = w->dtype == FNL_BF16;
+  if (!bf && w->dtype != FNL_F32) return FNL_ERR_INVALID;
+  const int kc = bf ? Tr<unsigned short>::KC : Tr<float>::KC;
+  if (w->k_pad < kc || w->k_pad % kc != 0 || aev_stride < w->k_pad ||
+      aev_stride % 8 != 0)
+    return FNL_ERR_INVALID;
+  if (grad && (grad_stride < w->k_pad || grad_stride % 4 != 0)) return FNL_ERR_INVALID;
   MlpArgs a;
   a.k_pad = w->k_pad;
   a.alpha = w->celu_alpha;
-  const bool bf = w->dtype == FNL_BF16;
-  if (!bf && w->dtype != FNL_F32) return FNL_ERR_INVALID;
   for (int n = 0; n < FNL_MLP_MAX_NETS; ++n) {
     NetPtrs& p = a.nets[n];
     if (n >= w->n_nets) {
@@ -387,8 +455,8 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
     const fnl_mlp_net& s = w->nets[n];
     if (!s.w1t || !s.w2t || !s.w3t || !s.w1 || !s.w2 || !s.w3 || !s.w4 || !s.b1 || !s.b2 || !s.b3 || !s.b4)
       return FNL_ERR_INVALID;
-    // forward B operands: K-contiguous for BF16, N-contiguous for FP32 (and the
-    // opposite for the backward GEMMs)
+    // A operands: K-contiguous for BF16 (8 k per lane), N-contiguous for FP32
+    // (16 lanes read 16 consecutive n); forward uses W^T, backward W.
     p.fw[0] = bf ? s.w1t : s.w1;
     p.fw[1] = bf ? s.w2t : s.w2;
     p.fw[2] = bf ? s.w3t : s.w3;

@@ -385,8 +385,12 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
       atomicOr(&status[FNL_STATUS_OVERFLOW], 1);
       atomicOr(&status[FNL_STATUS_STICKY], 1);
     }
-    atomicMax(&status[FNL_STATUS_MAX_NBR], count);
-    atomicMax(&status[FNL_STATUS_MAX_NBR_ANG], count_a);
+    // Same-address atomics from every centre serialise in one L2 channel:
+    // only centres above the running maximum (a relaxed read) update it.
+    if (count > __hip_atomic_load(&status[FNL_STATUS_MAX_NBR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(&status[FNL_STATUS_MAX_NBR], count);
+    if (count_a > __hip_atomic_load(&status[FNL_STATUS_MAX_NBR_ANG], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(&status[FNL_STATUS_MAX_NBR_ANG], count_a);
   }
 }
 

@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from . import _abi, units
-from .potential import ANIPotential, SystemBuffers
+from .potential import SEG_ATOMS, ANIPotential, SystemBuffers
 
 
 class LangevinMD:
@@ -121,8 +121,9 @@ class LangevinMD:
         """0.5 sum m v v^T per system in kcal/mol (integrate.py:320-339)."""
         sb = self.sb
         out = torch.zeros((sb.n_sys, 9), dtype=torch.float64, device=self.pot.device)
-        _abi.check(self.lib.fnl_system_reduce(sb.sys_offsets.data_ptr(), sb.n_sys, None, None, None, None,
-                                              self.v.data_ptr(), self.t_mass.data_ptr(), out.data_ptr(),
+        _abi.check(self.lib.fnl_system_reduce(sb.sys_offsets.data_ptr(), sb.seg_offsets.data_ptr(), sb.n_sys,
+                                              sb.n_seg, SEG_ATOMS, None, None, None, None, self.v.data_ptr(),
+                                              self.t_mass.data_ptr(), out.data_ptr(), sb.partials.data_ptr(),
                                               _abi.stream_handle()), "fnl_system_reduce")
         return out.cpu().numpy().reshape(-1, 3, 3)
 

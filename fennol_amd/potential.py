@@ -24,6 +24,7 @@ from . import _abi
 from .units import PERIODIC_TABLE, SYMBOL_TO_Z
 
 CELU_ALPHA = 0.1
+SEG_ATOMS = 512  # atoms per partial-sum segment of the per-system reductions
 
 
 @dataclass(frozen=True)
@@ -57,7 +58,8 @@ class ANIConfig:
 
     @property
     def k_pad(self) -> int:
-        return ((self.aev_length + 63) // 64) * 64
+        """AEV row length padded to the MLP kernel's K chunk (128)."""
+        return ((self.aev_length + 127) // 128) * 128
 
     def c_params(self) -> _abi.AevParams:
         """Kernel parameters; shifts computed in float64 and rounded to float32 (ani.py:73-110)."""
@@ -182,6 +184,9 @@ class SystemBuffers:
     vir_atom: torch.Tensor = None
     energy: torch.Tensor = None         # float64 [B]
     virial: torch.Tensor = None         # float64 [B,9]
+    seg_offsets: torch.Tensor = None    # int32 [B+1] reduction segments per system
+    n_seg: int = 0
+    partials: torch.Tensor = None       # float64 [n_seg, 19]
     extra: dict = field(default_factory=dict)
 
 
@@ -299,6 +304,11 @@ class ANIPotential:
         sb.vir_atom = torch.zeros((N, 9), dtype=torch.float32, device=dev)
         sb.energy = torch.zeros(B, dtype=torch.float64, device=dev)
         sb.virial = torch.zeros((B, 9), dtype=torch.float64, device=dev)
+        nseg = (natoms.astype(np.int64) + SEG_ATOMS - 1) // SEG_ATOMS
+        segoff = np.concatenate([[0], np.cumsum(nseg)]).astype(np.int32)
+        sb.seg_offsets = torch.tensor(segoff, device=dev)
+        sb.n_seg = int(segoff[-1])
+        sb.partials = torch.zeros((max(sb.n_seg, 1), _abi.FNL_REDUCE_VALUES), dtype=torch.float64, device=dev)
         return sb
 
     def _alloc_nblist(self, sb: SystemBuffers) -> None:
@@ -397,8 +407,8 @@ class ANIPotential:
         s = _abi.stream_handle(stream)
         P = _abi.ptr
         rc = self.lib.fnl_system_reduce(
-            P(sb.sys_offsets), sb.n_sys, P(sb.e_atom), P(sb.energy), P(sb.vir_atom) if with_virial else None,
-            P(sb.virial), None, None, None, s)
+            P(sb.sys_offsets), P(sb.seg_offsets), sb.n_sys, sb.n_seg, SEG_ATOMS, P(sb.e_atom), P(sb.energy),
+            P(sb.vir_atom) if with_virial else None, P(sb.virial), None, None, None, P(sb.partials), s)
         _abi.check(rc, "fnl_system_reduce")
 
     def compute(self, sb: SystemBuffers, coords32: torch.Tensor, forces: bool = True,

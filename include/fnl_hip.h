@@ -210,15 +210,21 @@ int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_mlp_weights* 
 /* ------------------------------------------------------------------------
  * Per-system reductions in FP64 (total_energy segment_sum, fennix.py:240-246;
  * virial tensor, fennix.py:320-347; kinetic tensor, md/integrate.py:320-339).
- * Deterministic: one workgroup per system, fixed summation tree.
+ * Deterministic two-stage tree: each system is cut into segments of at most
+ * seg_atoms atoms; seg_offsets[b] = index of the first segment of system b
+ * (exclusive prefix sum of ceil(natoms_b / seg_atoms)), seg_offsets[n_sys] =
+ * n_seg; partials is caller scratch of n_seg * 19 doubles.
  * energy_atom may be NULL (energy_sys untouched), likewise virial_atom /
  * virial_sys, and vel (kinetic_sys untouched; mass is then ignored).
  * ---------------------------------------------------------------------- */
-int fnl_system_reduce(const int32_t* sys_offsets, int32_t n_sys,
+#define FNL_REDUCE_VALUES 19
+
+int fnl_system_reduce(const int32_t* sys_offsets, const int32_t* seg_offsets,
+                      int32_t n_sys, int32_t n_seg, int32_t seg_atoms,
                       const float* energy_atom, double* energy_sys,
                       const float* virial_atom, double* virial_sys,
                       const double* vel, const double* mass, double* kinetic_sys,
-                      fnl_stream_t stream);
+                      double* partials, fnl_stream_t stream);
 
 /* forces = -(grad_self + grad_nbr) * scale  (float32[N][3]) */
 int fnl_forces(const float* grad_self, const float* grad_nbr, int32_t n_atoms,
