@@ -110,11 +110,26 @@ __device__ __forceinline__ int pair_index(int a, int b, int S) {
 }
 
 // per-wave LDS footprint (in 4-byte words) of the forward kernel
-__host__ __device__ inline int fwd_lds_words(int max_nbr, int max_nbr_ang, int S, int nR, int P, int NA,
-                                             int NZ) {
-  return 6 * max_nbr + 6 * max_nbr_ang + 4 * S * 16 + 2 * P * 32 + 64 * (NA + NZ + 1);
+__host__ __device__ inline int fwd_lds_words(int max_nbr, int max_nbr_ang, int aev_len) {
+  return 6 * max_nbr + 6 * max_nbr_ang + ((aev_len + 3) & ~3) + 64 * 16;
 }
 
+__device__ __forceinline__ void lds_add(float* p, float v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Forward, one wave per centre.  LDS slice of the wave (4-byte words):
+//   nbv  [max_nbr]     float4 (vx, vy, vz, d) per radial slot
+//   nbf  [max_nbr]     float2 (0.25 fc(d), species)
+//   av   [max_nbr_ang] float4 (ux, uy, uz, d) per angular slot
+//   af   [max_nbr_ang] float2 (fc_ang(d), species)
+//   acc  [aev_len]     the AEV row being accumulated, in output channel order
+//   trip [64][16]      per-triplet factors: R_d (NA), F_z (NZ), pair index
+// Radial: lanes (q = lane/16, k = lane%16) add 0.25 fc exp(-eta (d-R_k)^2)
+// for neighbours e = q, q+4, ... into acc[s*nR + k] (LDS float adds).
+// Angular: per chunk of 64 triplets, lane t evaluates R_d and F_z of triplet
+// t; then lanes (g = lane/NZ, z = lane%NZ) add R_d F_z into
+// acc[rad_len + p*NA*NZ + d*NZ + z] for the triplets g, g + 64/NZ, ...
 template <int NA, int NZ, typename TOut>
 __global__ void __launch_bounds__(64 * kWaves)
 k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
@@ -123,30 +138,20 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           const int32_t* __restrict__ nbr, const int32_t* __restrict__ n_nbr, int max_nbr,
           const int32_t* __restrict__ nbr_ang, const int32_t* __restrict__ n_nbr_ang, int max_nbr_ang,
           fnl_aev_params p, AevDims dm, TOut* __restrict__ aev, int stride) {
-  extern __shared__ float lds[];
+  static_assert(NA * NZ == 32 && (NA == 4 || NA == 8), "angular channel layout");
+  extern __shared__ float4 lds4[];
   const int w = threadIdx.x >> 6;
   const int lane = fnl_lane();
   const int i = blockIdx.x * kWaves + w;
   if (i >= n_atoms) return;
-  const int words = fwd_lds_words(max_nbr, max_nbr_ang, dm.S, dm.nR, dm.P, NA, NZ);
-  float* base = lds + (size_t)w * words;
-  RadialRow rr;
-  rr.d = base;
-  rr.fc = rr.d + max_nbr;
-  rr.s = (int*)(rr.fc + max_nbr);
-  rr.vx = (float*)(rr.s + max_nbr);
-  rr.vy = rr.vx + max_nbr;
-  rr.vz = rr.vy + max_nbr;
-  float* ax = rr.vz + max_nbr;  // angular rows: ux uy uz d fc s
-  float* ay = ax + max_nbr_ang;
-  float* az = ay + max_nbr_ang;
-  float* ad = az + max_nbr_ang;
-  float* afc = ad + max_nbr_ang;
-  int* as = (int*)(afc + max_nbr_ang);
-  float* racc = (float*)(as + max_nbr_ang);  // [4][S*16]
-  float* aacc = racc + 4 * dm.S * 16;        // [2][P][32]
-  float* trip = aacc + 2 * dm.P * 32;        // [64][NZ + NA + 1]
-  constexpr int TW = NA + NZ + 1;
+  const int words = fwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
+  float* base = (float*)lds4 + (size_t)w * words;
+  float4* nbv = (float4*)base;
+  float2* nbf = (float2*)(base + 4 * max_nbr);
+  float4* av = (float4*)(base + 6 * max_nbr);
+  float2* af = (float2*)(base + 6 * max_nbr + 4 * max_nbr_ang);
+  float* acc = base + 6 * max_nbr + 6 * max_nbr_ang;
+  float* trip = acc + ((dm.aev_len + 3) & ~3);
 
   TOut* out = aev + (size_t)i * stride;
   const int si = species_idx[i];
@@ -160,101 +165,135 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const int nr = min(n_nbr[i], max_nbr);
   const int na = min(n_nbr_ang[i], max_nbr_ang);
 
-  for (int c = lane; c < 4 * dm.S * 16; c += 64) racc[c] = 0.0f;
-  for (int c = lane; c < 2 * dm.P * 32; c += 64) aacc[c] = 0.0f;
-  load_radial_rows(i, nr, coords, species_idx, nbr + (size_t)i * max_nbr, pbc, cl, p.cutoff, 0.25f, rr);
+  for (int c = lane; c < dm.aev_len; c += 64) acc[c] = 0.0f;
+  {
+    const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
+    const int32_t* row = nbr + (size_t)i * max_nbr;
+    for (int e = lane; e < nr; e += 64) {
+      const int j = row[e];
+      float vx = coords[3 * j] - xi, vy = coords[3 * j + 1] - yi, vz = coords[3 * j + 2] - zi;
+      if (pbc) fnl_min_image(vx, vy, vz, cl);
+      const float d = sqrtf(fmaxf(vx * vx + vy * vy + vz * vz, 1.0e-5f));
+      nbv[e] = make_float4(vx, vy, vz, d);
+      nbf[e] = make_float2(0.25f * cos_switch(d, p.cutoff), __int_as_float(species_idx[j]));
+    }
+  }
   wave_sync();
 
   // radial channels
   {
     const int q = lane >> 4, k = lane & 15;
-    const bool kv = k < dm.nR;
-    const float shift = kv ? p.radial_shifts[k] : 0.0f;
-    float* col = racc + q * dm.S * 16 + k;
-    for (int e = q; e < nr; e += 4) {
-      const float x = rr.d[e] - shift;
-      const float term = rr.fc[e] * __expf(-p.radial_eta * x * x);
-      if (kv) col[rr.s[e] * 16] += term;
+    if (k < dm.nR) {
+      const float shift = p.radial_shifts[k];
+      for (int e = q; e < nr; e += 4) {
+        const float d = nbv[e].w;
+        const float2 f = nbf[e];
+        const float x = d - shift;
+        lds_add(acc + __float_as_int(f.y) * dm.nR + k, f.x * __expf(-p.radial_eta * x * x));
+      }
     }
   }
 
-  // angular rows
-  const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
-  for (int a = lane; a < na; a += 64) {
-    const int e = arow[a];
-    const float d = rr.d[e];
-    const float inv = 1.0f / fmaxf(d, 1.0e-5f);
-    ax[a] = rr.vx[e] * inv;
-    ay[a] = rr.vy[e] * inv;
-    az[a] = rr.vz[e] * inv;
-    ad[a] = d;
-    afc[a] = cos_switch(d, p.cutoff_ang);
-    as[a] = rr.s[e];
+  // angular rows (unit vectors of the radial slots listed in the angular row)
+  {
+    const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
+    for (int a = lane; a < na; a += 64) {
+      const int e = arow[a];
+      const float4 v = nbv[e];
+      const float inv = 1.0f / fmaxf(v.w, 1.0e-5f);
+      av[a] = make_float4(v.x * inv, v.y * inv, v.z * inv, v.w);
+      af[a] = make_float2(cos_switch(v.w, p.cutoff_ang), nbf[e].y);
+    }
   }
   wave_sync();
 
   const float pref = 2.0f * exp2f(-p.zeta);  // 2 * 0.5^zeta (ani.py:112)
   const int T = na * (na - 1) / 2;
-  const int h = lane >> 5, ch = lane & 31;
-  const int cd = ch / NZ, cz = ch % NZ;
+  float* aacc = acc + dm.rad_len;
+  constexpr int G = 64 / NZ;  // triplet groups in the accumulation phase
+  const int g = lane / NZ, z = lane % NZ;
   for (int base_t = 0; base_t < T; base_t += 64) {
     const int t = base_t + lane;
-    float* tr = trip + lane * TW;
+    float* tr = trip + lane * 16;
     if (t < T) {
       int a, b;
       tri_pair(t, na, a, b);
-      const float c = ax[a] * ax[b] + ay[a] * ay[b] + az[a] * az[b];
+      const float4 ua = av[a], ub = av[b];
+      const float2 fa = af[a], fb = af[b];
+      const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
       const float x = 0.95f * c;
       const float st = sqrtf(fmaxf(1.0f - x * x, 0.0f));
-      const float sw = pref * afc[a] * afc[b];
-#pragma unroll
-      for (int z = 0; z < NZ; ++z) {
-        const float bz = 1.0f + x * p.ang_cos[z] + st * p.ang_sin[z];
-        tr[z] = bz > 0.0f ? sw * exp2f(p.zeta * __log2f(bz)) : 0.0f;
-      }
-      const float m = 0.5f * (ad[a] + ad[b]);
+      const float sw = pref * fa.x * fb.x;
+      const float m = 0.5f * (ua.w + ub.w);
 #pragma unroll
       for (int d = 0; d < NA; ++d) {
         const float y = m - p.ang_shifts[d];
-        tr[NZ + d] = __expf(-p.angular_eta * y * y);
+        tr[d] = __expf(-p.angular_eta * y * y);
       }
-      tr[NZ + NA] = __int_as_float(pair_index(as[a], as[b], dm.S));
+#pragma unroll
+      for (int zz = 0; zz < NZ; ++zz) {
+        const float bz = 1.0f + x * p.ang_cos[zz] + st * p.ang_sin[zz];
+        tr[NA + zz] = bz > 0.0f ? sw * exp2f(p.zeta * __log2f(bz)) : 0.0f;
+      }
+      tr[NA + NZ] = __int_as_float(pair_index(__float_as_int(fa.y), __float_as_int(fb.y), dm.S) * 32);
     } else {
-      tr[NZ + NA] = __int_as_float(-1);
+      tr[NA + NZ] = __int_as_float(-1);
     }
     wave_sync();
-    float* col = aacc + h * dm.P * 32 + ch;
-#pragma unroll 4
-    for (int tt = 0; tt < 32; ++tt) {
-      const float* q = trip + (h * 32 + tt) * TW;
-      const int pi = __float_as_int(q[NZ + NA]);
-      if (pi >= 0) col[pi * 32] += q[cz] * q[NZ + cd];
+    const int nt = min(64, T - base_t);
+    for (int tt = g; tt < nt; tt += G) {
+      const float* q = trip + tt * 16;
+      const int po = __float_as_int(q[NA + NZ]);
+      const float f = q[NA + z];
+      float R[NA];
+      *(float4*)R = *(const float4*)q;
+      if (NA == 8) *(float4*)(R + 4) = *(const float4*)(q + 4);
+#pragma unroll
+      for (int d = 0; d < NA; ++d) lds_add(aacc + po + d * NZ + z, R[d] * f);
     }
     wave_sync();
   }
-  wave_sync();
 
-  // write the row: radial | angular | zero padding
-  const int ang_len = dm.P * NA * NZ;
-  for (int c = lane; c < stride; c += 64) {
-    float v = 0.0f;
-    if (c < dm.rad_len) {
-      const int s = c / dm.nR, k = c % dm.nR;
-      const int o = s * 16 + k;
-      v = racc[o] + racc[dm.S * 16 + o] + racc[2 * dm.S * 16 + o] + racc[3 * dm.S * 16 + o];
-    } else if (c < dm.rad_len + ang_len) {
-      const int o = c - dm.rad_len;
-      const int pi = o / (NA * NZ), cc = o % (NA * NZ);
-      v = aacc[pi * 32 + cc] + aacc[dm.P * 32 + pi * 32 + cc];
+  // the row: acc (radial | angular) then zero padding
+  if constexpr (sizeof(TOut) == 4) {
+    for (int c = 4 * lane; c < stride; c += 256) {
+      float4 v;
+      v.x = c < dm.aev_len ? acc[c] : 0.0f;
+      v.y = c + 1 < dm.aev_len ? acc[c + 1] : 0.0f;
+      v.z = c + 2 < dm.aev_len ? acc[c + 2] : 0.0f;
+      v.w = c + 3 < dm.aev_len ? acc[c + 3] : 0.0f;
+      if (c + 3 < stride) {
+        *(float4*)(out + c) = v;
+      } else {
+        for (int k = 0; k < 4 && c + k < stride; ++k) out[c + k] = (&v.x)[k];
+      }
     }
-    out[c] = to_out<TOut>(v);
+  } else {
+    for (int c = 4 * lane; c < stride; c += 256) {
+      TOut o[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) o[k] = to_out<TOut>(c + k < dm.aev_len ? acc[c + k] : 0.0f);
+      if (c + 3 < stride) {
+        *(uint2*)(out + c) = *(const uint2*)o;
+      } else {
+        for (int k = 0; k < 4 && c + k < stride; ++k) out[c + k] = o[k];
+      }
+    }
   }
 }
 
 __host__ __device__ inline int bwd_lds_words(int max_nbr, int max_nbr_ang, int aev_len) {
-  return aev_len + 9 * max_nbr + 9 * max_nbr_ang;
+  return ((aev_len + 3) & ~3) + 10 * max_nbr + 8 * max_nbr_ang;
 }
 
+// Adjoint, one wave per centre.  LDS slice of the wave (4-byte words):
+//   G   [aev_len]      dE/dAEV row of the centre
+//   nbv [max_nbr]      float4 (vx, vy, vz, d) per radial slot
+//   nbf [max_nbr]      float2 (fc(d), species)
+//   gs  [max_nbr]      float4 dE/dvec of the slot (radial part stored by the
+//                      slot's lane, angular part added with LDS float adds)
+//   av  [max_nbr_ang]  float4 (ux, uy, uz, d) per angular slot
+//   af  [max_nbr_ang]  float4 (fc_ang, fc_ang', species, radial slot)
 template <int NA, int NZ>
 __global__ void __launch_bounds__(64 * kWaves)
 k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
@@ -264,32 +303,19 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           const int32_t* __restrict__ nbr_ang, const int32_t* __restrict__ n_nbr_ang, int max_nbr_ang,
           fnl_aev_params p, AevDims dm, const float* __restrict__ g, int g_stride,
           float* __restrict__ grad_self, float* __restrict__ grad_nbr, float* __restrict__ virial_atom) {
-  extern __shared__ float lds[];
+  static_assert(NA * NZ == 32, "angular channel layout");
+  extern __shared__ float4 lds4[];
   const int w = threadIdx.x >> 6;
   const int lane = fnl_lane();
   const int i = blockIdx.x * kWaves + w;
   if (i >= n_atoms) return;
   const int words = bwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
-  float* G = lds + (size_t)w * words;
-  RadialRow rr;
-  rr.d = G + dm.aev_len;
-  rr.fc = rr.d + max_nbr;  // unscaled switch
-  rr.s = (int*)(rr.fc + max_nbr);
-  rr.vx = (float*)(rr.s + max_nbr);
-  rr.vy = rr.vx + max_nbr;
-  rr.vz = rr.vy + max_nbr;
-  float* gx = rr.vz + max_nbr;  // per radial slot dE/dvec
-  float* gy = gx + max_nbr;
-  float* gz = gy + max_nbr;
-  float* ax = gz + max_nbr;  // angular rows
-  float* ay = ax + max_nbr_ang;
-  float* az = ay + max_nbr_ang;
-  float* ad = az + max_nbr_ang;
-  float* afc = ad + max_nbr_ang;
-  float* adfc = afc + max_nbr_ang;
-  float* aok = adfc + max_nbr_ang;  // 1 if d2 > 1e-5 (distance differentiable)
-  int* as = (int*)(aok + max_nbr_ang);
-  int* ae = as + max_nbr_ang;
+  float* G = (float*)lds4 + (size_t)w * words;
+  float4* nbv = (float4*)(G + ((dm.aev_len + 3) & ~3));
+  float2* nbf = (float2*)(nbv + max_nbr);
+  float4* gs = (float4*)((float*)nbf + 2 * max_nbr);
+  float4* av = gs + max_nbr;
+  float4* af = av + max_nbr_ang;
 
   const int si = species_idx[i];
   if (si < 0) {
@@ -302,61 +328,47 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   if (pbc) fnl_load_cell(cells, rec, batch_index[i], cl);
   const int nr = min(n_nbr[i], max_nbr);
   const int na = min(n_nbr_ang[i], max_nbr_ang);
-  const float* grow = g + (size_t)i * g_stride;
-  for (int c = lane; c < dm.aev_len; c += 64) G[c] = grow[c];
-  for (int e = lane; e < nr; e += 64) gx[e] = gy[e] = gz[e] = 0.0f;
-  load_radial_rows(i, nr, coords, species_idx, nbr + (size_t)i * max_nbr, pbc, cl, p.cutoff, 1.0f, rr);
-  wave_sync();
-
-  // radial: dE/dd_e = sum_k G[s_e, k] * d/dd (0.25 fc exp(-eta (d - R_k)^2))
   {
-    const int q = lane >> 4, k = lane & 15;
-    const bool kv = k < dm.nR;
-    const float shift = kv ? p.radial_shifts[k] : 0.0f;
-    const int nrr = (nr + 3) & ~3;
-    for (int e = q; e < nrr; e += 4) {
+    const float4* grow = (const float4*)(g + (size_t)i * g_stride);
+    for (int c = lane; c < (dm.aev_len + 3) / 4; c += 64) ((float4*)G)[c] = grow[c];
+  }
+  const int32_t* row = nbr + (size_t)i * max_nbr;
+  {
+    const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
+    for (int e = lane; e < nr; e += 64) {
+      const int j = row[e];
+      float vx = coords[3 * j] - xi, vy = coords[3 * j + 1] - yi, vz = coords[3 * j + 2] - zi;
+      if (pbc) fnl_min_image(vx, vy, vz, cl);
+      const float d2 = vx * vx + vy * vy + vz * vz;
+      const float d = sqrtf(fmaxf(d2, 1.0e-5f));
+      nbv[e] = make_float4(vx, vy, vz, d);
+      const float fc = cos_switch(d, p.cutoff);
+      const int sj = species_idx[j];
+      nbf[e] = make_float2(fc, __int_as_float(sj));
+      // radial: dE/dd = sum_k G[s, k] * 0.25 exp(-eta x^2) (fc' - 2 eta x fc), x = d - R_k
+      const float dfc = cos_switch_grad(d, p.cutoff);
+      const float* gr = G + sj * dm.nR;
       float v = 0.0f;
-      if (e < nr && kv) {
-        const float d = rr.d[e];
-        const float x = d - shift;
-        const float ex = __expf(-p.radial_eta * x * x);
-        const float fc = rr.fc[e];
-        const float dfc = cos_switch_grad(d, p.cutoff);
-        v = G[rr.s[e] * dm.nR + k] * 0.25f * ex * (dfc - 2.0f * p.radial_eta * x * fc);
+      for (int k = 0; k < dm.nR; ++k) {
+        const float x = d - p.radial_shifts[k];
+        v += gr[k] * __expf(-p.radial_eta * x * x) * (dfc - 2.0f * p.radial_eta * x * fc);
       }
-      v += __shfl_xor(v, 8, 64);
-      v += __shfl_xor(v, 4, 64);
-      v += __shfl_xor(v, 2, 64);
-      v += __shfl_xor(v, 1, 64);
-      if (k == 0 && e < nr) {
-        const float vx = rr.vx[e], vy = rr.vy[e], vz = rr.vz[e];
-        const float d2 = vx * vx + vy * vy + vz * vz;
-        if (d2 > 1.0e-5f) {
-          const float s = v / rr.d[e];
-          gx[e] += s * vx;
-          gy[e] += s * vy;
-          gz[e] += s * vz;
-        }
-      }
+      const float sc = d2 > 1.0e-5f ? 0.25f * v / d : 0.0f;
+      gs[e] = make_float4(sc * vx, sc * vy, sc * vz, 0.0f);
     }
   }
-
-  // angular rows
-  const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
-  for (int a = lane; a < na; a += 64) {
-    const int e = arow[a];
-    const float d = rr.d[e];
-    const float inv = 1.0f / fmaxf(d, 1.0e-5f);
-    ax[a] = rr.vx[e] * inv;
-    ay[a] = rr.vy[e] * inv;
-    az[a] = rr.vz[e] * inv;
-    ad[a] = d;
-    afc[a] = cos_switch(d, p.cutoff_ang);
-    adfc[a] = cos_switch_grad(d, p.cutoff_ang);
-    const float d2 = rr.vx[e] * rr.vx[e] + rr.vy[e] * rr.vy[e] + rr.vz[e] * rr.vz[e];
-    aok[a] = d2 > 1.0e-5f ? 1.0f : 0.0f;
-    as[a] = rr.s[e];
-    ae[a] = e;
+  wave_sync();
+  {
+    const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
+    for (int a = lane; a < na; a += 64) {
+      const int e = arow[a];
+      const float4 v = nbv[e];
+      const float inv = 1.0f / fmaxf(v.w, 1.0e-5f);
+      av[a] = make_float4(v.x * inv, v.y * inv, v.z * inv, v.w);
+      const bool ok = v.x * v.x + v.y * v.y + v.z * v.z > 1.0e-5f;
+      af[a] = make_float4(cos_switch(v.w, p.cutoff_ang), ok ? cos_switch_grad(v.w, p.cutoff_ang) : 0.0f,
+                          nbf[e].y, __int_as_float(e | (ok ? 0 : (1 << 30))));
+    }
   }
   wave_sync();
 
@@ -366,84 +378,80 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   for (int t = lane; t < T; t += 64) {
     int a, b;
     tri_pair(t, na, a, b);
-    const float c = ax[a] * ax[b] + ay[a] * ay[b] + az[a] * az[b];
+    const float4 ua = av[a], ub = av[b];
+    const float4 fa = af[a], fb = af[b];
+    const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
     const float x = 0.95f * c;
     const float st = sqrtf(fmaxf(1.0f - x * x, 1.0e-12f));
-    float F[NZ], dF[NZ], R[NA], dR[NA];
+    const float ist = 1.0f / st;
+    float F[NZ], dF[NZ];
 #pragma unroll
     for (int z = 0; z < NZ; ++z) {
       const float bz = 1.0f + x * p.ang_cos[z] + st * p.ang_sin[z];
       if (bz > 0.0f) {
-        const float fz = exp2f(p.zeta * __log2f(bz));
-        F[z] = fz;
+        const float lb = __log2f(bz);
+        F[z] = exp2f(p.zeta * lb);
         // d/dc (1 + cos(theta - theta_z))^zeta with theta = arccos(0.95 c)
-        dF[z] = 0.95f * p.zeta * (fz / bz) * (p.ang_cos[z] - x * p.ang_sin[z] / st);
+        dF[z] = 0.95f * p.zeta * exp2f((p.zeta - 1.0f) * lb) * (p.ang_cos[z] - x * p.ang_sin[z] * ist);
       } else {
         F[z] = 0.0f;
         dF[z] = 0.0f;
       }
     }
-    const float m = 0.5f * (ad[a] + ad[b]);
-#pragma unroll
-    for (int d = 0; d < NA; ++d) {
-      const float y = m - p.ang_shifts[d];
-      R[d] = __expf(-p.angular_eta * y * y);
-      dR[d] = -2.0f * p.angular_eta * y * R[d];
-    }
-    const float* gp = Gang + pair_index(as[a], as[b], dm.S) * (NA * NZ);
+    const float m = 0.5f * (ua.w + ub.w);
+    const float* gp = Gang + pair_index(__float_as_int(fa.z), __float_as_int(fb.z), dm.S) * (NA * NZ);
     float S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
 #pragma unroll
     for (int d = 0; d < NA; ++d) {
+      const float y = m - p.ang_shifts[d];
+      const float R = __expf(-p.angular_eta * y * y);
+      float gz[NZ];
+#pragma unroll
+      for (int q = 0; q < NZ; q += 4) *(float4*)(gz + q) = *(const float4*)(gp + d * NZ + q);
       float gf = 0.0f, gdf = 0.0f;
 #pragma unroll
       for (int z = 0; z < NZ; ++z) {
-        const float gv = gp[d * NZ + z];
-        gf += gv * F[z];
-        gdf += gv * dF[z];
+        gf += gz[z] * F[z];
+        gdf += gz[z] * dF[z];
       }
-      S0 += gf * R[d];
-      S1 += gf * dR[d];
-      S2 += gdf * R[d];
+      S0 += gf * R;
+      S1 += gf * R * (-2.0f * p.angular_eta * y);
+      S2 += gdf * R;
     }
-    const float sw = pref * afc[a] * afc[b];
-    const float dEda = pref * adfc[a] * afc[b] * S0 + 0.5f * sw * S1;
-    const float dEdb = pref * afc[a] * adfc[b] * S0 + 0.5f * sw * S1;
+    const float sw = pref * fa.x * fb.x;
+    const float dEda = pref * fa.y * fb.x * S0 + 0.5f * sw * S1;
+    const float dEdb = pref * fa.x * fb.y * S0 + 0.5f * sw * S1;
     const float dEdc = sw * S2;
-    const float ia = 1.0f / fmaxf(ad[a], 1.0e-5f), ib = 1.0f / fmaxf(ad[b], 1.0e-5f);
-    const float ra = dEda * aok[a], rb = dEdb * aok[b];
-    const float gax = ra * ax[a] + dEdc * (ax[b] - c * ax[a]) * ia;
-    const float gay = ra * ay[a] + dEdc * (ay[b] - c * ay[a]) * ia;
-    const float gaz = ra * az[a] + dEdc * (az[b] - c * az[a]) * ia;
-    const float gbx = rb * ax[b] + dEdc * (ax[a] - c * ax[b]) * ib;
-    const float gby = rb * ay[b] + dEdc * (ay[a] - c * ay[b]) * ib;
-    const float gbz = rb * az[b] + dEdc * (az[a] - c * az[b]) * ib;
-    const int ea = ae[a], eb = ae[b];
-    atomicAdd(&gx[ea], gax);
-    atomicAdd(&gy[ea], gay);
-    atomicAdd(&gz[ea], gaz);
-    atomicAdd(&gx[eb], gbx);
-    atomicAdd(&gy[eb], gby);
-    atomicAdd(&gz[eb], gbz);
+    const int ea = __float_as_int(fa.w), eb = __float_as_int(fb.w);
+    const float ra = (ea >> 30) ? 0.0f : dEda, rb = (eb >> 30) ? 0.0f : dEdb;
+    const float ia = 1.0f / fmaxf(ua.w, 1.0e-5f), ib = 1.0f / fmaxf(ub.w, 1.0e-5f);
+    float* pa = (float*)(gs + (ea & 0x3fffffff));
+    float* pb = (float*)(gs + (eb & 0x3fffffff));
+    lds_add(pa + 0, ra * ua.x + dEdc * (ub.x - c * ua.x) * ia);
+    lds_add(pa + 1, ra * ua.y + dEdc * (ub.y - c * ua.y) * ia);
+    lds_add(pa + 2, ra * ua.z + dEdc * (ub.z - c * ua.z) * ia);
+    lds_add(pb + 0, rb * ub.x + dEdc * (ua.x - c * ub.x) * ib);
+    lds_add(pb + 1, rb * ub.y + dEdc * (ua.y - c * ub.y) * ib);
+    lds_add(pb + 2, rb * ub.z + dEdc * (ua.z - c * ub.z) * ib);
   }
   wave_sync();
 
   // scatter: neighbours by atomics, centre by a wave reduction; virial
   float sx = 0.f, sy = 0.f, sz = 0.f;
   float v[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  const int32_t* row = nbr + (size_t)i * max_nbr;
   for (int e = lane; e < nr; e += 64) {
-    const float x = gx[e], y = gy[e], z = gz[e];
+    const float4 gg = gs[e];
+    const float4 vv = nbv[e];
     const int j = row[e];
-    atomicAdd(&grad_nbr[3 * j], x);
-    atomicAdd(&grad_nbr[3 * j + 1], y);
-    atomicAdd(&grad_nbr[3 * j + 2], z);
-    sx += x;
-    sy += y;
-    sz += z;
-    const float vx = rr.vx[e], vy = rr.vy[e], vz = rr.vz[e];
-    v[0] += vx * x; v[1] += vx * y; v[2] += vx * z;
-    v[3] += vy * x; v[4] += vy * y; v[5] += vy * z;
-    v[6] += vz * x; v[7] += vz * y; v[8] += vz * z;
+    atomicAdd(&grad_nbr[3 * j], gg.x);
+    atomicAdd(&grad_nbr[3 * j + 1], gg.y);
+    atomicAdd(&grad_nbr[3 * j + 2], gg.z);
+    sx += gg.x;
+    sy += gg.y;
+    sz += gg.z;
+    v[0] += vv.x * gg.x; v[1] += vv.x * gg.y; v[2] += vv.x * gg.z;
+    v[3] += vv.y * gg.x; v[4] += vv.y * gg.y; v[5] += vv.y * gg.z;
+    v[6] += vv.z * gg.x; v[7] += vv.z * gg.y; v[8] += vv.z * gg.z;
   }
   sx = fnl_wave_sum(sx);
   sy = fnl_wave_sum(sy);
@@ -459,12 +467,170 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   }
 }
 
+// Tangent (forward-mode JVP) of the AEV along a displacement field r:
+//   dA_i = sum_e dA_i/dvec_e . (r_j - r_i)
+// (the same edge vectors and terms as the forward; used for weight gradients
+// through forces, where F . r = -dE/dx . r = -sum_i dE/dA_i . dA_i).
+// Per triplet the tangent of K fc_a fc_b F_z(c) R_d(m) is
+//   F_z (dsw R_d + sw R_d' dm) + F_z' (sw R_d dc) = F_z P_d + F_z' Q_d.
+__host__ __device__ inline int tan_lds_words(int max_nbr, int max_nbr_ang, int aev_len) {
+  return 10 * max_nbr + 12 * max_nbr_ang + ((aev_len + 3) & ~3) + 64 * 32;
+}
+
+template <int NA, int NZ>
+__global__ void __launch_bounds__(64 * kWaves)
+k_aev_tangent(const float* __restrict__ coords, const float* __restrict__ disp,
+              const int32_t* __restrict__ species_idx, const int32_t* __restrict__ batch_index, int n_atoms,
+              int pbc_mode, const float* __restrict__ cells, const float* __restrict__ rec,
+              const int32_t* __restrict__ nbr, const int32_t* __restrict__ n_nbr, int max_nbr,
+              const int32_t* __restrict__ nbr_ang, const int32_t* __restrict__ n_nbr_ang, int max_nbr_ang,
+              fnl_aev_params p, AevDims dm, float* __restrict__ out_rows, int stride) {
+  static_assert(NA * NZ == 32, "angular channel layout");
+  extern __shared__ float4 lds4[];
+  const int w = threadIdx.x >> 6;
+  const int lane = fnl_lane();
+  const int i = blockIdx.x * kWaves + w;
+  if (i >= n_atoms) return;
+  const int words = tan_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
+  float* base = (float*)lds4 + (size_t)w * words;
+  float4* nbv = (float4*)base;                        // vx vy vz d
+  float4* nbt = nbv + max_nbr;                        // dvec (x y z), dd
+  float2* nbf = (float2*)(nbt + max_nbr);             // fc, species
+  float4* av = (float4*)((float*)nbf + 2 * max_nbr);  // u, d
+  float4* at = av + max_nbr_ang;                      // du, dd
+  float4* af = at + max_nbr_ang;                      // fc_a, fc_a', species
+  float* acc = (float*)(af + max_nbr_ang);
+  float* trip = acc + ((dm.aev_len + 3) & ~3);
+
+  float* out = out_rows + (size_t)i * stride;
+  const int si = species_idx[i];
+  if (si < 0) {
+    for (int c = lane; c < stride; c += 64) out[c] = 0.0f;
+    return;
+  }
+  const bool pbc = pbc_mode == FNL_PBC_MINIMUM_IMAGE;
+  FnlCell cl;
+  if (pbc) fnl_load_cell(cells, rec, batch_index[i], cl);
+  const int nr = min(n_nbr[i], max_nbr);
+  const int na = min(n_nbr_ang[i], max_nbr_ang);
+  for (int c = lane; c < dm.aev_len; c += 64) acc[c] = 0.0f;
+  {
+    const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
+    const float ri0 = disp[3 * i], ri1 = disp[3 * i + 1], ri2 = disp[3 * i + 2];
+    const int32_t* row = nbr + (size_t)i * max_nbr;
+    for (int e = lane; e < nr; e += 64) {
+      const int j = row[e];
+      float vx = coords[3 * j] - xi, vy = coords[3 * j + 1] - yi, vz = coords[3 * j + 2] - zi;
+      if (pbc) fnl_min_image(vx, vy, vz, cl);
+      const float d2 = vx * vx + vy * vy + vz * vz;
+      const float d = sqrtf(fmaxf(d2, 1.0e-5f));
+      const float tx = disp[3 * j] - ri0, ty = disp[3 * j + 1] - ri1, tz = disp[3 * j + 2] - ri2;
+      const float dd = d2 > 1.0e-5f ? (vx * tx + vy * ty + vz * tz) / d : 0.0f;
+      nbv[e] = make_float4(vx, vy, vz, d);
+      nbt[e] = make_float4(tx, ty, tz, dd);
+      nbf[e] = make_float2(cos_switch(d, p.cutoff), __int_as_float(species_idx[j]));
+    }
+  }
+  wave_sync();
+  // radial: d/dd (0.25 fc exp(-eta x^2)) * dd
+  {
+    const int q = lane >> 4, k = lane & 15;
+    if (k < dm.nR) {
+      const float shift = p.radial_shifts[k];
+      for (int e = q; e < nr; e += 4) {
+        const float4 v = nbv[e];
+        const float dd = nbt[e].w;
+        const float2 f = nbf[e];
+        const float x = v.w - shift;
+        const float ex = __expf(-p.radial_eta * x * x);
+        const float dfc = cos_switch_grad(v.w, p.cutoff);
+        lds_add(acc + __float_as_int(f.y) * dm.nR + k, 0.25f * ex * (dfc - 2.0f * p.radial_eta * x * f.x) * dd);
+      }
+    }
+  }
+  // angular rows: u, du = (dvec - u (u . dvec)) / d, dd, fc, fc'
+  {
+    const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
+    for (int a = lane; a < na; a += 64) {
+      const int e = arow[a];
+      const float4 v = nbv[e];
+      const float4 t = nbt[e];
+      const float inv = 1.0f / fmaxf(v.w, 1.0e-5f);
+      const float ux = v.x * inv, uy = v.y * inv, uz = v.z * inv;
+      const float ut = ux * t.x + uy * t.y + uz * t.z;
+      av[a] = make_float4(ux, uy, uz, v.w);
+      at[a] = make_float4((t.x - ux * ut) * inv, (t.y - uy * ut) * inv, (t.z - uz * ut) * inv, t.w);
+      af[a] = make_float4(cos_switch(v.w, p.cutoff_ang), cos_switch_grad(v.w, p.cutoff_ang), nbf[e].y, 0.0f);
+    }
+  }
+  wave_sync();
+  const float pref = 2.0f * exp2f(-p.zeta);
+  const int T = na * (na - 1) / 2;
+  float* aacc = acc + dm.rad_len;
+  constexpr int G = 64 / NZ;
+  const int g = lane / NZ, z = lane % NZ;
+  for (int base_t = 0; base_t < T; base_t += 64) {
+    const int t = base_t + lane;
+    float* tr = trip + lane * 32;
+    if (t < T) {
+      int a, b;
+      tri_pair(t, na, a, b);
+      const float4 ua = av[a], ub = av[b], ta = at[a], tb = at[b];
+      const float4 fa = af[a], fb = af[b];
+      const int sa = __float_as_int(fa.z), sb = __float_as_int(fb.z);
+      const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
+      const float dc = ta.x * ub.x + ta.y * ub.y + ta.z * ub.z + ua.x * tb.x + ua.y * tb.y + ua.z * tb.z;
+      const float x = 0.95f * c;
+      const float st = sqrtf(fmaxf(1.0f - x * x, 1.0e-12f));
+      const float ist = 1.0f / st;
+      const float sw = pref * fa.x * fb.x;
+      const float dsw = pref * (fa.y * fb.x * ta.w + fa.x * fb.y * tb.w);
+      const float m = 0.5f * (ua.w + ub.w);
+      const float dm_ = 0.5f * (ta.w + tb.w);
+#pragma unroll
+      for (int d = 0; d < NA; ++d) {
+        const float y = m - p.ang_shifts[d];
+        const float R = __expf(-p.angular_eta * y * y);
+        tr[d] = dsw * R + sw * (-2.0f * p.angular_eta * y * R) * dm_;  // P_d
+        tr[NA + d] = sw * R * dc;                                       // Q_d
+      }
+#pragma unroll
+      for (int zz = 0; zz < NZ; ++zz) {
+        const float bz = 1.0f + x * p.ang_cos[zz] + st * p.ang_sin[zz];
+        float F = 0.0f, dF = 0.0f;
+        if (bz > 0.0f) {
+          const float lb = __log2f(bz);
+          F = exp2f(p.zeta * lb);
+          dF = 0.95f * p.zeta * exp2f((p.zeta - 1.0f) * lb) * (p.ang_cos[zz] - x * p.ang_sin[zz] * ist);
+        }
+        tr[2 * NA + zz] = F;
+        tr[2 * NA + NZ + zz] = dF;
+      }
+      tr[31] = __int_as_float(pair_index(sa, sb, dm.S) * 32);
+    } else {
+      tr[31] = __int_as_float(-1);
+    }
+    wave_sync();
+    const int nt = min(64, T - base_t);
+    for (int tt = g; tt < nt; tt += G) {
+      const float* q = trip + tt * 32;
+      const int po = __float_as_int(q[31]);
+      const float F = q[2 * NA + z], dF = q[2 * NA + NZ + z];
+#pragma unroll
+      for (int d = 0; d < NA; ++d) lds_add(aacc + po + d * NZ + z, F * q[d] + dF * q[NA + d]);
+    }
+    wave_sync();
+  }
+  for (int c = lane; c < stride; c += 64) out[c] = c < dm.aev_len ? acc[c] : 0.0f;
+}
+
 bool check_params(const fnl_aev_params* p, AevDims* dm) {
   if (!p) return false;
   if (p->n_species < 1 || p->n_species > FNL_MAX_SPECIES) return false;
   if (p->n_radial < 1 || p->n_radial > 16) return false;
   if (p->n_ang_dist < 1 || p->n_ang_sect < 1 || p->n_ang_dist * p->n_ang_sect > 32) return false;
   if (!(p->cutoff > 0.0f) || !(p->cutoff_ang > 0.0f) || p->cutoff_ang > p->cutoff) return false;
+  if ((p->n_species * p->n_radial) % 4 != 0) return false;  // angular block 16-byte aligned
   dm->S = p->n_species;
   dm->P = p->n_species * (p->n_species + 1) / 2;
   dm->nR = p->n_radial;
@@ -479,13 +645,19 @@ int launch_fwd(const float* coords, const int32_t* species_idx, const int32_t* b
                const int32_t* n_nbr, int max_nbr, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
                int max_nbr_ang, const fnl_aev_params& p, const AevDims& dm, void* aev, int stride,
                int out_dtype, hipStream_t s) {
-  const size_t lds = (size_t)kWaves * 4 * fwd_lds_words(max_nbr, max_nbr_ang, dm.S, dm.nR, dm.P, NA, NZ);
+  const size_t lds = (size_t)kWaves * 4 * fwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
   if (lds > 160 * 1024) return FNL_ERR_UNSUPPORTED;
   dim3 grid(fnl_blocks(n_atoms, kWaves)), block(64 * kWaves);
   const void* kf = out_dtype == FNL_F32 ? reinterpret_cast<const void*>(&k_aev_fwd<NA, NZ, float>)
                                         : reinterpret_cast<const void*>(&k_aev_fwd<NA, NZ, hip_bfloat16>);
-  if (hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-    return FNL_ERR_HIP;
+  // raise the dynamic-LDS limit once per size (not a stream operation: kept out of graph captures
+  // that follow a first eager call)
+  static size_t lds_set[2] = {0, 0};
+  if (lds > lds_set[out_dtype == FNL_F32 ? 0 : 1]) {
+    if (hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+      return FNL_ERR_HIP;
+    lds_set[out_dtype == FNL_F32 ? 0 : 1] = lds;
+  }
   if (out_dtype == FNL_F32) {
     hipLaunchKernelGGL((k_aev_fwd<NA, NZ, float>), grid, block, lds, s, coords, species_idx, batch_index,
                        n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang,
@@ -507,9 +679,13 @@ int launch_bwd(const float* coords, const int32_t* species_idx, const int32_t* b
                float* grad_self, float* grad_nbr, float* virial_atom, hipStream_t s) {
   const size_t lds = (size_t)kWaves * 4 * bwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
   if (lds > 160 * 1024) return FNL_ERR_UNSUPPORTED;
-  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_aev_bwd<NA, NZ>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-    return FNL_ERR_HIP;
+  static size_t lds_set = 0;
+  if (lds > lds_set) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_aev_bwd<NA, NZ>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+      return FNL_ERR_HIP;
+    lds_set = lds;
+  }
   hipLaunchKernelGGL((k_aev_bwd<NA, NZ>), dim3(fnl_blocks(n_atoms, kWaves)), dim3(64 * kWaves), lds, s,
                      coords, species_idx, batch_index, n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr,
                      nbr_ang, n_nbr_ang, max_nbr_ang, p, dm, g, g_stride, grad_self, grad_nbr, virial_atom);
@@ -570,7 +746,7 @@ extern "C" int fnl_aev_backward(const float* coords, const int32_t* species_idx,
   if (!common_args_ok(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells, nbr,
                       n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang))
     return FNL_ERR_INVALID;
-  if (g_stride < dm.aev_len) return FNL_ERR_INVALID;
+  if (g_stride < ((dm.aev_len + 3) & ~3) || g_stride % 4 != 0) return FNL_ERR_INVALID;
   if (n_atoms == 0) return FNL_OK;
   if (!g || !grad_self || !grad_nbr || !virial_atom) return FNL_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
@@ -584,4 +760,46 @@ extern "C" int fnl_aev_backward(const float* coords, const int32_t* species_idx,
                             nbr, n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm, g, g_stride,
                             grad_self, grad_nbr, virial_atom, s);
   return FNL_ERR_UNSUPPORTED;
+}
+
+extern "C" int fnl_aev_tangent(const float* coords, const float* disp, const int32_t* species_idx,
+                               const int32_t* batch_index, int32_t n_atoms, int32_t pbc_mode, const float* cells,
+                               const float* reciprocal_cells, const int32_t* nbr, const int32_t* n_nbr,
+                               int32_t max_nbr, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
+                               int32_t max_nbr_ang, const fnl_aev_params* params, float* daev, int32_t daev_stride,
+                               fnl_stream_t stream) {
+  AevDims dm;
+  if (!check_params(params, &dm)) return FNL_ERR_INVALID;
+  if (!common_args_ok(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells, nbr,
+                      n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang))
+    return FNL_ERR_INVALID;
+  if (daev_stride < dm.aev_len) return FNL_ERR_INVALID;
+  if (n_atoms == 0) return FNL_OK;
+  if (!disp || !daev) return FNL_ERR_INVALID;
+  const size_t lds = (size_t)kWaves * 4 * tan_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
+  if (lds > 160 * 1024) return FNL_ERR_UNSUPPORTED;
+  hipStream_t s = (hipStream_t)stream;
+  const int na = params->n_ang_dist, nz = params->n_ang_sect;
+  const void* kf = nullptr;
+  if (na == 4 && nz == 8) kf = reinterpret_cast<const void*>(&k_aev_tangent<4, 8>);
+  else if (na == 8 && nz == 4) kf = reinterpret_cast<const void*>(&k_aev_tangent<8, 4>);
+  else return FNL_ERR_UNSUPPORTED;
+  static size_t lds_set[2] = {0, 0};
+  const int slot = na == 4 ? 0 : 1;
+  if (lds > lds_set[slot]) {
+    if (hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+      return FNL_ERR_HIP;
+    lds_set[slot] = lds;
+  }
+  dim3 grid(fnl_blocks(n_atoms, kWaves)), block(64 * kWaves);
+  if (na == 4)
+    hipLaunchKernelGGL((k_aev_tangent<4, 8>), grid, block, lds, s, coords, disp, species_idx, batch_index, n_atoms,
+                       pbc_mode, cells, reciprocal_cells, nbr, n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang,
+                       *params, dm, daev, daev_stride);
+  else
+    hipLaunchKernelGGL((k_aev_tangent<8, 4>), grid, block, lds, s, coords, disp, species_idx, batch_index, n_atoms,
+                       pbc_mode, cells, reciprocal_cells, nbr, n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang,
+                       *params, dm, daev, daev_stride);
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
 }

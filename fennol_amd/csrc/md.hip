@@ -89,9 +89,10 @@ __global__ void k_baoab(double* __restrict__ x, double* __restrict__ v, float* _
                         const float* __restrict__ gs, const float* __restrict__ gn, float force_scale,
                         const double* __restrict__ dtm, double half_dt, double a1,
                         const double* __restrict__ a2, const double* __restrict__ noise, uint64_t seed,
-                        uint64_t step, int n_atoms) {
+                        uint64_t step_arg, const uint64_t* __restrict__ step_dev, int n_atoms) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_atoms) return;
+  const uint64_t step = step_dev ? *step_dev : step_arg;
   double xi[3];
   if (a2 && !noise) fnl_normals3(seed, step, 0u, (uint32_t)i, xi);
   const double m = dtm[i];
@@ -109,6 +110,39 @@ __global__ void k_baoab(double* __restrict__ x, double* __restrict__ v, float* _
     x[3 * i + k] = xx;
     x32[3 * i + k] = (float)xx;
   }
+}
+
+// Batched form: every system b carries its own step number and stream key, so
+// the noise of an atom depends only on (key_b, step_b, local atom index) and
+// not on where the system sits in the batch.
+__global__ void k_baoab_batched(double* __restrict__ x, double* __restrict__ v, float* __restrict__ x32,
+                                const float* __restrict__ gs, const float* __restrict__ gn, float force_scale,
+                                const double* __restrict__ dtm, double half_dt, double a1,
+                                const double* __restrict__ a2, const int32_t* __restrict__ batch_index,
+                                const int32_t* __restrict__ sys_offsets, const uint64_t* __restrict__ sys_step,
+                                const uint32_t* __restrict__ sys_key, uint64_t seed, int n_atoms) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_atoms) return;
+  const double m = dtm[i];
+  if (m == 0.0) return;  // atoms of systems that are not integrated this step
+  const int b = batch_index[i];
+  double xi[3];
+  fnl_normals3(seed, sys_step[b], sys_key[b], (uint32_t)(i - sys_offsets[b]), xi);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double f = -((double)gs[3 * i + k] + (double)gn[3 * i + k]) * (double)force_scale;
+    double vv = v[3 * i + k] + m * f;
+    double xx = x[3 * i + k] + half_dt * vv;
+    vv = a1 * vv + a2[i] * xi[k];
+    xx = xx + half_dt * vv;
+    v[3 * i + k] = vv;
+    x[3 * i + k] = xx;
+    x32[3 * i + k] = (float)xx;
+  }
+}
+
+__global__ void k_counter_add(uint64_t* __restrict__ p, uint64_t v) {
+  if (threadIdx.x == 0) *p += v;
 }
 
 __global__ void k_normals(double* __restrict__ out, int n_atoms, uint64_t seed, uint64_t step, uint32_t sid) {
@@ -159,13 +193,37 @@ extern "C" int fnl_forces(const float* grad_self, const float* grad_nbr, int32_t
 
 extern "C" int fnl_baoab_step(double* x, double* v, float* x32, const float* grad_self, const float* grad_nbr,
                               float force_scale, const double* dtm, double half_dt, double a1, const double* a2,
-                              const double* noise, uint64_t seed, uint64_t step, int32_t n_atoms,
-                              fnl_stream_t stream) {
+                              const double* noise, uint64_t seed, uint64_t step, const uint64_t* step_dev,
+                              int32_t n_atoms, fnl_stream_t stream) {
   if (n_atoms < 0) return FNL_ERR_INVALID;
   if (n_atoms == 0) return FNL_OK;
   if (!x || !v || !x32 || !grad_self || !grad_nbr || !dtm) return FNL_ERR_INVALID;
   hipLaunchKernelGGL(k_baoab, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, (hipStream_t)stream, x, v, x32,
-                     grad_self, grad_nbr, force_scale, dtm, half_dt, a1, a2, noise, seed, step, n_atoms);
+                     grad_self, grad_nbr, force_scale, dtm, half_dt, a1, a2, noise, seed, step, step_dev, n_atoms);
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
+}
+
+extern "C" int fnl_baoab_step_batched(double* x, double* v, float* x32, const float* grad_self,
+                                      const float* grad_nbr, float force_scale, const double* dtm, double half_dt,
+                                      double a1, const double* a2, const int32_t* batch_index,
+                                      const int32_t* sys_offsets, const uint64_t* sys_step, const uint32_t* sys_key,
+                                      uint64_t seed, int32_t n_atoms, fnl_stream_t stream) {
+  if (n_atoms < 0) return FNL_ERR_INVALID;
+  if (n_atoms == 0) return FNL_OK;
+  if (!x || !v || !x32 || !grad_self || !grad_nbr || !dtm || !a2 || !batch_index || !sys_offsets || !sys_step ||
+      !sys_key)
+    return FNL_ERR_INVALID;
+  hipLaunchKernelGGL(k_baoab_batched, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, (hipStream_t)stream, x, v, x32,
+                     grad_self, grad_nbr, force_scale, dtm, half_dt, a1, a2, batch_index, sys_offsets, sys_step,
+                     sys_key, seed, n_atoms);
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
+}
+
+extern "C" int fnl_counter_add(uint64_t* counter, uint64_t value, fnl_stream_t stream) {
+  if (!counter) return FNL_ERR_INVALID;
+  hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, (hipStream_t)stream, counter, value);
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }

@@ -66,6 +66,9 @@ class LangevinMD:
             self.t_a2 = None
         else:
             raise NotImplementedError(f"thermostat {thermostat}")
+        # step counter on the device: the O-step noise of a captured graph advances on replay
+        self.step_dev = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.graph = None
         self.x = torch.tensor(np.asarray(system["coordinates"], np.float64), device=dev).contiguous()
         self.x32 = self.x.float().contiguous()
         self.v = torch.empty((self.n, 3), dtype=torch.float64, device=dev)
@@ -97,8 +100,9 @@ class LangevinMD:
         rc = self.lib.fnl_baoab_step(
             self.x.data_ptr(), self.v.data_ptr(), self.x32.data_ptr(), sb.grad_self.data_ptr(),
             sb.grad_nbr.data_ptr(), self.force_scale, self.t_dtm.data_ptr(), 0.5 * self.dt, self.a1,
-            _abi.ptr(self.t_a2), None, self.seed, self.istep, self.n, s)
+            _abi.ptr(self.t_a2), None, self.seed, 0, self.step_dev.data_ptr(), self.n, s)
         _abi.check(rc, "fnl_baoab_step")
+        _abi.check(self.lib.fnl_counter_add(self.step_dev.data_ptr(), 1, s), "fnl_counter_add")
 
     def step(self, check: bool | None = None, stream=None) -> None:
         """integrate -> rebuild neighbour lists -> energy and forces at the new positions."""
@@ -111,6 +115,32 @@ class LangevinMD:
     def run(self, nsteps: int) -> None:
         for _ in range(nsteps):
             self.step()
+
+    # ---------------------------------------------------------- graph replay
+    def capture(self, steps_per_graph: int = 1) -> None:
+        """Record `steps_per_graph` MD steps in a HIP graph (no host checks inside;
+        capacities must have headroom: construct with check_every=0)."""
+        if self.check_every != 0:
+            raise ValueError("graph capture needs check_every=0 (no host reads inside a step)")
+        timer, self.pot.timer = self.pot.timer, None
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):  # warm the allocator and kernels outside the capture
+            self.step(check=False)
+        torch.cuda.current_stream().wait_stream(side)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(steps_per_graph):
+                self.step(check=False)
+        self.istep -= steps_per_graph  # capture does not execute the steps
+        self.graph = g
+        self.steps_per_graph = steps_per_graph
+        self.pot.timer = timer
+
+    def replay(self, n_graphs: int = 1) -> None:
+        for _ in range(n_graphs):
+            self.graph.replay()
+            self.istep += self.steps_per_graph
 
     # -------------------------------------------------------------- observables
     def energies(self) -> np.ndarray:

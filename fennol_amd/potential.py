@@ -311,15 +311,127 @@ class ANIPotential:
         sb.partials = torch.zeros((max(sb.n_seg, 1), _abi.FNL_REDUCE_VALUES), dtype=torch.float64, device=dev)
         return sb
 
+    # ------------------------------------------------------- capacity pools
+    def allocate_pool(self, max_atoms: int, max_sys: int, max_nbr: int = 208, max_nbr_ang: int = 72,
+                      cells_per_atom: int = 4) -> SystemBuffers:
+        """Buffers sized for up to max_atoms atoms in up to max_sys open-boundary systems.
+
+        The layout is set (and changed) by :meth:`set_pool_layout`; the kernels only touch the
+        first n_atoms rows, so one pool serves batches of changing composition (serving)."""
+        dev = self.device
+        n_cells = cells_per_atom * max_atoms + max_sys
+        sb = SystemBuffers(
+            n_atoms=0, n_sys=0, pbc=_abi.FNL_PBC_NONE, species=np.zeros(0, np.int32),
+            natoms=np.zeros(0, np.int32),
+            sys_offsets=torch.zeros(max_sys + 1, dtype=torch.int32, device=dev),
+            batch_index=torch.zeros(max_atoms, dtype=torch.int32, device=dev),
+            species_idx=torch.full((max_atoms,), -1, dtype=torch.int32, device=dev),
+            cells=None, rcells=None, grid=torch.zeros((max_sys, 4), dtype=torch.int32, device=dev),
+            n_cells=n_cells, max_nbr=max_nbr, max_nbr_ang=max_nbr_ang)
+        sb.extra["max_atoms"] = max_atoms
+        sb.extra["max_sys"] = max_sys
+        sb.extra["max_cells"] = n_cells
+        sb.n_atoms = max_atoms  # size the neighbour and workspace buffers for the capacity
+        sb.n_sys = max_sys
+        self._alloc_nblist(sb)
+        M = self.nets.tile_rows
+        max_tiles = max_atoms // M + max_sys * len(self.nets.species) + 1
+        sb.perm = torch.zeros(max_atoms, dtype=torch.int32, device=dev)
+        sb.tile_start = torch.zeros(max_tiles, dtype=torch.int32, device=dev)
+        sb.tile_rows = torch.zeros(max_tiles, dtype=torch.int32, device=dev)
+        sb.tile_net = torch.zeros(max_tiles, dtype=torch.int32, device=dev)
+        kp = self.config.k_pad
+        adt = torch.float32 if self.precision == "fp32" else torch.bfloat16
+        sb.aev = torch.zeros((max_atoms, kp), dtype=adt, device=dev)
+        sb.g_aev = torch.zeros((max_atoms, kp), dtype=torch.float32, device=dev)
+        sb.e_atom = torch.zeros(max_atoms, dtype=torch.float32, device=dev)
+        sb.grad_self = torch.zeros((max_atoms, 3), dtype=torch.float32, device=dev)
+        sb.grad_nbr = torch.zeros((max_atoms, 3), dtype=torch.float32, device=dev)
+        sb.vir_atom = torch.zeros((max_atoms, 9), dtype=torch.float32, device=dev)
+        sb.energy = torch.zeros(max_sys, dtype=torch.float64, device=dev)
+        sb.virial = torch.zeros((max_sys, 9), dtype=torch.float64, device=dev)
+        max_seg = max_atoms // SEG_ATOMS + max_sys + 1
+        sb.seg_offsets = torch.zeros(max_sys + 1, dtype=torch.int32, device=dev)
+        sb.partials = torch.zeros((max_seg, _abi.FNL_REDUCE_VALUES), dtype=torch.float64, device=dev)
+        sb.n_atoms = 0
+        sb.n_sys = 0
+        return sb
+
+    def pool_layout_arrays(self, species: np.ndarray, natoms: np.ndarray) -> dict:
+        """Host arrays of a pool layout (systems packed from atom 0, no padding between them)."""
+        species = np.asarray(species, np.int32)
+        natoms = np.asarray(natoms, np.int32)
+        B = natoms.shape[0]
+        N = int(natoms.sum())
+        offs = np.concatenate([[0], np.cumsum(natoms)]).astype(np.int32)
+        batch_index = np.repeat(np.arange(B, dtype=np.int32), natoms)
+        conv = self.config.species_map()
+        sidx = np.where(species > 0, conv[np.clip(species, 0, conv.shape[0] - 1)], -1).astype(np.int32)
+        k = np.maximum(1, np.ceil(np.maximum(natoms, 1) ** (1.0 / 3.0) - 1e-9)).astype(np.int32)
+        ncell = k.astype(np.int64) ** 3
+        grid = np.stack([k, k, k, np.concatenate([[0], np.cumsum(ncell)[:-1]]).astype(np.int32)], 1)
+        n_cells = int(ncell.sum())
+        net = np.full(N, -1, np.int64)
+        for z, n in self.net_of_z.items():
+            net[species == z] = n
+        if np.any((net < 0) & (species > 0)):
+            missing = sorted({PERIODIC_TABLE[z] for z in np.unique(species[(net < 0) & (species > 0)])})
+            raise ValueError(f"Species {missing} not found in networks")
+        order = np.argsort(net, kind="stable")
+        order = order[net[order] >= 0]
+        counts = np.bincount(net[net >= 0], minlength=len(self.nets.species))
+        M = self.nets.tile_rows
+        ts, tr, tn = [], [], []
+        base = 0
+        for n, c in enumerate(counts):
+            for s0 in range(0, int(c), M):
+                ts.append(base + s0)
+                tr.append(min(M, int(c) - s0))
+                tn.append(n)
+            base += int(c)
+        nseg = (natoms.astype(np.int64) + SEG_ATOMS - 1) // SEG_ATOMS
+        segoff = np.concatenate([[0], np.cumsum(nseg)]).astype(np.int32)
+        return dict(n_atoms=N, n_sys=B, sys_offsets=offs, batch_index=batch_index, species_idx=sidx, grid=grid,
+                    n_cells=max(n_cells, 1), perm=order.astype(np.int32), tile_start=np.array(ts, np.int32),
+                    tile_rows=np.array(tr, np.int32), tile_net=np.array(tn, np.int32), seg_offsets=segoff,
+                    n_seg=int(segoff[-1]), species=species, natoms=natoms)
+
+    def set_pool_layout(self, sb: SystemBuffers, lay: dict) -> None:
+        """Copy a layout (from pool_layout_arrays) into the pool's device tables."""
+        if lay["n_atoms"] > sb.extra["max_atoms"] or lay["n_sys"] > sb.extra["max_sys"]:
+            raise ValueError("layout exceeds the pool capacity")
+        if lay["n_cells"] > sb.extra["max_cells"] or len(lay["tile_start"]) > sb.tile_start.numel():
+            raise ValueError("layout exceeds the pool's cell or tile capacity")
+        N, B, T = lay["n_atoms"], lay["n_sys"], len(lay["tile_start"])
+
+        def put(dst, src):
+            if src.size:
+                dst[: src.shape[0]].copy_(torch.from_numpy(np.ascontiguousarray(src)), non_blocking=False)
+
+        put(sb.sys_offsets, lay["sys_offsets"])
+        put(sb.batch_index, lay["batch_index"])
+        put(sb.species_idx, lay["species_idx"])
+        put(sb.grid, lay["grid"])
+        put(sb.perm, lay["perm"])
+        put(sb.tile_start, lay["tile_start"])
+        put(sb.tile_rows, lay["tile_rows"])
+        put(sb.tile_net, lay["tile_net"])
+        put(sb.seg_offsets, lay["seg_offsets"])
+        sb.n_atoms, sb.n_sys, sb.n_tiles, sb.n_seg = N, B, T, lay["n_seg"]
+        sb.n_cells = lay["n_cells"]
+        sb.species, sb.natoms = lay["species"], lay["natoms"]
+
     def _alloc_nblist(self, sb: SystemBuffers) -> None:
         dev = self.device
-        N = sb.n_atoms
+        N = sb.extra.get("max_atoms", sb.n_atoms)
+        n_sys = sb.extra.get("max_sys", sb.n_sys)
+        n_cells = sb.extra.get("max_cells", sb.n_cells)
         sb.nbr = torch.zeros((max(N, 1), sb.max_nbr), dtype=torch.int32, device=dev)
         sb.n_nbr = torch.zeros(max(N, 1), dtype=torch.int32, device=dev)
         sb.nbr_ang = torch.zeros((max(N, 1), sb.max_nbr_ang), dtype=torch.int32, device=dev)
         sb.n_nbr_ang = torch.zeros(max(N, 1), dtype=torch.int32, device=dev)
         sb.status = torch.zeros(_abi.FNL_STATUS_WORDS, dtype=torch.int32, device=dev)
-        wsb = self.lib.fnl_nblist_workspace(N, sb.n_sys, sb.n_cells)
+        wsb = self.lib.fnl_nblist_workspace(N, max(n_sys, 1), max(n_cells, 1))
         sb.workspace = torch.empty(max(int(wsb), 256), dtype=torch.uint8, device=dev)
 
     def _setup_tiles(self, sb: SystemBuffers) -> None:

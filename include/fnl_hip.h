@@ -163,6 +163,21 @@ int fnl_aev_backward(const float* coords, const int32_t* species_idx,
                      float* grad_self, float* grad_nbr, float* virial_atom,
                      fnl_stream_t stream);
 
+/* Tangent (forward-mode derivative) of the AEV along a displacement field
+ * disp (float32[N][3]):  daev_i = sum_e dAEV_i/dvec_e . (disp_j - disp_i).
+ * With disp = F - F_ref it gives the input tangent whose contraction with
+ * dE/dAEV is F . disp -- the term a force loss differentiates with respect to
+ * the weights (the reference obtains it by jax.grad through jax.grad,
+ * training/utils.py:508 over fennix.py:260-272).  Rows float32, stride
+ * daev_stride, columns past the AEV length written as zero. */
+int fnl_aev_tangent(const float* coords, const float* disp, const int32_t* species_idx,
+                    const int32_t* batch_index, int32_t n_atoms, int32_t pbc_mode,
+                    const float* cells, const float* reciprocal_cells,
+                    const int32_t* nbr, const int32_t* n_nbr, int32_t max_nbr,
+                    const int32_t* nbr_ang, const int32_t* n_nbr_ang, int32_t max_nbr_ang,
+                    const fnl_aev_params* params, float* daev, int32_t daev_stride,
+                    fnl_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * Species MLPs as grouped GEMMs on MFMA (SpeciesIndexNet models/misc/nets.py:
  * 301-426 over FullyConnectedNet :13-102, activation CELU(alpha)).
@@ -237,14 +252,31 @@ int fnl_forces(const float* grad_self, const float* grad_nbr, int32_t n_atoms,
  *   v += dtm * F;  x += dt/2 v;  v = a1 v + a2 xi;  x += dt/2 v
  * xi ~ N(0,1) from Philox4x32-10 with counter (atom, step_lo, step_hi, 0) and
  * key (seed_lo, seed_hi), Box-Muller on (r0,r1), (r2,r3); or, if noise != NULL,
- * xi = noise (float64[N][3]).  a2 == NULL means no O-step (NVE).
- * Also writes the float32 copy of x the model reads (x32).
+ * xi = noise (float64[N][3]).  a2 == NULL means no O-step (NVE).  If step_dev
+ * is not NULL the step number is read from that device word instead of
+ * `step` (so that a captured graph replays with a fresh counter; advance it
+ * with fnl_counter_add).  Also writes the float32 copy of x the model reads.
  * ---------------------------------------------------------------------- */
 int fnl_baoab_step(double* x, double* v, float* x32,
                    const float* grad_self, const float* grad_nbr, float force_scale,
                    const double* dtm, double half_dt, double a1, const double* a2,
                    const double* noise, uint64_t seed, uint64_t step,
-                   int32_t n_atoms, fnl_stream_t stream);
+                   const uint64_t* step_dev, int32_t n_atoms, fnl_stream_t stream);
+
+/* Batched BAOAB for many independent systems (continuous-batching server):
+ * atoms with dtm == 0 are left untouched (systems not integrated this step);
+ * the O-step noise of atom i in system b uses counter (i - sys_offsets[b],
+ * step_lo, step_hi, sys_key[b]) with step = sys_step[b] and key (seed_lo,
+ * seed_hi), so it does not depend on the system's place in the batch. */
+int fnl_baoab_step_batched(double* x, double* v, float* x32,
+                           const float* grad_self, const float* grad_nbr, float force_scale,
+                           const double* dtm, double half_dt, double a1, const double* a2,
+                           const int32_t* batch_index, const int32_t* sys_offsets,
+                           const uint64_t* sys_step, const uint32_t* sys_key, uint64_t seed,
+                           int32_t n_atoms, fnl_stream_t stream);
+
+/* *counter += value on the device (single thread). */
+int fnl_counter_add(uint64_t* counter, uint64_t value, fnl_stream_t stream);
 
 /* Standard normals float64[N][3] from the same generator (initial velocities
  * use stream = 1; the O-step uses stream = 0). */

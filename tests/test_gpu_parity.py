@@ -238,7 +238,7 @@ def test_philox_and_baoab_injected_noise(gpu):
     tgs = torch.tensor(gs, device=dev)
     tgn = torch.tensor(gn, device=dev)
     _abi.check(lib.fnl_baoab_step(tx.data_ptr(), tv.data_ptr(), x32.data_ptr(), tgs.data_ptr(), tgn.data_ptr(),
-                                  scale, tdtm.data_ptr(), 0.5 * dt, a1, ta2.data_ptr(), tno.data_ptr(), 0, 0, n,
+                                  scale, tdtm.data_ptr(), 0.5 * dt, a1, ta2.data_ptr(), tno.data_ptr(), 0, 0, None, n,
                                   _abi.stream_handle()), "baoab")
     np.testing.assert_allclose(tx.cpu().numpy(), xr, rtol=1e-13, atol=1e-13)
     np.testing.assert_allclose(tv.cpu().numpy(), vr, rtol=1e-13, atol=1e-13)
