@@ -138,6 +138,74 @@ def cpu_reference(system, cfg, params):
     return (E, F), t1 - t0
 
 
+def bench_serving(cfg, params, n_requests: int, n_baseline: int) -> dict:
+    """cfg 3: continuous batching (BF16 MLP) vs plain FP32 per-request launches."""
+    import torch
+
+    from fennol_amd.potential import ANIConfig, ANIPotential
+    from fennol_amd.serving import make_requests, per_request_baseline, poisson_arrivals, serve_stream
+
+    cfg4 = ANIConfig()  # 4-species AEV (clusters of H, C, N, O)
+    from fennol_amd.params import init_species_params
+
+    p4 = init_species_params(MODEL_SEED, cfg4.aev_length)
+    reqs = make_requests(n_requests, seed=3)
+    pot = ANIPotential(cfg4, p4, precision="bf16")
+    log(f"[bench] serving: {n_requests} requests, saturated")
+    sat = serve_stream(pot, reqs, None, max_atoms=65536, max_systems=2048)
+    rate = 0.7 * sat["requests_per_s"]
+    log(f"[bench] serving: Poisson arrivals at {rate:.1f} req/s")
+    poi = serve_stream(pot, reqs, poisson_arrivals(len(reqs), rate, seed=1), max_atoms=65536, max_systems=2048)
+    del pot
+    torch.cuda.empty_cache()
+    log(f"[bench] serving: FP32 per-request baseline on {n_baseline} requests")
+    pot32 = ANIPotential(cfg4, p4, precision="fp32")
+    base = per_request_baseline(pot32, reqs[:n_baseline])
+    keep = ("wall_s", "requests", "atom_steps", "atom_steps_per_s", "requests_per_s", "iterations", "p50_ms",
+            "p99_ms", "ef_p50_ms", "ef_p99_ms", "md_p50_ms", "md_p99_ms", "service_p50_ms", "service_p99_ms")
+    rnd = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items() if k in keep}
+    return {"workload": f"{n_requests} cluster requests, 8-512 atoms log-uniform, 30% 200-step Langevin, "
+                        f"70% single energy+forces", "bf16_saturated": rnd(sat),
+            "bf16_poisson": {**rnd(poi), "arrival_rate_per_s": round(rate, 2)},
+            "fp32_per_request": rnd(base),
+            "speedup_atom_steps": round(sat["atom_steps_per_s"] / base["atom_steps_per_s"], 2)}
+
+
+def bench_training(n_boxes: int, steps: int) -> dict:
+    """cfg 4: energy + force loss and weight gradient through forces on boxes of 64 waters."""
+    import torch
+
+    from fennol_amd import systems
+    from fennol_amd.params import init_species_params
+    from fennol_amd.potential import ANIConfig, ANIPotential
+    from fennol_amd.training import TrainingStep
+
+    cfg = ANIConfig(species_order=SPECIES7)
+    params = init_species_params(MODEL_SEED, cfg.aev_length)
+    ref = init_species_params(MODEL_SEED + 1, cfg.aev_length)
+    batch = systems.training_boxes(n_boxes=n_boxes, seed=4)
+    N = len(batch["species"])
+    ref_pot = ANIPotential(cfg, ref, precision="fp32")
+    e_ref, f_ref, _ = ref_pot.energy_forces_virial(batch["species"], batch["coordinates"], batch["natoms"],
+                                                   batch["cells"])
+    del ref_pot
+    pot = ANIPotential(cfg, params, precision="fp32")
+    sb = pot.setup(batch["species"], batch["natoms"], batch["cells"])
+    x32 = torch.tensor(np.asarray(batch["coordinates"], np.float32), device="cuda")
+    ts = TrainingStep(pot, params)
+    er = torch.tensor(e_ref, device="cuda")
+    fr = torch.tensor(f_ref, device="cuda")
+    ts(sb, x32, er, fr)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss, grads, _, _ = ts(sb, x32, er, fr)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return {"workload": f"{n_boxes} periodic boxes x 64 waters ({N} atoms), loss = MSE(E) + 10 MSE(F)",
+            "ms_per_step": round(1000 * dt, 3), "atoms_per_s": round(N / dt, 1), "loss": loss}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +213,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="water23k", choices=["water23k", "water7k"])
     ap.add_argument("--skip-cpu", action="store_true", help="skip the FP64 oracle (accuracy and CPU baseline)")
+    ap.add_argument("--serving-requests", type=int, default=1024, help="0 skips the serving measurement")
+    ap.add_argument("--serving-baseline", type=int, default=64)
+    ap.add_argument("--training-boxes", type=int, default=256, help="0 skips the training measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (single-GPU build)",
@@ -217,6 +288,9 @@ def main() -> int:
         cpu = {"value": round(N / t_cpu, 1), "unit": "atom-steps/s", "cores": cores, "kind": "port",
                "sample": f"1 MD step (all-pairs nblist, AEV, MLP, forces, BAOAB) of the same {N}-atom box, "
                          f"FP64 NumPy oracle, {t_cpu:.1f} s"}
+    serving = bench_serving(cfg, params, args.serving_requests, args.serving_baseline) \
+        if args.serving_requests > 0 else None
+    training = bench_training(args.training_boxes, 3) if args.training_boxes > 0 else None
     out = {
         "metric": "atom-steps/s and p50/p99 latency, batched BF16 vs plain FP32, at <=1 meV/atom err",
         "value": round(value, 1),
@@ -244,6 +318,8 @@ def main() -> int:
         "stages": stage_table,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
+        "training": training,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -16,8 +16,9 @@
 //   k_cell_sort one wave per cell: order each segment by atom index
 //               (deterministic rows) and write cell-sorted float4 positions
 //   k_pairs     one wave per centre, centres in cell order: candidates of the
-//               neighbouring cells tested 64 at a time, kept pairs compacted
-//               with a wave ballot + prefix popcount into the centre's row.
+//               neighbouring cells (contiguous z-runs of the cell-sorted
+//               arrays) tested 64 at a time, kept pairs compacted with a wave
+//               ballot + prefix popcount into the centre's row.
 
 #include "fnl_common.h"
 
@@ -295,6 +296,39 @@ __device__ __forceinline__ int axis_offsets(int c, int n, bool pbc, int* out) {
   return k;
 }
 
+// z-runs of the neighbourhood along one (x, y) column of cells: cells
+// cz-1..cz+1 are consecutive in memory, so their atoms form one contiguous
+// range of the cell-sorted arrays (two ranges when the periodic wrap splits it)
+__device__ __forceinline__ int z_runs(int cz, int nz, bool pbc, int* lo, int* hi) {
+  if (!pbc) {
+    lo[0] = max(cz - 1, 0);
+    hi[0] = min(cz + 1, nz - 1);
+    return 1;
+  }
+  if (nz <= 3) {  // every cell of the column
+    lo[0] = 0;
+    hi[0] = nz - 1;
+    return 1;
+  }
+  if (cz == 0) {
+    lo[0] = 0; hi[0] = 1;
+    lo[1] = nz - 1; hi[1] = nz - 1;
+    return 2;
+  }
+  if (cz == nz - 1) {
+    lo[0] = nz - 2; hi[0] = nz - 1;
+    lo[1] = 0; hi[1] = 0;
+    return 2;
+  }
+  lo[0] = cz - 1;
+  hi[0] = cz + 1;
+  return 1;
+}
+
+// One wave per centre (centres in cell order).  The candidates are the atoms
+// of up to 18 contiguous ranges (3x3 columns x z-runs); each range is scanned
+// 64 candidates at a time, kept pairs are compacted with a ballot and a prefix
+// popcount into the radial row and, for d12 < rca^2, the angular row.
 template <int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
 k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_index,
@@ -304,43 +338,33 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
         const float4* __restrict__ pos, float rc2, float rca2, int max_nbr, int max_nbr_ang,
         int32_t* __restrict__ nbr, int32_t* __restrict__ n_nbr, int32_t* __restrict__ nbr_ang,
         int32_t* __restrict__ n_nbr_ang, int32_t* __restrict__ status, int n_cells) {
-  __shared__ int s_cstart[WAVES][27];
-  __shared__ int s_cpre[WAVES][28];
   const int w = threadIdx.x >> 6;
   const int lane = fnl_lane();
   const int slot = blockIdx.x * WAVES + w;
   const int n_binned = cell_start[n_cells];
-  const bool active = slot < n_binned && slot < n_binned_cap;
-  int i = 0, b = 0, ncell = 0, total = 0, first = 0, nx = 1, ny = 1, nz = 1;
-  bool pbc = false;
-  if (active) {
-    i = sorted[slot];
-    b = batch_index[i];
-    const int32_t* g = grid + b * kGridWords;
-    nx = g[0]; ny = g[1]; nz = g[2]; first = g[3];
-    pbc = g[10] != 0;
-    const int cell = atom_cell[i] - first;
-    const int cz = cell % nz, cy = (cell / nz) % ny, cx = cell / (ny * nz);
-    int ox[3], oy[3], oz[3];
-    const int nox = axis_offsets(cx, nx, pbc, ox);
-    const int noy = axis_offsets(cy, ny, pbc, oy);
-    const int noz = axis_offsets(cz, nz, pbc, oz);
-    ncell = nox * noy * noz;
-    int cnt = 0;
-    if (lane < ncell) {
-      const int a = lane / (noy * noz), r = lane % (noy * noz);
-      const int bb = r / noz, c = r % noz;
-      const int nc = first + ((cx + ox[a]) * ny + (cy + oy[bb])) * nz + (cz + oz[c]);
-      s_cstart[w][lane] = cell_start[nc];
-      cnt = cell_start[nc + 1] - cell_start[nc];
-    }
-    const int incl = fnl_wave_incl_scan(lane < ncell ? cnt : 0);
-    if (lane < ncell) s_cpre[w][lane + 1] = incl;
-    if (lane == 0) s_cpre[w][0] = 0;
-    total = __shfl(incl, ncell - 1, 64);
+  if (slot >= n_binned || slot >= n_binned_cap) return;
+  const int i = sorted[slot];
+  const int b = batch_index[i];
+  const int32_t* g = grid + b * kGridWords;
+  const int nx = g[0], ny = g[1], nz = g[2], first = g[3];
+  const bool pbc = g[10] != 0;
+  const int cell = atom_cell[i] - first;
+  const int cz = cell % nz, cy = (cell / nz) % ny, cx = cell / (ny * nz);
+  int ox[3], oy[3];
+  const int nox = axis_offsets(cx, nx, pbc, ox);
+  const int noy = axis_offsets(cy, ny, pbc, oy);
+  int zlo[2], zhi[2];
+  const int nzr = z_runs(cz, nz, pbc, zlo, zhi);
+  // lane r < nox*noy*nzr holds range r
+  int r_lo = 0, r_hi = 0;
+  const int nrange = nox * noy * nzr;
+  if (lane < nrange) {
+    const int zr = lane % nzr, xy = lane / nzr;
+    const int a = xy / noy, bb = xy % noy;
+    const int col = first + ((cx + ox[a]) * ny + (cy + oy[bb])) * nz;
+    r_lo = cell_start[col + zlo[zr]];
+    r_hi = cell_start[col + zhi[zr] + 1];
   }
-  __syncthreads();
-  if (!active) return;
 
   FnlCell cl;
   if (pbc) fnl_load_cell(cells, rec, b, cl);
@@ -348,35 +372,36 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
   int count = 0, count_a = 0;
   int32_t* row = nbr + (size_t)i * max_nbr;
   int32_t* row_a = nbr_ang + (size_t)i * max_nbr_ang;
-  for (int base = 0; base < total; base += 64) {
-    const int t = base + lane;
-    bool keep = false, keep_a = false;
-    int j = -1;
-    if (t < total) {
-      int c = 0;
-      while (s_cpre[w][c + 1] <= t) ++c;
-      const float4 pj = pos[s_cstart[w][c] + (t - s_cpre[w][c])];
-      j = __float_as_int(pj.w);
-      if (j != i) {
-        float vx = pj.x - xi, vy = pj.y - yi, vz = pj.z - zi;
-        if (pbc) fnl_min_image(vx, vy, vz, cl);
-        const float d12 = fnl_d12(vx, vy, vz);
-        keep = d12 < rc2;
-        keep_a = d12 < rca2;
+  for (int r = 0; r < nrange; ++r) {
+    const int lo = __shfl(r_lo, r, 64), hi = __shfl(r_hi, r, 64);
+    for (int base = lo; base < hi; base += 64) {
+      const int p = base + lane;
+      bool keep = false, keep_a = false;
+      int j = -1;
+      if (p < hi) {
+        const float4 pj = pos[p];
+        j = __float_as_int(pj.w);
+        if (j != i) {
+          float vx = pj.x - xi, vy = pj.y - yi, vz = pj.z - zi;
+          if (pbc) fnl_min_image(vx, vy, vz, cl);
+          const float d12 = fnl_d12(vx, vy, vz);
+          keep = d12 < rc2;
+          keep_a = d12 < rca2;
+        }
       }
-    }
-    const unsigned long long m = __ballot(keep);
-    const unsigned long long ma = __ballot(keep_a);
-    if (keep) {
-      const int e = count + fnl_rank_below(m);
-      if (e < max_nbr) row[e] = j;
-      if (keep_a) {
-        const int ea = count_a + fnl_rank_below(ma);
-        if (ea < max_nbr_ang && e < max_nbr) row_a[ea] = e;
+      const unsigned long long m = __ballot(keep);
+      const unsigned long long ma = __ballot(keep_a);
+      if (keep) {
+        const int e = count + fnl_rank_below(m);
+        if (e < max_nbr) row[e] = j;
+        if (keep_a) {
+          const int ea = count_a + fnl_rank_below(ma);
+          if (ea < max_nbr_ang && e < max_nbr) row_a[ea] = e;
+        }
       }
+      count += __popcll(m);
+      count_a += __popcll(ma);
     }
-    count += __popcll(m);
-    count_a += __popcll(ma);
   }
   if (lane == 0) {
     n_nbr[i] = count;

@@ -263,3 +263,71 @@ class ServingEngine:
                 "atom_steps_per_s": self.atom_steps / wall, "iterations": self.iterations,
                 "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
                 "mean_ms": float(lat.mean() * 1e3)}
+
+
+def make_requests(n_requests: int, seed: int = 3, md_fraction: float = 0.3, md_steps: int = 200,
+                  n_min: int = 8, n_max: int = 512) -> list[Request]:
+    """Request stream of clusters with log-uniform sizes, a fraction of them short MD runs."""
+    from .systems import serving_requests
+
+    out = []
+    for k, q in enumerate(serving_requests(n_requests, seed, n_min, n_max, md_fraction, md_steps)):
+        s = q["system"]
+        out.append(Request(s["species"], s["coordinates"], md_steps=q["md_steps"], rid=k))
+    return out
+
+
+def poisson_arrivals(n: int, rate: float, seed: int = 0) -> np.ndarray:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))
+
+
+def serve_stream(potential: ANIPotential, requests: list[Request], arrivals: np.ndarray | None,
+                 **engine_kw) -> dict:
+    """Serve copies of `requests` (arrival times `arrivals`, None = all at t=0)."""
+    eng = ServingEngine(potential, **engine_kw)
+    reqs = [Request(r.species, r.coordinates, r.md_steps, 0.0 if arrivals is None else float(arrivals[k]), r.rid)
+            for k, r in enumerate(requests)]
+    res = eng.run(reqs, realtime=arrivals is not None)
+    lat_ef = np.array([r.t_done - r.arrival for r in reqs if r.md_steps == 0])
+    lat_md = np.array([r.t_done - r.arrival for r in reqs if r.md_steps > 0])
+    res["requests_per_s"] = len(reqs) / res["wall_s"]
+    if lat_ef.size:
+        res["ef_p50_ms"] = float(np.percentile(lat_ef, 50) * 1e3)
+        res["ef_p99_ms"] = float(np.percentile(lat_ef, 99) * 1e3)
+    if lat_md.size:
+        res["md_p50_ms"] = float(np.percentile(lat_md, 50) * 1e3)
+        res["md_p99_ms"] = float(np.percentile(lat_md, 99) * 1e3)
+    res["pool_bytes"] = eng.pool_bytes
+    return res
+
+
+def per_request_baseline(potential: ANIPotential, requests: list[Request], dt_fs: float = 0.5,
+                         temperature: float = 300.0, gamma_thz: float = 1.0, seed: int = 0) -> dict:
+    """Plain per-request launches: each request alone (own buffers and launches), in order."""
+    from .md import LangevinMD
+
+    times = []
+    atom_steps = 0
+    t0 = time.perf_counter()
+    for r in requests:
+        t1 = time.perf_counter()
+        system = {"species": r.species, "coordinates": r.coordinates}
+        if r.md_steps == 0:
+            sb = potential.setup(r.species)
+            x = torch.as_tensor(np.asarray(r.coordinates, np.float32), device=potential.device)
+            potential.compute(sb, x)
+            potential.forces(sb).cpu()
+        else:
+            md = LangevinMD(potential, system, dt_fs=dt_fs, temperature=temperature, gamma_thz=gamma_thz,
+                            seed=seed + r.rid, check_every=0)
+            md.run(r.md_steps)
+            md.energies()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t1)
+        atom_steps += r.atom_steps
+    wall = time.perf_counter() - t0
+    t = np.array(times)
+    return {"wall_s": wall, "requests": len(requests), "atom_steps": atom_steps,
+            "atom_steps_per_s": atom_steps / wall, "requests_per_s": len(requests) / wall,
+            "service_p50_ms": float(np.percentile(t, 50) * 1e3), "service_p99_ms": float(np.percentile(t, 99) * 1e3)}

@@ -1,0 +1,76 @@
+"""CPU tests of the host-side logic (no kernel launches)."""
+
+from __future__ import annotations
+
+from collections import OrderedDict
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fennol_amd import _abi, systems
+
+needs_lib = pytest.mark.skipif(not _abi.LIB_PATH.exists(), reason="HIP library not built")
+
+
+@needs_lib
+def test_pool_layout_tables():
+    from fennol_amd.potential import ANIConfig, ANIPotential
+
+    cfg = ANIConfig()
+    pot = ANIPotential(cfg, O.init_mlp_params(1, cfg.aev_length), device="cpu")
+    rng = np.random.Generator(np.random.PCG64(0))
+    sysl = [systems.lattice_cluster(int(n), rng) for n in (8, 100, 513 - 1, 37)]
+    b = systems.batch_systems(sysl)
+    lay = pot.pool_layout_arrays(b["species"], b["natoms"])
+    N = len(b["species"])
+    assert lay["n_atoms"] == N and lay["n_sys"] == 4
+    np.testing.assert_array_equal(lay["sys_offsets"], np.concatenate([[0], np.cumsum(b["natoms"])]))
+    # perm: every atom once, grouped by network in species order
+    assert sorted(lay["perm"].tolist()) == list(range(N))
+    z_of_net = [1, 6, 7, 8]
+    covered = []
+    for s, r, n in zip(lay["tile_start"], lay["tile_rows"], lay["tile_net"]):
+        assert 0 < r <= pot.nets.tile_rows
+        rows = lay["perm"][s:s + r]
+        assert np.all(b["species"][rows] == z_of_net[n])
+        covered.extend(rows.tolist())
+    assert sorted(covered) == list(range(N))
+    # cell grids: caps k^3 with k = ceil(n^(1/3)), consecutive first cells
+    k = lay["grid"][:, 0]
+    assert np.all(k == np.ceil(b["natoms"] ** (1 / 3) - 1e-9).astype(int))
+    np.testing.assert_array_equal(lay["grid"][1:, 3], np.cumsum(k.astype(np.int64) ** 3)[:-1])
+    assert lay["n_cells"] == int((k.astype(np.int64) ** 3).sum())
+    np.testing.assert_array_equal(np.diff(lay["seg_offsets"]), np.ceil(b["natoms"] / 512).astype(int))
+
+
+@needs_lib
+def test_unhandled_species_is_rejected():
+    from fennol_amd.potential import ANIConfig, ANIPotential
+
+    cfg = ANIConfig(species_order=("H", "C", "N", "O", "S", "F", "Cl"))
+    params = O.init_mlp_params(1, cfg.aev_length, species=("H", "O"))
+    pot = ANIPotential(cfg, params, device="cpu")
+    with pytest.raises(ValueError):
+        pot.pool_layout_arrays(np.array([1, 6, 8], np.int32), np.array([3], np.int32))
+
+
+def test_fennix_rejects_other_modules():
+    from fennol_amd.fennix import FENNIX
+
+    with pytest.raises(NotImplementedError):
+        FENNIX(cutoff=5.0, modules=OrderedDict(emb={"module_name": "SCHNET_EMBEDDING"}), device="cpu")
+
+
+def test_serving_request_stream_properties():
+    from fennol_amd.serving import make_requests, poisson_arrivals
+
+    reqs = make_requests(400, seed=3)
+    n = np.array([r.n for r in reqs])
+    assert n.min() >= 8 and n.max() <= 512
+    md = np.array([r.md_steps > 0 for r in reqs])
+    assert 0.2 < md.mean() < 0.4
+    assert np.all(np.array([r.md_steps for r in reqs])[md] == 200)
+    t = poisson_arrivals(20000, 50.0, seed=1)
+    assert np.all(np.diff(t) > 0)
+    assert abs(20000 / t[-1] - 50.0) < 2.0
