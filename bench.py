@@ -75,13 +75,13 @@ def algorithmic_units(sb, cfg, precision: str) -> dict:
     }
 
 
-def run_md(system, cfg, params, precision, steps, warmup, timed_stages=True):
+def run_md(system, cfg, params, precision, steps, warmup, timed_stages=True, tile_rows=0):
     import torch
 
     from fennol_amd.md import LangevinMD
     from fennol_amd.potential import ANIPotential, StageTimer
 
-    pot = ANIPotential(cfg, params, precision=precision)
+    pot = ANIPotential(cfg, params, precision=precision, mlp_tile_rows=tile_rows)
     md = LangevinMD(pot, system, dt_fs=0.5, temperature=300.0, gamma_thz=1.0, seed=MD_SEED, check_every=0)
     for _ in range(warmup):
         md.step()
@@ -216,6 +216,8 @@ def main() -> int:
     ap.add_argument("--serving-requests", type=int, default=1024, help="0 skips the serving measurement")
     ap.add_argument("--serving-baseline", type=int, default=64)
     ap.add_argument("--training-boxes", type=int, default=256, help="0 skips the training measurement")
+    ap.add_argument("--tile-bf16", type=int, default=0, help="MLP atoms per tile, BF16 form (0: default)")
+    ap.add_argument("--tile-fp32", type=int, default=0, help="MLP atoms per tile, FP32 form (0: default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (single-GPU build)",
@@ -238,7 +240,9 @@ def main() -> int:
     results = {}
     for precision in ("fp32", "bf16"):
         log(f"[bench] MD {precision}: warmup {args.warmup}, steps {args.steps}")
-        pot, md, wall, lat, stages = run_md(system, cfg, params, precision, args.steps, args.warmup)
+        tile = args.tile_bf16 if precision == "bf16" else args.tile_fp32
+        pot, md, wall, lat, stages = run_md(system, cfg, params, precision, args.steps, args.warmup,
+                                            tile_rows=tile)
         units = algorithmic_units(md.sb, cfg, precision)
         results[precision] = dict(wall=wall, lat=lat, stages=stages, units=units,
                                   max_nbr=md.sb.max_nbr, max_nbr_ang=md.sb.max_nbr_ang)

@@ -70,6 +70,7 @@ class MlpWeights(C.Structure):
         ("k_pad", i32),
         ("dtype", i32),
         ("celu_alpha", f32),
+        ("tile_rows", i32),
         ("nets", MlpNet * FNL_MLP_MAX_NETS),
     ]
 

@@ -57,10 +57,8 @@ struct Tr;
 template <>
 struct Tr<float> {
   static constexpr int KS = 4;    // K per MFMA
-  static constexpr int M = 32;    // atoms per tile
   static constexpr int PAD = 4;   // activation row padding (elements)
   static constexpr int KC = 64;   // layer-1 K chunk
-  static constexpr int PF = 4;    // weight prefetch depth (k-steps)
   typedef float frag;
   static __device__ __forceinline__ f4 mma(frag a, frag b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -82,10 +80,8 @@ struct Tr<float> {
 template <>
 struct Tr<unsigned short> {  // bf16 stored as raw bits
   static constexpr int KS = 32;
-  static constexpr int M = 64;
   static constexpr int PAD = 8;
   static constexpr int KC = 128;
-  static constexpr int PF = 2;
   typedef bf8 frag;
   static __device__ __forceinline__ f4 mma(frag a, frag b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -113,9 +109,17 @@ struct Tr<unsigned short> {  // bf16 stored as raw bits
   }
 };
 
-template <typename T>
+// Kernel variants: element type, atoms per tile (M) and weight prefetch depth
+// (PF k-steps, in registers).  LDS footprint bounds the workgroups per CU.
+template <typename T, int M_, int PF_>
+struct Cfg {
+  static constexpr int M = M_;
+  static constexpr int PF = PF_;
+};
+
+template <typename T, int M_>
 struct Lds {
-  static constexpr int M = Tr<T>::M;
+  static constexpr int M = M_;
   static constexpr int LD1 = H1 + Tr<T>::PAD, LD2 = H2 + Tr<T>::PAD, LD3 = H3 + Tr<T>::PAD;
   static constexpr int LDX = Tr<T>::KC + Tr<T>::PAD;
   static constexpr int OFF1 = 0;
@@ -125,7 +129,12 @@ struct Lds {
   static constexpr int BYTES = OFFE + 4 * M * 4;
   static constexpr int XBUF = M * LDX * (int)sizeof(T);  // one staging buffer, at OFF2
   static_assert(2 * XBUF <= OFFE - OFF2, "layer-1 staging must fit in the H2/H3 buffers");
-  static_assert(BYTES <= 80 * 1024, "two workgroups per CU");
+  static_assert(BYTES <= 80 * 1024, "at least two workgroups per CU");
+  // workgroups per CU allowed by the LDS footprint, at most 3 for BF16 (the
+  // prefetch ring then keeps <= 168 VGPRs per lane without spills)
+  static constexpr int BLOCKS_LDS = (160 * 1024) / BYTES;
+  static constexpr int BLOCKS_PER_CU =
+      sizeof(T) == 2 ? (BLOCKS_LDS > 3 ? 3 : BLOCKS_LDS) : (BLOCKS_LDS > 4 ? 4 : BLOCKS_LDS);
 };
 
 template <int NB, int MB>
@@ -138,11 +147,10 @@ __device__ __forceinline__ void zero_acc(f4 (&acc)[NB][MB]) {
 
 // acc[nb][mb] += sum_k W(n, k) act(k, m) over k in [0, K) for the n-blocks
 // nb0 + 4*nb (< ncb) of this wave.  W prefetched PF steps ahead.
-template <typename T, int NB, int MB>
+template <typename T, int NB, int MB, int PF>
 __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ W, int ldw, int nb0, int ncb,
                                        const T* act, int lda, int K) {
   typedef Tr<T> tr;
-  constexpr int PF = tr::PF;
   const int nsteps = K / tr::KS;
   typename tr::frag wq[PF][NB];
 #pragma unroll
@@ -218,15 +226,16 @@ __device__ __forceinline__ void bwd_epilogue(const f4 (&acc)[NB][MB], T* HG, int
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256, 2)
+template <typename T, int M_, int PF_>
+__global__ void __launch_bounds__(256, (Lds<T, M_>::BLOCKS_PER_CU))
 k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int32_t* __restrict__ perm,
               const int32_t* __restrict__ tile_start, const int32_t* __restrict__ tile_rows,
               const int32_t* __restrict__ tile_net, float* __restrict__ energy, float* __restrict__ grad,
               int grad_stride) {
   typedef Tr<T> tr;
-  typedef Lds<T> L;
-  constexpr int M = tr::M;
+  typedef Lds<T, M_> L;
+  constexpr int M = M_;
+  constexpr int PF = PF_;
   constexpr int MB = M / 16;
   constexpr int KC = tr::KC;
   __shared__ __attribute__((aligned(16))) unsigned char smem[L::BYTES];
@@ -276,7 +285,6 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   f4 acc1[4][MB];
   zero_acc(acc1);
   {
-    constexpr int PF = tr::PF;
     constexpr int SPC = KC / tr::KS;  // k-steps per chunk
     static_assert(SPC % PF == 0, "prefetch ring aligned to chunks");
     const T* W = (const T*)P.fw[0];
@@ -320,7 +328,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   {
     f4 acc2[3][MB];
     zero_acc(acc2);
-    gemm_t<T, 3, MB>(acc2, (const T*)P.fw[1], sizeof(T) == 2 ? H1 : H2, wave, H2 / 16, B1, L::LD1, H1);
+    gemm_t<T, 3, MB, PF>(acc2, (const T*)P.fw[1], sizeof(T) == 2 ? H1 : H2, wave, H2 / 16, B1, L::LD1, H1);
     fwd_epilogue<T, 3, MB>(acc2, P.b[1], B2, L::LD2, wave, H2 / 16, alpha);
   }
   __syncthreads();
@@ -328,7 +336,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   // ---------------- layer 3 + energy ----------------
   f4 acc3[3][MB];
   zero_acc(acc3);
-  gemm_t<T, 3, MB>(acc3, (const T*)P.fw[2], sizeof(T) == 2 ? H2 : H3, wave, H3 / 16, B2, L::LD2, H2);
+  gemm_t<T, 3, MB, PF>(acc3, (const T*)P.fw[2], sizeof(T) == 2 ? H2 : H3, wave, H3 / 16, B2, L::LD2, H2);
   fwd_epilogue<T, 3, MB>(acc3, P.b[2], B3, L::LD3, wave, H3 / 16, alpha);
   {
     float part[MB];
@@ -385,14 +393,14 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   {
     f4 acc2[3][MB];
     zero_acc(acc2);
-    gemm_t<T, 3, MB>(acc2, (const T*)P.bw[2], sizeof(T) == 2 ? H3 : H2, wave, H2 / 16, B3, L::LD3, H3);
+    gemm_t<T, 3, MB, PF>(acc2, (const T*)P.bw[2], sizeof(T) == 2 ? H3 : H2, wave, H2 / 16, B3, L::LD3, H3);
     bwd_epilogue<T, 3, MB>(acc2, B2, L::LD2, wave, H2 / 16, alpha);
   }
   __syncthreads();
   {
     f4 accg[4][MB];
     zero_acc(accg);
-    gemm_t<T, 4, MB>(accg, (const T*)P.bw[1], sizeof(T) == 2 ? H2 : H1, wave, H1 / 16, B2, L::LD2, H2);
+    gemm_t<T, 4, MB, PF>(accg, (const T*)P.bw[1], sizeof(T) == 2 ? H2 : H1, wave, H1 / 16, B2, L::LD2, H2);
     bwd_epilogue<T, 4, MB>(accg, B1, L::LD1, wave, H1 / 16, alpha);
   }
   __syncthreads();
@@ -407,7 +415,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   for (int base = 0; base < ncb_x; base += 16) {
     f4 accx[4][MB];
     zero_acc(accx);
-    gemm_t<T, 4, MB>(accx, (const T*)P.bw[0], sizeof(T) == 2 ? H1 : K, base + wave, ncb_x, B1, L::LD1, H1);
+    gemm_t<T, 4, MB, PF>(accx, (const T*)P.bw[0], sizeof(T) == 2 ? H1 : K, base + wave, ncb_x, B1, L::LD1, H1);
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
       const int cb = base + wave + 4 * nb;
@@ -423,8 +431,8 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
 }  // namespace
 
 extern "C" int fnl_mlp_tile_rows(int32_t dtype) {
-  if (dtype == FNL_F32) return Tr<float>::M;
-  if (dtype == FNL_BF16) return Tr<unsigned short>::M;
+  if (dtype == FNL_F32) return 32;
+  if (dtype == FNL_BF16) return 64;
   return FNL_ERR_INVALID;
 }
 
@@ -470,13 +478,23 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
     p.b[3] = s.b4;
   }
   hipStream_t st = (hipStream_t)stream;
-  if (bf) {
-    hipLaunchKernelGGL(k_species_mlp<unsigned short>, dim3(n_tiles), dim3(256), 0, st,
+  const int M = w->tile_rows > 0 ? w->tile_rows : fnl_mlp_tile_rows(w->dtype);
+  if (bf && M == 64) {
+    hipLaunchKernelGGL((k_species_mlp<unsigned short, 64, 2>), dim3(n_tiles), dim3(256), 0, st,
                        (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, energy,
                        grad, grad_stride);
+  } else if (bf && M == 32) {
+    hipLaunchKernelGGL((k_species_mlp<unsigned short, 32, 2>), dim3(n_tiles), dim3(256), 0, st,
+                       (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, energy,
+                       grad, grad_stride);
+  } else if (!bf && M == 32) {
+    hipLaunchKernelGGL((k_species_mlp<float, 32, 4>), dim3(n_tiles), dim3(256), 0, st, (const float*)aev,
+                       aev_stride, a, perm, tile_start, tile_rows, tile_net, energy, grad, grad_stride);
+  } else if (!bf && M == 16) {
+    hipLaunchKernelGGL((k_species_mlp<float, 16, 4>), dim3(n_tiles), dim3(256), 0, st, (const float*)aev,
+                       aev_stride, a, perm, tile_start, tile_rows, tile_net, energy, grad, grad_stride);
   } else {
-    hipLaunchKernelGGL(k_species_mlp<float>, dim3(n_tiles), dim3(256), 0, st, (const float*)aev, aev_stride, a,
-                       perm, tile_start, tile_rows, tile_net, energy, grad, grad_stride);
+    return FNL_ERR_UNSUPPORTED;
   }
   FNL_LAUNCH_CHECK();
   return FNL_OK;

@@ -260,3 +260,20 @@ class FENNIX:
     @property
     def params(self):
         return from_variables(self.variables, self.energy_module)
+
+    # ------------------------------------------------------------- files
+    def save(self, filename) -> None:
+        """Write the model as a .fnx file (msgpack, fennix.py:647-662)."""
+        from .fnx import save
+
+        save(self, filename)
+
+    @classmethod
+    def load(cls, filename, precision: str = "fp32", device="cuda", graph_config: dict = {},
+             use_atom_padding: bool = False) -> "FENNIX":
+        """Build a model from a .fnx file (fennix.py:664-688)."""
+        from .fnx import load_state
+
+        state = load_state(filename)
+        return cls(**state, graph_config=graph_config, use_atom_padding=use_atom_padding, precision=precision,
+                   device=device, seed=None)

@@ -105,7 +105,7 @@ class SpeciesNetworks:
     """
 
     def __init__(self, params: dict, k_in: int, k_pad: int, dtype: str = "fp32", device="cuda",
-                 celu_alpha: float = CELU_ALPHA):
+                 celu_alpha: float = CELU_ALPHA, tile_rows: int = 0):
         if dtype not in ("fp32", "bf16"):
             raise ValueError("dtype must be 'fp32' or 'bf16'")
         if len(params) > _abi.FNL_MLP_MAX_NETS:
@@ -144,7 +144,8 @@ class SpeciesNetworks:
             self._keep += [w4, *bs]
             net.w4 = w4.data_ptr()
             net.b1, net.b2, net.b3, net.b4 = (x.data_ptr() for x in bs)
-        self.tile_rows = _abi.load().fnl_mlp_tile_rows(self.cw.dtype)
+        self.tile_rows = tile_rows or _abi.load().fnl_mlp_tile_rows(self.cw.dtype)
+        self.cw.tile_rows = self.tile_rows
 
 
 @dataclass
@@ -219,13 +220,14 @@ class ANIPotential:
     """
 
     def __init__(self, config: ANIConfig, params: dict, precision: str = "fp32", device="cuda",
-                 max_nbr: int = 64, max_nbr_ang: int = 32):
+                 max_nbr: int = 64, max_nbr_ang: int = 32, mlp_tile_rows: int = 0):
         self.lib = _abi.load()
         self.config = config
         self.precision = precision
         self.device = torch.device(device)
         self.cparams = config.c_params()
-        self.nets = SpeciesNetworks(params, config.aev_length, config.k_pad, precision, self.device)
+        self.nets = SpeciesNetworks(params, config.aev_length, config.k_pad, precision, self.device,
+                                    tile_rows=mlp_tile_rows)
         self.net_of_z = {SYMBOL_TO_Z[s]: n for n, s in enumerate(self.nets.species)}
         self.init_max_nbr = max_nbr
         self.init_max_nbr_ang = max_nbr_ang

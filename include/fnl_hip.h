@@ -191,7 +191,7 @@ int fnl_aev_tangent(const float* coords, const float* disp, const int32_t* speci
  *   w4 [H3], b1 [H1], b2 [H2], b3 [H3], b4 [1]   float32
  * Atoms are processed in tiles of rows of one network: tile t covers
  * perm[tile_start[t] .. tile_start[t] + tile_rows[t]) with network
- * tile_net[t]; rows per tile <= fnl_mlp_tile_rows(dtype).
+ * tile_net[t]; rows per tile <= the kernel's tile size (tile_rows).
  * Outputs: energy[perm[r]] and, if grad != NULL, dE/dAEV rows grad[perm[r]]
  * (float32, stride grad_stride >= k_pad; columns >= k_pad untouched).
  * ---------------------------------------------------------------------- */
@@ -211,9 +211,12 @@ typedef struct fnl_mlp_weights {
   int32_t k_pad;
   int32_t dtype;              /* FNL_F32 or FNL_BF16 (weights and GEMM inputs) */
   float celu_alpha;
+  int32_t tile_rows;          /* atoms per tile: 0 = fnl_mlp_tile_rows(dtype);
+                                 FP32: 16 or 32, BF16: 32 or 64            */
   fnl_mlp_net nets[FNL_MLP_MAX_NETS];
 } fnl_mlp_weights;
 
+/* default atoms per tile for a dtype */
 int fnl_mlp_tile_rows(int32_t dtype);
 
 int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_mlp_weights* w,

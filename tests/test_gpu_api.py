@@ -205,3 +205,16 @@ def test_serving_results_same_alone_and_in_mix(gpu):
         if r.md_steps == 0:
             e, f, _ = pot.energy_forces_virial(r.species, r.coordinates)
             np.testing.assert_allclose(mix[r.rid].energy, e[0], rtol=1e-5, atol=1e-5)
+
+
+def test_fnx_save_load_roundtrip(gpu, tmp_path):
+    print("[api] fnx roundtrip", flush=True)
+    s = systems.cluster_config0(6)
+    model = ani_model(seed=9)
+    e, f, _ = model.energy_and_forces(species=s["species"], coordinates=s["coordinates"])
+    path = tmp_path / "ani.fnx"
+    model.save(path)
+    back = FENNIX.load(path)
+    e2, f2, _ = back.energy_and_forces(species=s["species"], coordinates=s["coordinates"])
+    np.testing.assert_array_equal(e, e2)
+    np.testing.assert_allclose(f, f2, rtol=1e-6, atol=1e-6)

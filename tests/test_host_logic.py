@@ -74,3 +74,27 @@ def test_serving_request_stream_properties():
     t = poisson_arrivals(20000, 50.0, seed=1)
     assert np.all(np.diff(t) > 0)
     assert abs(20000 / t[-1] - 50.0) < 2.0
+
+
+def test_fnx_roundtrip_and_layout(tmp_path):
+    """msgpack .fnx encoding: arrays as ext type 1 (shape, dtype, bytes), scalars as ext type 3."""
+    import msgpack
+
+    from fennol_amd import fnx
+    from fennol_amd.params import init_species_params, to_variables
+
+    variables = to_variables(init_species_params(2, 40, species=("H", "O")))
+    state = {"cutoff": 5.2, "modules": [["energy", {"module_name": "SPECIES_INDEX_NET"}]],
+             "preprocessing": [], "energy_unit": "Ha", "energy_terms": ["energy"], "variables": variables,
+             "scalar": np.float32(1.5)}
+    data = fnx.dumps(state)
+    back = fnx.loads(data)
+    W = back["variables"]["params"]["energy"]["H"]["Layer_1"]["kernel"]
+    np.testing.assert_array_equal(W, variables["params"]["energy"]["H"]["Layer_1"]["kernel"])
+    assert back["scalar"] == np.float32(1.5) and back["cutoff"] == 5.2
+    # raw structure: ext code 1 with a msgpack (shape, dtype name, bytes) payload
+    raw = msgpack.unpackb(data, raw=False, strict_map_key=False)
+    ext = raw["variables"]["params"]["energy"]["O"]["Layer_2"]["bias"]
+    assert isinstance(ext, msgpack.ExtType) and ext.code == 1
+    shape, dt, buf = msgpack.unpackb(ext.data, raw=False)
+    assert shape == [192] and dt == "float64" and len(buf) == 192 * 8
