@@ -26,6 +26,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# the stages that are one kernel launch each (candidates for the roofline entry)
+KERNEL_OF_STAGE = {"aev_fwd": "k_aev_fwd", "mlp": "k_species_mlp", "aev_bwd": "k_aev_bwd"}
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
 MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp32": 157.3}
 SPECIES7 = ("H", "C", "N", "O", "S", "F", "Cl")
 MODEL_SEED = 11
@@ -73,6 +76,33 @@ def algorithmic_units(sb, cfg, precision: str) -> dict:
         "reduce": ("hbm", float(N * (4 + 36))),
         "baoab": ("hbm", float(N * (48 + 24 + 8 + 8 + 48 + 12))),
     }
+
+
+def measured_traffic(kernel: str, workload_name: str, precision: str):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter's reading for a plain copy of
+    known size), or None when that kernel/workload has not been profiled."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    ent = table.get(f"{workload_name}/{precision}/{kernel}")
+    return None if ent is None else float(ent["bytes_per_launch"])
+
+
+def copy_calibration(mib: int) -> None:
+    """One plain streaming kernel of known size (reads and writes ``mib`` MiB each), launched so
+    that a counter run of this command can scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: {mib} MiB read + {mib} MiB written by one elementwise kernel")
+    del a, b
 
 
 def run_md(system, cfg, params, precision, steps, warmup, timed_stages=True, tile_rows=0):
@@ -218,6 +248,9 @@ def main() -> int:
     ap.add_argument("--training-boxes", type=int, default=256, help="0 skips the training measurement")
     ap.add_argument("--tile-bf16", type=int, default=0, help="MLP atoms per tile, BF16 form (0: default)")
     ap.add_argument("--tile-fp32", type=int, default=0, help="MLP atoms per tile, FP32 form (0: default)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="launch one plain copy kernel of this size first (for counter runs)")
+    ap.add_argument("--precisions", default="fp32,bf16", help="MD forms to run (bf16 gives value)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (single-GPU build)",
@@ -237,8 +270,14 @@ def main() -> int:
     params = model_params(cfg)
     log(f"[bench] workload {args.workload}: {N} atoms, AEV {cfg.aev_length}")
 
+    if args.calibrate_copy_mib > 0:
+        copy_calibration(args.calibrate_copy_mib)
+    precisions = [p for p in ("fp32", "bf16") if p in args.precisions.split(",")]
+    if "bf16" not in precisions:
+        print("the bf16 form gives the headline value; --precisions must include it", file=sys.stderr)
+        return 2
     results = {}
-    for precision in ("fp32", "bf16"):
+    for precision in precisions:
         log(f"[bench] MD {precision}: warmup {args.warmup}, steps {args.steps}")
         tile = args.tile_bf16 if precision == "bf16" else args.tile_fp32
         pot, md, wall, lat, stages = run_md(system, cfg, params, precision, args.steps, args.warmup,
@@ -251,23 +290,26 @@ def main() -> int:
         del pot, md
         torch.cuda.empty_cache()
 
-    bf, fp = results["bf16"], results["fp32"]
+    bf = results["bf16"]
+    fp = results.get("fp32")
     ms_bf = 1000.0 * bf["wall"] / args.steps
-    ms_fp = 1000.0 * fp["wall"] / args.steps
     value = N * args.steps / bf["wall"]
-    value_fp32 = N * args.steps / fp["wall"]
 
-    # roofline of the dominant kernel of the headline (BF16) form
+    # roofline of the dominant kernel of the headline (BF16) form: the longest single-kernel
+    # stage, its algorithmic bytes (or flops) per launch over its mean launch time (HIP events on
+    # the launch stream)
     st = bf["stages"]
-    dom = max((k for k in st if k in bf["units"]), key=lambda k: st[k].mean())
+    dom = max((k for k in KERNEL_OF_STAGE if k in st), key=lambda k: st[k].mean())
     bound, work = bf["units"][dom]
     t_s = st[dom].mean() / 1000.0
     if bound == "hbm":
         achieved, peak, unit = work / t_s / 1e9, HBM_PEAK_GBS, "GB/s"
     else:
         achieved, peak, unit = work / t_s / 1e12, MFMA_PEAK_TFLOPS["bf16"], "TFLOP/s"
-    roofline = {"kernel": dom, "bound": bound, "achieved": round(achieved, 2), "peak": peak, "unit": unit,
-                "frac": round(achieved / peak, 4), "traffic": None}
+    traffic = measured_traffic(KERNEL_OF_STAGE[dom], args.workload, "bf16")
+    roofline = {"bound": bound, "achieved": round(achieved, 2), "peak": peak, "unit": unit,
+                "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": KERNEL_OF_STAGE[dom],
+                "launch_us": round(1e6 * t_s, 2), "algorithmic_per_launch": work}
     stage_table = {}
     for prec, r in results.items():
         stage_table[prec] = {}
@@ -286,7 +328,7 @@ def main() -> int:
     if not args.skip_cpu:
         log("[bench] FP64 oracle on the host (accuracy reference + CPU baseline)")
         fp64, t_cpu = cpu_reference(system, cfg, params)
-        for precision in ("fp32", "bf16"):
+        for precision in precisions:
             acc[precision] = accuracy(system, cfg, params, precision, fp64)
         cores = len(os.sched_getaffinity(0))
         cpu = {"value": round(N / t_cpu, 1), "unit": "atom-steps/s", "cores": cores, "kind": "port",
@@ -312,10 +354,11 @@ def main() -> int:
                                f"rebuilt every step, BAOAB Langevin dt=0.5 fs 300 K gamma=1/ps",
                    "aev_length": cfg.aev_length, "mlp": "256-192-160-1 CELU(0.1), 4 species networks",
                    "precision": "BF16 MLP GEMMs (FP32 accumulate), FP32 AEV, FP64 reductions and integrator"},
-        "fp32": {"value": round(value_fp32, 1), "ms_per_step": round(ms_fp, 4),
-                 "p50_ms": round(float(np.percentile(fp["lat"], 50)), 4),
-                 "p99_ms": round(float(np.percentile(fp["lat"], 99)), 4)},
-        "bf16_over_fp32": round(value / value_fp32, 3),
+        "fp32": None if fp is None else {
+            "value": round(N * args.steps / fp["wall"], 1), "ms_per_step": round(1000.0 * fp["wall"] / args.steps, 4),
+            "p50_ms": round(float(np.percentile(fp["lat"], 50)), 4),
+            "p99_ms": round(float(np.percentile(fp["lat"], 99)), 4)},
+        "bf16_over_fp32": None if fp is None else round(fp["wall"] / bf["wall"], 3),
         "p50_ms": round(float(np.percentile(bf["lat"], 50)), 4),
         "p99_ms": round(float(np.percentile(bf["lat"], 99)), 4),
         "accuracy_vs_fp64": acc,

@@ -110,8 +110,8 @@ __device__ __forceinline__ int pair_index(int a, int b, int S) {
 }
 
 // per-wave LDS footprint (in 4-byte words) of the forward kernel
-__host__ __device__ inline int fwd_lds_words(int max_nbr, int max_nbr_ang, int aev_len) {
-  return 6 * max_nbr + 6 * max_nbr_ang + ((aev_len + 3) & ~3) + 64 * 16;
+__host__ __device__ inline int fwd_lds_words(int max_nbr, int max_nbr_ang, int aev_len, int rad_len) {
+  return 6 * max_nbr + 6 * max_nbr_ang + ((aev_len + 3) & ~3) + 3 * ((rad_len + 3) & ~3) + 64 * 16;
 }
 
 __device__ __forceinline__ void lds_add(float* p, float v) {
@@ -121,15 +121,21 @@ __device__ __forceinline__ void lds_add(float* p, float v) {
 // Forward, one wave per centre.  LDS slice of the wave (4-byte words):
 //   nbv  [max_nbr]     float4 (vx, vy, vz, d) per radial slot
 //   nbf  [max_nbr]     float2 (0.25 fc(d), species)
-//   av   [max_nbr_ang] float4 (ux, uy, uz, d) per angular slot
-//   af   [max_nbr_ang] float2 (fc_ang(d), species)
+//   av   [max_nbr_ang] float4 (ux, uy, uz, d) per angular slot, sorted by species
+//   af   [max_nbr_ang] float2 (fc_ang(d), species), same order
 //   acc  [aev_len]     the AEV row being accumulated, in output channel order
-//   trip [64][16]      per-triplet factors: R_d (NA), F_z (NZ), pair index
-// Radial: lanes (q = lane/16, k = lane%16) add 0.25 fc exp(-eta (d-R_k)^2)
-// for neighbours e = q, q+4, ... into acc[s*nR + k] (LDS float adds).
-// Angular: per chunk of 64 triplets, lane t evaluates R_d and F_z of triplet
-// t; then lanes (g = lane/NZ, z = lane%NZ) add R_d F_z into
-// acc[rad_len + p*NA*NZ + d*NZ + z] for the triplets g, g + 64/NZ, ...
+//   racc [3][rad_len]  radial partial rows of lane groups 1..3
+//   trip [64][16]      per-triplet factors: R_d (NA), F_z (NZ), pair offset
+// Radial: lane (q = lane/16, k = lane%16) adds 0.25 fc exp(-eta (d-R_k)^2) of
+// neighbours e = q, q+4, ... into its own group's copy of column s*nR + k, so
+// no two lanes ever add to one word; the copies are summed at the end.
+// Angular: the angular row is sorted by species, so that in row-major (a<b)
+// order the species pair of consecutive triplets changes at most a few times
+// per a.  Per chunk of 64 triplets, lane t evaluates R_d and F_z of triplet t;
+// then lane (h = lane/32, c = d*NZ + z = lane%32) walks the triplets
+// h, h+2, ... of the chunk, keeps a running sum of R_d F_z for the current
+// pair in a register and adds it to acc[rad_len + pair*32 + c] when the pair
+// changes (the two halves may meet on one word: LDS float adds).
 template <int NA, int NZ, typename TOut>
 __global__ void __launch_bounds__(64 * kWaves)
 k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
@@ -144,14 +150,16 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const int lane = fnl_lane();
   const int i = blockIdx.x * kWaves + w;
   if (i >= n_atoms) return;
-  const int words = fwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
+  const int words = fwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len, dm.rad_len);
+  const int rad4 = (dm.rad_len + 3) & ~3;
   float* base = (float*)lds4 + (size_t)w * words;
   float4* nbv = (float4*)base;
   float2* nbf = (float2*)(base + 4 * max_nbr);
   float4* av = (float4*)(base + 6 * max_nbr);
   float2* af = (float2*)(base + 6 * max_nbr + 4 * max_nbr_ang);
   float* acc = base + 6 * max_nbr + 6 * max_nbr_ang;
-  float* trip = acc + ((dm.aev_len + 3) & ~3);
+  float* racc = acc + ((dm.aev_len + 3) & ~3);
+  float* trip = racc + 3 * rad4;
 
   TOut* out = aev + (size_t)i * stride;
   const int si = species_idx[i];
@@ -166,6 +174,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const int na = min(n_nbr_ang[i], max_nbr_ang);
 
   for (int c = lane; c < dm.aev_len; c += 64) acc[c] = 0.0f;
+  for (int c = lane; c < 3 * rad4; c += 64) racc[c] = 0.0f;
   {
     const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
     const int32_t* row = nbr + (size_t)i * max_nbr;
@@ -183,26 +192,46 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   // radial channels
   {
     const int q = lane >> 4, k = lane & 15;
+    float* dst = q == 0 ? acc : racc + (q - 1) * rad4;
     if (k < dm.nR) {
       const float shift = p.radial_shifts[k];
       for (int e = q; e < nr; e += 4) {
         const float d = nbv[e].w;
         const float2 f = nbf[e];
         const float x = d - shift;
-        lds_add(acc + __float_as_int(f.y) * dm.nR + k, f.x * __expf(-p.radial_eta * x * x));
+        lds_add(dst + __float_as_int(f.y) * dm.nR + k, f.x * __expf(-p.radial_eta * x * x));
       }
     }
   }
 
-  // angular rows (unit vectors of the radial slots listed in the angular row)
+  // angular rows: unit vectors of the radial slots listed in the angular row, stably sorted by
+  // species (position = #{species < s} + #{earlier slots of species s}, counted with ballots)
   {
     const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
-    for (int a = lane; a < na; a += 64) {
-      const int e = arow[a];
-      const float4 v = nbv[e];
-      const float inv = 1.0f / fmaxf(v.w, 1.0e-5f);
-      av[a] = make_float4(v.x * inv, v.y * inv, v.z * inv, v.w);
-      af[a] = make_float2(cos_switch(v.w, p.cutoff_ang), nbf[e].y);
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int ca = 0; ca < na; ca += 64) {
+      const int a = ca + lane;
+      int e = 0, sa = dm.S;
+      if (a < na) {
+        e = arow[a];
+        sa = __float_as_int(nbf[e].y);
+      }
+      int pos = 0;
+      for (int cb = 0; cb < na; cb += 64) {
+        const int b = cb + lane;
+        const int sb = b < na ? __float_as_int(nbf[arow[b]].y) : dm.S;
+        for (int s = 0; s < dm.S; ++s) {
+          const uint64_t m = __ballot(sb == s);
+          if (s < sa) pos += __popcll(m);
+          else if (s == sa) pos += cb < ca ? __popcll(m) : (cb == ca ? __popcll(m & below) : 0);
+        }
+      }
+      if (a < na) {
+        const float4 v = nbv[e];
+        const float inv = 1.0f / fmaxf(v.w, 1.0e-5f);
+        av[pos] = make_float4(v.x * inv, v.y * inv, v.z * inv, v.w);
+        af[pos] = make_float2(cos_switch(v.w, p.cutoff_ang), __int_as_float(sa));
+      }
     }
   }
   wave_sync();
@@ -210,8 +239,10 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const float pref = 2.0f * exp2f(-p.zeta);  // 2 * 0.5^zeta (ani.py:112)
   const int T = na * (na - 1) / 2;
   float* aacc = acc + dm.rad_len;
-  constexpr int G = 64 / NZ;  // triplet groups in the accumulation phase
-  const int g = lane / NZ, z = lane % NZ;
+  const int h = lane >> 5, ch = lane & 31;
+  const int dd = ch / NZ, zz = ch % NZ;
+  float run = 0.0f;
+  int cur = -1;
   for (int base_t = 0; base_t < T; base_t += 64) {
     const int t = base_t + lane;
     float* tr = trip + lane * 16;
@@ -231,28 +262,30 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
         tr[d] = __expf(-p.angular_eta * y * y);
       }
 #pragma unroll
-      for (int zz = 0; zz < NZ; ++zz) {
-        const float bz = 1.0f + x * p.ang_cos[zz] + st * p.ang_sin[zz];
-        tr[NA + zz] = bz > 0.0f ? sw * exp2f(p.zeta * __log2f(bz)) : 0.0f;
+      for (int z = 0; z < NZ; ++z) {
+        const float bz = 1.0f + x * p.ang_cos[z] + st * p.ang_sin[z];
+        tr[NA + z] = bz > 0.0f ? sw * exp2f(p.zeta * __log2f(bz)) : 0.0f;
       }
       tr[NA + NZ] = __int_as_float(pair_index(__float_as_int(fa.y), __float_as_int(fb.y), dm.S) * 32);
-    } else {
-      tr[NA + NZ] = __int_as_float(-1);
     }
     wave_sync();
     const int nt = min(64, T - base_t);
-    for (int tt = g; tt < nt; tt += G) {
-      const float* q = trip + tt * 16;
+    for (int j = h; j < nt; j += 2) {
+      const float* q = trip + j * 16;
       const int po = __float_as_int(q[NA + NZ]);
-      const float f = q[NA + z];
-      float R[NA];
-      *(float4*)R = *(const float4*)q;
-      if (NA == 8) *(float4*)(R + 4) = *(const float4*)(q + 4);
-#pragma unroll
-      for (int d = 0; d < NA; ++d) lds_add(aacc + po + d * NZ + z, R[d] * f);
+      if (po != cur) {
+        if (cur >= 0) lds_add(aacc + cur + ch, run);
+        run = 0.0f;
+        cur = po;
+      }
+      run += q[dd] * q[NA + zz];
     }
     wave_sync();
   }
+  if (cur >= 0) lds_add(aacc + cur + ch, run);
+  wave_sync();
+  for (int c = lane; c < dm.rad_len; c += 64) acc[c] += (racc[c] + racc[rad4 + c]) + racc[2 * rad4 + c];
+  wave_sync();
 
   // the row: acc (radial | angular) then zero padding
   if constexpr (sizeof(TOut) == 4) {
@@ -375,9 +408,12 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const float pref = 2.0f * exp2f(-p.zeta);
   const int T = na * (na - 1) / 2;
   const float* Gang = G + dm.rad_len;
+  // triplets in diagonal order (b - a = 1, then 2, ...): the 64 triplets a wave holds at once
+  // share a slot only across diagonals, which bounds the LDS adds that meet on one gs word
   for (int t = lane; t < T; t += 64) {
-    int a, b;
-    tri_pair(t, na, a, b);
+    int r, q;
+    tri_pair(t, na, r, q);
+    const int a = q - r - 1, b = q;
     const float4 ua = av[a], ub = av[b];
     const float4 fa = af[a], fb = af[b];
     const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
@@ -645,7 +681,7 @@ int launch_fwd(const float* coords, const int32_t* species_idx, const int32_t* b
                const int32_t* n_nbr, int max_nbr, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
                int max_nbr_ang, const fnl_aev_params& p, const AevDims& dm, void* aev, int stride,
                int out_dtype, hipStream_t s) {
-  const size_t lds = (size_t)kWaves * 4 * fwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
+  const size_t lds = (size_t)kWaves * 4 * fwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len, dm.rad_len);
   if (lds > 160 * 1024) return FNL_ERR_UNSUPPORTED;
   dim3 grid(fnl_blocks(n_atoms, kWaves)), block(64 * kWaves);
   const void* kf = out_dtype == FNL_F32 ? reinterpret_cast<const void*>(&k_aev_fwd<NA, NZ, float>)

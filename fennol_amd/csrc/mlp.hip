@@ -32,6 +32,7 @@ typedef unsigned short us4 __attribute__((ext_vector_type(4)));
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
 constexpr int H1 = FNL_MLP_H1, H2 = FNL_MLP_H2, H3 = FNL_MLP_H3;
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
 struct NetPtrs {
   const void* fw[3];  // A operand of the forward GEMMs
@@ -230,8 +231,8 @@ template <typename T, int M_, int PF_>
 __global__ void __launch_bounds__(256, (Lds<T, M_>::BLOCKS_PER_CU))
 k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int32_t* __restrict__ perm,
               const int32_t* __restrict__ tile_start, const int32_t* __restrict__ tile_rows,
-              const int32_t* __restrict__ tile_net, float* __restrict__ energy, float* __restrict__ grad,
-              int grad_stride) {
+              const int32_t* __restrict__ tile_net, int n_tiles, float* __restrict__ energy,
+              float* __restrict__ grad, int grad_stride) {
   typedef Tr<T> tr;
   typedef Lds<T, M_> L;
   constexpr int M = M_;
@@ -249,7 +250,11 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   const int t = threadIdx.x;
   const int wave = t >> 6;
   const int l = fnl_lane();
-  const int tile = blockIdx.x;
+  // Workgroups go to the XCDs round-robin (workgroup b -> XCD b % 8).  Each XCD takes a contiguous
+  // range of the network-sorted tile list, so its L2 holds the weights of one or two networks.
+  const int per_xcd = (n_tiles + kXcds - 1) / kXcds;
+  const int tile = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (tile >= n_tiles) return;
   const int net = tile_net[tile];
   const int nrows = tile_rows[tile];
   const int r0 = tile_start[tile];
@@ -479,20 +484,23 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
   }
   hipStream_t st = (hipStream_t)stream;
   const int M = w->tile_rows > 0 ? w->tile_rows : fnl_mlp_tile_rows(w->dtype);
+  const int grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
   if (bf && M == 64) {
-    hipLaunchKernelGGL((k_species_mlp<unsigned short, 64, 2>), dim3(n_tiles), dim3(256), 0, st,
-                       (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, energy,
-                       grad, grad_stride);
+    hipLaunchKernelGGL((k_species_mlp<unsigned short, 64, 2>), dim3(grid), dim3(256), 0, st,
+                       (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
+                       energy, grad, grad_stride);
   } else if (bf && M == 32) {
-    hipLaunchKernelGGL((k_species_mlp<unsigned short, 32, 2>), dim3(n_tiles), dim3(256), 0, st,
-                       (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, energy,
-                       grad, grad_stride);
+    hipLaunchKernelGGL((k_species_mlp<unsigned short, 32, 2>), dim3(grid), dim3(256), 0, st,
+                       (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
+                       energy, grad, grad_stride);
   } else if (!bf && M == 32) {
-    hipLaunchKernelGGL((k_species_mlp<float, 32, 4>), dim3(n_tiles), dim3(256), 0, st, (const float*)aev,
-                       aev_stride, a, perm, tile_start, tile_rows, tile_net, energy, grad, grad_stride);
+    hipLaunchKernelGGL((k_species_mlp<float, 32, 4>), dim3(grid), dim3(256), 0, st, (const float*)aev,
+                       aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles, energy, grad,
+                       grad_stride);
   } else if (!bf && M == 16) {
-    hipLaunchKernelGGL((k_species_mlp<float, 16, 4>), dim3(n_tiles), dim3(256), 0, st, (const float*)aev,
-                       aev_stride, a, perm, tile_start, tile_rows, tile_net, energy, grad, grad_stride);
+    hipLaunchKernelGGL((k_species_mlp<float, 16, 4>), dim3(grid), dim3(256), 0, st, (const float*)aev,
+                       aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles, energy, grad,
+                       grad_stride);
   } else {
     return FNL_ERR_UNSUPPORTED;
   }

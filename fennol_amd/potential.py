@@ -506,10 +506,18 @@ class ANIPotential:
             P(sb.tile_net), sb.n_tiles, P(sb.e_atom), P(sb.g_aev) if with_grad else None, sb.g_aev.shape[1], s)
         _abi.check(rc, "fnl_species_mlp")
 
-    def aev_backward(self, sb: SystemBuffers, coords32: torch.Tensor, stream=None) -> None:
+    def zero_grad_nbr(self, sb: SystemBuffers, stream=None) -> None:
+        """Clear the neighbour-gradient rows the adjoint accumulates into (a fill kernel)."""
+        _abi.check(self.lib.fnl_fill_f32(_abi.ptr(sb.grad_nbr), 0.0, sb.grad_nbr.numel(),
+                                         _abi.stream_handle(stream)), "fnl_fill_f32")
+
+    def aev_backward(self, sb: SystemBuffers, coords32: torch.Tensor, stream=None, zero: bool = True) -> None:
+        """dE/dx from dE/dAEV: grad_self (centre terms) and grad_nbr (neighbour terms, accumulated
+        with atomics; cleared first unless ``zero`` is False), virial per atom."""
+        if zero:
+            self.zero_grad_nbr(sb, stream)
         s = _abi.stream_handle(stream)
         P = _abi.ptr
-        _abi.check(self.lib.fnl_fill_f32(P(sb.grad_nbr), 0.0, sb.grad_nbr.numel(), s), "fnl_fill_f32")
         rc = self.lib.fnl_aev_backward(
             P(coords32), P(sb.species_idx), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
             P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(sb.nbr_ang), P(sb.n_nbr_ang), sb.max_nbr_ang,
@@ -538,8 +546,10 @@ class ANIPotential:
         with self._stage("mlp"):
             self.mlp(sb, with_grad=forces, stream=stream)
         if forces:
+            with self._stage("zero_grad"):
+                self.zero_grad_nbr(sb, stream)
             with self._stage("aev_bwd"):
-                self.aev_backward(sb, coords32, stream)
+                self.aev_backward(sb, coords32, stream, zero=False)
         with self._stage("reduce"):
             self.reduce(sb, with_virial=forces, stream=stream)
 

@@ -7,10 +7,11 @@ one device-resident pool of atoms:
 
 * every iteration the engine admits waiting requests (FIFO) while the pool's
   atom and system capacity -- its HBM budget -- allows, packs the active
-  requests as consecutive systems (segment offsets), moves the positions and
-  velocities of continuing requests to their new rows on the device, then runs
-  one batched step: BAOAB for the MD requests whose forces are known, the
-  neighbour lists, AEV, MLP, forces and FP64 per-system energies for all;
+  requests as consecutive systems (segment offsets), moves the positions,
+  velocities and last gradients of continuing requests to their new rows on
+  the device, then runs one batched step: BAOAB for the MD requests whose
+  forces are known, the neighbour lists, AEV, MLP, forces and FP64 per-system
+  energies for all;
 * a single energy+force request finishes after its first evaluation; an MD
   request after ``md_steps`` Langevin steps;
 * per-request state (x, v in FP64, step counter, noise key) stays on the
@@ -136,7 +137,8 @@ class ServingEngine:
         if src:
             s_idx = torch.from_numpy(np.concatenate(src)).to(self.dev)
             d_idx = torch.from_numpy(np.concatenate(dst)).to(self.dev)
-            for t in (self.x, self.v):
+            # the next BAOAB kick reads the gradient rows of the last evaluation, so they move too
+            for t in (self.x, self.v, self.sb.grad_self, self.sb.grad_nbr):
                 t[d_idx] = t[s_idx].clone()
         # new requests: positions, velocities ~ N(0, kT/m) for MD requests
         n_cont = len(self.active)

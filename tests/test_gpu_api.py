@@ -52,7 +52,8 @@ def test_fennix_energy_forces_virial_match_oracle(gpu):
     # gradient function (MD's energy_and_gradient)
     gfun = model.get_gradient_function("coordinates", "strain")
     eg, de, _ = gfun(model.variables, {"species": s["species"], "coordinates": s["coordinates"]})
-    np.testing.assert_allclose(de["coordinates"], -f)
+    # forces are summed over neighbours with float atomics: equal to float32 rounding only
+    np.testing.assert_allclose(de["coordinates"], -f, rtol=1e-5, atol=1e-6)
     out2 = model(species=s["species"], coordinates=s["coordinates"])
     g = O.build_graphs(x32, np.array([64]), p)
     np.testing.assert_allclose(out2["embedding"], O.ani_aev(x32, s["species"], g, p), rtol=2e-5, atol=2e-5)
@@ -115,7 +116,11 @@ def test_langevin_temperature(gpu):
 
     s = water_box_small(216, 9)
     cfg = ANIConfig(species_order=("H", "C", "N", "O", "S", "F", "Cl"))
-    pot = ANIPotential(cfg, O.init_mlp_params(11, cfg.aev_length), precision="bf16")
+    # a random-init network is far from equilibrium at the lattice start and keeps releasing
+    # potential energy; scaled down 20x the thermostat, not the release, sets the temperature
+    params = {s_: [(W, b) for W, b in layers[:-1]] + [(0.05 * layers[-1][0], layers[-1][1])]
+              for s_, layers in O.init_mlp_params(11, cfg.aev_length).items()}
+    pot = ANIPotential(cfg, params, precision="bf16")
     md = LangevinMD(pot, s, dt_fs=0.5, temperature=300.0, gamma_thz=10.0, seed=5, check_every=0)
     temps = []
     for _ in range(40):
@@ -160,8 +165,10 @@ def test_md_graph_replay_matches_eager(gpu):
         runs.append((traj, md.istep))
     (te, ne), (tg, ng) = runs
     assert ne == ng == 8
+    # the neighbour force scatter uses float atomics, so runs agree to rounding (~1e-9 A after
+    # 8 steps); a repeated or skipped noise draw would move atoms by ~1e-3 A
     for a, b in zip(te, tg):
-        np.testing.assert_allclose(a, b, rtol=0, atol=1e-9)
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-6)
 
 
 def test_serving_results_same_alone_and_in_mix(gpu):
