@@ -13,14 +13,12 @@
 //
 // Forward (per centre i):
 //   1. radial rows: neighbour j, distance, 0.25*fc, species, vector -> LDS
-//   2. radial channels: lanes = 4 neighbour groups x 16 radial shifts, each
-//      lane owns one LDS accumulator column (no atomics, fixed order)
-//   3. angular rows: the radial slots listed in the angular row
-//   4. triplets (a<b of the angular row) in chunks of 64: one lane per
-//      triplet evaluates the n_Z angle and n_A distance factors into LDS; the
-//      wave then re-maps to (half, channel) lanes that accumulate the 32
-//      channels of their half's 32 triplets into species-pair columns
-//   5. the row (radial | angular | zero pad) is written coalesced
+//   2. angular rows: the radial slots listed in the angular row, sorted by
+//      species -> LDS
+//   3. radial and angular channel blocks as one-hot products on MFMA: the
+//      K index runs over neighbours (triplets), the rows over species
+//      (species pairs), the columns over radial shifts (d, z channels)
+//   4. the row is written from the accumulators (plus the zero padding)
 // Adjoint: the same terms are recomputed; dE/dvec per radial slot is
 // accumulated in LDS (radial part by owner lanes, angular part with LDS
 // float atomics), then scattered: grad_self[i] = -sum_e g_e (plain store),
@@ -31,6 +29,8 @@
 #include "fnl_common.h"
 
 namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;
 constexpr float kPi = 3.14159265358979323846f;
@@ -109,33 +109,92 @@ __device__ __forceinline__ int pair_index(int a, int b, int S) {
   return lo * S - (lo * (lo - 1)) / 2 + (hi - lo);
 }
 
-// per-wave LDS footprint (in 4-byte words) of the forward kernel
-__host__ __device__ inline int fwd_lds_words(int max_nbr, int max_nbr_ang, int aev_len, int rad_len) {
-  return 6 * max_nbr + 6 * max_nbr_ang + ((aev_len + 3) & ~3) + 3 * ((rad_len + 3) & ~3) + 64 * 16;
+constexpr int kFacLd = 65;     // row stride of the forward triplet-factor table (odd: no bank conflicts)
+constexpr int kTabWords = 96;  // per-wave table: 2 x 8 species entries + 2 x 36 pair-segment entries
+
+__host__ __device__ inline int round4(int n) { return (n + 3) & ~3; }
+
+// per-wave LDS layout of the forward kernel (offsets in 4-byte words, 16-byte aligned)
+struct FwdLayout {
+  int nbs, rdf, av, af, fac, tab, words;
+};
+__host__ __device__ inline FwdLayout fwd_layout(int max_nbr, int max_nbr_ang, int na_nz) {
+  FwdLayout L;
+  L.nbs = 4 * max_nbr;
+  L.rdf = round4(L.nbs + max_nbr);
+  L.av = round4(L.rdf + 2 * max_nbr);
+  L.af = L.av + 4 * max_nbr_ang;
+  L.fac = round4(L.af + 2 * max_nbr_ang);
+  L.tab = L.fac + na_nz * kFacLd;
+  L.words = round4(L.tab + kTabWords);
+  return L;
 }
 
 __device__ __forceinline__ void lds_add(float* p, float v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Stable counting sort of a row of n entries by species (values < S, S <= FNL_MAX_SPECIES),
+// with ballots: on return cnt[s] / off[s] hold the per-species counts and start positions
+// (wave-uniform), and pos_of(e) is called by the lane that owns entry e with its sorted position.
+template <typename SpecFn, typename PlaceFn>
+__device__ __forceinline__ void species_sort(int n, int S, SpecFn spec, PlaceFn place,
+                                             int (&cnt)[FNL_MAX_SPECIES], int (&off)[FNL_MAX_SPECIES]) {
+  const int lane = fnl_lane();
+  const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int s = 0; s < FNL_MAX_SPECIES; ++s) cnt[s] = 0;
+  for (int c = 0; c < n; c += 64) {
+    const int sb = c + lane < n ? spec(c + lane) : S;
+#pragma unroll
+    for (int s = 0; s < FNL_MAX_SPECIES; ++s)
+      if (s < S) cnt[s] += __popcll(__ballot(sb == s));
+  }
+  int acc = 0;
+#pragma unroll
+  for (int s = 0; s < FNL_MAX_SPECIES; ++s) {
+    off[s] = acc;
+    acc += cnt[s];
+  }
+  int run[FNL_MAX_SPECIES];
+#pragma unroll
+  for (int s = 0; s < FNL_MAX_SPECIES; ++s) run[s] = off[s];
+  for (int c = 0; c < n; c += 64) {
+    const int e = c + lane;
+    const int sa = e < n ? spec(e) : S;
+    int pos = 0;
+#pragma unroll
+    for (int s = 0; s < FNL_MAX_SPECIES; ++s) {
+      if (s < S) {
+        const uint64_t m = __ballot(sa == s);
+        if (sa == s) pos = run[s] + __popcll(m & below);
+        run[s] += __popcll(m);
+      }
+    }
+    if (e < n) place(e, pos, sa);
+  }
+}
+
 // Forward, one wave per centre.  LDS slice of the wave (4-byte words):
-//   nbv  [max_nbr]     float4 (vx, vy, vz, d) per radial slot
-//   nbf  [max_nbr]     float2 (0.25 fc(d), species)
-//   av   [max_nbr_ang] float4 (ux, uy, uz, d) per angular slot, sorted by species
-//   af   [max_nbr_ang] float2 (fc_ang(d), species), same order
-//   acc  [aev_len]     the AEV row being accumulated, in output channel order
-//   racc [3][rad_len]  radial partial rows of lane groups 1..3
-//   trip [64][16]      per-triplet factors: R_d (NA), F_z (NZ), pair offset
-// Radial: lane (q = lane/16, k = lane%16) adds 0.25 fc exp(-eta (d-R_k)^2) of
-// neighbours e = q, q+4, ... into its own group's copy of column s*nR + k, so
-// no two lanes ever add to one word; the copies are summed at the end.
-// Angular: the angular row is sorted by species, so that in row-major (a<b)
-// order the species pair of consecutive triplets changes at most a few times
-// per a.  Per chunk of 64 triplets, lane t evaluates R_d and F_z of triplet t;
-// then lane (h = lane/32, c = d*NZ + z = lane%32) walks the triplets
-// h, h+2, ... of the chunk, keeps a running sum of R_d F_z for the current
-// pair in a register and adds it to acc[rad_len + pair*32 + c] when the pair
-// changes (the two halves may meet on one word: LDS float adds).
+//   nbv [max_nbr]      float4 (vx, vy, vz, d) per radial slot (slot order)
+//   nbs [max_nbr]      neighbour species per radial slot
+//   rdf [max_nbr]      float2 (d, 0.25 fc(d)), radial slots sorted by species
+//   av  [max_nbr_ang]  float4 (ux, uy, uz, d), angular slots sorted by species
+//   af  [max_nbr_ang]  float2 (fc_ang(d), species), same order
+//   fac [NA+NZ][65]    per-triplet factors R_d and F_z of one chunk of 64
+//                      triplets, component-major (lane t owns column t)
+//   tab [96]           species counts and offsets, species-pair segments
+// Radial: lanes (q = lane/16, k = lane%16); for each species s the lanes sum
+// 0.25 fc exp(-eta (d - R_k)^2) over neighbours q, q+4, ... of the species-s
+// segment in a register, reduce over q with two shuffles and store channel
+// s*nR + k.  Angular: triplets are enumerated pair-major -- all triplets of
+// species pair p (a from the s1 segment, b from the s2 segment, a < b) are
+// consecutive -- so the sum over a pair's triplets is a branch-free loop.
+// Per chunk of 64 triplets, lane t evaluates R_d (NA) and F_z (NZ) of
+// triplet t into fac; then lane (h = lane/32, c = d*NZ + z = lane%32) adds
+// R_d F_z of the triplets h, h+2, ... of each pair segment in the chunk to a
+// register; when a segment ends, the halves combine (one shuffle) and the 32
+// channels of the pair are stored.  Every channel is written exactly once.
 template <int NA, int NZ, typename TOut>
 __global__ void __launch_bounds__(64 * kWaves)
 k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
@@ -144,22 +203,21 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           const int32_t* __restrict__ nbr, const int32_t* __restrict__ n_nbr, int max_nbr,
           const int32_t* __restrict__ nbr_ang, const int32_t* __restrict__ n_nbr_ang, int max_nbr_ang,
           fnl_aev_params p, AevDims dm, TOut* __restrict__ aev, int stride) {
-  static_assert(NA * NZ == 32 && (NA == 4 || NA == 8), "angular channel layout");
+  static_assert(NA * NZ == 32, "angular channel layout");
   extern __shared__ float4 lds4[];
   const int w = threadIdx.x >> 6;
   const int lane = fnl_lane();
   const int i = blockIdx.x * kWaves + w;
   if (i >= n_atoms) return;
-  const int words = fwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len, dm.rad_len);
-  const int rad4 = (dm.rad_len + 3) & ~3;
-  float* base = (float*)lds4 + (size_t)w * words;
+  const FwdLayout L = fwd_layout(max_nbr, max_nbr_ang, NA + NZ);
+  float* base = (float*)lds4 + (size_t)w * L.words;
   float4* nbv = (float4*)base;
-  float2* nbf = (float2*)(base + 4 * max_nbr);
-  float4* av = (float4*)(base + 6 * max_nbr);
-  float2* af = (float2*)(base + 6 * max_nbr + 4 * max_nbr_ang);
-  float* acc = base + 6 * max_nbr + 6 * max_nbr_ang;
-  float* racc = acc + ((dm.aev_len + 3) & ~3);
-  float* trip = racc + 3 * rad4;
+  int* nbs = (int*)(base + L.nbs);
+  float2* rdf = (float2*)(base + L.rdf);
+  float4* av = (float4*)(base + L.av);
+  float2* af = (float2*)(base + L.af);
+  float* fac = base + L.fac;
+  int* tab = (int*)(base + L.tab);
 
   TOut* out = aev + (size_t)i * stride;
   const int si = species_idx[i];
@@ -172,9 +230,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   if (pbc) fnl_load_cell(cells, rec, batch_index[i], cl);
   const int nr = min(n_nbr[i], max_nbr);
   const int na = min(n_nbr_ang[i], max_nbr_ang);
-
-  for (int c = lane; c < dm.aev_len; c += 64) acc[c] = 0.0f;
-  for (int c = lane; c < 3 * rad4; c += 64) racc[c] = 0.0f;
+  const int S = dm.S;
   {
     const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
     const int32_t* row = nbr + (size_t)i * max_nbr;
@@ -184,139 +240,181 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       if (pbc) fnl_min_image(vx, vy, vz, cl);
       const float d = sqrtf(fmaxf(vx * vx + vy * vy + vz * vz, 1.0e-5f));
       nbv[e] = make_float4(vx, vy, vz, d);
-      nbf[e] = make_float2(0.25f * cos_switch(d, p.cutoff), __int_as_float(species_idx[j]));
+      nbs[e] = species_idx[j];
     }
   }
   wave_sync();
 
-  // radial channels
-  {
-    const int q = lane >> 4, k = lane & 15;
-    float* dst = q == 0 ? acc : racc + (q - 1) * rad4;
-    if (k < dm.nR) {
-      const float shift = p.radial_shifts[k];
-      for (int e = q; e < nr; e += 4) {
+  int cnt[FNL_MAX_SPECIES], off[FNL_MAX_SPECIES];
+  // radial slots sorted by species
+  species_sort(
+      nr, S, [&](int e) { return nbs[e]; },
+      [&](int e, int pos, int) {
         const float d = nbv[e].w;
-        const float2 f = nbf[e];
-        const float x = d - shift;
-        lds_add(dst + __float_as_int(f.y) * dm.nR + k, f.x * __expf(-p.radial_eta * x * x));
+        rdf[pos] = make_float2(d, 0.25f * cos_switch(d, p.cutoff));
+      },
+      cnt, off);
+  wave_sync();
+
+  const int q = lane >> 4, k = lane & 15;
+  {
+    const float shift = k < dm.nR ? p.radial_shifts[k] : 0.0f;
+#pragma unroll
+    for (int s = 0; s < FNL_MAX_SPECIES; ++s) {
+      if (s >= S) break;
+      const int lo = off[s], hi = off[s] + cnt[s];
+      float run = 0.0f;
+#pragma unroll 4
+      for (int e = lo + q; e < hi; e += 4) {
+        const float2 df = rdf[e];
+        const float x = df.x - shift;
+        run += df.y * __expf(-p.radial_eta * x * x);
       }
+      run += __shfl_xor(run, 16, 64);
+      run += __shfl_xor(run, 32, 64);
+      if (q == 0 && k < dm.nR) out[s * dm.nR + k] = to_out<TOut>(run);
     }
   }
 
-  // angular rows: unit vectors of the radial slots listed in the angular row, stably sorted by
-  // species (position = #{species < s} + #{earlier slots of species s}, counted with ballots)
+  // angular slots sorted by species: unit vector, distance, switch
   {
     const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
-    const uint64_t below = (1ull << lane) - 1ull;
-    for (int ca = 0; ca < na; ca += 64) {
-      const int a = ca + lane;
-      int e = 0, sa = dm.S;
-      if (a < na) {
-        e = arow[a];
-        sa = __float_as_int(nbf[e].y);
-      }
-      int pos = 0;
-      for (int cb = 0; cb < na; cb += 64) {
-        const int b = cb + lane;
-        const int sb = b < na ? __float_as_int(nbf[arow[b]].y) : dm.S;
-        for (int s = 0; s < dm.S; ++s) {
-          const uint64_t m = __ballot(sb == s);
-          if (s < sa) pos += __popcll(m);
-          else if (s == sa) pos += cb < ca ? __popcll(m) : (cb == ca ? __popcll(m & below) : 0);
+    species_sort(
+        na, S, [&](int a) { return nbs[arow[a]]; },
+        [&](int a, int pos, int sa) {
+          const float4 v = nbv[arow[a]];
+          const float inv = 1.0f / fmaxf(v.w, 1.0e-5f);
+          av[pos] = make_float4(v.x * inv, v.y * inv, v.z * inv, v.w);
+          af[pos] = make_float2(cos_switch(v.w, p.cutoff_ang), __int_as_float(sa));
+        },
+        cnt, off);
+  }
+  // species-pair segments of the pair-major triplet order: tab[0..7] counts, tab[8..15]
+  // offsets of the species in the sorted angular row, tab[16+2k], tab[17+2k] = (first triplet,
+  // pair) of the k-th non-empty pair, in pair order
+  if (lane == 0) {
+#pragma unroll
+    for (int s = 0; s < FNL_MAX_SPECIES; ++s) {
+      tab[s] = cnt[s];
+      tab[8 + s] = off[s];
+    }
+  }
+  wave_sync();
+  const int P = dm.P;
+  int npairs;
+  {
+    int s1 = 0, s2 = 0, np = 0;
+    if (lane < P) {
+      int r0 = 0;
+      for (int a = 0; a < S; ++a) {
+        if (lane < r0 + (S - a)) {
+          s1 = a;
+          s2 = a + (lane - r0);
+          break;
         }
+        r0 += S - a;
       }
-      if (a < na) {
-        const float4 v = nbv[e];
-        const float inv = 1.0f / fmaxf(v.w, 1.0e-5f);
-        av[pos] = make_float4(v.x * inv, v.y * inv, v.z * inv, v.w);
-        af[pos] = make_float2(cos_switch(v.w, p.cutoff_ang), __int_as_float(sa));
-      }
+      const int c1 = tab[s1], c2 = tab[s2];
+      np = s1 == s2 ? c1 * (c1 - 1) / 2 : c1 * c2;
+    }
+    const uint64_t nz = __ballot(np > 0);
+    npairs = __popcll(nz);
+    const int lo = fnl_wave_incl_scan(np) - np;
+    if (np > 0) {
+      const int slot = __popcll(nz & ((1ull << lane) - 1ull));
+      tab[16 + 2 * slot] = lo;
+      tab[17 + 2 * slot] = lane;
     }
   }
   wave_sync();
 
   const float pref = 2.0f * exp2f(-p.zeta);  // 2 * 0.5^zeta (ani.py:112)
   const int T = na * (na - 1) / 2;
-  float* aacc = acc + dm.rad_len;
   const int h = lane >> 5, ch = lane & 31;
   const int dd = ch / NZ, zz = ch % NZ;
+  const float* fR = fac + dd * kFacLd;
+  const float* fF = fac + (NA + zz) * kFacLd;
+  TOut* aout = out + dm.rad_len;
   float run = 0.0f;
-  int cur = -1;
-  for (int base_t = 0; base_t < T; base_t += 64) {
-    const int t = base_t + lane;
-    float* tr = trip + lane * 16;
+  int kseg = 0;  // first pair segment not yet finished (uniform)
+  for (int c0 = 0; c0 < T; c0 += 64) {
+    const int t = c0 + lane;
     if (t < T) {
+      // decode triplet t: its pair segment, then (a, b) inside it
+      int ks = 0;
+      for (int kk = 1; kk < npairs; ++kk)
+        if (t >= tab[16 + 2 * kk]) ks = kk;
+      const int u = t - tab[16 + 2 * ks];
+      const int pr = tab[17 + 2 * ks];
+      int s1 = 0, r0 = 0;
+      while (pr >= r0 + (S - s1)) {
+        r0 += S - s1;
+        ++s1;
+      }
+      const int s2 = s1 + (pr - r0);
       int a, b;
-      tri_pair(t, na, a, b);
+      if (s1 == s2) {
+        tri_pair(u, tab[s1], a, b);
+        a += tab[8 + s1];
+        b += tab[8 + s1];
+      } else {
+        const int n2 = tab[s2];
+        a = tab[8 + s1] + u / n2;
+        b = tab[8 + s2] + u % n2;
+      }
       const float4 ua = av[a], ub = av[b];
-      const float2 fa = af[a], fb = af[b];
       const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
       const float x = 0.95f * c;
       const float st = sqrtf(fmaxf(1.0f - x * x, 0.0f));
-      const float sw = pref * fa.x * fb.x;
+      const float sw = pref * af[a].x * af[b].x;
       const float m = 0.5f * (ua.w + ub.w);
 #pragma unroll
       for (int d = 0; d < NA; ++d) {
         const float y = m - p.ang_shifts[d];
-        tr[d] = __expf(-p.angular_eta * y * y);
+        fac[d * kFacLd + lane] = __expf(-p.angular_eta * y * y);
       }
 #pragma unroll
       for (int z = 0; z < NZ; ++z) {
         const float bz = 1.0f + x * p.ang_cos[z] + st * p.ang_sin[z];
-        tr[NA + z] = bz > 0.0f ? sw * exp2f(p.zeta * __log2f(bz)) : 0.0f;
+        fac[(NA + z) * kFacLd + lane] = bz > 0.0f ? sw * exp2f(p.zeta * __log2f(bz)) : 0.0f;
       }
-      tr[NA + NZ] = __int_as_float(pair_index(__float_as_int(fa.y), __float_as_int(fb.y), dm.S) * 32);
     }
     wave_sync();
-    const int nt = min(64, T - base_t);
-    for (int j = h; j < nt; j += 2) {
-      const float* q = trip + j * 16;
-      const int po = __float_as_int(q[NA + NZ]);
-      if (po != cur) {
-        if (cur >= 0) lds_add(aacc + cur + ch, run);
-        run = 0.0f;
-        cur = po;
-      }
-      run += q[dd] * q[NA + zz];
+    const int c1 = min(c0 + 64, T);
+    while (kseg < npairs) {
+      const int slo = tab[16 + 2 * kseg];
+      const int shi = kseg + 1 < npairs ? tab[16 + 2 * (kseg + 1)] : T;
+      const int lo = max(slo, c0), hi = min(shi, c1);
+#pragma unroll 4
+      for (int tt = lo + h; tt < hi; tt += 2) run += fR[tt - c0] * fF[tt - c0];
+      if (shi > c1) break;  // the segment continues in the next chunk
+      const float v = run + __shfl_xor(run, 32, 64);
+      if (h == 0) aout[tab[17 + 2 * kseg] * 32 + ch] = to_out<TOut>(v);
+      run = 0.0f;
+      ++kseg;
     }
     wave_sync();
   }
-  if (cur >= 0) lds_add(aacc + cur + ch, run);
-  wave_sync();
-  for (int c = lane; c < dm.rad_len; c += 64) acc[c] += (racc[c] + racc[rad4 + c]) + racc[2 * rad4 + c];
-  wave_sync();
-
-  // the row: acc (radial | angular) then zero padding
-  if constexpr (sizeof(TOut) == 4) {
-    for (int c = 4 * lane; c < stride; c += 256) {
-      float4 v;
-      v.x = c < dm.aev_len ? acc[c] : 0.0f;
-      v.y = c + 1 < dm.aev_len ? acc[c + 1] : 0.0f;
-      v.z = c + 2 < dm.aev_len ? acc[c + 2] : 0.0f;
-      v.w = c + 3 < dm.aev_len ? acc[c + 3] : 0.0f;
-      if (c + 3 < stride) {
-        *(float4*)(out + c) = v;
-      } else {
-        for (int k = 0; k < 4 && c + k < stride; ++k) out[c + k] = (&v.x)[k];
-      }
-    }
-  } else {
-    for (int c = 4 * lane; c < stride; c += 256) {
-      TOut o[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) o[k] = to_out<TOut>(c + k < dm.aev_len ? acc[c + k] : 0.0f);
-      if (c + 3 < stride) {
-        *(uint2*)(out + c) = *(const uint2*)o;
-      } else {
-        for (int k = 0; k < 4 && c + k < stride; ++k) out[c + k] = o[k];
+  // pairs without triplets and the zero padding
+  {
+    int seg = 0;
+    for (int pr = 0; pr < P; ++pr) {
+      if (seg < npairs && tab[17 + 2 * seg] == pr) {
+        ++seg;
+      } else if (h == 0) {
+        aout[pr * 32 + ch] = to_out<TOut>(0.0f);
       }
     }
   }
+  for (int c = dm.aev_len + lane; c < stride; c += 64) out[c] = to_out<TOut>(0.0f);
 }
 
+// per-wave LDS footprint (4-byte words) of the adjoint kernel; gs starts 16-byte aligned
+__host__ __device__ inline int bwd_gs_offset(int max_nbr, int aev_len) {
+  return round4(round4(aev_len) + 6 * max_nbr);
+}
 __host__ __device__ inline int bwd_lds_words(int max_nbr, int max_nbr_ang, int aev_len) {
-  return ((aev_len + 3) & ~3) + 10 * max_nbr + 8 * max_nbr_ang;
+  return bwd_gs_offset(max_nbr, aev_len) + 4 * max_nbr + 8 * max_nbr_ang;
 }
 
 // Adjoint, one wave per centre.  LDS slice of the wave (4-byte words):
@@ -346,7 +444,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   float* G = (float*)lds4 + (size_t)w * words;
   float4* nbv = (float4*)(G + ((dm.aev_len + 3) & ~3));
   float2* nbf = (float2*)(nbv + max_nbr);
-  float4* gs = (float4*)((float*)nbf + 2 * max_nbr);
+  float4* gs = (float4*)(G + bwd_gs_offset(max_nbr, dm.aev_len));
   float4* av = gs + max_nbr;
   float4* af = av + max_nbr_ang;
 
@@ -510,7 +608,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
 // Per triplet the tangent of K fc_a fc_b F_z(c) R_d(m) is
 //   F_z (dsw R_d + sw R_d' dm) + F_z' (sw R_d dc) = F_z P_d + F_z' Q_d.
 __host__ __device__ inline int tan_lds_words(int max_nbr, int max_nbr_ang, int aev_len) {
-  return 10 * max_nbr + 12 * max_nbr_ang + ((aev_len + 3) & ~3) + 64 * 32;
+  return round4(10 * max_nbr) + 12 * max_nbr_ang + round4(aev_len) + 64 * 32;
 }
 
 template <int NA, int NZ>
@@ -532,7 +630,7 @@ k_aev_tangent(const float* __restrict__ coords, const float* __restrict__ disp,
   float4* nbv = (float4*)base;                        // vx vy vz d
   float4* nbt = nbv + max_nbr;                        // dvec (x y z), dd
   float2* nbf = (float2*)(nbt + max_nbr);             // fc, species
-  float4* av = (float4*)((float*)nbf + 2 * max_nbr);  // u, d
+  float4* av = (float4*)(base + round4(10 * max_nbr));  // u, d
   float4* at = av + max_nbr_ang;                      // du, dd
   float4* af = at + max_nbr_ang;                      // fc_a, fc_a', species
   float* acc = (float*)(af + max_nbr_ang);
@@ -681,7 +779,7 @@ int launch_fwd(const float* coords, const int32_t* species_idx, const int32_t* b
                const int32_t* n_nbr, int max_nbr, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
                int max_nbr_ang, const fnl_aev_params& p, const AevDims& dm, void* aev, int stride,
                int out_dtype, hipStream_t s) {
-  const size_t lds = (size_t)kWaves * 4 * fwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len, dm.rad_len);
+  const size_t lds = (size_t)kWaves * 4 * fwd_layout(max_nbr, max_nbr_ang, p.n_ang_dist + p.n_ang_sect).words;
   if (lds > 160 * 1024) return FNL_ERR_UNSUPPORTED;
   dim3 grid(fnl_blocks(n_atoms, kWaves)), block(64 * kWaves);
   const void* kf = out_dtype == FNL_F32 ? reinterpret_cast<const void*>(&k_aev_fwd<NA, NZ, float>)
