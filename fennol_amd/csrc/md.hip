@@ -61,22 +61,38 @@ k_reduce_segments(const int32_t* __restrict__ sys_offsets, const int32_t* __rest
   if (t < kRedVals) partials[(size_t)sgi * kRedVals + t] = red[t][0];
 }
 
-__global__ void k_reduce_systems(const int32_t* __restrict__ seg_offsets, int n_sys,
-                                 const double* __restrict__ partials, double* __restrict__ energy_sys,
-                                 double* __restrict__ virial_sys, double* __restrict__ kinetic_sys) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per system: lane k sums the partials of segments k, k+64, ...; then a fixed
+// xor-butterfly over the wave (deterministic order).
+__global__ void __launch_bounds__(64) k_reduce_systems(const int32_t* __restrict__ seg_offsets, int n_sys,
+                                                       const double* __restrict__ partials,
+                                                       double* __restrict__ energy_sys, double* __restrict__ virial_sys,
+                                                       double* __restrict__ kinetic_sys) {
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
   if (b >= n_sys) return;
   double acc[kRedVals];
 #pragma unroll
   for (int k = 0; k < kRedVals; ++k) acc[k] = 0.0;
-  for (int s = seg_offsets[b]; s < seg_offsets[b + 1]; ++s)
+  for (int s = seg_offsets[b] + lane; s < seg_offsets[b + 1]; s += 64)
 #pragma unroll
     for (int k = 0; k < kRedVals; ++k) acc[k] += partials[(size_t)s * kRedVals + k];
-  if (energy_sys) energy_sys[b] = acc[0];
-  if (virial_sys)
-    for (int k = 0; k < 9; ++k) virial_sys[9 * b + k] = acc[1 + k];
-  if (kinetic_sys)
-    for (int k = 0; k < 9; ++k) kinetic_sys[9 * b + k] = acc[10 + k];
+#pragma unroll
+  for (int k = 0; k < kRedVals; ++k)
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+  if (lane == 0 && energy_sys) energy_sys[b] = acc[0];
+  if (lane < 9) {
+    // lane k writes component k (the values are wave-uniform after the butterfly)
+    double vk = 0.0, kk = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      if (lane == k) {
+        vk = acc[1 + k];
+        kk = acc[10 + k];
+      }
+    if (virial_sys) virial_sys[9 * b + lane] = vk;
+    if (kinetic_sys) kinetic_sys[9 * b + lane] = kk;
+  }
 }
 
 __global__ void k_forces(const float* __restrict__ gs, const float* __restrict__ gn, int n3, float scale,
@@ -173,7 +189,7 @@ extern "C" int fnl_system_reduce(const int32_t* sys_offsets, const int32_t* seg_
                        seg_atoms, energy_atom, virial_atom, vel, mass, partials);
     FNL_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_reduce_systems, dim3(fnl_blocks(n_sys, 64)), dim3(64), 0, st, seg_offsets, n_sys, partials,
+  hipLaunchKernelGGL(k_reduce_systems, dim3(n_sys), dim3(64), 0, st, seg_offsets, n_sys, partials,
                      energy_atom ? energy_sys : nullptr, virial_atom ? virial_sys : nullptr,
                      vel ? kinetic_sys : nullptr);
   FNL_LAUNCH_CHECK();

@@ -9,7 +9,8 @@
 // cutoff wide.
 //
 // Pipeline (all on the caller's stream):
-//   k_grid      one workgroup per system: grid dims (+ bounding box if open)
+//   k_grid      one workgroup per system: grid dims (+ bounding box if open);
+//               clears the cell counters
 //   k_bin       one thread per atom: cell id, population count
 //   k_scan      one workgroup: exclusive scan of cell populations
 //   k_scatter   one thread per atom: place atom in its cell segment
@@ -18,7 +19,8 @@
 //   k_pairs     one wave per centre, centres in cell order: candidates of the
 //               neighbouring cells (contiguous z-runs of the cell-sorted
 //               arrays) tested 64 at a time, kept pairs compacted with a wave
-//               ballot + prefix popcount into the centre's row.
+//               ballot + prefix popcount into the centre's row
+//   k_row_max   one workgroup: row maxima into the status words
 
 #include "fnl_common.h"
 
@@ -89,10 +91,15 @@ __device__ __forceinline__ int cells_along(float width, float rc, int cap) {
 __global__ void k_grid(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
                        const int32_t* __restrict__ sys_offsets, int pbc_mode,
                        const float* __restrict__ rec, const int32_t* __restrict__ grid_cap,
-                       float cutoff, int32_t* __restrict__ grid, int32_t* __restrict__ status) {
+                       float cutoff, int32_t* __restrict__ grid, int32_t* __restrict__ status,
+                       int32_t* __restrict__ cell_count, int32_t* __restrict__ cell_fill, int n_cells) {
   const int b = blockIdx.x;
   const int t = threadIdx.x;
   if (b == 0 && t < FNL_STATUS_STICKY) status[t] = 0;  // the sticky word persists
+  for (int c = b * blockDim.x + t; c < n_cells; c += gridDim.x * blockDim.x) {
+    cell_count[c] = 0;
+    cell_fill[c] = 0;
+  }
   const int capx = grid_cap[b * 4 + 0], capy = grid_cap[b * 4 + 1], capz = grid_cap[b * 4 + 2];
   const int first = grid_cap[b * 4 + 3];
   int32_t* g = grid + b * kGridWords;
@@ -326,9 +333,15 @@ __device__ __forceinline__ int z_runs(int cz, int nz, bool pbc, int* lo, int* hi
 }
 
 // One wave per centre (centres in cell order).  The candidates are the atoms
-// of up to 18 contiguous ranges (3x3 columns x z-runs); each range is scanned
-// 64 candidates at a time, kept pairs are compacted with a ballot and a prefix
-// popcount into the radial row and, for d12 < rca^2, the angular row.
+// of up to 18 contiguous ranges (3x3 columns x z-runs) of the cell-sorted
+// arrays, concatenated into one list (a wave prefix sum over the range
+// lengths held in lanes; candidate v finds its range by a binary search over
+// the shuffled offsets), so every pass tests 64 candidates, and the next
+// pass's positions are loaded before the current pass is compacted.  Kept
+// pairs are compacted with a ballot and a prefix popcount into the radial
+// row and, for d12 < rca^2, the angular row, in candidate order.  The row
+// maxima are reduced afterwards by k_row_max (a same-address atomic per
+// centre serialises at the device's coherence point).
 template <int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
 k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_index,
@@ -355,16 +368,30 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
   const int noy = axis_offsets(cy, ny, pbc, oy);
   int zlo[2], zhi[2];
   const int nzr = z_runs(cz, nz, pbc, zlo, zhi);
-  // lane r < nox*noy*nzr holds range r
-  int r_lo = 0, r_hi = 0;
+  // lane r < nox*noy*nzr holds range r: start and length
+  int r_lo = 0, r_len = 0;
   const int nrange = nox * noy * nzr;
   if (lane < nrange) {
     const int zr = lane % nzr, xy = lane / nzr;
     const int a = xy / noy, bb = xy % noy;
     const int col = first + ((cx + ox[a]) * ny + (cy + oy[bb])) * nz;
     r_lo = cell_start[col + zlo[zr]];
-    r_hi = cell_start[col + zhi[zr] + 1];
+    r_len = cell_start[col + zhi[zr] + 1] - r_lo;
   }
+  const int r_end = fnl_wave_incl_scan(r_len);
+  const int r_off = r_end - r_len;
+  const int total = __shfl(r_end, 63, 64);
+  // position in the cell-sorted arrays of candidate v < total
+  auto candidate = [&](int v) {
+    int r = 0;
+#pragma unroll
+    for (int step = 16; step > 0; step >>= 1) {
+      const int c = r + step;
+      const int o = __shfl(r_off, c < 64 ? c : 63, 64);
+      if (c < nrange && o <= v) r = c;
+    }
+    return __shfl(r_lo, r, 64) + (v - __shfl(r_off, r, 64));
+  };
 
   FnlCell cl;
   if (pbc) fnl_load_cell(cells, rec, b, cl);
@@ -372,36 +399,39 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
   int count = 0, count_a = 0;
   int32_t* row = nbr + (size_t)i * max_nbr;
   int32_t* row_a = nbr_ang + (size_t)i * max_nbr_ang;
-  for (int r = 0; r < nrange; ++r) {
-    const int lo = __shfl(r_lo, r, 64), hi = __shfl(r_hi, r, 64);
-    for (int base = lo; base < hi; base += 64) {
-      const int p = base + lane;
-      bool keep = false, keep_a = false;
-      int j = -1;
-      if (p < hi) {
-        const float4 pj = pos[p];
-        j = __float_as_int(pj.w);
-        if (j != i) {
-          float vx = pj.x - xi, vy = pj.y - yi, vz = pj.z - zi;
-          if (pbc) fnl_min_image(vx, vy, vz, cl);
-          const float d12 = fnl_d12(vx, vy, vz);
-          keep = d12 < rc2;
-          keep_a = d12 < rca2;
-        }
-      }
-      const unsigned long long m = __ballot(keep);
-      const unsigned long long ma = __ballot(keep_a);
-      if (keep) {
-        const int e = count + fnl_rank_below(m);
-        if (e < max_nbr) row[e] = j;
-        if (keep_a) {
-          const int ea = count_a + fnl_rank_below(ma);
-          if (ea < max_nbr_ang && e < max_nbr) row_a[ea] = e;
-        }
-      }
-      count += __popcll(m);
-      count_a += __popcll(ma);
+  float4 next = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(i));
+  {
+    const int pc = candidate(lane);
+    if (lane < total) next = pos[pc];
+  }
+  for (int base = 0; base < total; base += 64) {
+    const float4 pj = next;
+    const bool valid = base + lane < total;
+    if (base + 64 < total) {
+      const int pc = candidate(base + 64 + lane);
+      if (base + 64 + lane < total) next = pos[pc];
     }
+    bool keep = false, keep_a = false;
+    const int j = __float_as_int(pj.w);
+    if (valid && j != i) {
+      float vx = pj.x - xi, vy = pj.y - yi, vz = pj.z - zi;
+      if (pbc) fnl_min_image(vx, vy, vz, cl);
+      const float d12 = fnl_d12(vx, vy, vz);
+      keep = d12 < rc2;
+      keep_a = d12 < rca2;
+    }
+    const unsigned long long m = __ballot(keep);
+    const unsigned long long ma = __ballot(keep_a);
+    if (keep) {
+      const int e = count + fnl_rank_below(m);
+      if (e < max_nbr) row[e] = j;
+      if (keep_a) {
+        const int ea = count_a + fnl_rank_below(ma);
+        if (ea < max_nbr_ang && e < max_nbr) row_a[ea] = e;
+      }
+    }
+    count += __popcll(m);
+    count_a += __popcll(ma);
   }
   if (lane == 0) {
     n_nbr[i] = count;
@@ -410,12 +440,34 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
       atomicOr(&status[FNL_STATUS_OVERFLOW], 1);
       atomicOr(&status[FNL_STATUS_STICKY], 1);
     }
-    // Same-address atomics from every centre serialise in one L2 channel:
-    // only centres above the running maximum (a relaxed read) update it.
-    if (count > __hip_atomic_load(&status[FNL_STATUS_MAX_NBR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-      atomicMax(&status[FNL_STATUS_MAX_NBR], count);
-    if (count_a > __hip_atomic_load(&status[FNL_STATUS_MAX_NBR_ANG], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-      atomicMax(&status[FNL_STATUS_MAX_NBR_ANG], count_a);
+  }
+}
+
+// Row maxima (status words MAX_NBR, MAX_NBR_ANG): one workgroup, plain stores.
+__global__ void __launch_bounds__(1024) k_row_max(const int32_t* __restrict__ n_nbr,
+                                                  const int32_t* __restrict__ n_nbr_ang, int n_atoms,
+                                                  int32_t* __restrict__ status) {
+  __shared__ int sm[2][16];
+  int m = 0, ma = 0;
+  for (int a = threadIdx.x; a < n_atoms; a += 1024) {
+    m = max(m, n_nbr[a]);
+    ma = max(ma, n_nbr_ang[a]);
+  }
+  m = fnl_wave_max_i(m);
+  ma = fnl_wave_max_i(ma);
+  const int w = threadIdx.x >> 6;
+  if (fnl_lane() == 0) {
+    sm[0][w] = m;
+    sm[1][w] = ma;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 16; ++k) {
+      m = max(m, sm[0][k]);
+      ma = max(ma, sm[1][k]);
+    }
+    status[FNL_STATUS_MAX_NBR] = max(m, sm[0][0]);
+    status[FNL_STATUS_MAX_NBR_ANG] = max(ma, sm[1][0]);
   }
 }
 
@@ -515,13 +567,9 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
   const float rca2 = (float)((double)cutoff_ang * (double)cutoff_ang);
 
   hipLaunchKernelGGL(k_grid, dim3(n_sys), dim3(256), 0, s, coords, species_idx, sys_offsets, pbc_mode,
-                     reciprocal_cells, grid, cutoff, ws.grid, status);
+                     reciprocal_cells, grid, cutoff, ws.grid, status, ws.cell_count, ws.cell_fill, n_cells);
   FNL_LAUNCH_CHECK();
   if (n_atoms == 0) return FNL_OK;
-  int rc = fnl_fill_i32(ws.cell_count, 0, (size_t)n_cells, stream);
-  if (rc) return rc;
-  rc = fnl_fill_i32(ws.cell_fill, 0, (size_t)n_cells, stream);
-  if (rc) return rc;
   hipLaunchKernelGGL(k_bin, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, s, coords, species_idx,
                      batch_index, n_atoms, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_count,
                      n_nbr, n_nbr_ang, n_cells, status);
@@ -539,6 +587,8 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
                      n_atoms, cells, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_start, ws.sorted,
                      ws.pos, rc2, rca2, max_nbr, max_nbr_ang, nbr, n_nbr, nbr_ang, n_nbr_ang, status,
                      n_cells);
+  FNL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_row_max, dim3(1), dim3(1024), 0, s, n_nbr, n_nbr_ang, n_atoms, status);
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }
