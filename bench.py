@@ -3,6 +3,8 @@
 One step = BAOAB integrate -> cell-list neighbour lists (rebuilt every step) ->
 AEV -> species MLP (forward + input gradient) -> AEV adjoint (forces, virial)
 -> FP64 per-system reductions, all on one GPU with inputs resident in HBM.
+The timed steps replay a HIP graph of one MD step (one graph launch per step);
+a separate eager run gives the per-stage breakdown.
 
 Prints ONE JSON line.  ``value`` is the BF16-MLP form's atom-steps/s; the
 plain FP32 form, their ratio, the per-step p50/p99 latency, the energy and
@@ -105,7 +107,12 @@ def copy_calibration(mib: int) -> None:
     del a, b
 
 
-def run_md(system, cfg, params, precision, steps, warmup, timed_stages=True, tile_rows=0):
+def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_steps=20):
+    """Warm-up, then a per-stage breakdown (eager steps, HIP events around each stage on the
+    launch stream), then the timed run: one MD step per HIP graph, replayed ``steps`` times back
+    to back (the host enqueues one graph launch per step).  Returns the potential, the MD state,
+    the wall time of the timed steps, the per-step latencies (ms), the stage times and the eager
+    wall time per step."""
     import torch
 
     from fennol_amd.md import LangevinMD
@@ -117,22 +124,31 @@ def run_md(system, cfg, params, precision, steps, warmup, timed_stages=True, til
         md.step()
     torch.cuda.synchronize()
     md.verify()
-    timer = StageTimer() if timed_stages else None
+    timer = StageTimer()
     pot.timer = timer
+    torch.cuda.synchronize()
+    te0 = time.perf_counter()
+    for _ in range(stage_steps):
+        md.step()
+    torch.cuda.synchronize()
+    eager_ms = 1000.0 * (time.perf_counter() - te0) / stage_steps
+    pot.timer = None
+    stages = timer.times_ms()
+    md.capture(steps_per_graph=1)
+    md.replay(1)
+    torch.cuda.synchronize()
     step_ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     step_ev[0].record()
     for k in range(steps):
-        md.step()
+        md.replay(1)
         step_ev[k + 1].record()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    pot.timer = None
     md.verify()
     lat = np.array([step_ev[k].elapsed_time(step_ev[k + 1]) for k in range(steps)])
-    stages = timer.times_ms() if timer else {}
-    return pot, md, (t1 - t0), lat, stages
+    return pot, md, (t1 - t0), lat, stages, eager_ms
 
 
 def accuracy(system, cfg, params, precision, fp64):
@@ -280,13 +296,13 @@ def main() -> int:
     for precision in precisions:
         log(f"[bench] MD {precision}: warmup {args.warmup}, steps {args.steps}")
         tile = args.tile_bf16 if precision == "bf16" else args.tile_fp32
-        pot, md, wall, lat, stages = run_md(system, cfg, params, precision, args.steps, args.warmup,
-                                            tile_rows=tile)
+        pot, md, wall, lat, stages, eager_ms = run_md(system, cfg, params, precision, args.steps, args.warmup,
+                                                      tile_rows=tile)
         units = algorithmic_units(md.sb, cfg, precision)
-        results[precision] = dict(wall=wall, lat=lat, stages=stages, units=units,
+        results[precision] = dict(wall=wall, lat=lat, stages=stages, units=units, eager_ms=eager_ms,
                                   max_nbr=md.sb.max_nbr, max_nbr_ang=md.sb.max_nbr_ang)
-        log(f"[bench]   {precision}: {1000 * wall / args.steps:.3f} ms/step; stages (ms): "
-            + ", ".join(f"{k} {v.mean():.3f}" for k, v in stages.items()))
+        log(f"[bench]   {precision}: {1000 * wall / args.steps:.3f} ms/step (graph replay), eager "
+            f"{eager_ms:.3f} ms/step; stages (ms): " + ", ".join(f"{k} {v.mean():.3f}" for k, v in stages.items()))
         del pot, md
         torch.cuda.empty_cache()
 
@@ -354,8 +370,10 @@ def main() -> int:
                                f"rebuilt every step, BAOAB Langevin dt=0.5 fs 300 K gamma=1/ps",
                    "aev_length": cfg.aev_length, "mlp": "256-192-160-1 CELU(0.1), 4 species networks",
                    "precision": "BF16 MLP GEMMs (FP32 accumulate), FP32 AEV, FP64 reductions and integrator"},
+        "eager_ms_per_step": round(bf["eager_ms"], 4),
         "fp32": None if fp is None else {
             "value": round(N * args.steps / fp["wall"], 1), "ms_per_step": round(1000.0 * fp["wall"] / args.steps, 4),
+            "eager_ms_per_step": round(fp["eager_ms"], 4),
             "p50_ms": round(float(np.percentile(fp["lat"], 50)), 4),
             "p99_ms": round(float(np.percentile(fp["lat"], 99)), 4)},
         "bf16_over_fp32": None if fp is None else round(fp["wall"] / bf["wall"], 3),
