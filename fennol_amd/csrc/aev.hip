@@ -134,6 +134,26 @@ __device__ __forceinline__ void lds_add(float* p, float v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// DPP lane moves (VALU, no LDS traffic): row_ror:n = 0x120 + n, quad_perm = 0x00..0xFF
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+// sum over the wave, returned in every lane: rotations within each row of 16 lanes, then the
+// four row sums read as scalars
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+  v += dpp<0x128>(v);
+  v += dpp<0x124>(v);
+  v += dpp<0x122>(v);
+  v += dpp<0x121>(v);
+  const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+  const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+  const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+  const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+  return (r0 + r1) + (r2 + r3);
+}
+
 // Stable counting sort of a row of n entries by species (values < S, S <= FNL_MAX_SPECIES),
 // with ballots: on return cnt[s] / off[s] hold the per-species counts and start positions
 // (wave-uniform), and pos_of(e) is called by the lane that owns entry e with its sorted position.
@@ -506,67 +526,100 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const float pref = 2.0f * exp2f(-p.zeta);
   const int T = na * (na - 1) / 2;
   const float* Gang = G + dm.rad_len;
-  // triplets in diagonal order (b - a = 1, then 2, ...): the 64 triplets a wave holds at once
-  // share a slot only across diagonals, which bounds the LDS adds that meet on one gs word
-  for (int t = lane; t < T; t += 64) {
-    int r, q;
-    tri_pair(t, na, r, q);
-    const int a = q - r - 1, b = q;
-    const float4 ua = av[a], ub = av[b];
-    const float4 fa = af[a], fb = af[b];
-    const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
-    const float x = 0.95f * c;
-    const float st = sqrtf(fmaxf(1.0f - x * x, 1.0e-12f));
-    const float ist = 1.0f / st;
-    float F[NZ], dF[NZ];
-#pragma unroll
-    for (int z = 0; z < NZ; ++z) {
-      const float bz = 1.0f + x * p.ang_cos[z] + st * p.ang_sin[z];
-      if (bz > 0.0f) {
-        const float lb = __log2f(bz);
-        F[z] = exp2f(p.zeta * lb);
-        // d/dc (1 + cos(theta - theta_z))^zeta with theta = arccos(0.95 c)
-        dF[z] = 0.95f * p.zeta * exp2f((p.zeta - 1.0f) * lb) * (p.ang_cos[z] - x * p.ang_sin[z] * ist);
-      } else {
-        F[z] = 0.0f;
-        dF[z] = 0.0f;
+  // Triplets in 4x4 blocks of the (a, b) table: a row of 16 lanes holds block (A, B), lane c
+  // the pair a = 4A + c%4, b = 4B + c/4 (a < b).  The lanes that share a (c%4 equal) and the
+  // lanes that share b (c/4 equal) pre-sum their dE/dvec terms with DPP moves, so that only one
+  // lane per slot and block adds to gs (LDS float adds cost in proportion to the active lanes).
+  const int nb = (na + 3) >> 2;
+  const int nblk = nb * (nb + 1) / 2;
+  const int cc = lane & 15;
+  for (int kb0 = 0; kb0 < nblk; kb0 += 4) {
+    const int kb = kb0 + (lane >> 4);
+    int A = 0, B = 0;
+    if (kb < nblk) {
+      int r0 = 0;
+      while (kb >= r0 + (nb - A)) {
+        r0 += nb - A;
+        ++A;
       }
+      B = A + (kb - r0);
     }
-    const float m = 0.5f * (ua.w + ub.w);
-    const float* gp = Gang + pair_index(__float_as_int(fa.z), __float_as_int(fb.z), dm.S) * (NA * NZ);
-    float S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
-#pragma unroll
-    for (int d = 0; d < NA; ++d) {
-      const float y = m - p.ang_shifts[d];
-      const float R = __expf(-p.angular_eta * y * y);
-      float gz[NZ];
-#pragma unroll
-      for (int q = 0; q < NZ; q += 4) *(float4*)(gz + q) = *(const float4*)(gp + d * NZ + q);
-      float gf = 0.0f, gdf = 0.0f;
+    const int a = 4 * A + (cc & 3), b = 4 * B + (cc >> 2);
+    const bool valid = kb < nblk && a < b && b < na;
+    float gax = 0.f, gay = 0.f, gaz = 0.f, gbx = 0.f, gby = 0.f, gbz = 0.f;
+    if (valid) {
+      const float4 ua = av[a], ub = av[b];
+      const float4 fa = af[a], fb = af[b];
+      const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
+      const float x = 0.95f * c;
+      const float st = sqrtf(fmaxf(1.0f - x * x, 1.0e-12f));
+      const float ist = 1.0f / st;
+      float F[NZ], dF[NZ];
 #pragma unroll
       for (int z = 0; z < NZ; ++z) {
-        gf += gz[z] * F[z];
-        gdf += gz[z] * dF[z];
+        const float bz = 1.0f + x * p.ang_cos[z] + st * p.ang_sin[z];
+        if (bz > 0.0f) {
+          const float lb = __log2f(bz);
+          F[z] = exp2f(p.zeta * lb);
+          // d/dc (1 + cos(theta - theta_z))^zeta with theta = arccos(0.95 c)
+          dF[z] = 0.95f * p.zeta * exp2f((p.zeta - 1.0f) * lb) * (p.ang_cos[z] - x * p.ang_sin[z] * ist);
+        } else {
+          F[z] = 0.0f;
+          dF[z] = 0.0f;
+        }
       }
-      S0 += gf * R;
-      S1 += gf * R * (-2.0f * p.angular_eta * y);
-      S2 += gdf * R;
+      const float m = 0.5f * (ua.w + ub.w);
+      const float* gp = Gang + pair_index(__float_as_int(fa.z), __float_as_int(fb.z), dm.S) * (NA * NZ);
+      float S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
+#pragma unroll
+      for (int d = 0; d < NA; ++d) {
+        const float y = m - p.ang_shifts[d];
+        const float R = __expf(-p.angular_eta * y * y);
+        float gz[NZ];
+#pragma unroll
+        for (int q = 0; q < NZ; q += 4) *(float4*)(gz + q) = *(const float4*)(gp + d * NZ + q);
+        float gf = 0.0f, gdf = 0.0f;
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) {
+          gf += gz[z] * F[z];
+          gdf += gz[z] * dF[z];
+        }
+        S0 += gf * R;
+        S1 += gf * R * (-2.0f * p.angular_eta * y);
+        S2 += gdf * R;
+      }
+      const float sw = pref * fa.x * fb.x;
+      const float dEda = pref * fa.y * fb.x * S0 + 0.5f * sw * S1;
+      const float dEdb = pref * fa.x * fb.y * S0 + 0.5f * sw * S1;
+      const float dEdc = sw * S2;
+      const int ea = __float_as_int(fa.w), eb = __float_as_int(fb.w);
+      const float ra = (ea >> 30) ? 0.0f : dEda, rb = (eb >> 30) ? 0.0f : dEdb;
+      const float ia = 1.0f / fmaxf(ua.w, 1.0e-5f), ib = 1.0f / fmaxf(ub.w, 1.0e-5f);
+      gax = ra * ua.x + dEdc * (ub.x - c * ua.x) * ia;
+      gay = ra * ua.y + dEdc * (ub.y - c * ua.y) * ia;
+      gaz = ra * ua.z + dEdc * (ub.z - c * ua.z) * ia;
+      gbx = rb * ub.x + dEdc * (ua.x - c * ub.x) * ib;
+      gby = rb * ub.y + dEdc * (ua.y - c * ub.y) * ib;
+      gbz = rb * ub.z + dEdc * (ua.z - c * ub.z) * ib;
     }
-    const float sw = pref * fa.x * fb.x;
-    const float dEda = pref * fa.y * fb.x * S0 + 0.5f * sw * S1;
-    const float dEdb = pref * fa.x * fb.y * S0 + 0.5f * sw * S1;
-    const float dEdc = sw * S2;
-    const int ea = __float_as_int(fa.w), eb = __float_as_int(fb.w);
-    const float ra = (ea >> 30) ? 0.0f : dEda, rb = (eb >> 30) ? 0.0f : dEdb;
-    const float ia = 1.0f / fmaxf(ua.w, 1.0e-5f), ib = 1.0f / fmaxf(ub.w, 1.0e-5f);
-    float* pa = (float*)(gs + (ea & 0x3fffffff));
-    float* pb = (float*)(gs + (eb & 0x3fffffff));
-    lds_add(pa + 0, ra * ua.x + dEdc * (ub.x - c * ua.x) * ia);
-    lds_add(pa + 1, ra * ua.y + dEdc * (ub.y - c * ua.y) * ia);
-    lds_add(pa + 2, ra * ua.z + dEdc * (ub.z - c * ua.z) * ia);
-    lds_add(pb + 0, rb * ub.x + dEdc * (ua.x - c * ub.x) * ib);
-    lds_add(pb + 1, rb * ub.y + dEdc * (ua.y - c * ub.y) * ib);
-    lds_add(pb + 2, rb * ub.z + dEdc * (ua.z - c * ub.z) * ib);
+    // lanes c, c+4, c+8, c+12 share a; lanes 4k..4k+3 share b
+    gax += dpp<0x124>(gax); gay += dpp<0x124>(gay); gaz += dpp<0x124>(gaz);
+    gax += dpp<0x128>(gax); gay += dpp<0x128>(gay); gaz += dpp<0x128>(gaz);
+    gbx += dpp<0xB1>(gbx); gby += dpp<0xB1>(gby); gbz += dpp<0xB1>(gbz);
+    gbx += dpp<0x4E>(gbx); gby += dpp<0x4E>(gby); gbz += dpp<0x4E>(gbz);
+    // one lane per slot and block, and only where the block holds a valid triplet of the slot
+    if (kb < nblk && cc < 4 && max(a + 1, 4 * B) <= min(4 * B + 3, na - 1)) {
+      float* pa = (float*)(gs + (__float_as_int(af[a].w) & 0x3fffffff));
+      lds_add(pa + 0, gax);
+      lds_add(pa + 1, gay);
+      lds_add(pa + 2, gaz);
+    }
+    if (kb < nblk && (cc & 3) == 0 && b < na && b > 4 * A) {
+      float* pb = (float*)(gs + (__float_as_int(af[b].w) & 0x3fffffff));
+      lds_add(pb + 0, gbx);
+      lds_add(pb + 1, gby);
+      lds_add(pb + 2, gbz);
+    }
   }
   wave_sync();
 
@@ -587,11 +640,11 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     v[3] += vv.y * gg.x; v[4] += vv.y * gg.y; v[5] += vv.y * gg.z;
     v[6] += vv.z * gg.x; v[7] += vv.z * gg.y; v[8] += vv.z * gg.z;
   }
-  sx = fnl_wave_sum(sx);
-  sy = fnl_wave_sum(sy);
-  sz = fnl_wave_sum(sz);
+  sx = wave_sum_dpp(sx);
+  sy = wave_sum_dpp(sy);
+  sz = wave_sum_dpp(sz);
 #pragma unroll
-  for (int k = 0; k < 9; ++k) v[k] = fnl_wave_sum(v[k]);
+  for (int k = 0; k < 9; ++k) v[k] = wave_sum_dpp(v[k]);
   if (lane == 0) {
     grad_self[3 * i] = -sx;
     grad_self[3 * i + 1] = -sy;
