@@ -35,10 +35,14 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 constexpr int kWaves = 4;
 constexpr float kPi = 3.14159265358979323846f;
 
+// Each wave owns its LDS slice, so the lanes of one wave only exchange data among
+// themselves: the LDS executes a wave's DS instructions in issue order, and wavefront-scope
+// fences keep the compiler from moving LDS accesses across this point without waiting for
+// outstanding global loads (a workgroup-scope fence would drain them).
 __device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ float cos_switch(float d, float rc) {
@@ -110,13 +114,13 @@ __device__ __forceinline__ int pair_index(int a, int b, int S) {
 }
 
 constexpr int kFacLd = 65;     // row stride of the forward triplet-factor table (odd: no bank conflicts)
-constexpr int kTabWords = 96;  // per-wave table: 2 x 8 species entries + 2 x 36 pair-segment entries
+constexpr int kTabWords = 16;  // per-wave table: species counts and offsets
 
 __host__ __device__ inline int round4(int n) { return (n + 3) & ~3; }
 
 // per-wave LDS layout of the forward kernel (offsets in 4-byte words, 16-byte aligned)
 struct FwdLayout {
-  int nbs, rdf, av, af, fac, tab, words;
+  int nbs, rdf, av, af, fac, tab, ae, words;
 };
 __host__ __device__ inline FwdLayout fwd_layout(int max_nbr, int max_nbr_ang, int na_nz) {
   FwdLayout L;
@@ -126,7 +130,8 @@ __host__ __device__ inline FwdLayout fwd_layout(int max_nbr, int max_nbr_ang, in
   L.af = L.av + 4 * max_nbr_ang;
   L.fac = round4(L.af + 2 * max_nbr_ang);
   L.tab = L.fac + na_nz * kFacLd;
-  L.words = round4(L.tab + kTabWords);
+  L.ae = L.tab + kTabWords;
+  L.words = round4(L.ae + max_nbr_ang);
   return L;
 }
 
@@ -162,6 +167,23 @@ __device__ __forceinline__ void species_sort(int n, int S, SpecFn spec, PlaceFn 
                                              int (&cnt)[FNL_MAX_SPECIES], int (&off)[FNL_MAX_SPECIES]) {
   const int lane = fnl_lane();
   const uint64_t below = (1ull << lane) - 1ull;
+  if (n <= 64) {  // one ballot per species serves both the counts and the ranks
+    const int sa = lane < n ? spec(lane) : S;
+    int pos = 0, acc = 0;
+#pragma unroll
+    for (int s = 0; s < FNL_MAX_SPECIES; ++s) {
+      cnt[s] = 0;
+      off[s] = acc;
+      if (s < S) {
+        const uint64_t m = __ballot(sa == s);
+        cnt[s] = __popcll(m);
+        if (sa == s) pos = acc + __popcll(m & below);
+        acc += cnt[s];
+      }
+    }
+    if (lane < n) place(lane, pos, sa);
+    return;
+  }
 #pragma unroll
   for (int s = 0; s < FNL_MAX_SPECIES; ++s) cnt[s] = 0;
   for (int c = 0; c < n; c += 64) {
@@ -203,7 +225,7 @@ __device__ __forceinline__ void species_sort(int n, int S, SpecFn spec, PlaceFn 
 //   af  [max_nbr_ang]  float2 (fc_ang(d), species), same order
 //   fac [NA+NZ][65]    per-triplet factors R_d and F_z of one chunk of 64
 //                      triplets, component-major (lane t owns column t)
-//   tab [96]           species counts and offsets, species-pair segments
+//   tab [16]           species counts and offsets in the sorted angular row
 // Radial: lanes (q = lane/16, k = lane%16); for each species s the lanes sum
 // 0.25 fc exp(-eta (d - R_k)^2) over neighbours q, q+4, ... of the species-s
 // segment in a register, reduce over q with two shuffles and store channel
@@ -238,6 +260,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   float2* af = (float2*)(base + L.af);
   float* fac = base + L.fac;
   int* tab = (int*)(base + L.tab);
+  int* ae = (int*)(base + L.ae);
 
   TOut* out = aev + (size_t)i * stride;
   const int si = species_idx[i];
@@ -262,6 +285,9 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       nbv[e] = make_float4(vx, vy, vz, d);
       nbs[e] = species_idx[j];
     }
+    // the angular row (radial slot numbers), fetched in the same round of loads
+    const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
+    for (int a = lane; a < na; a += 64) ae[a] = arow[a];
   }
   wave_sync();
 
@@ -298,20 +324,18 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
 
   // angular slots sorted by species: unit vector, distance, switch
   {
-    const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
     species_sort(
-        na, S, [&](int a) { return nbs[arow[a]]; },
+        na, S, [&](int a) { return nbs[ae[a]]; },
         [&](int a, int pos, int sa) {
-          const float4 v = nbv[arow[a]];
+          const float4 v = nbv[ae[a]];
           const float inv = 1.0f / fmaxf(v.w, 1.0e-5f);
           av[pos] = make_float4(v.x * inv, v.y * inv, v.z * inv, v.w);
           af[pos] = make_float2(cos_switch(v.w, p.cutoff_ang), __int_as_float(sa));
         },
         cnt, off);
   }
-  // species-pair segments of the pair-major triplet order: tab[0..7] counts, tab[8..15]
-  // offsets of the species in the sorted angular row, tab[16+2k], tab[17+2k] = (first triplet,
-  // pair) of the k-th non-empty pair, in pair order
+  // species pairs: lane p < P holds pair p's species blocks in the sorted angular row (sizes,
+  // offsets) and its triplet count; the present pairs form a wave-uniform bit mask
   if (lane == 0) {
 #pragma unroll
     for (int s = 0; s < FNL_MAX_SPECIES; ++s) {
@@ -321,110 +345,98 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   }
   wave_sync();
   const int P = dm.P;
-  int npairs;
-  {
-    int s1 = 0, s2 = 0, np = 0;
-    if (lane < P) {
-      int r0 = 0;
-      for (int a = 0; a < S; ++a) {
-        if (lane < r0 + (S - a)) {
-          s1 = a;
-          s2 = a + (lane - r0);
-          break;
-        }
-        r0 += S - a;
+  int pn1 = 0, pn2 = 0, po1 = 0, po2 = 0, np = 0;
+  if (lane < P) {
+    int s1 = 0, s2 = 0, r0 = 0;
+    for (int a = 0; a < S; ++a) {
+      if (lane < r0 + (S - a)) {
+        s1 = a;
+        s2 = a + (lane - r0);
+        break;
       }
-      const int c1 = tab[s1], c2 = tab[s2];
-      np = s1 == s2 ? c1 * (c1 - 1) / 2 : c1 * c2;
+      r0 += S - a;
     }
-    const uint64_t nz = __ballot(np > 0);
-    npairs = __popcll(nz);
-    const int lo = fnl_wave_incl_scan(np) - np;
-    if (np > 0) {
-      const int slot = __popcll(nz & ((1ull << lane) - 1ull));
-      tab[16 + 2 * slot] = lo;
-      tab[17 + 2 * slot] = lane;
-    }
+    pn1 = tab[s1];
+    pn2 = tab[s2];
+    po1 = tab[8 + s1];
+    po2 = tab[8 + s2];
+    np = s1 == s2 ? pn1 * (pn1 - 1) / 2 : pn1 * pn2;
   }
-  wave_sync();
+  const uint64_t present = __ballot(np > 0);
 
   const float pref = 2.0f * exp2f(-p.zeta);  // 2 * 0.5^zeta (ani.py:112)
-  const int T = na * (na - 1) / 2;
   const int h = lane >> 5, ch = lane & 31;
   const int dd = ch / NZ, zz = ch % NZ;
   const float* fR = fac + dd * kFacLd;
   const float* fF = fac + (NA + zz) * kFacLd;
   TOut* aout = out + dm.rad_len;
-  float run = 0.0f;
-  int kseg = 0;  // first pair segment not yet finished (uniform)
-  for (int c0 = 0; c0 < T; c0 += 64) {
-    const int t = c0 + lane;
-    if (t < T) {
-      // decode triplet t: its pair segment, then (a, b) inside it
-      int ks = 0;
-      for (int kk = 1; kk < npairs; ++kk)
-        if (t >= tab[16 + 2 * kk]) ks = kk;
-      const int u = t - tab[16 + 2 * ks];
-      const int pr = tab[17 + 2 * ks];
-      int s1 = 0, r0 = 0;
-      while (pr >= r0 + (S - s1)) {
-        r0 += S - s1;
-        ++s1;
-      }
-      const int s2 = s1 + (pr - r0);
-      int a, b;
-      if (s1 == s2) {
-        tri_pair(u, tab[s1], a, b);
-        a += tab[8 + s1];
-        b += tab[8 + s1];
-      } else {
-        const int n2 = tab[s2];
-        a = tab[8 + s1] + u / n2;
-        b = tab[8 + s2] + u % n2;
-      }
-      const float4 ua = av[a], ub = av[b];
-      const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
-      const float x = 0.95f * c;
-      const float st = sqrtf(fmaxf(1.0f - x * x, 0.0f));
-      const float sw = pref * af[a].x * af[b].x;
-      const float m = 0.5f * (ua.w + ub.w);
-#pragma unroll
-      for (int d = 0; d < NA; ++d) {
-        const float y = m - p.ang_shifts[d];
-        fac[d * kFacLd + lane] = __expf(-p.angular_eta * y * y);
-      }
-#pragma unroll
-      for (int z = 0; z < NZ; ++z) {
-        const float bz = 1.0f + x * p.ang_cos[z] + st * p.ang_sin[z];
-        fac[(NA + z) * kFacLd + lane] = bz > 0.0f ? sw * exp2f(p.zeta * __log2f(bz)) : 0.0f;
-      }
-    }
-    wave_sync();
-    const int c1 = min(c0 + 64, T);
-    while (kseg < npairs) {
-      const int slo = tab[16 + 2 * kseg];
-      const int shi = kseg + 1 < npairs ? tab[16 + 2 * (kseg + 1)] : T;
-      const int lo = max(slo, c0), hi = min(shi, c1);
-#pragma unroll 4
-      for (int tt = lo + h; tt < hi; tt += 2) run += fR[tt - c0] * fF[tt - c0];
-      if (shi > c1) break;  // the segment continues in the next chunk
-      const float v = run + __shfl_xor(run, 32, 64);
-      if (h == 0) aout[tab[17 + 2 * kseg] * 32 + ch] = to_out<TOut>(v);
-      run = 0.0f;
-      ++kseg;
-    }
-    wave_sync();
-  }
-  // pairs without triplets and the zero padding
+  // pairs without triplets: zero channels, two pairs per instruction
   {
-    int seg = 0;
-    for (int pr = 0; pr < P; ++pr) {
-      if (seg < npairs && tab[17 + 2 * seg] == pr) {
-        ++seg;
-      } else if (h == 0) {
-        aout[pr * 32 + ch] = to_out<TOut>(0.0f);
+    uint64_t zm = ~present & (P >= 64 ? ~0ull : ((1ull << P) - 1ull));
+    while (zm) {
+      const int p0 = __builtin_ctzll(zm);
+      zm &= zm - 1ull;
+      int p1 = -1;
+      if (zm) {
+        p1 = __builtin_ctzll(zm);
+        zm &= zm - 1ull;
       }
+      const int pw = h == 0 ? p0 : p1;
+      if (pw >= 0) aout[pw * 32 + ch] = to_out<TOut>(0.0f);
     }
+  }
+  // present pairs, one at a time: chunks of 64 triplets (a from block 1, b from block 2,
+  // a < b); lane t evaluates triplet t into fac, then lane (h, c) sums R_d F_z over the
+  // chunk's triplets h, h+2, ... in a register
+  uint64_t pm = present;
+  while (pm) {
+    const int pr = __builtin_ctzll(pm);
+    pm &= pm - 1ull;
+    const int n1 = __builtin_amdgcn_readlane(pn1, pr), n2 = __builtin_amdgcn_readlane(pn2, pr);
+    const int o1 = __builtin_amdgcn_readlane(po1, pr), o2 = __builtin_amdgcn_readlane(po2, pr);
+    const int npp = __builtin_amdgcn_readlane(np, pr);
+    const float rn2 = 1.0f / (float)n2;
+    float run = 0.0f;
+    for (int u0 = 0; u0 < npp; u0 += 64) {
+      const int u = u0 + lane;
+      if (u < npp) {
+        int a, b;
+        if (o1 == o2) {
+          tri_pair(u, n1, a, b);
+          a += o1;
+          b += o1;
+        } else {
+          int qd = (int)((float)u * rn2);
+          if (qd * n2 > u) --qd;
+          if ((qd + 1) * n2 <= u) ++qd;
+          a = o1 + qd;
+          b = o2 + (u - qd * n2);
+        }
+        const float4 ua = av[a], ub = av[b];
+        const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
+        const float x = 0.95f * c;
+        const float st = sqrtf(fmaxf(1.0f - x * x, 0.0f));
+        const float sw = pref * af[a].x * af[b].x;
+        const float m = 0.5f * (ua.w + ub.w);
+#pragma unroll
+        for (int d = 0; d < NA; ++d) {
+          const float y = m - p.ang_shifts[d];
+          fac[d * kFacLd + lane] = __expf(-p.angular_eta * y * y);
+        }
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) {
+          const float bz = 1.0f + x * p.ang_cos[z] + st * p.ang_sin[z];
+          fac[(NA + z) * kFacLd + lane] = bz > 0.0f ? sw * exp2f(p.zeta * __log2f(bz)) : 0.0f;
+        }
+      }
+      wave_sync();
+      const int nt = min(64, npp - u0);
+#pragma unroll 4
+      for (int tt = h; tt < nt; tt += 2) run += fR[tt] * fF[tt];
+      wave_sync();
+    }
+    const float v = run + __shfl_xor(run, 32, 64);
+    if (h == 0) aout[pr * 32 + ch] = to_out<TOut>(v);
   }
   for (int c = dm.aev_len + lane; c < stride; c += 64) out[c] = to_out<TOut>(0.0f);
 }
@@ -479,6 +491,9 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   if (pbc) fnl_load_cell(cells, rec, batch_index[i], cl);
   const int nr = min(n_nbr[i], max_nbr);
   const int na = min(n_nbr_ang[i], max_nbr_ang);
+  // the first 64 entries of the angular row, fetched with the dE/dAEV and radial rows
+  const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
+  const int ae0 = lane < na ? arow[lane] : 0;
   {
     const float4* grow = (const float4*)(g + (size_t)i * g_stride);
     for (int c = lane; c < (dm.aev_len + 3) / 4; c += 64) ((float4*)G)[c] = grow[c];
@@ -510,9 +525,8 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   }
   wave_sync();
   {
-    const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
     for (int a = lane; a < na; a += 64) {
-      const int e = arow[a];
+      const int e = a < 64 ? ae0 : arow[a];
       const float4 v = nbv[e];
       const float inv = 1.0f / fmaxf(v.w, 1.0e-5f);
       av[a] = make_float4(v.x * inv, v.y * inv, v.z * inv, v.w);
