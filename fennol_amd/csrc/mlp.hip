@@ -30,6 +30,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned short us4 __attribute__((ext_vector_type(4)));
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 
 constexpr int H1 = FNL_MLP_H1, H2 = FNL_MLP_H2, H3 = FNL_MLP_H3;
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
@@ -153,12 +154,17 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
                                        const T* act, int lda, int K) {
   typedef Tr<T> tr;
   const int nsteps = K / tr::KS;
+  // Every prefetch load is issued unconditionally (k-step and n-block clamped into range), so
+  // that the compiler can count the loads in flight and wait only for the one it needs; a
+  // conditional load makes it wait for all of them.
+  int ncol[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) ncol[nb] = min(nb0 + 4 * nb, ncb - 1) * 16;
   typename tr::frag wq[PF][NB];
 #pragma unroll
   for (int u = 0; u < PF; ++u)
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb)
-      if (u < nsteps && nb0 + 4 * nb < ncb) wq[u][nb] = tr::wload(W, ldw, (nb0 + 4 * nb) * 16, u * tr::KS);
+    for (int nb = 0; nb < NB; ++nb) wq[u][nb] = tr::wload(W, ldw, ncol[nb], min(u, nsteps - 1) * tr::KS);
   for (int s0 = 0; s0 < nsteps; s0 += PF) {
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
@@ -167,13 +173,14 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
         typename tr::frag b[MB];
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(act, lda, mb * 16, s * tr::KS);
+        const int sn = min(s + PF, nsteps - 1);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
           if (nb0 + 4 * nb < ncb) {
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb) acc[nb][mb] = tr::mma(wq[u][nb], b[mb], acc[nb][mb]);
-            if (s + PF < nsteps) wq[u][nb] = tr::wload(W, ldw, (nb0 + 4 * nb) * 16, (s + PF) * tr::KS);
           }
+          wq[u][nb] = tr::wload(W, ldw, ncol[nb], sn * tr::KS);
         }
       }
     }
@@ -269,22 +276,25 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   constexpr int NPASS = M / RPP;
   static_assert(M % RPP == 0, "staging passes");
   int srow[NPASS];
+  const int row_safe = perm[r0];
   const int scol = (t % TPR) * VEC;
 #pragma unroll
   for (int p = 0; p < NPASS; ++p) {
     const int row = p * RPP + t / TPR;
     srow[p] = row < nrows ? perm[r0 + row] : -1;
   }
-  uint4 xr[NPASS];
+  u4v xr[NPASS];
   auto stage_load = [&](int kc) {
 #pragma unroll
-    for (int p = 0; p < NPASS; ++p)
-      xr[p] = srow[p] >= 0 ? *(const uint4*)(aev + (size_t)srow[p] * aev_stride + kc + scol)
-                           : make_uint4(0u, 0u, 0u, 0u);
+    for (int p = 0; p < NPASS; ++p) {
+      const int rr = srow[p] >= 0 ? srow[p] : row_safe;  // a real row of the tile: the load is always issued
+      const u4v v = __builtin_nontemporal_load((const u4v*)(aev + (size_t)rr * aev_stride + kc + scol));
+      xr[p] = srow[p] >= 0 ? v : u4v{0u, 0u, 0u, 0u};
+    }
   };
   auto stage_store = [&](T* buf) {
 #pragma unroll
-    for (int p = 0; p < NPASS; ++p) *(uint4*)(buf + (p * RPP + t / TPR) * L::LDX + scol) = xr[p];
+    for (int p = 0; p < NPASS; ++p) *(u4v*)(buf + (p * RPP + t / TPR) * L::LDX + scol) = xr[p];
   };
 
   f4 acc1[4][MB];
@@ -318,7 +328,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
           for (int nb = 0; nb < 4; ++nb) {
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb) acc1[nb][mb] = tr::mma(wq[u][nb], b[mb], acc1[nb][mb]);
-            if (s + PF < nsteps) wq[u][nb] = tr::wload(W, ldw, (wave + 4 * nb) * 16, (s + PF) * tr::KS);
+            wq[u][nb] = tr::wload(W, ldw, (wave + 4 * nb) * 16, min(s + PF, nsteps - 1) * tr::KS);
           }
         }
       }
@@ -428,7 +438,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
       const int n = cb * 16 + 4 * (l >> 4);
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb)
-        if (atom[mb] >= 0) *(f4*)(grad + (size_t)atom[mb] * grad_stride + n) = accx[nb][mb];
+        if (atom[mb] >= 0) __builtin_nontemporal_store(accx[nb][mb], (f4*)(grad + (size_t)atom[mb] * grad_stride + n));
     }
   }
 }
@@ -498,7 +508,7 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
                        aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles, energy, grad,
                        grad_stride);
   } else if (!bf && M == 16) {
-    hipLaunchKernelGGL((k_species_mlp<float, 16, 4>), dim3(grid), dim3(256), 0, st, (const float*)aev,
+    hipLaunchKernelGGL((k_species_mlp<float, 16, 2>), dim3(grid), dim3(256), 0, st, (const float*)aev,
                        aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles, energy, grad,
                        grad_stride);
   } else {
