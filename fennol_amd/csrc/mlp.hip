@@ -131,7 +131,7 @@ struct Lds {
   static constexpr int BYTES = OFFE + 4 * M * 4;
   static constexpr int XBUF = M * LDX * (int)sizeof(T);  // one staging buffer, at OFF2
   static_assert(2 * XBUF <= OFFE - OFF2, "layer-1 staging must fit in the H2/H3 buffers");
-  static_assert(BYTES <= 80 * 1024, "at least two workgroups per CU");
+  static_assert(BYTES <= 160 * 1024, "fits the LDS of one CU");
   // workgroups per CU allowed by the LDS footprint, at most 3 for BF16 (the
   // prefetch ring then keeps <= 168 VGPRs per lane without spills)
   static constexpr int BLOCKS_LDS = (160 * 1024) / BYTES;
@@ -165,14 +165,18 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
   for (int u = 0; u < PF; ++u)
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) wq[u][nb] = tr::wload(W, ldw, ncol[nb], min(u, nsteps - 1) * tr::KS);
+  typename tr::frag b[MB];
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(act, lda, mb * 16, 0);
   for (int s0 = 0; s0 < nsteps; s0 += PF) {
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       const int s = s0 + u;
       if (s < nsteps) {
-        typename tr::frag b[MB];
+        // activations of the next k-step, read before this step's MFMAs
+        typename tr::frag bn[MB];
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(act, lda, mb * 16, s * tr::KS);
+        for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(act, lda, mb * 16, min(s + 1, nsteps - 1) * tr::KS);
         const int sn = min(s + PF, nsteps - 1);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
@@ -182,6 +186,8 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
           }
           wq[u][nb] = tr::wload(W, ldw, ncol[nb], sn * tr::KS);
         }
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) b[mb] = bn[mb];
       }
     }
   }
@@ -316,20 +322,27 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     const int nchunks = K / KC;
     for (int c = 0; c < nchunks; ++c) {
       const T* X = (c & 1) ? Xs1 : Xs0;
-      if (c + 1 < nchunks) stage_load((c + 1) * KC);
+      stage_load(min(c + 1, nchunks - 1) * KC);  // always issued (see gemm_t)
+      typename tr::frag b[MB];
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(X, L::LDX, mb * 16, 0);
       for (int s0 = 0; s0 < SPC; s0 += PF) {
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
           const int s = c * SPC + s0 + u;  // global k-step
-          typename tr::frag b[MB];
+          // activations of the next k-step of the chunk, read before this step's MFMAs
+          typename tr::frag bn[MB];
+          const int kn = min(s0 + u + 1, SPC - 1) * tr::KS;
 #pragma unroll
-          for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(X, L::LDX, mb * 16, (s0 + u) * tr::KS);
+          for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(X, L::LDX, mb * 16, kn);
 #pragma unroll
           for (int nb = 0; nb < 4; ++nb) {
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb) acc1[nb][mb] = tr::mma(wq[u][nb], b[mb], acc1[nb][mb]);
             wq[u][nb] = tr::wload(W, ldw, (wave + 4 * nb) * 16, min(s + PF, nsteps - 1) * tr::KS);
           }
+#pragma unroll
+          for (int mb = 0; mb < MB; ++mb) b[mb] = bn[mb];
         }
       }
       if (c + 1 < nchunks) stage_store((c & 1) ? Xs0 : Xs1);
@@ -447,7 +460,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
 
 extern "C" int fnl_mlp_tile_rows(int32_t dtype) {
   if (dtype == FNL_F32) return 32;
-  if (dtype == FNL_BF16) return 64;
+  if (dtype == FNL_BF16) return 96;
   return FNL_ERR_INVALID;
 }
 
@@ -497,6 +510,10 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
   const int grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
   if (bf && M == 64) {
     hipLaunchKernelGGL((k_species_mlp<unsigned short, 64, 2>), dim3(grid), dim3(256), 0, st,
+                       (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
+                       energy, grad, grad_stride);
+  } else if (bf && M == 96) {
+    hipLaunchKernelGGL((k_species_mlp<unsigned short, 96, 4>), dim3(grid), dim3(256), 0, st,
                        (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
                        energy, grad, grad_stride);
   } else if (bf && M == 32) {

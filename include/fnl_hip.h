@@ -212,7 +212,8 @@ typedef struct fnl_mlp_weights {
   int32_t dtype;              /* FNL_F32 or FNL_BF16 (weights and GEMM inputs) */
   float celu_alpha;
   int32_t tile_rows;          /* atoms per tile: 0 = fnl_mlp_tile_rows(dtype);
-                                 FP32: 16 or 32, BF16: 32 or 64            */
+                                 FP32: 16 or 32; BF16: 32, 64 or 96 (96:
+                                 one 120 KB workgroup per CU, the default) */
   fnl_mlp_net nets[FNL_MLP_MAX_NETS];
 } fnl_mlp_weights;
 

@@ -191,6 +191,27 @@ def test_energy_forces_bf16(gpu, name):
     assert f_rms <= 0.02
 
 
+@pytest.mark.parametrize("precision,tiles", [("bf16", (32, 64, 96)), ("fp32", (16, 32))])
+def test_mlp_tile_sizes_agree(gpu, precision, tiles):
+    """Every atom's MLP result depends only on its own row: all tile sizes give the same atom
+    energies and dE/dAEV rows (same k order per atom)."""
+    _flush(f"[mlp tiles] {precision} {tiles}")
+    s = water_box_small(216, 4)
+    cfg = configs(("H", "C", "N", "O", "S", "F", "Cl"))
+    params = O.init_mlp_params(13, O.aev_length(cfg[0]))
+    outs = []
+    for m in tiles:
+        pot = ANIPotential(cfg[1], params, precision=precision, mlp_tile_rows=m)
+        sb = pot.setup(s["species"], s["natoms"], s["cells"])
+        x = torch.tensor(np.asarray(s["coordinates"], np.float32), device="cuda")
+        pot.compute(sb, x, forces=True)
+        torch.cuda.synchronize()
+        outs.append((sb.e_atom[: sb.n_atoms].cpu().numpy().copy(), sb.g_aev[: sb.n_atoms].cpu().numpy().copy()))
+    for e, g in outs[1:]:
+        np.testing.assert_allclose(e, outs[0][0], rtol=1e-6, atol=1e-7)
+        np.testing.assert_allclose(g, outs[0][1], rtol=1e-5, atol=1e-6 * np.abs(outs[0][1]).max())
+
+
 def test_batched_systems_match_single(gpu):
     _flush("[efv] batched vs single")
     rng = np.random.Generator(np.random.PCG64(9))
