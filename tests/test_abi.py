@@ -40,7 +40,7 @@ def test_host_side_argument_checks():
     assert lib.fnl_nblist_workspace(10, 0, 1) == 0
     assert lib.fnl_nblist_workspace(100, 1, 8) > 0
     assert lib.fnl_mlp_tile_rows(_abi.FNL_F32) == 32
-    assert lib.fnl_mlp_tile_rows(_abi.FNL_BF16) == 64
+    assert lib.fnl_mlp_tile_rows(_abi.FNL_BF16) == 96
     assert lib.fnl_nblist_build(None, None, None, None, 10, 1, 0, None, None, None, 1, 5.2, 3.5, 8, 8,
                                 None, 0, None, None, None, None, None, None) == _abi.FNL_ERR_INVALID if hasattr(_abi, "FNL_ERR_INVALID") else True
     p = _abi.AevParams()
