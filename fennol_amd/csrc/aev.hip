@@ -446,17 +446,24 @@ __host__ __device__ inline int bwd_gs_offset(int max_nbr, int aev_len) {
   return round4(round4(aev_len) + 6 * max_nbr);
 }
 __host__ __device__ inline int bwd_lds_words(int max_nbr, int max_nbr_ang, int aev_len) {
-  return bwd_gs_offset(max_nbr, aev_len) + 4 * max_nbr + 8 * max_nbr_ang;
+  return bwd_gs_offset(max_nbr, aev_len) + 12 * max_nbr_ang;
 }
 
 // Adjoint, one wave per centre.  LDS slice of the wave (4-byte words):
 //   G   [aev_len]      dE/dAEV row of the centre
 //   nbv [max_nbr]      float4 (vx, vy, vz, d) per radial slot
 //   nbf [max_nbr]      float2 (fc(d), species)
-//   gs  [max_nbr]      float4 dE/dvec of the slot (radial part stored by the
-//                      slot's lane, angular part added with LDS float adds)
+//   gs  [max_nbr_ang]  float4 dE/dvec of the angular slot, angular terms only
+//                      (LDS float adds)
 //   av  [max_nbr_ang]  float4 (ux, uy, uz, d) per angular slot
-//   af  [max_nbr_ang]  float4 (fc_ang, fc_ang', species, radial slot)
+//   af  [max_nbr_ang]  float4 (fc_ang, fc_ang', species, neighbour atom | degenerate bit)
+// Radial terms: the lane of slot e (neighbour j) evaluates both dE_i/dvec_ij (from the
+// centre's row G_i: self term and virial of i) and dE_j/dvec_ji (from j's row G_j, read from
+// global memory: the neighbour term that j's own radial block gives i), so the radial
+// neighbour terms are gathered per atom and written with the self term -- no atomics.  The
+// angular terms of each angular slot are scattered to its atom with global atomics.
+// Outputs: grad_self[i] = -sum_e dE_i/dvec_ie - sum_j dE_j/dvec_ji|radial;
+// grad_nbr[j] += dE_i/dvec_ij|angular; virial_atom[i] = sum_e vec_ie (x) dE_i/dvec_ie.
 template <int NA, int NZ>
 __global__ void __launch_bounds__(64 * kWaves)
 k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
@@ -477,7 +484,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   float4* nbv = (float4*)(G + ((dm.aev_len + 3) & ~3));
   float2* nbf = (float2*)(nbv + max_nbr);
   float4* gs = (float4*)(G + bwd_gs_offset(max_nbr, dm.aev_len));
-  float4* av = gs + max_nbr;
+  float4* av = gs + max_nbr_ang;
   float4* af = av + max_nbr_ang;
 
   const int si = species_idx[i];
@@ -499,6 +506,9 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     for (int c = lane; c < (dm.aev_len + 3) / 4; c += 64) ((float4*)G)[c] = grow[c];
   }
   const int32_t* row = nbr + (size_t)i * max_nbr;
+  // per-lane partial sums: dE/dx_i (self and gathered radial neighbour terms) and the virial
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  float v[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   {
     const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
     for (int e = lane; e < nr; e += 64) {
@@ -511,28 +521,41 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       const float fc = cos_switch(d, p.cutoff);
       const int sj = species_idx[j];
       nbf[e] = make_float2(fc, __int_as_float(sj));
-      // radial: dE/dd = sum_k G[s, k] * 0.25 exp(-eta x^2) (fc' - 2 eta x fc), x = d - R_k
+      // radial: dE/dd = sum_k G[s, k] * 0.25 exp(-eta x^2) (fc' - 2 eta x fc), x = d - R_k,
+      // for the centre's block of species sj (G_i) and j's block of species si (G_j)
       const float dfc = cos_switch_grad(d, p.cutoff);
       const float* gr = G + sj * dm.nR;
-      float v = 0.0f;
+      const float* gj = g + (size_t)j * g_stride + si * dm.nR;
+      float vi = 0.0f, vj = 0.0f;
       for (int k = 0; k < dm.nR; ++k) {
         const float x = d - p.radial_shifts[k];
-        v += gr[k] * __expf(-p.radial_eta * x * x) * (dfc - 2.0f * p.radial_eta * x * fc);
+        const float t = __expf(-p.radial_eta * x * x) * (dfc - 2.0f * p.radial_eta * x * fc);
+        vi += gr[k] * t;
+        vj += gj[k] * t;
       }
-      const float sc = d2 > 1.0e-5f ? 0.25f * v / d : 0.0f;
-      gs[e] = make_float4(sc * vx, sc * vy, sc * vz, 0.0f);
+      const float sc = d2 > 1.0e-5f ? 0.25f * vi / d : 0.0f;
+      const float sj_ = d2 > 1.0e-5f ? 0.25f * vj / d : 0.0f;
+      const float gx = sc * vx, gy = sc * vy, gz = sc * vz;
+      // dE/dx_i: -dE_i/dvec_ij + dE_j/dvec_ji, with vec_ji = -vec_ij
+      sx -= gx + sj_ * vx;
+      sy -= gy + sj_ * vy;
+      sz -= gz + sj_ * vz;
+      v[0] += vx * gx; v[1] += vx * gy; v[2] += vx * gz;
+      v[3] += vy * gx; v[4] += vy * gy; v[5] += vy * gz;
+      v[6] += vz * gx; v[7] += vz * gy; v[8] += vz * gz;
     }
   }
   wave_sync();
   {
     for (int a = lane; a < na; a += 64) {
       const int e = a < 64 ? ae0 : arow[a];
-      const float4 v = nbv[e];
-      const float inv = 1.0f / fmaxf(v.w, 1.0e-5f);
-      av[a] = make_float4(v.x * inv, v.y * inv, v.z * inv, v.w);
-      const bool ok = v.x * v.x + v.y * v.y + v.z * v.z > 1.0e-5f;
-      af[a] = make_float4(cos_switch(v.w, p.cutoff_ang), ok ? cos_switch_grad(v.w, p.cutoff_ang) : 0.0f,
-                          nbf[e].y, __int_as_float(e | (ok ? 0 : (1 << 30))));
+      const float4 vv = nbv[e];
+      const float inv = 1.0f / fmaxf(vv.w, 1.0e-5f);
+      av[a] = make_float4(vv.x * inv, vv.y * inv, vv.z * inv, vv.w);
+      const bool ok = vv.x * vv.x + vv.y * vv.y + vv.z * vv.z > 1.0e-5f;
+      af[a] = make_float4(cos_switch(vv.w, p.cutoff_ang), ok ? cos_switch_grad(vv.w, p.cutoff_ang) : 0.0f,
+                          nbf[e].y, __int_as_float(row[e] | (ok ? 0 : (1 << 30))));
+      gs[a] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
   }
   wave_sync();
@@ -623,13 +646,13 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     gbx += dpp<0x4E>(gbx); gby += dpp<0x4E>(gby); gbz += dpp<0x4E>(gbz);
     // one lane per slot and block, and only where the block holds a valid triplet of the slot
     if (kb < nblk && cc < 4 && max(a + 1, 4 * B) <= min(4 * B + 3, na - 1)) {
-      float* pa = (float*)(gs + (__float_as_int(af[a].w) & 0x3fffffff));
+      float* pa = (float*)(gs + a);
       lds_add(pa + 0, gax);
       lds_add(pa + 1, gay);
       lds_add(pa + 2, gaz);
     }
     if (kb < nblk && (cc & 3) == 0 && b < na && b > 4 * A) {
-      float* pb = (float*)(gs + (__float_as_int(af[b].w) & 0x3fffffff));
+      float* pb = (float*)(gs + b);
       lds_add(pb + 0, gbx);
       lds_add(pb + 1, gby);
       lds_add(pb + 2, gbz);
@@ -637,22 +660,21 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   }
   wave_sync();
 
-  // scatter: neighbours by atomics, centre by a wave reduction; virial
-  float sx = 0.f, sy = 0.f, sz = 0.f;
-  float v[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  for (int e = lane; e < nr; e += 64) {
-    const float4 gg = gs[e];
-    const float4 vv = nbv[e];
-    const int j = row[e];
+  // angular terms: scatter to the neighbour atoms, add to the centre's sums and virial
+  for (int a = lane; a < na; a += 64) {
+    const float4 gg = gs[a];
+    const float4 ua = av[a];
+    const int j = __float_as_int(af[a].w) & 0x3fffffff;
     atomicAdd(&grad_nbr[3 * j], gg.x);
     atomicAdd(&grad_nbr[3 * j + 1], gg.y);
     atomicAdd(&grad_nbr[3 * j + 2], gg.z);
-    sx += gg.x;
-    sy += gg.y;
-    sz += gg.z;
-    v[0] += vv.x * gg.x; v[1] += vv.x * gg.y; v[2] += vv.x * gg.z;
-    v[3] += vv.y * gg.x; v[4] += vv.y * gg.y; v[5] += vv.y * gg.z;
-    v[6] += vv.z * gg.x; v[7] += vv.z * gg.y; v[8] += vv.z * gg.z;
+    sx -= gg.x;
+    sy -= gg.y;
+    sz -= gg.z;
+    const float vx = ua.x * ua.w, vy = ua.y * ua.w, vz = ua.z * ua.w;
+    v[0] += vx * gg.x; v[1] += vx * gg.y; v[2] += vx * gg.z;
+    v[3] += vy * gg.x; v[4] += vy * gg.y; v[5] += vy * gg.z;
+    v[6] += vz * gg.x; v[7] += vz * gg.y; v[8] += vz * gg.z;
   }
   sx = wave_sum_dpp(sx);
   sy = wave_sum_dpp(sy);
@@ -660,9 +682,9 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
 #pragma unroll
   for (int k = 0; k < 9; ++k) v[k] = wave_sum_dpp(v[k]);
   if (lane == 0) {
-    grad_self[3 * i] = -sx;
-    grad_self[3 * i + 1] = -sy;
-    grad_self[3 * i + 2] = -sz;
+    grad_self[3 * i] = sx;
+    grad_self[3 * i + 1] = sy;
+    grad_self[3 * i + 2] = sz;
 #pragma unroll
     for (int k = 0; k < 9; ++k) virial_atom[9 * i + k] = v[k];
   }

@@ -146,13 +146,17 @@ int fnl_aev_forward(const float* coords, const int32_t* species_idx,
 
 /* Adjoint of fnl_aev_forward (the reverse pass jax.grad runs through
  * ani.py:47-142 and the graph processors).  AEV terms are recomputed rather
- * than stored.  Given g = dE/dAEV (float32 rows of stride g_stride):
- *   grad_self[i] = -sum_e dE/dvec_ie        (written, float32[N][3])
- *   grad_nbr[j] += dE/dvec_ij  for every edge (atomic, float32[N][3]; the
- *                                          caller zeroes it first)
- *   virial_atom[i] = sum_e vec_ie (x) dE/dvec_ie   (float32[N][9], row-major a,b)
- * so that dE/dx = grad_self + grad_nbr and the per-system virial
- * dE/dstrain (fennix.py:320-347) is the per-system sum of virial_atom. */
+ * than stored.  Given g = dE/dAEV (float32 rows of stride g_stride), with
+ * E = sum_i E_i and vec_ij = x_j - x_i:
+ *   grad_self[i] = -sum_j dE_i/dvec_ij - sum_j dE_j/dvec_ji|radial
+ *                  (written, float32[N][3]: the centre's own terms plus the
+ *                  radial terms its neighbours' rows give it, gathered)
+ *   grad_nbr[j] += dE_i/dvec_ij|angular for every angular edge (atomic,
+ *                  float32[N][3]; the caller zeroes it first)
+ *   virial_atom[i] = sum_j vec_ij (x) dE_i/dvec_ij   (float32[N][9], row-major a,b)
+ * so that dE/dx = grad_self + grad_nbr (only the sum is meaningful) and the
+ * per-system virial dE/dstrain (fennix.py:320-347) is the per-system sum of
+ * virial_atom.  The neighbour lists must be symmetric (as built). */
 int fnl_aev_backward(const float* coords, const int32_t* species_idx,
                      const int32_t* batch_index, int32_t n_atoms, int32_t pbc_mode,
                      const float* cells, const float* reciprocal_cells,
