@@ -20,13 +20,14 @@
 //               neighbouring cells (contiguous z-runs of the cell-sorted
 //               arrays) tested 64 at a time, kept pairs compacted with a wave
 //               ballot + prefix popcount into the centre's row
-//   k_row_max   one workgroup: row maxima into the status words
+//   k_row_max   a few workgroups: row maxima into the status words
 
 #include "fnl_common.h"
 
 namespace {
 
-constexpr int kGridWords = 12;  // per system: nx ny nz first | ox oy oz | iwx iwy iwz | pbc pad
+constexpr int kGridWords = 12;
+constexpr int kMaxRanges = 18;  // 3 x 3 columns x at most 2 z-runs  // per system: nx ny nz first | ox oy oz | iwx iwy iwz | pbc pad
 
 struct NblistWs {
   int32_t* grid;       // [n_sys][kGridWords] (floats stored as bits)
@@ -380,17 +381,22 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
   }
   const int r_end = fnl_wave_incl_scan(r_len);
   const int r_off = r_end - r_len;
-  const int total = __shfl(r_end, 63, 64);
-  // position in the cell-sorted arrays of candidate v < total
-  auto candidate = [&](int v) {
-    int r = 0;
+  const int total = __builtin_amdgcn_readlane(r_end, 63);
+  // range starts (in the candidate list) and position shifts as wave-uniform scalars
+  int roff[kMaxRanges], rdel[kMaxRanges];
 #pragma unroll
-    for (int step = 16; step > 0; step >>= 1) {
-      const int c = r + step;
-      const int o = __shfl(r_off, c < 64 ? c : 63, 64);
-      if (c < nrange && o <= v) r = c;
-    }
-    return __shfl(r_lo, r, 64) + (v - __shfl(r_off, r, 64));
+  for (int r = 0; r < kMaxRanges; ++r) {
+    roff[r] = __builtin_amdgcn_readlane(r_off, r);
+    rdel[r] = __builtin_amdgcn_readlane(r_lo, r) - roff[r];
+  }
+  // position in the cell-sorted arrays of candidate v < total: v + shift of the last range
+  // that starts at or before v (empty ranges start where the next one does)
+  auto candidate = [&](int v) {
+    int dl = rdel[0];
+#pragma unroll
+    for (int r = 1; r < kMaxRanges; ++r)
+      if (r < nrange && roff[r] <= v) dl = rdel[r];
+    return v + dl;
   };
 
   FnlCell cl;
@@ -443,13 +449,14 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
   }
 }
 
-// Row maxima (status words MAX_NBR, MAX_NBR_ANG): one workgroup, plain stores.
+// Row maxima (status words MAX_NBR, MAX_NBR_ANG, zeroed by k_grid): a few workgroups, one
+// atomicMax per workgroup and word.
 __global__ void __launch_bounds__(1024) k_row_max(const int32_t* __restrict__ n_nbr,
                                                   const int32_t* __restrict__ n_nbr_ang, int n_atoms,
                                                   int32_t* __restrict__ status) {
   __shared__ int sm[2][16];
   int m = 0, ma = 0;
-  for (int a = threadIdx.x; a < n_atoms; a += 1024) {
+  for (int a = blockIdx.x * 1024 + threadIdx.x; a < n_atoms; a += gridDim.x * 1024) {
     m = max(m, n_nbr[a]);
     ma = max(ma, n_nbr_ang[a]);
   }
@@ -466,8 +473,8 @@ __global__ void __launch_bounds__(1024) k_row_max(const int32_t* __restrict__ n_
       m = max(m, sm[0][k]);
       ma = max(ma, sm[1][k]);
     }
-    status[FNL_STATUS_MAX_NBR] = max(m, sm[0][0]);
-    status[FNL_STATUS_MAX_NBR_ANG] = max(ma, sm[1][0]);
+    atomicMax(&status[FNL_STATUS_MAX_NBR], max(m, sm[0][0]));
+    atomicMax(&status[FNL_STATUS_MAX_NBR_ANG], max(ma, sm[1][0]));
   }
 }
 
@@ -588,7 +595,8 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
                      ws.pos, rc2, rca2, max_nbr, max_nbr_ang, nbr, n_nbr, nbr_ang, n_nbr_ang, status,
                      n_cells);
   FNL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_row_max, dim3(1), dim3(1024), 0, s, n_nbr, n_nbr_ang, n_atoms, status);
+  hipLaunchKernelGGL(k_row_max, dim3(std::min(16, (n_atoms + 4095) / 4096)), dim3(1024), 0, s, n_nbr, n_nbr_ang,
+                     n_atoms, status);
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }

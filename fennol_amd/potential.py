@@ -512,8 +512,9 @@ class ANIPotential:
                                          _abi.stream_handle(stream)), "fnl_fill_f32")
 
     def aev_backward(self, sb: SystemBuffers, coords32: torch.Tensor, stream=None, zero: bool = True) -> None:
-        """dE/dx from dE/dAEV: grad_self (centre terms) and grad_nbr (neighbour terms, accumulated
-        with atomics; cleared first unless ``zero`` is False), virial per atom."""
+        """dE/dx from dE/dAEV as grad_self + grad_nbr (grad_self: written per atom, the centre's
+        terms and the radial terms gathered from its neighbours; grad_nbr: angular neighbour
+        terms accumulated with atomics, cleared first unless ``zero`` is False), virial per atom."""
         if zero:
             self.zero_grad_nbr(sb, stream)
         s = _abi.stream_handle(stream)
