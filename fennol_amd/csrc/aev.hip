@@ -303,11 +303,14 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   wave_sync();
 
   const int q = lane >> 4, k = lane & 15;
+  uint32_t rad_present = 0;  // species with at least one radial neighbour
   {
     const float shift = k < dm.nR ? p.radial_shifts[k] : 0.0f;
 #pragma unroll
     for (int s = 0; s < FNL_MAX_SPECIES; ++s) {
       if (s >= S) break;
+      if (cnt[s] == 0) continue;  // written as zeros with the other empty blocks
+      rad_present |= 1u << s;
       const int lo = off[s], hi = off[s] + cnt[s];
       float run = 0.0f;
 #pragma unroll 4
@@ -370,19 +373,28 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const float* fR = fac + dd * kFacLd;
   const float* fF = fac + (NA + zz) * kFacLd;
   TOut* aout = out + dm.rad_len;
-  // pairs without triplets: zero channels, two pairs per instruction
+  // empty blocks (species without radial neighbours, pairs without triplets) and the padding:
+  // zeros in 16-byte stores
   {
-    uint64_t zm = ~present & (P >= 64 ? ~0ull : ((1ull << P) - 1ull));
-    while (zm) {
-      const int p0 = __builtin_ctzll(zm);
-      zm &= zm - 1ull;
-      int p1 = -1;
-      if (zm) {
-        p1 = __builtin_ctzll(zm);
-        zm &= zm - 1ull;
+    constexpr int CH = 16 / (int)sizeof(TOut);  // channels per 16-byte chunk
+    const bool aligned = (dm.nR % CH) == 0 && (dm.rad_len % CH) == 0 && (stride % CH) == 0 &&
+                         ((uintptr_t)aev & 15) == 0;
+    if (aligned) {
+      for (int c0 = lane * CH; c0 < stride; c0 += 64 * CH) {
+        bool empty;
+        if (c0 < dm.rad_len) empty = !((rad_present >> (c0 / dm.nR)) & 1u);
+        else if (c0 < dm.aev_len) empty = !((present >> ((c0 - dm.rad_len) >> 5)) & 1ull);
+        else empty = true;
+        if (empty) *(uint4*)(out + c0) = make_uint4(0u, 0u, 0u, 0u);
       }
-      const int pw = h == 0 ? p0 : p1;
-      if (pw >= 0) aout[pw * 32 + ch] = to_out<TOut>(0.0f);
+    } else {
+      for (int c = lane; c < stride; c += 64) {
+        bool empty;
+        if (c < dm.rad_len) empty = !((rad_present >> (c / dm.nR)) & 1u);
+        else if (c < dm.aev_len) empty = !((present >> ((c - dm.rad_len) >> 5)) & 1ull);
+        else empty = true;
+        if (empty) out[c] = to_out<TOut>(0.0f);
+      }
     }
   }
   // present pairs, one at a time: chunks of 64 triplets (a from block 1, b from block 2,
@@ -438,7 +450,6 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     const float v = run + __shfl_xor(run, 32, 64);
     if (h == 0) aout[pr * 32 + ch] = to_out<TOut>(v);
   }
-  for (int c = dm.aev_len + lane; c < stride; c += 64) out[c] = to_out<TOut>(0.0f);
 }
 
 // per-wave LDS footprint (4-byte words) of the adjoint kernel; gs starts 16-byte aligned
