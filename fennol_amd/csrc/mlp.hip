@@ -119,7 +119,7 @@ struct Cfg {
   static constexpr int PF = PF_;
 };
 
-template <typename T, int M_>
+template <typename T, int M_, int WV_ = 4>
 struct Lds {
   static constexpr int M = M_;
   static constexpr int LD1 = H1 + Tr<T>::PAD, LD2 = H2 + Tr<T>::PAD, LD3 = H3 + Tr<T>::PAD;
@@ -128,7 +128,7 @@ struct Lds {
   static constexpr int OFF2 = OFF1 + M * LD1 * (int)sizeof(T);
   static constexpr int OFF3 = OFF2 + M * LD2 * (int)sizeof(T);
   static constexpr int OFFE = OFF3 + M * LD3 * (int)sizeof(T);
-  static constexpr int BYTES = OFFE + 4 * M * 4;
+  static constexpr int BYTES = OFFE + WV_ * M * 4;  // + per-wave energy partials
   static constexpr int XBUF = M * LDX * (int)sizeof(T);  // one staging buffer, at OFF2
   static_assert(2 * XBUF <= OFFE - OFF2, "layer-1 staging must fit in the H2/H3 buffers");
   static_assert(BYTES <= 160 * 1024, "fits the LDS of one CU");
@@ -148,8 +148,8 @@ __device__ __forceinline__ void zero_acc(f4 (&acc)[NB][MB]) {
 }
 
 // acc[nb][mb] += sum_k W(n, k) act(k, m) over k in [0, K) for the n-blocks
-// nb0 + 4*nb (< ncb) of this wave.  W prefetched PF steps ahead.
-template <typename T, int NB, int MB, int PF>
+// nb0 + WV*nb (< ncb) of this wave.  W prefetched PF steps ahead.
+template <typename T, int NB, int MB, int PF, int WV>
 __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ W, int ldw, int nb0, int ncb,
                                        const T* act, int lda, int K) {
   typedef Tr<T> tr;
@@ -159,7 +159,7 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
   // conditional load makes it wait for all of them.
   int ncol[NB];
 #pragma unroll
-  for (int nb = 0; nb < NB; ++nb) ncol[nb] = min(nb0 + 4 * nb, ncb - 1) * 16;
+  for (int nb = 0; nb < NB; ++nb) ncol[nb] = min(nb0 + WV * nb, ncb - 1) * 16;
   typename tr::frag wq[PF][NB];
 #pragma unroll
   for (int u = 0; u < PF; ++u)
@@ -173,35 +173,43 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
     for (int u = 0; u < PF; ++u) {
       const int s = s0 + u;
       if (s < nsteps) {
-        // activations of the next k-step, read before this step's MFMAs
+        // activations of the next k-step, read before this step's MFMAs (with 4 waves per
+        // workgroup; with 8, the other wave on the SIMD covers the LDS latency)
         typename tr::frag bn[MB];
+        if constexpr (WV == 4) {
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(act, lda, mb * 16, min(s + 1, nsteps - 1) * tr::KS);
+          for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(act, lda, mb * 16, min(s + 1, nsteps - 1) * tr::KS);
+        }
         const int sn = min(s + PF, nsteps - 1);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
-          if (nb0 + 4 * nb < ncb) {
+          if (nb0 + WV * nb < ncb) {
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb) acc[nb][mb] = tr::mma(wq[u][nb], b[mb], acc[nb][mb]);
           }
           wq[u][nb] = tr::wload(W, ldw, ncol[nb], sn * tr::KS);
         }
+        if constexpr (WV == 4) {
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) b[mb] = bn[mb];
+          for (int mb = 0; mb < MB; ++mb) b[mb] = bn[mb];
+        } else if (s + 1 < nsteps) {
+#pragma unroll
+          for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(act, lda, mb * 16, (s + 1) * tr::KS);
+        }
       }
     }
   }
 }
 
 // forward epilogue: h = celu(acc + b) -> acc and LDS [m][n]
-template <typename T, int NB, int MB>
+template <typename T, int NB, int MB, int WV>
 __device__ __forceinline__ void fwd_epilogue(f4 (&acc)[NB][MB], const float* __restrict__ bias, T* H, int ldh,
                                              int nb0, int ncb, float alpha) {
   const int l = fnl_lane();
   const float ia = 1.0f / alpha;
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
-    const int cb = nb0 + 4 * nb;
+    const int cb = nb0 + WV * nb;
     if (cb >= ncb) continue;
     const int n = cb * 16 + 4 * (l >> 4);
     const f4 bv = *(const f4*)(bias + n);
@@ -218,14 +226,14 @@ __device__ __forceinline__ void fwd_epilogue(f4 (&acc)[NB][MB], const float* __r
 
 // backward epilogue: G = acc * celu'(z) with celu'(z) = h > 0 ? 1 : 1 + h/alpha,
 // h read from the LDS buffer that G then overwrites
-template <typename T, int NB, int MB>
+template <typename T, int NB, int MB, int WV>
 __device__ __forceinline__ void bwd_epilogue(const f4 (&acc)[NB][MB], T* HG, int ld, int nb0, int ncb,
                                              float alpha) {
   const int l = fnl_lane();
   const float ia = 1.0f / alpha;
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
-    const int cb = nb0 + 4 * nb;
+    const int cb = nb0 + WV * nb;
     if (cb >= ncb) continue;
     const int n = cb * 16 + 4 * (l >> 4);
 #pragma unroll
@@ -240,23 +248,27 @@ __device__ __forceinline__ void bwd_epilogue(const f4 (&acc)[NB][MB], T* HG, int
   }
 }
 
-template <typename T, int M_, int PF_>
-__global__ void __launch_bounds__(256, (Lds<T, M_>::BLOCKS_PER_CU))
+template <typename T, int M_, int PF_, int WV_>
+__global__ void __launch_bounds__(64 * WV_, (Lds<T, M_, WV_>::BLOCKS_PER_CU))
 k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int32_t* __restrict__ perm,
               const int32_t* __restrict__ tile_start, const int32_t* __restrict__ tile_rows,
               const int32_t* __restrict__ tile_net, int n_tiles, float* __restrict__ energy,
               float* __restrict__ grad, int grad_stride) {
   typedef Tr<T> tr;
-  typedef Lds<T, M_> L;
+  typedef Lds<T, M_, WV_> L;
   constexpr int M = M_;
   constexpr int PF = PF_;
+  constexpr int WV = WV_;                    // waves per workgroup
+  constexpr int NB1 = 16 / WV;               // n-blocks per wave: H1 = 256 features
+  constexpr int NB2 = (H2 / 16 + WV - 1) / WV;
+  constexpr int NB3 = (H3 / 16 + WV - 1) / WV;
   constexpr int MB = M / 16;
   constexpr int KC = tr::KC;
   __shared__ __attribute__((aligned(16))) unsigned char smem[L::BYTES];
   T* B1 = (T*)(smem + L::OFF1);
   T* B2 = (T*)(smem + L::OFF2);
   T* B3 = (T*)(smem + L::OFF3);
-  float* Epart = (float*)(smem + L::OFFE);  // [4][M]
+  float* Epart = (float*)(smem + L::OFFE);  // [WV][M]
   T* Xs0 = B2;
   T* Xs1 = (T*)(smem + L::OFF2 + L::XBUF);
 
@@ -278,7 +290,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   // ---------------- layer 1: X (global, staged) -> H1 ----------------
   constexpr int VEC = 16 / sizeof(T);          // elements per 16-byte load
   constexpr int TPR = KC / VEC;                // threads per row of a chunk
-  constexpr int RPP = 256 / TPR;               // rows per pass
+  constexpr int RPP = 64 * WV / TPR;           // rows per pass
   constexpr int NPASS = M / RPP;
   static_assert(M % RPP == 0, "staging passes");
   int srow[NPASS];
@@ -303,7 +315,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     for (int p = 0; p < NPASS; ++p) *(u4v*)(buf + (p * RPP + t / TPR) * L::LDX + scol) = xr[p];
   };
 
-  f4 acc1[4][MB];
+  f4 acc1[NB1][MB];
   zero_acc(acc1);
   {
     constexpr int SPC = KC / tr::KS;  // k-steps per chunk
@@ -311,11 +323,11 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     const T* W = (const T*)P.fw[0];
     const int ldw = sizeof(T) == 2 ? K : H1;
     const int nsteps = K / tr::KS;
-    typename tr::frag wq[PF][4];
+    typename tr::frag wq[PF][NB1];
 #pragma unroll
     for (int u = 0; u < PF; ++u)
 #pragma unroll
-      for (int nb = 0; nb < 4; ++nb) wq[u][nb] = tr::wload(W, ldw, (wave + 4 * nb) * 16, u * tr::KS);
+      for (int nb = 0; nb < NB1; ++nb) wq[u][nb] = tr::wload(W, ldw, (wave + WV * nb) * 16, u * tr::KS);
     stage_load(0);
     stage_store(Xs0);
     __syncthreads();
@@ -333,46 +345,53 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
           // activations of the next k-step of the chunk, read before this step's MFMAs
           typename tr::frag bn[MB];
           const int kn = min(s0 + u + 1, SPC - 1) * tr::KS;
+          if constexpr (WV == 4) {
 #pragma unroll
-          for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(X, L::LDX, mb * 16, kn);
-#pragma unroll
-          for (int nb = 0; nb < 4; ++nb) {
-#pragma unroll
-            for (int mb = 0; mb < MB; ++mb) acc1[nb][mb] = tr::mma(wq[u][nb], b[mb], acc1[nb][mb]);
-            wq[u][nb] = tr::wload(W, ldw, (wave + 4 * nb) * 16, min(s + PF, nsteps - 1) * tr::KS);
+            for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(X, L::LDX, mb * 16, kn);
           }
 #pragma unroll
-          for (int mb = 0; mb < MB; ++mb) b[mb] = bn[mb];
+          for (int nb = 0; nb < NB1; ++nb) {
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) acc1[nb][mb] = tr::mma(wq[u][nb], b[mb], acc1[nb][mb]);
+            wq[u][nb] = tr::wload(W, ldw, (wave + WV * nb) * 16, min(s + PF, nsteps - 1) * tr::KS);
+          }
+          if constexpr (WV == 4) {
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) b[mb] = bn[mb];
+          } else {
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(X, L::LDX, mb * 16, kn);
+          }
         }
       }
       if (c + 1 < nchunks) stage_store((c & 1) ? Xs0 : Xs1);
       __syncthreads();
     }
   }
-  fwd_epilogue<T, 4, MB>(acc1, P.b[0], B1, L::LD1, wave, H1 / 16, alpha);
+  fwd_epilogue<T, NB1, MB, WV>(acc1, P.b[0], B1, L::LD1, wave, H1 / 16, alpha);
   __syncthreads();
 
   // ---------------- layer 2 ----------------
   {
-    f4 acc2[3][MB];
+    f4 acc2[NB2][MB];
     zero_acc(acc2);
-    gemm_t<T, 3, MB, PF>(acc2, (const T*)P.fw[1], sizeof(T) == 2 ? H1 : H2, wave, H2 / 16, B1, L::LD1, H1);
-    fwd_epilogue<T, 3, MB>(acc2, P.b[1], B2, L::LD2, wave, H2 / 16, alpha);
+    gemm_t<T, NB2, MB, PF, WV>(acc2, (const T*)P.fw[1], sizeof(T) == 2 ? H1 : H2, wave, H2 / 16, B1, L::LD1, H1);
+    fwd_epilogue<T, NB2, MB, WV>(acc2, P.b[1], B2, L::LD2, wave, H2 / 16, alpha);
   }
   __syncthreads();
 
   // ---------------- layer 3 + energy ----------------
-  f4 acc3[3][MB];
+  f4 acc3[NB3][MB];
   zero_acc(acc3);
-  gemm_t<T, 3, MB, PF>(acc3, (const T*)P.fw[2], sizeof(T) == 2 ? H2 : H3, wave, H3 / 16, B2, L::LD2, H2);
-  fwd_epilogue<T, 3, MB>(acc3, P.b[2], B3, L::LD3, wave, H3 / 16, alpha);
+  gemm_t<T, NB3, MB, PF, WV>(acc3, (const T*)P.fw[2], sizeof(T) == 2 ? H2 : H3, wave, H3 / 16, B2, L::LD2, H2);
+  fwd_epilogue<T, NB3, MB, WV>(acc3, P.b[2], B3, L::LD3, wave, H3 / 16, alpha);
   {
     float part[MB];
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) part[mb] = 0.0f;
 #pragma unroll
-    for (int nb = 0; nb < 3; ++nb) {
-      const int cb = wave + 4 * nb;
+    for (int nb = 0; nb < NB3; ++nb) {
+      const int cb = wave + WV * nb;
       if (cb >= H3 / 16) continue;
       const f4 wv = *(const f4*)(P.w4 + cb * 16 + 4 * (l >> 4));
 #pragma unroll
@@ -390,8 +409,10 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   }
   __syncthreads();
   if (t < nrows) {
-    energy[perm[r0 + t]] =
-        ((Epart[t] + Epart[M + t]) + (Epart[2 * M + t] + Epart[3 * M + t])) + P.b[3][0];
+    float e = 0.0f;
+#pragma unroll
+    for (int w2 = 0; w2 < WV; ++w2) e += Epart[w2 * M + t];
+    energy[perm[r0 + t]] = e + P.b[3][0];
   }
   if (grad == nullptr) return;
 
@@ -400,8 +421,8 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     // G3 = w4 * celu'(z3) -> B3 (h3 is in acc3)
     const float ia = 1.0f / alpha;
 #pragma unroll
-    for (int nb = 0; nb < 3; ++nb) {
-      const int cb = wave + 4 * nb;
+    for (int nb = 0; nb < NB3; ++nb) {
+      const int cb = wave + WV * nb;
       if (cb >= H3 / 16) continue;
       const int n = cb * 16 + 4 * (l >> 4);
       const f4 wv = *(const f4*)(P.w4 + n);
@@ -419,20 +440,20 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   }
   __syncthreads();
   {
-    f4 acc2[3][MB];
+    f4 acc2[NB2][MB];
     zero_acc(acc2);
-    gemm_t<T, 3, MB, PF>(acc2, (const T*)P.bw[2], sizeof(T) == 2 ? H3 : H2, wave, H2 / 16, B3, L::LD3, H3);
-    bwd_epilogue<T, 3, MB>(acc2, B2, L::LD2, wave, H2 / 16, alpha);
+    gemm_t<T, NB2, MB, PF, WV>(acc2, (const T*)P.bw[2], sizeof(T) == 2 ? H3 : H2, wave, H2 / 16, B3, L::LD3, H3);
+    bwd_epilogue<T, NB2, MB, WV>(acc2, B2, L::LD2, wave, H2 / 16, alpha);
   }
   __syncthreads();
   {
-    f4 accg[4][MB];
+    f4 accg[NB1][MB];
     zero_acc(accg);
-    gemm_t<T, 4, MB, PF>(accg, (const T*)P.bw[1], sizeof(T) == 2 ? H2 : H1, wave, H1 / 16, B2, L::LD2, H2);
-    bwd_epilogue<T, 4, MB>(accg, B1, L::LD1, wave, H1 / 16, alpha);
+    gemm_t<T, NB1, MB, PF, WV>(accg, (const T*)P.bw[1], sizeof(T) == 2 ? H2 : H1, wave, H1 / 16, B2, L::LD2, H2);
+    bwd_epilogue<T, NB1, MB, WV>(accg, B1, L::LD1, wave, H1 / 16, alpha);
   }
   __syncthreads();
-  // dE/dX = W1 G1: input features in passes of 16 n-blocks (4 per wave)
+  // dE/dX = W1 G1: input features in passes of 16 n-blocks (16/WV per wave)
   int atom[MB];
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb) {
@@ -441,12 +462,12 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   }
   const int ncb_x = K / 16;
   for (int base = 0; base < ncb_x; base += 16) {
-    f4 accx[4][MB];
+    f4 accx[NB1][MB];
     zero_acc(accx);
-    gemm_t<T, 4, MB, PF>(accx, (const T*)P.bw[0], sizeof(T) == 2 ? H1 : K, base + wave, ncb_x, B1, L::LD1, H1);
+    gemm_t<T, NB1, MB, PF, WV>(accx, (const T*)P.bw[0], sizeof(T) == 2 ? H1 : K, base + wave, ncb_x, B1, L::LD1, H1);
 #pragma unroll
-    for (int nb = 0; nb < 4; ++nb) {
-      const int cb = base + wave + 4 * nb;
+    for (int nb = 0; nb < NB1; ++nb) {
+      const int cb = base + wave + WV * nb;
       if (cb >= ncb_x) continue;
       const int n = cb * 16 + 4 * (l >> 4);
 #pragma unroll
@@ -509,23 +530,23 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
   const int M = w->tile_rows > 0 ? w->tile_rows : fnl_mlp_tile_rows(w->dtype);
   const int grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
   if (bf && M == 64) {
-    hipLaunchKernelGGL((k_species_mlp<unsigned short, 64, 2>), dim3(grid), dim3(256), 0, st,
+    hipLaunchKernelGGL((k_species_mlp<unsigned short, 64, 2, 4>), dim3(grid), dim3(256), 0, st,
                        (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
                        energy, grad, grad_stride);
   } else if (bf && M == 96) {
-    hipLaunchKernelGGL((k_species_mlp<unsigned short, 96, 4>), dim3(grid), dim3(256), 0, st,
+    hipLaunchKernelGGL((k_species_mlp<unsigned short, 96, 4, 4>), dim3(grid), dim3(256), 0, st,
                        (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
                        energy, grad, grad_stride);
   } else if (bf && M == 32) {
-    hipLaunchKernelGGL((k_species_mlp<unsigned short, 32, 2>), dim3(grid), dim3(256), 0, st,
+    hipLaunchKernelGGL((k_species_mlp<unsigned short, 32, 2, 4>), dim3(grid), dim3(256), 0, st,
                        (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
                        energy, grad, grad_stride);
   } else if (!bf && M == 32) {
-    hipLaunchKernelGGL((k_species_mlp<float, 32, 4>), dim3(grid), dim3(256), 0, st, (const float*)aev,
+    hipLaunchKernelGGL((k_species_mlp<float, 32, 4, 4>), dim3(grid), dim3(256), 0, st, (const float*)aev,
                        aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles, energy, grad,
                        grad_stride);
   } else if (!bf && M == 16) {
-    hipLaunchKernelGGL((k_species_mlp<float, 16, 2>), dim3(grid), dim3(256), 0, st, (const float*)aev,
+    hipLaunchKernelGGL((k_species_mlp<float, 16, 2, 4>), dim3(grid), dim3(256), 0, st, (const float*)aev,
                        aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles, energy, grad,
                        grad_stride);
   } else {
