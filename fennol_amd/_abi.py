@@ -80,8 +80,10 @@ _SIGNATURES = {
     "fnl_version": (C.c_int, []),
     "fnl_strerror": (C.c_char_p, [C.c_int]),
     "fnl_nblist_workspace": (sz, [i32, i32, i32]),
-    "fnl_nblist_build": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, i32, f32, f32, i32, i32,
+    "fnl_nblist_build": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, i32, f64, f64, i32, i32,
                                    vp, sz, vp, vp, vp, vp, vp, vp]),
+    "fnl_nblist_skin_update": (C.c_int, [vp, vp, i32, i32, vp, vp, vp, vp, i32, f64, f64, i32, i32, vp, vp, vp,
+                                         vp, vp, vp]),
     "fnl_nblist_export": (C.c_int, [vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp]),
     "fnl_aev_forward": (C.c_int, [vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, i32,
                                   C.POINTER(AevParams), vp, i32, i32, vp]),

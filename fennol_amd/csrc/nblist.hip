@@ -21,6 +21,10 @@
 //               arrays) tested 64 at a time, kept pairs compacted with a wave
 //               ballot + prefix popcount into the centre's row
 //   k_row_max   a few workgroups: row maxima into the status words
+//
+// Skin update (a list built with cutoff + skin, re-filtered at the current
+// positions on the steps between builds):
+//   k_status_clear, k_skin_filter (one wave per centre), k_row_max
 
 #include "fnl_common.h"
 
@@ -207,7 +211,7 @@ __global__ void k_bin(const float* __restrict__ coords, const int32_t* __restric
   if (species_idx[i] < 0) {
     atom_cell[i] = -1;
     n_nbr[i] = 0;
-    n_nbr_ang[i] = 0;
+    if (n_nbr_ang) n_nbr_ang[i] = 0;
     return;
   }
   const int b = batch_index[i];
@@ -217,7 +221,7 @@ __global__ void k_bin(const float* __restrict__ coords, const int32_t* __restric
   if (cell < 0 || cell >= n_cells) {  // inconsistent grid capacities: drop and flag
     atom_cell[i] = -1;
     n_nbr[i] = 0;
-    n_nbr_ang[i] = 0;
+    if (n_nbr_ang) n_nbr_ang[i] = 0;
     atomicOr(&status[FNL_STATUS_OVERFLOW], 2);
     atomicOr(&status[FNL_STATUS_STICKY], 2);
     return;
@@ -441,7 +445,7 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
   }
   if (lane == 0) {
     n_nbr[i] = count;
-    n_nbr_ang[i] = count_a;
+    if (n_nbr_ang) n_nbr_ang[i] = count_a;
     if (count > max_nbr || count_a > max_nbr_ang) {
       atomicOr(&status[FNL_STATUS_OVERFLOW], 1);
       atomicOr(&status[FNL_STATUS_STICKY], 1);
@@ -458,7 +462,7 @@ __global__ void __launch_bounds__(1024) k_row_max(const int32_t* __restrict__ n_
   int m = 0, ma = 0;
   for (int a = blockIdx.x * 1024 + threadIdx.x; a < n_atoms; a += gridDim.x * 1024) {
     m = max(m, n_nbr[a]);
-    ma = max(ma, n_nbr_ang[a]);
+    if (n_nbr_ang) ma = max(ma, n_nbr_ang[a]);
   }
   m = fnl_wave_max_i(m);
   ma = fnl_wave_max_i(ma);
@@ -475,6 +479,77 @@ __global__ void __launch_bounds__(1024) k_row_max(const int32_t* __restrict__ n_
     }
     atomicMax(&status[FNL_STATUS_MAX_NBR], max(m, sm[0][0]));
     atomicMax(&status[FNL_STATUS_MAX_NBR_ANG], max(ma, sm[1][0]));
+  }
+}
+
+// Status words of one list update: the overflow flag and the row maxima start at zero (the
+// sticky word persists).
+__global__ void k_status_clear(int32_t* __restrict__ status) {
+  const int t = threadIdx.x;
+  if (t < FNL_STATUS_STICKY) status[t] = 0;
+}
+
+// Skin update (GraphGenerator.update_skin, preprocessing.py:640-696, with the
+// GraphFilter re-filter :958-960): one wave per centre reads the centre's row
+// of the list built with cutoff + skin, 64 slots a pass, recomputes d12 at the
+// current positions with the same float32 arithmetic as the full build, and
+// compacts the slots with d12 < rc^2 (radial row) and d12 < rca^2 (angular row)
+// in row order, as k_pairs does.  For the minimum-image convention the image
+// chosen at the build is the image the current positions select for every
+// pair still inside the cutoff (rc below half the lattice-plane spacing), so
+// the recomputed image reproduces the reference's stored pbc_shifts.  The
+// next pass's slot indices are loaded before the current pass is tested.
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
+k_skin_filter(const float* __restrict__ coords, const int32_t* __restrict__ batch_index, int n_atoms,
+              int pbc_mode, const float* __restrict__ cells, const float* __restrict__ rec,
+              const int32_t* __restrict__ nbr_skin, const int32_t* __restrict__ n_skin, int max_skin,
+              float rc2, float rca2, int max_nbr, int max_nbr_ang, int32_t* __restrict__ nbr,
+              int32_t* __restrict__ n_nbr, int32_t* __restrict__ nbr_ang, int32_t* __restrict__ n_nbr_ang,
+              int32_t* __restrict__ status) {
+  const int lane = fnl_lane();
+  const int i = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (i >= n_atoms) return;
+  const int ns = n_skin[i];
+  const int n = min(ns, max_skin);
+  const int32_t* srow = nbr_skin + (size_t)i * max_skin;
+  FnlCell cl;
+  const bool pbc = pbc_mode == FNL_PBC_MINIMUM_IMAGE;
+  if (pbc) fnl_load_cell(cells, rec, batch_index[i], cl);
+  const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
+  int32_t* row = nbr + (size_t)i * max_nbr;
+  int32_t* row_a = nbr_ang + (size_t)i * max_nbr_ang;
+  int count = 0, count_a = 0;
+  int jr = srow[min(lane, max_skin - 1)];
+  for (int base = 0; base < n; base += 64) {
+    const bool valid = base + lane < n;
+    const int j = valid ? jr : i;  // slots past the row end hold stale values: never dereferenced
+    jr = srow[min(base + 64 + lane, max_skin - 1)];
+    float vx = coords[3 * j] - xi, vy = coords[3 * j + 1] - yi, vz = coords[3 * j + 2] - zi;
+    if (pbc) fnl_min_image(vx, vy, vz, cl);
+    const float d12 = fnl_d12(vx, vy, vz);
+    const bool keep = valid && d12 < rc2;
+    const bool keep_a = valid && d12 < rca2;
+    const unsigned long long m = __ballot(keep);
+    const unsigned long long ma = __ballot(keep_a);
+    if (keep) {
+      const int e = count + fnl_rank_below(m);
+      if (e < max_nbr) row[e] = j;
+      if (keep_a) {
+        const int ea = count_a + fnl_rank_below(ma);
+        if (ea < max_nbr_ang && e < max_nbr) row_a[ea] = e;
+      }
+    }
+    count += __popcll(m);
+    count_a += __popcll(ma);
+  }
+  if (lane == 0) {
+    n_nbr[i] = count;
+    n_nbr_ang[i] = count_a;
+    if (count > max_nbr || count_a > max_nbr_ang || ns > max_skin) {
+      atomicOr(&status[FNL_STATUS_OVERFLOW], 1);
+      atomicOr(&status[FNL_STATUS_STICKY], 1);
+    }
   }
 }
 
@@ -554,27 +629,30 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
                                 const int32_t* batch_index, const int32_t* sys_offsets,
                                 int32_t n_atoms, int32_t n_sys, int32_t pbc_mode,
                                 const float* cells, const float* reciprocal_cells,
-                                const int32_t* grid, int32_t n_cells, float cutoff,
-                                float cutoff_ang, int32_t max_nbr, int32_t max_nbr_ang,
+                                const int32_t* grid, int32_t n_cells, double cutoff,
+                                double cutoff_ang, int32_t max_nbr, int32_t max_nbr_ang,
                                 void* workspace, size_t workspace_bytes, int32_t* nbr,
                                 int32_t* n_nbr, int32_t* nbr_ang, int32_t* n_nbr_ang,
                                 int32_t* status, fnl_stream_t stream) {
   if (n_atoms < 0 || n_sys < 1 || n_cells < 1 || max_nbr < 1 || max_nbr_ang < 1) return FNL_ERR_INVALID;
-  if (!(cutoff > 0.0f) || cutoff_ang > cutoff) return FNL_ERR_INVALID;
+  if (!(cutoff > 0.0) || cutoff_ang > cutoff) return FNL_ERR_INVALID;
   if (pbc_mode != FNL_PBC_NONE && pbc_mode != FNL_PBC_MINIMUM_IMAGE) return FNL_ERR_UNSUPPORTED;
   if (pbc_mode == FNL_PBC_MINIMUM_IMAGE && (!cells || !reciprocal_cells)) return FNL_ERR_INVALID;
   if (!status || !grid || !sys_offsets) return FNL_ERR_INVALID;
-  if (n_atoms > 0 && (!coords || !species_idx || !batch_index || !nbr || !n_nbr || !nbr_ang || !n_nbr_ang))
-    return FNL_ERR_INVALID;
+  if (n_atoms > 0 && (!coords || !species_idx || !batch_index || !nbr || !n_nbr)) return FNL_ERR_INVALID;
+  if ((nbr_ang == nullptr) != (n_nbr_ang == nullptr)) return FNL_ERR_INVALID;
   if (!workspace || workspace_bytes < fnl_nblist_workspace(n_atoms, n_sys, n_cells)) return FNL_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
   NblistWs ws;
   ws_layout(n_atoms, n_sys, n_cells, (char*)workspace, &ws);
-  const float rc2 = (float)((double)cutoff * (double)cutoff);
-  const float rca2 = (float)((double)cutoff_ang * (double)cutoff_ang);
+  // thresholds: the float32 rounding of the squared double cutoff, as the reference's
+  // float32 d12 < python-float rc**2 comparison (preprocessing.py:266, :557)
+  const float rc2 = (float)(cutoff * cutoff);
+  // without angular rows no slot passes the angular test
+  const float rca2 = nbr_ang ? (float)(cutoff_ang * cutoff_ang) : -1.0f;
 
   hipLaunchKernelGGL(k_grid, dim3(n_sys), dim3(256), 0, s, coords, species_idx, sys_offsets, pbc_mode,
-                     reciprocal_cells, grid, cutoff, ws.grid, status, ws.cell_count, ws.cell_fill, n_cells);
+                     reciprocal_cells, grid, (float)cutoff, ws.grid, status, ws.cell_count, ws.cell_fill, n_cells);
   FNL_LAUNCH_CHECK();
   if (n_atoms == 0) return FNL_OK;
   hipLaunchKernelGGL(k_bin, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, s, coords, species_idx,
@@ -594,6 +672,35 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
                      n_atoms, cells, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_start, ws.sorted,
                      ws.pos, rc2, rca2, max_nbr, max_nbr_ang, nbr, n_nbr, nbr_ang, n_nbr_ang, status,
                      n_cells);
+  FNL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_row_max, dim3(std::min(16, (n_atoms + 4095) / 4096)), dim3(1024), 0, s, n_nbr, n_nbr_ang,
+                     n_atoms, status);
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
+}
+
+extern "C" int fnl_nblist_skin_update(const float* coords, const int32_t* batch_index, int32_t n_atoms,
+                                      int32_t pbc_mode, const float* cells, const float* reciprocal_cells,
+                                      const int32_t* nbr_skin, const int32_t* n_skin, int32_t max_skin,
+                                      double cutoff, double cutoff_ang, int32_t max_nbr, int32_t max_nbr_ang,
+                                      int32_t* nbr, int32_t* n_nbr, int32_t* nbr_ang, int32_t* n_nbr_ang,
+                                      int32_t* status, fnl_stream_t stream) {
+  if (n_atoms < 0 || max_skin < 1 || max_nbr < 1 || max_nbr_ang < 1) return FNL_ERR_INVALID;
+  if (!(cutoff > 0.0) || cutoff_ang > cutoff || !status) return FNL_ERR_INVALID;
+  if (pbc_mode != FNL_PBC_NONE && pbc_mode != FNL_PBC_MINIMUM_IMAGE) return FNL_ERR_UNSUPPORTED;
+  if (pbc_mode == FNL_PBC_MINIMUM_IMAGE && (!cells || !reciprocal_cells)) return FNL_ERR_INVALID;
+  if (n_atoms > 0 && (!coords || !batch_index || !nbr_skin || !n_skin || !nbr || !n_nbr || !nbr_ang || !n_nbr_ang))
+    return FNL_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  const float rc2 = (float)(cutoff * cutoff);
+  const float rca2 = (float)(cutoff_ang * cutoff_ang);
+  hipLaunchKernelGGL(k_status_clear, dim3(1), dim3(64), 0, s, status);
+  FNL_LAUNCH_CHECK();
+  if (n_atoms == 0) return FNL_OK;
+  constexpr int W = 4;
+  hipLaunchKernelGGL(k_skin_filter<W>, dim3(fnl_blocks(n_atoms, W)), dim3(64 * W), 0, s, coords, batch_index,
+                     n_atoms, pbc_mode, cells, reciprocal_cells, nbr_skin, n_skin, max_skin, rc2, rca2, max_nbr,
+                     max_nbr_ang, nbr, n_nbr, nbr_ang, n_nbr_ang, status);
   FNL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_row_max, dim3(std::min(16, (n_atoms + 4095) / 4096)), dim3(1024), 0, s, n_nbr, n_nbr_ang,
                      n_atoms, status);

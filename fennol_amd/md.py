@@ -2,8 +2,10 @@
 
 Mirrors fennol_md's step (md/integrate.py:570-600): ``integrate`` (BAOAB with
 merged half kicks, :286-294, LGV O-step md/thermostats.py:108-120), then the
-neighbour-list rebuild (``update_graphs``, :515-566, every step without a
-skin), then ``update_observables`` (energy and forces, :298-339).  Positions
+neighbour-list update (``update_graphs``, :515-566: a full rebuild every
+step without a skin; with ``nblist_skin`` a full rebuild every
+``nblist_stride`` steps and re-filters of the skin list in between), then
+``update_observables`` (energy and forces, :298-339).  Positions
 and velocities are FP64 on the device; the model reads a float32 copy written
 by the integrator kernel.  Units are fennol_md's (Angstrom, ps, kcal/mol);
 the model's energies are converted with its energy unit (integrate.py:303-305).
@@ -20,6 +22,17 @@ from . import _abi, units
 from .potential import SEG_ATOMS, ANIPotential, SystemBuffers
 
 
+def nblist_schedule(skin: float, dt_fs: float, stride: int = -1) -> int:
+    """Steps between full rebuilds (md/integrate.py:488-507): 1 without a skin; else the
+    given stride, or by default the time a 2 A skin lasts at 300 K (40 fs) scaled by
+    skin / 2 A."""
+    if skin <= 0:
+        return 1
+    if stride <= 0:
+        stride = int(math.floor(skin / 2.0 * 40.0 / dt_fs))
+    return max(int(stride), 1)
+
+
 class LangevinMD:
     """Device-resident BAOAB Langevin (``thermostat="LGV"``) or NVE dynamics.
 
@@ -28,18 +41,30 @@ class LangevinMD:
     velocities v ~ N(0, kT/m) (thermostats.py:64-67), stream 0 the O-step
     noise of step n.  For NVE the initial velocities are rescaled to exactly
     kT per degree of freedom (thermostats.py:172-189).
+
+    ``nblist_skin`` (A, > 0 enables it), ``nblist_stride`` (steps between full
+    rebuilds; <= 0: from the skin, md/integrate.py:498-504) and
+    ``nblist_warmup_fs`` (full rebuilds every step before that time) follow
+    fennol_md's keywords of the same names (integrate.py:477-513).
     """
 
     def __init__(self, potential: ANIPotential, system: dict, dt_fs: float = 0.5, temperature: float = 300.0,
                  gamma_thz: float = 1.0, seed: int = 0, thermostat: str = "LGV", energy_unit: str = "eV",
-                 check_every: int = 1, sb: SystemBuffers | None = None):
+                 check_every: int = 1, sb: SystemBuffers | None = None, nblist_skin: float = -1.0,
+                 nblist_stride: int = -1, nblist_warmup_fs: float = -1.0):
         self.pot = potential
         self.lib = potential.lib
         dev = potential.device
         species = np.asarray(system["species"], np.int32)
         natoms = system.get("natoms")
         cells = system.get("cells")
-        self.sb = sb if sb is not None else potential.setup(species, natoms, cells)
+        skin = float(nblist_skin) if nblist_skin > 0 else 0.0
+        self.sb = sb if sb is not None else potential.setup(species, natoms, cells, skin=skin)
+        if self.sb.skin != skin:
+            raise ValueError(f"buffers built with skin {self.sb.skin}, MD asks for {skin}")
+        self.nblist_stride = nblist_schedule(skin, dt_fs, nblist_stride)
+        self.nblist_warmup = int(nblist_warmup_fs / dt_fs) if nblist_warmup_fs > 0 else 0
+        self.nblist_countdown = 0
         self.n = species.shape[0]
         mass_da = units.ATOMIC_MASSES[np.clip(species, 0, None)]
         self.mass = np.where(species > 0, mass_da / units.DA, 1.0)       # md/initial.py:79-108
@@ -69,6 +94,7 @@ class LangevinMD:
         # step counter on the device: the O-step noise of a captured graph advances on replay
         self.step_dev = torch.zeros(1, dtype=torch.int64, device=dev)
         self.graph = None
+        self.graph_skin = None
         self.x = torch.tensor(np.asarray(system["coordinates"], np.float64), device=dev).contiguous()
         self.x32 = self.x.float().contiguous()
         self.v = torch.empty((self.n, 3), dtype=torch.float64, device=dev)
@@ -104,43 +130,73 @@ class LangevinMD:
         _abi.check(rc, "fnl_baoab_step")
         _abi.check(self.lib.fnl_counter_add(self.step_dev.data_ptr(), 1, s), "fnl_counter_add")
 
+    def _graph_update(self) -> bool | str:
+        """The neighbour-list action of the next step (update_graphs, integrate.py:515-566):
+        True = full rebuild, "skin" = skin re-filter; advances the countdown."""
+        if self.nblist_countdown <= 0 or self.istep < self.nblist_warmup:
+            self.nblist_countdown = self.nblist_stride - 1
+            return True
+        self.nblist_countdown -= 1
+        return "skin"
+
     def step(self, check: bool | None = None, stream=None) -> None:
-        """integrate -> rebuild neighbour lists -> energy and forces at the new positions."""
+        """integrate -> update neighbour lists -> energy and forces at the new positions."""
+        rebuild = self._graph_update()
         self._integrate(stream)
         self.istep += 1
         if check is None:
             check = self.check_every > 0 and (self.istep % self.check_every) == 0
-        self.pot.compute(self.sb, self.x32, forces=True, rebuild=True, check=check, stream=stream)
+        self.pot.compute(self.sb, self.x32, forces=True, rebuild=rebuild, check=check, stream=stream)
 
     def run(self, nsteps: int) -> None:
         for _ in range(nsteps):
             self.step()
 
     # ---------------------------------------------------------- graph replay
+    def _record(self, rebuild: bool | str, stream=None) -> None:
+        """One step with a fixed neighbour-list action and no host reads or counters."""
+        self._integrate(stream)
+        self.pot.compute(self.sb, self.x32, forces=True, rebuild=rebuild, check=False, stream=stream)
+
     def capture(self, steps_per_graph: int = 1) -> None:
-        """Record `steps_per_graph` MD steps in a HIP graph (no host checks inside;
-        capacities must have headroom: construct with check_every=0)."""
+        """Record MD steps in HIP graphs (no host checks inside; capacities must have
+        headroom: construct with check_every=0).  Without a skin list one graph holds
+        `steps_per_graph` steps; with one, two one-step graphs (full rebuild, skin
+        re-filter) are recorded and :meth:`replay` follows the rebuild schedule."""
         if self.check_every != 0:
             raise ValueError("graph capture needs check_every=0 (no host reads inside a step)")
+        skin = self.nblist_stride > 1
+        if skin and steps_per_graph != 1:
+            raise ValueError("with a skin list every graph holds one step")
         timer, self.pot.timer = self.pot.timer, None
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):  # warm the allocator and kernels outside the capture
             self.step(check=False)
-        torch.cuda.current_stream().wait_stream(side)
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            for _ in range(steps_per_graph):
+            if skin:
                 self.step(check=False)
-        self.istep -= steps_per_graph  # capture does not execute the steps
-        self.graph = g
+        torch.cuda.current_stream().wait_stream(side)
+        kinds = (True, "skin") if skin else (True,)
+        graphs = {}
+        for kind in kinds:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                for _ in range(steps_per_graph):
+                    self._record(kind)
+            graphs[kind] = g
+        self.graph = graphs[True]
+        self.graph_skin = graphs.get("skin")
         self.steps_per_graph = steps_per_graph
         self.pot.timer = timer
 
     def replay(self, n_graphs: int = 1) -> None:
         for _ in range(n_graphs):
-            self.graph.replay()
-            self.istep += self.steps_per_graph
+            if self.graph_skin is None:
+                self.graph.replay()
+                self.istep += self.steps_per_graph
+            else:
+                (self.graph if self._graph_update() is True else self.graph_skin).replay()
+                self.istep += 1
 
     # -------------------------------------------------------------- observables
     def energies(self) -> np.ndarray:

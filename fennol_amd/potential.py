@@ -188,6 +188,13 @@ class SystemBuffers:
     seg_offsets: torch.Tensor = None    # int32 [B+1] reduction segments per system
     n_seg: int = 0
     partials: torch.Tensor = None       # float64 [n_seg, 19]
+    # Verlet skin: rows of the list built with cutoff + skin (radial only), re-filtered to the
+    # radial/angular rows above on the steps between builds; skin == 0: no skin list
+    skin: float = 0.0
+    max_skin: int = 0
+    nbr_skin: torch.Tensor = None
+    n_skin: torch.Tensor = None
+    status_skin: torch.Tensor = None
     extra: dict = field(default_factory=dict)
 
 
@@ -256,7 +263,10 @@ class ANIPotential:
             first += int(np.prod(caps))
         return g, max(first, 1)
 
-    def setup(self, species, natoms=None, cells=None) -> SystemBuffers:
+    def setup(self, species, natoms=None, cells=None, skin: float = 0.0) -> SystemBuffers:
+        """Device buffers of one batch.  ``skin`` > 0 (A) adds a Verlet skin list: full builds
+        use cutoff + skin and :meth:`update_skin` re-filters it between builds
+        (preprocessing.py:640-696)."""
         species = np.asarray(species, np.int32)
         N = species.shape[0]
         natoms = np.array([N], np.int32) if natoms is None else np.asarray(natoms, np.int32)
@@ -294,6 +304,12 @@ class ANIPotential:
             cells=tc, rcells=tr, grid=torch.tensor(grid, device=dev), n_cells=n_cells,
             max_nbr=self.init_max_nbr, max_nbr_ang=self.init_max_nbr_ang,
         )
+        if skin < 0:
+            raise ValueError("skin must be >= 0")
+        if skin > 0:
+            sb.skin = float(skin)
+            grow = ((self.config.cutoff + skin) / self.config.cutoff) ** 3
+            sb.max_skin = _next_mult(int(grow * self.init_max_nbr), 8)
         self._alloc_nblist(sb)
         self._setup_tiles(sb)
         kp = self.config.k_pad
@@ -435,6 +451,14 @@ class ANIPotential:
         sb.status = torch.zeros(_abi.FNL_STATUS_WORDS, dtype=torch.int32, device=dev)
         wsb = self.lib.fnl_nblist_workspace(N, max(n_sys, 1), max(n_cells, 1))
         sb.workspace = torch.empty(max(int(wsb), 256), dtype=torch.uint8, device=dev)
+        if sb.skin > 0:
+            self._alloc_skin(sb)
+
+    def _alloc_skin(self, sb: SystemBuffers) -> None:
+        N = max(sb.extra.get("max_atoms", sb.n_atoms), 1)
+        sb.nbr_skin = torch.zeros((N, sb.max_skin), dtype=torch.int32, device=self.device)
+        sb.n_skin = torch.zeros(N, dtype=torch.int32, device=self.device)
+        sb.status_skin = torch.zeros(_abi.FNL_STATUS_WORDS, dtype=torch.int32, device=self.device)
 
     def _setup_tiles(self, sb: SystemBuffers) -> None:
         """Atoms grouped by network, cut into tiles of at most tile_rows rows (SpeciesIndexer role)."""
@@ -459,8 +483,19 @@ class ANIPotential:
 
     # --------------------------------------------------------------- kernels
     def build_nblist(self, sb: SystemBuffers, coords32: torch.Tensor, stream=None) -> None:
+        """Full build (GraphGenerator.process + GraphFilter.process, preprocessing.py:369-638,
+        909-956); with a skin, the cutoff + skin list followed by :meth:`update_skin`."""
         s = _abi.stream_handle(stream)
         P = _abi.ptr
+        if sb.skin > 0:
+            rc = self.lib.fnl_nblist_build(
+                P(coords32), P(sb.species_idx), P(sb.batch_index), P(sb.sys_offsets), sb.n_atoms, sb.n_sys,
+                sb.pbc, P(sb.cells), P(sb.rcells), P(sb.grid), sb.n_cells, self.config.cutoff + sb.skin,
+                self.config.cutoff_angular, sb.max_skin, 1, P(sb.workspace), sb.workspace.numel(),
+                P(sb.nbr_skin), P(sb.n_skin), None, None, P(sb.status_skin), s)
+            _abi.check(rc, "fnl_nblist_build")
+            self.update_skin(sb, coords32, stream)
+            return
         rc = self.lib.fnl_nblist_build(
             P(coords32), P(sb.species_idx), P(sb.batch_index), P(sb.sys_offsets), sb.n_atoms, sb.n_sys, sb.pbc,
             P(sb.cells), P(sb.rcells), P(sb.grid), sb.n_cells, self.config.cutoff, self.config.cutoff_angular,
@@ -468,21 +503,56 @@ class ANIPotential:
             P(sb.nbr_ang), P(sb.n_nbr_ang), P(sb.status), s)
         _abi.check(rc, "fnl_nblist_build")
 
+    def update_skin(self, sb: SystemBuffers, coords32: torch.Tensor, stream=None) -> None:
+        """Radial and angular rows at coords32 from the skin list of the last full build
+        (GraphGenerator.update_skin preprocessing.py:640-696, GraphFilter.update_skin :958-960,
+        GraphAngularExtension.update_skin :1278-1280)."""
+        if sb.skin <= 0:
+            raise ValueError("these buffers have no skin list (setup(..., skin>0))")
+        P = _abi.ptr
+        rc = self.lib.fnl_nblist_skin_update(
+            P(coords32), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells), P(sb.nbr_skin),
+            P(sb.n_skin), sb.max_skin, self.config.cutoff, self.config.cutoff_angular, sb.max_nbr,
+            sb.max_nbr_ang, P(sb.nbr), P(sb.n_nbr), P(sb.nbr_ang), P(sb.n_nbr_ang), P(sb.status),
+            _abi.stream_handle(stream))
+        _abi.check(rc, "fnl_nblist_skin_update")
+
     def check_reallocate(self, sb: SystemBuffers, coords32: torch.Tensor, mult: float = 1.05,
                          add: int = 4) -> bool:
         """Read the status words; on overflow grow the rows and rebuild (preprocessing.py:357-367).
 
-        Returns True if the lists were reallocated.  This synchronises the stream.
+        With a skin list, an overflow of the skin rows grows them and redoes the full build; an
+        overflow of the rows re-filtered from it grows those and redoes only the re-filter, so
+        the skin list of the last full build is kept.  Returns True if the lists were
+        reallocated.  This synchronises the stream.
         """
+        grown = False
+        if sb.skin > 0:
+            st = sb.status_skin.cpu().numpy()
+            if st[_abi.FNL_STATUS_OVERFLOW] & 2:
+                raise RuntimeError("cell grid capacities inconsistent with the system")
+            if st[_abi.FNL_STATUS_OVERFLOW]:
+                sb.max_skin = _next_mult(int(mult * max(st[_abi.FNL_STATUS_MAX_NBR], sb.max_skin)) + add, 8)
+                self._alloc_skin(sb)
+                self.build_nblist(sb, coords32)
+                if sb.status_skin.cpu().numpy()[_abi.FNL_STATUS_OVERFLOW]:
+                    raise RuntimeError("skin list still overflows after reallocation")
+                grown = True
         st = sb.status.cpu().numpy()
         if st[_abi.FNL_STATUS_OVERFLOW] & 2:
             raise RuntimeError("cell grid capacities inconsistent with the system")
         if st[_abi.FNL_STATUS_OVERFLOW] == 0:
-            return False
+            return grown
         sb.max_nbr = _next_mult(int(mult * max(st[_abi.FNL_STATUS_MAX_NBR], sb.max_nbr)) + add, 8)
         sb.max_nbr_ang = _next_mult(int(mult * max(st[_abi.FNL_STATUS_MAX_NBR_ANG], sb.max_nbr_ang)) + add, 8)
-        self._alloc_nblist(sb)
-        self.build_nblist(sb, coords32)
+        if sb.skin > 0:
+            nbr_skin, n_skin, status_skin = sb.nbr_skin, sb.n_skin, sb.status_skin
+            self._alloc_nblist(sb)
+            sb.nbr_skin, sb.n_skin, sb.status_skin = nbr_skin, n_skin, status_skin
+            self.update_skin(sb, coords32)
+        else:
+            self._alloc_nblist(sb)
+            self.build_nblist(sb, coords32)
         st = sb.status.cpu().numpy()
         if st[_abi.FNL_STATUS_OVERFLOW]:
             raise RuntimeError("neighbour list still overflows after reallocation")
@@ -535,9 +605,17 @@ class ANIPotential:
         _abi.check(rc, "fnl_system_reduce")
 
     def compute(self, sb: SystemBuffers, coords32: torch.Tensor, forces: bool = True,
-                rebuild: bool = True, check: bool = True, stream=None) -> None:
-        """Energy (and gradients) of the batch at coords32 (float32 [N,3] device tensor)."""
-        if rebuild:
+                rebuild: bool | str = True, check: bool = True, stream=None) -> None:
+        """Energy (and gradients) of the batch at coords32 (float32 [N,3] device tensor).
+
+        ``rebuild``: True = full neighbour-list build, "skin" = re-filter of the skin list
+        (:meth:`update_skin`), False = keep the current rows."""
+        if rebuild == "skin":
+            with self._stage("skin"):
+                self.update_skin(sb, coords32, stream)
+            if check:
+                self.check_reallocate(sb, coords32)
+        elif rebuild:
             with self._stage("nblist"):
                 self.build_nblist(sb, coords32, stream)
             if check:
@@ -557,8 +635,12 @@ class ANIPotential:
     def check_sticky(self, sb: SystemBuffers, clear: bool = True) -> None:
         """Raise if any build since the last clear overflowed a capacity (synchronises)."""
         v = int(sb.status[_abi.FNL_STATUS_STICKY].item())
+        if sb.skin > 0:
+            v |= int(sb.status_skin[_abi.FNL_STATUS_STICKY].item())
         if clear:
             sb.status[_abi.FNL_STATUS_STICKY:_abi.FNL_STATUS_STICKY + 1].zero_()
+            if sb.skin > 0:
+                sb.status_skin[_abi.FNL_STATUS_STICKY:_abi.FNL_STATUS_STICKY + 1].zero_()
         if v:
             raise RuntimeError("neighbour-list capacity overflowed during the run")
 
@@ -569,12 +651,18 @@ class ANIPotential:
         st = sb.status.cpu().numpy()
         want = _next_mult(int(margin * st[_abi.FNL_STATUS_MAX_NBR]) + add, 8)
         want_a = _next_mult(int(margin * st[_abi.FNL_STATUS_MAX_NBR_ANG]) + add, 8)
-        if want > sb.max_nbr or want_a > sb.max_nbr_ang:
+        want_s = sb.max_skin
+        if sb.skin > 0:
+            want_s = _next_mult(int(margin * sb.status_skin.cpu().numpy()[_abi.FNL_STATUS_MAX_NBR]) + add, 8)
+        if want > sb.max_nbr or want_a > sb.max_nbr_ang or want_s > sb.max_skin:
             sb.max_nbr = max(sb.max_nbr, want)
             sb.max_nbr_ang = max(sb.max_nbr_ang, want_a)
+            sb.max_skin = max(sb.max_skin, want_s)
             self._alloc_nblist(sb)
             self.build_nblist(sb, coords32)
         sb.status[_abi.FNL_STATUS_STICKY:_abi.FNL_STATUS_STICKY + 1].zero_()
+        if sb.skin > 0:
+            sb.status_skin[_abi.FNL_STATUS_STICKY:_abi.FNL_STATUS_STICKY + 1].zero_()
 
     def forces(self, sb: SystemBuffers, scale: float = 1.0, stream=None) -> torch.Tensor:
         out = torch.empty((sb.n_atoms, 3), dtype=torch.float32, device=self.device)

@@ -69,7 +69,9 @@ const char* fnl_strerror(int code);
  *   nbr_ang[i*max_nbr_ang + a]  radial slot e whose d12 < cutoff_ang^2
  * A pair (i, j) is kept iff d12 < cutoff^2 with d12 computed as the reference
  * does in float32: vec = x_j - x_i; s = -rint(vec . R); vec += s . C;
- * d12 = (vx*vx + vy*vy) + vz*vz (no contraction).  The triplets of centre i are
+ * d12 = (vx*vx + vy*vy) + vz*vz (no contraction).  The cutoffs are doubles
+ * (the model's Python floats): the threshold is cutoff^2 squared in double and
+ * rounded to float32, as the reference compares float32 d12 with rc**2.  The triplets of centre i are
  * all slot pairs a < b of its angular row (preprocessing.py:1237-1245).
  *
  * grid: int32[n_sys][4] = (cap_x, cap_y, cap_z, first_cell) -- the caller's
@@ -86,12 +88,36 @@ int fnl_nblist_build(const float* coords, const int32_t* species_idx,
                      int32_t n_atoms, int32_t n_sys, int32_t pbc_mode,
                      const float* cells, const float* reciprocal_cells,
                      const int32_t* grid, int32_t n_cells,
-                     float cutoff, float cutoff_ang,
+                     double cutoff, double cutoff_ang,
                      int32_t max_nbr, int32_t max_nbr_ang,
                      void* workspace, size_t workspace_bytes,
                      int32_t* nbr, int32_t* n_nbr,
                      int32_t* nbr_ang, int32_t* n_nbr_ang,
                      int32_t* status, fnl_stream_t stream);
+
+/* nbr_ang and n_nbr_ang may both be NULL: radial rows only (the list built
+ * with cutoff + skin for fnl_nblist_skin_update). */
+
+/* Skin update: the radial and angular rows at the current positions selected
+ * from the rows of a list built by fnl_nblist_build with cutoff + skin
+ * (nbr_skin / n_skin / max_skin, radial rows only).  Replaces
+ * GraphGenerator.update_skin (models/preprocessing.py:640-696) with the
+ * re-filter of GraphFilter.update_skin (:958-960) and the slot table of
+ * GraphAngularExtension.update_skin (:1278-1280).  d12 is recomputed with the
+ * full build's float32 arithmetic; rows keep the order of the skin rows.
+ * Exact as long as no atom moved more than skin/2 since the build (the
+ * reference's approximation); with the minimum-image convention, cutoff +
+ * skin must stay below half of every lattice-plane spacing.  status: as for
+ * fnl_nblist_build (overflow also if a skin row had overflowed). */
+int fnl_nblist_skin_update(const float* coords, const int32_t* batch_index,
+                           int32_t n_atoms, int32_t pbc_mode,
+                           const float* cells, const float* reciprocal_cells,
+                           const int32_t* nbr_skin, const int32_t* n_skin, int32_t max_skin,
+                           double cutoff, double cutoff_ang,
+                           int32_t max_nbr, int32_t max_nbr_ang,
+                           int32_t* nbr, int32_t* n_nbr,
+                           int32_t* nbr_ang, int32_t* n_nbr_ang,
+                           int32_t* status, fnl_stream_t stream);
 
 /* Flat directed edge list in the reference's dictionary layout
  * (edge_src / edge_dst / d12 / pbc_shifts, preprocessing.py:328-341) from the

@@ -11,6 +11,10 @@ What pins each part (checked in ``tests/test_oracle.py``):
   images (min-image result == nearest of the 27 images when rc < L/2),
   shell counts 6/12/8/6/24 of a perfect simple-cubic lattice, symmetry of the
   directed list.
+* ``skin_update`` (preprocessing.py:640-696): at the build positions it
+  equals ``half_pairs`` at the cutoff (pairs, d12 bits, shifts); after moves
+  below skin/2 per atom it equals ``half_pairs`` at the new positions (the
+  Verlet-list guarantee); after larger moves it is a subset of it.
 * ``angular_triplets`` (preprocessing.py:1085-1181): T = sum_i C(n_i, 2) and a
   brute-force enumeration of neighbour pairs.
 * ``cosine_switch`` (misc.py:674-678): fc(0)=1, fc(rc)=0, fc'(rc)=0.
@@ -40,6 +44,7 @@ __all__ = [
     "md_units",
     "check_input",
     "half_pairs",
+    "skin_update",
     "symmetrize",
     "filter_graph",
     "angular_triplets",
@@ -273,6 +278,38 @@ def half_pairs(coords, natoms, cutoff, cells=None, reciprocal_cells=None,
     d12 = np.concatenate(d12_all).astype(np.float32)
     sh = None if cells is None else np.concatenate(sh_all).astype(np.float32)
     return src, dst, d12, sh
+
+
+def skin_update(src_skin, dst_skin, shifts_skin, coords, cutoff, cells=None, batch_index=None):
+    """Pairs of a skin list still within ``cutoff`` at new coordinates.
+
+    Restates GraphGenerator.update_skin (preprocessing.py:640-696) on the half
+    list: ``vec = x[dst] - x[src] + shift . C`` with the shifts stored at the
+    build (the cell of each pair's system via ``batch_index[src]`` when there
+    are several, :655-659), ``d12 = sum(vec**2)`` in float32 and the strict test
+    ``d12 < rc^2`` (:662-663), keeping the skin list's order (mask_filter_1d).
+    Returns (src, dst, d12, shifts); shifts is None without cells.
+    """
+    src = np.asarray(src_skin, np.int64)
+    dst = np.asarray(dst_skin, np.int64)
+    x = np.asarray(coords, np.float32)
+    vec = (x[dst] - x[src]).astype(np.float32)
+    sh = None
+    if cells is not None:
+        C = np.asarray(cells, np.float32).reshape(-1, 3, 3)
+        sh = np.asarray(shifts_skin, np.float32)
+        if C.shape[0] == 1:
+            vec = vec + _f32_dot3(sh, C[0])
+        else:
+            bidx = np.asarray(batch_index)[src]
+            for b in range(C.shape[0]):
+                m = bidx == b
+                if np.any(m):
+                    vec[m] = vec[m] + _f32_dot3(sh[m], C[b])
+    d12 = _f32_d2(vec.astype(np.float32))
+    keep = d12 < np.float32(cutoff ** 2)
+    return (src[keep].astype(np.int32), dst[keep].astype(np.int32), d12[keep],
+            None if sh is None else sh[keep])
 
 
 def symmetrize(src, dst, d12, shifts):

@@ -49,3 +49,47 @@ def triclinic_box(n_atoms=120, seed=0):
     species = rng.choice(np.array([1, 6, 7, 8], np.int32), size=n_atoms)
     return {"species": species, "coordinates": x, "natoms": np.array([n_atoms], np.int32),
             "cells": cell[None], "reciprocal_cells": np.linalg.inv(cell)[None]}
+
+
+def device_half_list(pot, sb, x):
+    """Half list (src < dst, sorted) with float32 d12 from the device rows of ``sb`` at x,
+    after checking that the directed rows are symmetric and that every angular row holds
+    exactly the radial slots with d12 < rca^2."""
+    import torch
+
+    from fennol_amd import _abi
+
+    nbr, n_nbr = sb.nbr.cpu().numpy(), sb.n_nbr.cpu().numpy()
+    assert np.all(n_nbr <= sb.max_nbr)
+    rs = np.concatenate([[0], np.cumsum(n_nbr)]).astype(np.int32)
+    E = int(rs[-1])
+    dev = x.device
+    es = torch.empty(max(E, 1), dtype=torch.int32, device=dev)
+    ed = torch.empty(max(E, 1), dtype=torch.int32, device=dev)
+    d12 = torch.empty(max(E, 1), dtype=torch.float32, device=dev)
+    sh = torch.empty((max(E, 1), 3), dtype=torch.float32, device=dev)
+    rst = torch.tensor(rs[:-1], device=dev)
+    P = _abi.ptr
+    _abi.check(pot.lib.fnl_nblist_export(P(x), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
+                                         P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(rst), P(es), P(ed), P(d12), P(sh),
+                                         _abi.stream_handle()), "export")
+    es, ed, d12 = es.cpu().numpy()[:E], ed.cpu().numpy()[:E], d12.cpu().numpy()[:E]
+    i, j = rows_to_pairs(nbr, n_nbr, sb.max_nbr)
+    fwd = set(zip(i.tolist(), j.tolist()))
+    assert fwd == set(zip(j.tolist(), i.tolist())) and len(fwd) == len(i)
+    na, nba = sb.n_nbr_ang.cpu().numpy(), sb.nbr_ang.cpu().numpy()
+    assert np.all(na <= sb.max_nbr_ang)
+    rca2 = np.float32(pot.config.cutoff_angular ** 2)
+    for a in range(sb.n_atoms):
+        slots = np.sort(nba[a, : na[a]])
+        np.testing.assert_array_equal(slots, np.nonzero(d12[rs[a]: rs[a + 1]] < rca2)[0])
+    m = es < ed
+    order = np.lexsort((ed[m], es[m]))
+    return es[m][order], ed[m][order], d12[m][order]
+
+
+def canonical_half(src, dst, d12):
+    """Oracle half list sorted by (min, max) with its float32 d12."""
+    lo, hi = np.minimum(src, dst), np.maximum(src, dst)
+    o = np.lexsort((hi, lo))
+    return lo[o].astype(np.int32), hi[o].astype(np.int32), np.asarray(d12, np.float32)[o]

@@ -171,6 +171,83 @@ def test_md_graph_replay_matches_eager(gpu):
         np.testing.assert_allclose(a, b, rtol=0, atol=1e-6)
 
 
+def test_md_skin_matches_rebuild_every_step(gpu):
+    """Langevin MD with a 1 A skin, full rebuilds every 10 steps and re-filters in between,
+    follows the rebuild-every-step trajectory (same seed) to rounding: no atom moves 0.5 A
+    in 10 steps of 0.5 fs, so every re-filtered list is the fresh list (only the order of
+    the rows, hence of the AEV sums, differs)."""
+    print("[api] skin MD", flush=True)
+    from fennol_amd.md import LangevinMD
+    from fennol_amd.potential import ANIConfig, ANIPotential
+
+    s = water_box_small(64, 13)
+    cfg = ANIConfig(species_order=("H", "C", "N", "O", "S", "F", "Cl"))
+    params = O.init_mlp_params(11, cfg.aev_length)
+    out = []
+    for skin in (-1.0, 1.0):
+        pot = ANIPotential(cfg, params, precision="fp32", max_nbr=96, max_nbr_ang=48)  # no regrowth
+        md = LangevinMD(pot, s, dt_fs=0.5, temperature=300.0, seed=9, nblist_skin=skin, nblist_stride=10)
+        assert md.nblist_stride == (10 if skin > 0 else 1)
+        kinds = []
+        build, upd = pot.build_nblist, pot.update_skin
+        pot.build_nblist = lambda *a, **k: (kinds.append("full"), build(*a, **k))[1]
+        pot.update_skin = lambda *a, **k: (kinds.append("skin"), upd(*a, **k))[1]
+        energies, x_prev, moved = [], md.x.cpu().numpy(), 0.0
+        for k in range(60):
+            md.step()
+            energies.append(md.energies()[0])
+            if k % 10 == 9:
+                x = md.x.cpu().numpy()
+                moved = max(moved, float(np.linalg.norm(x - x_prev, axis=1).max()))
+                x_prev = x
+        md.verify()
+        out.append((md.x.cpu().numpy(), np.array(energies), kinds.count("full"), kinds.count("skin"), moved))
+    (x_a, e_a, f_a, s_a, _), (x_b, e_b, f_b, s_b, moved) = out
+    assert moved < 0.5, moved
+    assert (f_a, s_a) == (60, 0)
+    assert (f_b, s_b) == (6, 60)  # full builds at steps 1, 11, ..., 51, each with its re-filter
+    np.testing.assert_allclose(x_b, x_a, rtol=0, atol=1e-6)
+    np.testing.assert_allclose(e_b, e_a, rtol=1e-6, atol=1e-6)
+
+
+def test_md_skin_graph_replay_matches_eager(gpu):
+    """Two captured one-step graphs (full rebuild, skin re-filter) replayed on the rebuild
+    schedule give the eager skin trajectory."""
+    print("[api] skin graph replay", flush=True)
+    from fennol_amd.md import LangevinMD
+    from fennol_amd.potential import ANIConfig, ANIPotential
+
+    s = water_box_small(64, 11)
+    cfg = ANIConfig(species_order=("H", "C", "N", "O", "S", "F", "Cl"))
+    params = O.init_mlp_params(11, cfg.aev_length)
+    runs = []
+    for use_graph in (False, True):
+        pot = ANIPotential(cfg, params, precision="fp32")
+        md = LangevinMD(pot, s, dt_fs=0.5, temperature=300.0, seed=7, check_every=0, nblist_skin=1.0,
+                        nblist_stride=5)
+        traj = []
+        if use_graph:
+            md.capture()  # two eager warm-up steps
+            for _ in range(12):
+                md.replay(1)
+                torch.cuda.synchronize()
+                _ = md.x.sum().item()
+                traj.append(md.x.cpu().numpy().copy())
+        else:
+            md.step(check=False)
+            md.step(check=False)
+            for _ in range(12):
+                md.step(check=False)
+                torch.cuda.synchronize()
+                traj.append(md.x.cpu().numpy().copy())
+        md.verify()
+        runs.append((traj, md.istep, md.nblist_countdown))
+    (te, ne, ce), (tg, ng, cg) = runs
+    assert ne == ng == 14 and ce == cg
+    for a, b in zip(te, tg):
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-6)
+
+
 def test_serving_results_same_alone_and_in_mix(gpu):
     """Continuous batching: each request's result equals its result when served alone."""
     print("[api] serving mix", flush=True)

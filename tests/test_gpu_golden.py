@@ -15,11 +15,10 @@ import pytest
 import torch
 
 import oracle as O
-from fennol_amd import _abi
 from fennol_amd.md import LangevinMD
 from fennol_amd.potential import ANIPotential
 from tests.golden import make_golden as G
-from tests.helpers import configs, rows_to_pairs
+from tests.helpers import configs, device_half_list
 
 pytestmark = pytest.mark.gpu
 
@@ -41,29 +40,6 @@ def _potential():
     return c, ANIPotential(c, params, precision="fp32")
 
 
-def _export_pairs(pot, sb, x):
-    """Half list (src < dst, sorted) with float32 d12 from the device rows."""
-    nbr, n_nbr = sb.nbr.cpu().numpy(), sb.n_nbr.cpu().numpy()
-    rs = np.concatenate([[0], np.cumsum(np.minimum(n_nbr, sb.max_nbr))]).astype(np.int32)
-    E = int(rs[-1])
-    dev = torch.device("cuda")
-    es = torch.empty(E, dtype=torch.int32, device=dev)
-    ed = torch.empty(E, dtype=torch.int32, device=dev)
-    d12 = torch.empty(E, dtype=torch.float32, device=dev)
-    sh = torch.empty((E, 3), dtype=torch.float32, device=dev)
-    rst = torch.tensor(rs[:-1], device=dev)
-    P = _abi.ptr
-    _abi.check(pot.lib.fnl_nblist_export(P(x), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
-                                         P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(rst), P(es), P(ed), P(d12), P(sh),
-                                         _abi.stream_handle()), "export")
-    es, ed, d12 = es.cpu().numpy(), ed.cpu().numpy(), d12.cpu().numpy()
-    i, j = rows_to_pairs(nbr, n_nbr, sb.max_nbr)
-    assert set(zip(i.tolist(), j.tolist())) == set(zip(j.tolist(), i.tolist()))
-    m = es < ed
-    order = np.lexsort((ed[m], es[m]))
-    return es[m][order], ed[m][order], d12[m][order]
-
-
 @pytest.mark.parametrize("name", ["cluster64", "water2", "triclinic5"])
 def test_golden_system(gpu, name):
     _flush(f"[golden] {name}")
@@ -78,7 +54,7 @@ def test_golden_system(gpu, name):
     pot.compute(sb, x, forces=True)
     F = pot.forces(sb)
     torch.cuda.synchronize()
-    src, dst, d12 = _export_pairs(pot, sb, x)
+    src, dst, d12 = device_half_list(pot, sb, x)
     np.testing.assert_array_equal(src, f["pair_src"])
     np.testing.assert_array_equal(dst, f["pair_dst"])
     np.testing.assert_array_equal(d12.view(np.uint32), f["pair_d12"].view(np.uint32))

@@ -14,7 +14,8 @@ import torch
 import oracle as O
 from fennol_amd import _abi, systems
 from fennol_amd.potential import ANIPotential
-from tests.helpers import canonical_pairs_oracle, configs, rows_to_pairs, triclinic_box, water_box_small
+from tests.helpers import (canonical_half, canonical_pairs_oracle, configs, device_half_list, rows_to_pairs,
+                           triclinic_box, water_box_small)
 
 pytestmark = pytest.mark.gpu
 
@@ -128,6 +129,102 @@ def test_nblist_near_cutoff(gpu):
     half = np.array(sorted((a, b) for a, b in zip(i.tolist(), j.tolist()) if a < b), dtype=np.int64).reshape(-1, 2)
     ref, _ = canonical_pairs_oracle(coords, s["natoms"], cfg[1].cutoff)
     np.testing.assert_array_equal(half, ref.reshape(-1, 2))
+
+
+def _window_pairs(n_pairs, seed=3, rc=5.2):
+    """Isolated atom pairs whose float32 d12 lies in [float32(float32(rc)^2), float32(rc^2)):
+    inside the reference's cutoff, outside a threshold squared in float32."""
+    lo = np.float32(np.float64(np.float32(rc)) ** 2)
+    hi = np.float32(rc ** 2)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    xs = []
+    while len(xs) < 2 * n_pairs:
+        base = np.array([30.0 * (len(xs) // 2), 0.0, 0.0], np.float32) + rng.uniform(-1, 1, 3).astype(np.float32)
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        xj = (base.astype(np.float64) + u * rc * (1 + rng.uniform(-3e-7, 3e-7))).astype(np.float32)
+        v = (xj - base).astype(np.float32)
+        d12 = np.float32(np.float32(v[0] * v[0] + v[1] * v[1]) + v[2] * v[2])
+        if lo <= d12 < hi:
+            xs += [base, xj]
+    return np.array(xs, np.float32)
+
+
+def test_nblist_cutoff_rounding_window(gpu):
+    """Pairs whose d12 falls between the float32 squares of float32(rc) and of rc are kept,
+    as the reference keeps them (its threshold is float32(rc**2))."""
+    _flush("[nblist] cutoff rounding window")
+    coords = _window_pairs(24)
+    s = {"species": np.ones(len(coords), np.int32), "coordinates": coords,
+         "natoms": np.array([len(coords)], np.int32)}
+    cfg = configs()
+    _, pot = _potential(cfg)
+    sb, x = _nblist(pot, s)
+    src, dst, d12 = device_half_list(pot, sb, x)
+    ref, ref_d12 = canonical_pairs_oracle(coords, s["natoms"], cfg[1].cutoff)
+    assert ref.shape[0] == 24
+    np.testing.assert_array_equal(np.stack([src, dst], 1), ref)
+    np.testing.assert_array_equal(d12.view(np.uint32), ref_d12.view(np.uint32))
+
+
+def _skin_cases():
+    return {
+        "cluster64": systems.cluster_config0(0),
+        "water216": water_box_small(216, 2),
+        "triclinic": triclinic_box(150, 3),
+        "batched": systems.batch_systems([systems.lattice_cluster(int(n), np.random.Generator(
+            np.random.PCG64(n))) for n in (8, 40, 200)]),
+    }
+
+
+@pytest.mark.parametrize("name", ["cluster64", "water216", "triclinic", "batched"])
+def test_skin_update_matches_oracle(gpu, name):
+    """Build with a 1 A skin, move every atom by less than 0.5 A, re-filter on the device:
+    pair set and d12 bits equal the oracle's skin_update and a fresh cutoff build."""
+    _flush(f"[skin] {name}")
+    s = _skin_cases()[name]
+    cfg = configs()
+    rc, skin = cfg[1].cutoff, 1.0
+    _, pot = _potential(cfg, max_nbr=16, max_nbr_ang=8)  # small rows: every capacity regrows
+    cells = s.get("cells")
+    rec = None if cells is None else np.linalg.inv(cells)
+    sb = pot.setup(s["species"], s["natoms"], cells, skin=skin)
+    x0 = np.asarray(s["coordinates"], np.float32)
+    t0 = torch.tensor(x0, device="cuda")
+    pot.build_nblist(sb, t0)
+    pot.check_reallocate(sb, t0)
+    src, dst, d12 = device_half_list(pot, sb, t0)
+    ref, ref_d12 = canonical_pairs_oracle(x0, s["natoms"], rc, cells)
+    np.testing.assert_array_equal(np.stack([src, dst], 1), ref)
+    np.testing.assert_array_equal(d12.view(np.uint32), ref_d12.view(np.uint32))
+    # the skin rows hold the cutoff + skin pair set
+    i, j = rows_to_pairs(sb.nbr_skin.cpu().numpy(), sb.n_skin.cpu().numpy(), sb.max_skin)
+    half = np.array(sorted((a, b) for a, b in zip(i.tolist(), j.tolist()) if a < b), np.int64).reshape(-1, 2)
+    ss, sd, _, ssh = O.half_pairs(x0, s["natoms"], rc, cells, rec, skin=skin)
+    np.testing.assert_array_equal(half, np.stack(canonical_half(ss, sd, np.zeros(len(ss)))[:2], 1))
+    # move, re-filter
+    rng = np.random.Generator(np.random.PCG64(23))
+    u = rng.normal(size=x0.shape)
+    x1 = (x0 + 0.49 * skin * u / np.linalg.norm(u, axis=1, keepdims=True) * rng.random((len(x0), 1))).astype(
+        np.float32)
+    t1 = torch.tensor(x1, device="cuda")
+    sb.max_nbr, sb.max_nbr_ang = 16, 8  # regrow again from the re-filter's own status
+    pot._alloc_nblist(sb)
+    pot.build_nblist(sb, t0)
+    pot.check_reallocate(sb, t0)
+    nbr_skin_ptr = sb.nbr_skin.data_ptr()
+    pot.update_skin(sb, t1)
+    pot.check_reallocate(sb, t1)
+    assert sb.nbr_skin.data_ptr() == nbr_skin_ptr  # the skin list of the build is kept
+    src, dst, d12 = device_half_list(pot, sb, t1)
+    bidx = np.repeat(np.arange(len(s["natoms"])), s["natoms"])
+    us, ud, u12, _ = O.skin_update(ss, sd, ssh, x1, rc, cells, bidx)
+    cs, cd, c12 = canonical_half(us, ud, u12)
+    np.testing.assert_array_equal(src, cs)
+    np.testing.assert_array_equal(dst, cd)
+    np.testing.assert_array_equal(d12.view(np.uint32), c12.view(np.uint32))
+    fresh, _ = canonical_pairs_oracle(x1, s["natoms"], rc, cells)
+    np.testing.assert_array_equal(np.stack([src, dst], 1), fresh)
 
 
 @pytest.mark.parametrize("name", ["cluster64", "water64", "water216"])

@@ -98,3 +98,16 @@ def test_fnx_roundtrip_and_layout(tmp_path):
     assert isinstance(ext, msgpack.ExtType) and ext.code == 1
     shape, dt, buf = msgpack.unpackb(ext.data, raw=False)
     assert shape == [192] and dt == "float64" and len(buf) == 192 * 8
+
+
+def test_nblist_schedule_matches_fennol_md():
+    """Rebuild stride from the skin (md/integrate.py:488-507): a 2 A skin lasts 40 fs."""
+    from fennol_amd.md import nblist_schedule
+
+    assert nblist_schedule(-1.0, 0.5) == 1
+    assert nblist_schedule(0.0, 0.5) == 1
+    assert nblist_schedule(2.0, 0.5) == 80
+    assert nblist_schedule(2.0, 1.0) == 40
+    assert nblist_schedule(1.0, 0.5) == 40
+    assert nblist_schedule(1.5, 0.7) == 42  # floor(0.75 * 40 / 0.7)
+    assert nblist_schedule(2.0, 0.5, stride=10) == 10

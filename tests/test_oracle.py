@@ -124,6 +124,66 @@ def test_general_pbc_contains_all_images():
     assert count == src.shape[0]
 
 
+def _canon(src, dst, d12):
+    lo, hi = np.minimum(src, dst), np.maximum(src, dst)
+    o = np.lexsort((hi, lo))
+    return np.stack([lo[o], hi[o]], 1), np.asarray(d12, np.float32)[o]
+
+
+@pytest.mark.parametrize("case", ["cluster", "water", "triclinic"])
+def test_skin_update_verlet_guarantee(case):
+    """skin_update at the build positions is the cutoff list itself (bits); after moves
+    below skin/2 per atom it is the cutoff list at the new positions; after larger moves a
+    subset of it (never a pair beyond the cutoff)."""
+    rc, skin = 5.2, 1.0
+    s = {"cluster": lambda: systems.cluster_config0(1), "water": lambda: water_box_small(64, 3),
+         "triclinic": lambda: triclinic_box(120, 5)}[case]()
+    x0 = np.asarray(s["coordinates"], np.float32)
+    cells = s.get("cells")
+    rec = None if cells is None else np.linalg.inv(cells)
+    ss, sd, _, ssh = O.half_pairs(x0, s["natoms"], rc, cells, rec, skin=skin)
+    # zero displacement: bit-identical to the cutoff list (pairs, d12, shifts)
+    us, ud, u12, ush = O.skin_update(ss, sd, ssh, x0, rc, cells)
+    fs, fd, f12, fsh = O.half_pairs(x0, s["natoms"], rc, cells, rec)
+    assert us.shape[0] < ss.shape[0]
+    a, b = _canon(us, ud, u12), _canon(fs, fd, f12)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+    rng = np.random.Generator(np.random.PCG64(17))
+    n = x0.shape[0]
+    # moves below skin/2: same pair set as a fresh build
+    u = rng.normal(size=(n, 3))
+    dx = 0.49 * skin * u / np.linalg.norm(u, axis=1, keepdims=True) * rng.random((n, 1))
+    x1 = (x0 + dx).astype(np.float32)
+    us, ud, u12, _ = O.skin_update(ss, sd, ssh, x1, rc, cells)
+    fs, fd, f12, _ = O.half_pairs(x1, s["natoms"], rc, cells, rec)
+    a, b = _canon(us, ud, u12), _canon(fs, fd, f12)
+    np.testing.assert_array_equal(a[0], b[0])
+    if cells is not None:  # the stored image is the one a fresh build picks
+        np.testing.assert_array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+    # large moves: a strict subset (pairs that entered from beyond rc + skin are missed)
+    x2 = (x0 + 3.0 * rng.normal(size=(n, 3))).astype(np.float32)
+    us, ud, u12, _ = O.skin_update(ss, sd, ssh, x2, rc, cells)
+    fs, fd, _, _ = O.half_pairs(x2, s["natoms"], rc, cells, rec)
+    got = set(zip(np.minimum(us, ud).tolist(), np.maximum(us, ud).tolist()))
+    full = set(zip(np.minimum(fs, fd).tolist(), np.maximum(fs, fd).tolist()))
+    assert got < full
+    assert np.all(u12 < np.float32(rc ** 2))
+
+
+def test_cutoff_threshold_is_float32_of_double_square():
+    """d12 values between float32(float32(rc)^2) and float32(rc^2) are inside the cutoff:
+    the reference compares float32 d12 with the Python float rc**2."""
+    rc = 5.2
+    lo = np.float32(np.float64(np.float32(rc)) ** 2)
+    hi = np.float32(rc ** 2)
+    assert lo < hi
+    x = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 0.0]], np.float32)
+    x[1, 0] = np.float32(5.2)  # d12 = float32(5.2f^2) = lo
+    s, d, d12, _ = O.half_pairs(x, np.array([2]), rc)
+    assert d12.shape == (1,) and d12[0] == lo
+
+
 def test_symmetrized_list_and_filter():
     s = systems.cluster_config0(0)
     s_, d_, d12, _ = O.half_pairs(s["coordinates"], s["natoms"], 5.2)
