@@ -10,7 +10,9 @@ Prints ONE JSON line.  ``value`` is the BF16-MLP form's atom-steps/s; the
 plain FP32 form, their ratio, the per-step p50/p99 latency, the energy and
 force errors of both forms against an FP64 evaluation, the roofline of the
 dominant kernel and a CPU baseline (the NumPy oracle on the host cores) are
-reported beside it.
+reported beside it, and under "skin" the same BF16 MD with a Verlet skin
+(fennol_md's nblist_skin, 2 A by default: full rebuilds every 80 steps,
+re-filters of the skin list in between).
 """
 
 from __future__ import annotations
@@ -107,7 +109,7 @@ def copy_calibration(mib: int) -> None:
     del a, b
 
 
-def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_steps=20):
+def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_steps=20, skin=-1.0):
     """Warm-up, then a per-stage breakdown (eager steps, HIP events around each stage on the
     launch stream), then the timed run: one MD step per HIP graph, replayed ``steps`` times back
     to back (the host enqueues one graph launch per step).  Returns the potential, the MD state,
@@ -119,7 +121,8 @@ def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_ste
     from fennol_amd.potential import ANIPotential, StageTimer
 
     pot = ANIPotential(cfg, params, precision=precision, mlp_tile_rows=tile_rows)
-    md = LangevinMD(pot, system, dt_fs=0.5, temperature=300.0, gamma_thz=1.0, seed=MD_SEED, check_every=0)
+    md = LangevinMD(pot, system, dt_fs=0.5, temperature=300.0, gamma_thz=1.0, seed=MD_SEED, check_every=0,
+                    nblist_skin=skin)
     for _ in range(warmup):
         md.step()
     torch.cuda.synchronize()
@@ -267,6 +270,9 @@ def main() -> int:
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="launch one plain copy kernel of this size first (for counter runs)")
     ap.add_argument("--precisions", default="fp32,bf16", help="MD forms to run (bf16 gives value)")
+    ap.add_argument("--skin", type=float, default=2.0,
+                    help="also time BF16 MD with a Verlet skin of this many A (fennol_md's nblist_skin; "
+                         "rebuild stride from the skin); 0 skips")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (single-GPU build)",
@@ -303,6 +309,25 @@ def main() -> int:
                                   max_nbr=md.sb.max_nbr, max_nbr_ang=md.sb.max_nbr_ang)
         log(f"[bench]   {precision}: {1000 * wall / args.steps:.3f} ms/step (graph replay), eager "
             f"{eager_ms:.3f} ms/step; stages (ms): " + ", ".join(f"{k} {v.mean():.3f}" for k, v in stages.items()))
+        del pot, md
+        torch.cuda.empty_cache()
+
+    skin = None
+    if args.skin > 0:
+        from fennol_amd.md import nblist_schedule
+
+        stride = nblist_schedule(args.skin, 0.5)
+        k_skin = max(args.steps, 2 * stride)
+        log(f"[bench] MD bf16 with a {args.skin} A skin (rebuild every {stride} steps): {k_skin} steps")
+        pot, md, wall, lat, stages, eager_ms = run_md(system, cfg, params, "bf16", k_skin, args.warmup,
+                                                      tile_rows=args.tile_bf16, skin=args.skin)
+        skin = {"nblist_skin_A": args.skin, "rebuild_stride": stride, "steps": k_skin,
+                "value": round(N * k_skin / wall, 1), "unit": "atom-steps/s",
+                "ms_per_step": round(1000.0 * wall / k_skin, 4),
+                "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                "refilter_ms": round(float(stages["skin"].mean()), 4) if "skin" in stages else None,
+                "max_skin": md.sb.max_skin}
+        log(f"[bench]   skin: {skin['ms_per_step']:.3f} ms/step, re-filter {skin['refilter_ms']} ms")
         del pot, md
         torch.cuda.empty_cache()
 
@@ -383,6 +408,7 @@ def main() -> int:
         "stages": stage_table,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "skin": skin,
         "serving": serving,
         "training": training,
     }

@@ -17,6 +17,7 @@ FNL_ERR_HIP = -2
 FNL_ERR_UNSUPPORTED = -3
 FNL_PBC_NONE = 0
 FNL_PBC_MINIMUM_IMAGE = 1
+FNL_PBC_GENERAL = 2
 FNL_F32 = 0
 FNL_BF16 = 1
 FNL_STATUS_WORDS = 8
@@ -81,15 +82,15 @@ _SIGNATURES = {
     "fnl_strerror": (C.c_char_p, [C.c_int]),
     "fnl_nblist_workspace": (sz, [i32, i32, i32]),
     "fnl_nblist_build": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, i32, f64, f64, i32, i32,
-                                   vp, sz, vp, vp, vp, vp, vp, vp]),
-    "fnl_nblist_skin_update": (C.c_int, [vp, vp, i32, i32, vp, vp, vp, vp, i32, f64, f64, i32, i32, vp, vp, vp,
-                                         vp, vp, vp]),
-    "fnl_nblist_export": (C.c_int, [vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp]),
-    "fnl_aev_forward": (C.c_int, [vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, i32,
+                                   vp, sz, vp, vp, vp, vp, vp, vp, vp]),
+    "fnl_nblist_skin_update": (C.c_int, [vp, vp, i32, i32, vp, vp, vp, vp, vp, i32, f64, f64, i32, i32, vp, vp,
+                                         vp, vp, vp, vp, vp]),
+    "fnl_nblist_export": (C.c_int, [vp, vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp]),
+    "fnl_aev_forward": (C.c_int, [vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, i32,
                                   C.POINTER(AevParams), vp, i32, i32, vp]),
-    "fnl_aev_backward": (C.c_int, [vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, i32,
+    "fnl_aev_backward": (C.c_int, [vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, i32,
                                    C.POINTER(AevParams), vp, i32, vp, vp, vp, vp]),
-    "fnl_aev_tangent": (C.c_int, [vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, i32,
+    "fnl_aev_tangent": (C.c_int, [vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, i32,
                                   C.POINTER(AevParams), vp, i32, vp]),
     "fnl_mlp_tile_rows": (C.c_int, [i32]),
     "fnl_species_mlp": (C.c_int, [vp, i32, C.POINTER(MlpWeights), vp, vp, vp, vp, i32, vp, vp, i32, vp]),

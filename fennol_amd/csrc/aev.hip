@@ -65,37 +65,6 @@ __device__ __forceinline__ void tri_pair(int t, int n, int& a, int& b) {
   b = t - (r * n - (r * (r + 1)) / 2) + r + 1;
 }
 
-struct RadialRow {
-  float* d;    // distance
-  float* fc;   // radial switch (0.25 folded in for the forward)
-  int* s;      // neighbour species index
-  float* vx;
-  float* vy;
-  float* vz;
-};
-
-__device__ __forceinline__ void load_radial_rows(int i, int nr, const float* __restrict__ coords,
-                                                 const int32_t* __restrict__ species_idx,
-                                                 const int32_t* __restrict__ row, bool pbc,
-                                                 const FnlCell& cl, float rc, float fc_scale,
-                                                 RadialRow rr) {
-  const int lane = fnl_lane();
-  const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
-  for (int e = lane; e < nr; e += 64) {
-    const int j = row[e];
-    float vx = coords[3 * j] - xi, vy = coords[3 * j + 1] - yi, vz = coords[3 * j + 2] - zi;
-    if (pbc) fnl_min_image(vx, vy, vz, cl);
-    const float d2 = vx * vx + vy * vy + vz * vz;
-    const float d = sqrtf(fmaxf(d2, 1.0e-5f));
-    rr.d[e] = d;
-    rr.fc[e] = fc_scale * cos_switch(d, rc);
-    rr.s[e] = species_idx[j];
-    rr.vx[e] = vx;
-    rr.vy[e] = vy;
-    rr.vz[e] = vz;
-  }
-}
-
 template <typename T>
 __device__ __forceinline__ T to_out(float v);
 template <>
@@ -243,6 +212,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           const int32_t* __restrict__ batch_index, int n_atoms, int pbc_mode,
           const float* __restrict__ cells, const float* __restrict__ rec,
           const int32_t* __restrict__ nbr, const int32_t* __restrict__ n_nbr, int max_nbr,
+          const int32_t* __restrict__ nbr_shift,
           const int32_t* __restrict__ nbr_ang, const int32_t* __restrict__ n_nbr_ang, int max_nbr_ang,
           fnl_aev_params p, AevDims dm, TOut* __restrict__ aev, int stride) {
   static_assert(NA * NZ == 32, "angular channel layout");
@@ -270,7 +240,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   }
   const bool pbc = pbc_mode == FNL_PBC_MINIMUM_IMAGE;
   FnlCell cl;
-  if (pbc) fnl_load_cell(cells, rec, batch_index[i], cl);
+  if (pbc || nbr_shift) fnl_load_cell(cells, rec, batch_index[i], cl);
   const int nr = min(n_nbr[i], max_nbr);
   const int na = min(n_nbr_ang[i], max_nbr_ang);
   const int S = dm.S;
@@ -280,7 +250,10 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     for (int e = lane; e < nr; e += 64) {
       const int j = row[e];
       float vx = coords[3 * j] - xi, vy = coords[3 * j + 1] - yi, vz = coords[3 * j + 2] - zi;
-      if (pbc) fnl_min_image(vx, vy, vz, cl);
+      if (nbr_shift)  // general PBC: the stored image (GraphProcessor, preprocessing.py:727-735)
+        fnl_apply_shift(vx, vy, vz, cl, nbr_shift[(size_t)i * max_nbr + e]);
+      else if (pbc)
+        fnl_min_image(vx, vy, vz, cl);
       const float d = sqrtf(fmaxf(vx * vx + vy * vy + vz * vz, 1.0e-5f));
       nbv[e] = make_float4(vx, vy, vz, d);
       nbs[e] = species_idx[j];
@@ -481,6 +454,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           const int32_t* __restrict__ batch_index, int n_atoms, int pbc_mode,
           const float* __restrict__ cells, const float* __restrict__ rec,
           const int32_t* __restrict__ nbr, const int32_t* __restrict__ n_nbr, int max_nbr,
+          const int32_t* __restrict__ nbr_shift,
           const int32_t* __restrict__ nbr_ang, const int32_t* __restrict__ n_nbr_ang, int max_nbr_ang,
           fnl_aev_params p, AevDims dm, const float* __restrict__ g, int g_stride,
           float* __restrict__ grad_self, float* __restrict__ grad_nbr, float* __restrict__ virial_atom) {
@@ -506,7 +480,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   }
   const bool pbc = pbc_mode == FNL_PBC_MINIMUM_IMAGE;
   FnlCell cl;
-  if (pbc) fnl_load_cell(cells, rec, batch_index[i], cl);
+  if (pbc || nbr_shift) fnl_load_cell(cells, rec, batch_index[i], cl);
   const int nr = min(n_nbr[i], max_nbr);
   const int na = min(n_nbr_ang[i], max_nbr_ang);
   // the first 64 entries of the angular row, fetched with the dE/dAEV and radial rows
@@ -525,7 +499,10 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     for (int e = lane; e < nr; e += 64) {
       const int j = row[e];
       float vx = coords[3 * j] - xi, vy = coords[3 * j + 1] - yi, vz = coords[3 * j + 2] - zi;
-      if (pbc) fnl_min_image(vx, vy, vz, cl);
+      if (nbr_shift)  // general PBC: the stored image (GraphProcessor, preprocessing.py:727-735)
+        fnl_apply_shift(vx, vy, vz, cl, nbr_shift[(size_t)i * max_nbr + e]);
+      else if (pbc)
+        fnl_min_image(vx, vy, vz, cl);
       const float d2 = vx * vx + vy * vy + vz * vz;
       const float d = sqrtf(fmaxf(d2, 1.0e-5f));
       nbv[e] = make_float4(vx, vy, vz, d);
@@ -717,6 +694,7 @@ k_aev_tangent(const float* __restrict__ coords, const float* __restrict__ disp,
               const int32_t* __restrict__ species_idx, const int32_t* __restrict__ batch_index, int n_atoms,
               int pbc_mode, const float* __restrict__ cells, const float* __restrict__ rec,
               const int32_t* __restrict__ nbr, const int32_t* __restrict__ n_nbr, int max_nbr,
+          const int32_t* __restrict__ nbr_shift,
               const int32_t* __restrict__ nbr_ang, const int32_t* __restrict__ n_nbr_ang, int max_nbr_ang,
               fnl_aev_params p, AevDims dm, float* __restrict__ out_rows, int stride) {
   static_assert(NA * NZ == 32, "angular channel layout");
@@ -744,7 +722,7 @@ k_aev_tangent(const float* __restrict__ coords, const float* __restrict__ disp,
   }
   const bool pbc = pbc_mode == FNL_PBC_MINIMUM_IMAGE;
   FnlCell cl;
-  if (pbc) fnl_load_cell(cells, rec, batch_index[i], cl);
+  if (pbc || nbr_shift) fnl_load_cell(cells, rec, batch_index[i], cl);
   const int nr = min(n_nbr[i], max_nbr);
   const int na = min(n_nbr_ang[i], max_nbr_ang);
   for (int c = lane; c < dm.aev_len; c += 64) acc[c] = 0.0f;
@@ -755,7 +733,10 @@ k_aev_tangent(const float* __restrict__ coords, const float* __restrict__ disp,
     for (int e = lane; e < nr; e += 64) {
       const int j = row[e];
       float vx = coords[3 * j] - xi, vy = coords[3 * j + 1] - yi, vz = coords[3 * j + 2] - zi;
-      if (pbc) fnl_min_image(vx, vy, vz, cl);
+      if (nbr_shift)  // general PBC: the stored image (GraphProcessor, preprocessing.py:727-735)
+        fnl_apply_shift(vx, vy, vz, cl, nbr_shift[(size_t)i * max_nbr + e]);
+      else if (pbc)
+        fnl_min_image(vx, vy, vz, cl);
       const float d2 = vx * vx + vy * vy + vz * vz;
       const float d = sqrtf(fmaxf(d2, 1.0e-5f));
       const float tx = disp[3 * j] - ri0, ty = disp[3 * j + 1] - ri1, tz = disp[3 * j + 2] - ri2;
@@ -876,7 +857,7 @@ bool check_params(const fnl_aev_params* p, AevDims* dm) {
 template <int NA, int NZ>
 int launch_fwd(const float* coords, const int32_t* species_idx, const int32_t* batch_index, int n_atoms,
                int pbc_mode, const float* cells, const float* rec, const int32_t* nbr,
-               const int32_t* n_nbr, int max_nbr, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
+               const int32_t* n_nbr, int max_nbr, const int32_t* nbr_shift, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
                int max_nbr_ang, const fnl_aev_params& p, const AevDims& dm, void* aev, int stride,
                int out_dtype, hipStream_t s) {
   const size_t lds = (size_t)kWaves * 4 * fwd_layout(max_nbr, max_nbr_ang, p.n_ang_dist + p.n_ang_sect).words;
@@ -894,11 +875,11 @@ int launch_fwd(const float* coords, const int32_t* species_idx, const int32_t* b
   }
   if (out_dtype == FNL_F32) {
     hipLaunchKernelGGL((k_aev_fwd<NA, NZ, float>), grid, block, lds, s, coords, species_idx, batch_index,
-                       n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang,
+                       n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang,
                        p, dm, (float*)aev, stride);
   } else {
     hipLaunchKernelGGL((k_aev_fwd<NA, NZ, hip_bfloat16>), grid, block, lds, s, coords, species_idx,
-                       batch_index, n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr, nbr_ang, n_nbr_ang,
+                       batch_index, n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang,
                        max_nbr_ang, p, dm, (hip_bfloat16*)aev, stride);
   }
   FNL_LAUNCH_CHECK();
@@ -908,7 +889,7 @@ int launch_fwd(const float* coords, const int32_t* species_idx, const int32_t* b
 template <int NA, int NZ>
 int launch_bwd(const float* coords, const int32_t* species_idx, const int32_t* batch_index, int n_atoms,
                int pbc_mode, const float* cells, const float* rec, const int32_t* nbr,
-               const int32_t* n_nbr, int max_nbr, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
+               const int32_t* n_nbr, int max_nbr, const int32_t* nbr_shift, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
                int max_nbr_ang, const fnl_aev_params& p, const AevDims& dm, const float* g, int g_stride,
                float* grad_self, float* grad_nbr, float* virial_atom, hipStream_t s) {
   const size_t lds = (size_t)kWaves * 4 * bwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
@@ -922,20 +903,23 @@ int launch_bwd(const float* coords, const int32_t* species_idx, const int32_t* b
   }
   hipLaunchKernelGGL((k_aev_bwd<NA, NZ>), dim3(fnl_blocks(n_atoms, kWaves)), dim3(64 * kWaves), lds, s,
                      coords, species_idx, batch_index, n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr,
-                     nbr_ang, n_nbr_ang, max_nbr_ang, p, dm, g, g_stride, grad_self, grad_nbr, virial_atom);
+                     nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, p, dm, g, g_stride, grad_self, grad_nbr,
+                     virial_atom);
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }
 
+// nbr_shift: required for FNL_PBC_GENERAL (the stored image of every slot), NULL otherwise
 bool common_args_ok(const float* coords, const int32_t* species_idx, const int32_t* batch_index,
                     int n_atoms, int pbc_mode, const float* cells, const float* rec, const int32_t* nbr,
-                    const int32_t* n_nbr, int max_nbr, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
-                    int max_nbr_ang) {
+                    const int32_t* n_nbr, int max_nbr, const int32_t* nbr_shift, const int32_t* nbr_ang,
+                    const int32_t* n_nbr_ang, int max_nbr_ang) {
   if (n_atoms < 0 || max_nbr < 1 || max_nbr_ang < 1) return false;
   if (n_atoms == 0) return true;
   if (!coords || !species_idx || !batch_index || !nbr || !n_nbr || !nbr_ang || !n_nbr_ang) return false;
-  if (pbc_mode != FNL_PBC_NONE && pbc_mode != FNL_PBC_MINIMUM_IMAGE) return false;
-  if (pbc_mode == FNL_PBC_MINIMUM_IMAGE && (!cells || !rec)) return false;
+  if (pbc_mode != FNL_PBC_NONE && pbc_mode != FNL_PBC_MINIMUM_IMAGE && pbc_mode != FNL_PBC_GENERAL) return false;
+  if (pbc_mode != FNL_PBC_NONE && (!cells || !rec)) return false;
+  if ((pbc_mode == FNL_PBC_GENERAL) != (nbr_shift != nullptr)) return false;
   return true;
 }
 
@@ -944,13 +928,14 @@ bool common_args_ok(const float* coords, const int32_t* species_idx, const int32
 extern "C" int fnl_aev_forward(const float* coords, const int32_t* species_idx, const int32_t* batch_index,
                                int32_t n_atoms, int32_t pbc_mode, const float* cells,
                                const float* reciprocal_cells, const int32_t* nbr, const int32_t* n_nbr,
-                               int32_t max_nbr, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
+                               int32_t max_nbr, const int32_t* nbr_shift, const int32_t* nbr_ang,
+                               const int32_t* n_nbr_ang,
                                int32_t max_nbr_ang, const fnl_aev_params* params, void* aev,
                                int32_t aev_stride, int32_t out_dtype, fnl_stream_t stream) {
   AevDims dm;
   if (!check_params(params, &dm)) return FNL_ERR_INVALID;
   if (!common_args_ok(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells, nbr,
-                      n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang))
+                      n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang))
     return FNL_ERR_INVALID;
   if (aev_stride < dm.aev_len || (out_dtype != FNL_F32 && out_dtype != FNL_BF16)) return FNL_ERR_INVALID;
   if (n_atoms == 0) return FNL_OK;
@@ -959,11 +944,11 @@ extern "C" int fnl_aev_forward(const float* coords, const int32_t* species_idx, 
   const int na = params->n_ang_dist, nz = params->n_ang_sect;
   if (na == 4 && nz == 8)
     return launch_fwd<4, 8>(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells,
-                            nbr, n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm, aev,
+                            nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm, aev,
                             aev_stride, out_dtype, s);
   if (na == 8 && nz == 4)
     return launch_fwd<8, 4>(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells,
-                            nbr, n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm, aev,
+                            nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm, aev,
                             aev_stride, out_dtype, s);
   return FNL_ERR_UNSUPPORTED;
 }
@@ -971,14 +956,15 @@ extern "C" int fnl_aev_forward(const float* coords, const int32_t* species_idx, 
 extern "C" int fnl_aev_backward(const float* coords, const int32_t* species_idx, const int32_t* batch_index,
                                 int32_t n_atoms, int32_t pbc_mode, const float* cells,
                                 const float* reciprocal_cells, const int32_t* nbr, const int32_t* n_nbr,
-                                int32_t max_nbr, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
+                                int32_t max_nbr, const int32_t* nbr_shift, const int32_t* nbr_ang,
+                                const int32_t* n_nbr_ang,
                                 int32_t max_nbr_ang, const fnl_aev_params* params, const float* g,
                                 int32_t g_stride, float* grad_self, float* grad_nbr, float* virial_atom,
                                 fnl_stream_t stream) {
   AevDims dm;
   if (!check_params(params, &dm)) return FNL_ERR_INVALID;
   if (!common_args_ok(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells, nbr,
-                      n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang))
+                      n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang))
     return FNL_ERR_INVALID;
   if (g_stride < ((dm.aev_len + 3) & ~3) || g_stride % 4 != 0) return FNL_ERR_INVALID;
   if (n_atoms == 0) return FNL_OK;
@@ -987,11 +973,11 @@ extern "C" int fnl_aev_backward(const float* coords, const int32_t* species_idx,
   const int na = params->n_ang_dist, nz = params->n_ang_sect;
   if (na == 4 && nz == 8)
     return launch_bwd<4, 8>(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells,
-                            nbr, n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm, g, g_stride,
+                            nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm, g, g_stride,
                             grad_self, grad_nbr, virial_atom, s);
   if (na == 8 && nz == 4)
     return launch_bwd<8, 4>(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells,
-                            nbr, n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm, g, g_stride,
+                            nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm, g, g_stride,
                             grad_self, grad_nbr, virial_atom, s);
   return FNL_ERR_UNSUPPORTED;
 }
@@ -999,13 +985,14 @@ extern "C" int fnl_aev_backward(const float* coords, const int32_t* species_idx,
 extern "C" int fnl_aev_tangent(const float* coords, const float* disp, const int32_t* species_idx,
                                const int32_t* batch_index, int32_t n_atoms, int32_t pbc_mode, const float* cells,
                                const float* reciprocal_cells, const int32_t* nbr, const int32_t* n_nbr,
-                               int32_t max_nbr, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
+                               int32_t max_nbr, const int32_t* nbr_shift, const int32_t* nbr_ang,
+                               const int32_t* n_nbr_ang,
                                int32_t max_nbr_ang, const fnl_aev_params* params, float* daev, int32_t daev_stride,
                                fnl_stream_t stream) {
   AevDims dm;
   if (!check_params(params, &dm)) return FNL_ERR_INVALID;
   if (!common_args_ok(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells, nbr,
-                      n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang))
+                      n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang))
     return FNL_ERR_INVALID;
   if (daev_stride < dm.aev_len) return FNL_ERR_INVALID;
   if (n_atoms == 0) return FNL_OK;
@@ -1028,11 +1015,11 @@ extern "C" int fnl_aev_tangent(const float* coords, const float* disp, const int
   dim3 grid(fnl_blocks(n_atoms, kWaves)), block(64 * kWaves);
   if (na == 4)
     hipLaunchKernelGGL((k_aev_tangent<4, 8>), grid, block, lds, s, coords, disp, species_idx, batch_index, n_atoms,
-                       pbc_mode, cells, reciprocal_cells, nbr, n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang,
+                       pbc_mode, cells, reciprocal_cells, nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang,
                        *params, dm, daev, daev_stride);
   else
     hipLaunchKernelGGL((k_aev_tangent<8, 4>), grid, block, lds, s, coords, disp, species_idx, batch_index, n_atoms,
-                       pbc_mode, cells, reciprocal_cells, nbr, n_nbr, max_nbr, nbr_ang, n_nbr_ang, max_nbr_ang,
+                       pbc_mode, cells, reciprocal_cells, nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang,
                        *params, dm, daev, daev_stride);
   FNL_LAUNCH_CHECK();
   return FNL_OK;

@@ -90,6 +90,49 @@ __device__ __forceinline__ void fnl_min_image(float& vx, float& vy, float& vz, c
   }
 }
 
+// Integer cell shifts (s0, s1, s2) packed into one int32 (general PBC rows):
+// 10 bits per component, offset 512, so |s| <= 511 lattice vectors.
+__device__ __host__ __forceinline__ int32_t fnl_shift_pack(int s0, int s1, int s2) {
+  return (int32_t)((s0 + 512) | ((s1 + 512) << 10) | ((s2 + 512) << 20));
+}
+
+__device__ __host__ __forceinline__ bool fnl_shift_fits(int s0, int s1, int s2) {
+  return s0 >= -511 && s0 <= 511 && s1 >= -511 && s1 <= 511 && s2 >= -511 && s2 <= 511;
+}
+
+__device__ __forceinline__ void fnl_shift_unpack(int32_t p, float& s0, float& s1, float& s2) {
+  s0 = (float)((p & 1023) - 512);
+  s1 = (float)(((p >> 10) & 1023) - 512);
+  s2 = (float)(((p >> 20) & 1023) - 512);
+}
+
+// vec += s . C for a stored shift (general PBC), the operation order of fnl_min_image
+__device__ __forceinline__ void fnl_apply_shift(float& vx, float& vy, float& vz, const FnlCell& cl,
+                                                int32_t packed) {
+#pragma clang fp contract(off)
+  float s0, s1, s2;
+  fnl_shift_unpack(packed, s0, s1, s2);
+  vx = vx + ((s0 * cl.c[0] + s1 * cl.c[3]) + s2 * cl.c[6]);
+  vy = vy + ((s0 * cl.c[1] + s1 * cl.c[4]) + s2 * cl.c[7]);
+  vz = vz + ((s0 * cl.c[2] + s1 * cl.c[5]) + s2 * cl.c[8]);
+}
+
+// Atom wrapped into the cell for the general-PBC build (preprocessing.py:145-149):
+// at = -floor(x . R), xw = x + at . C, float32 in a fixed order.
+__device__ __forceinline__ void fnl_wrap(float x, float y, float z, const FnlCell& cl, float* xw, int* at) {
+#pragma clang fp contract(off)
+  const float f0 = (x * cl.r[0] + y * cl.r[3]) + z * cl.r[6];
+  const float f1 = (x * cl.r[1] + y * cl.r[4]) + z * cl.r[7];
+  const float f2 = (x * cl.r[2] + y * cl.r[5]) + z * cl.r[8];
+  const float a0 = -floorf(f0), a1 = -floorf(f1), a2 = -floorf(f2);
+  xw[0] = x + ((a0 * cl.c[0] + a1 * cl.c[3]) + a2 * cl.c[6]);
+  xw[1] = y + ((a0 * cl.c[1] + a1 * cl.c[4]) + a2 * cl.c[7]);
+  xw[2] = z + ((a0 * cl.c[2] + a1 * cl.c[5]) + a2 * cl.c[8]);
+  at[0] = (int)a0;
+  at[1] = (int)a1;
+  at[2] = (int)a2;
+}
+
 __device__ __forceinline__ float fnl_d12(float vx, float vy, float vz) {
 #pragma clang fp contract(off)
   return (vx * vx + vy * vy) + vz * vz;

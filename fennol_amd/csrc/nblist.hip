@@ -108,6 +108,21 @@ __global__ void k_grid(const float* __restrict__ coords, const int32_t* __restri
   const int capx = grid_cap[b * 4 + 0], capy = grid_cap[b * 4 + 1], capz = grid_cap[b * 4 + 2];
   const int first = grid_cap[b * 4 + 3];
   int32_t* g = grid + b * kGridWords;
+  if (pbc_mode == FNL_PBC_GENERAL) {
+    // images per direction: ceil(cutoff * |column a of R|) in float32 (preprocessing.py:152-154)
+    if (t == 0) {
+      const float* R = rec + b * 9;
+      for (int a = 0; a < 3; ++a) {
+        const float nrm = sqrtf((R[a] * R[a] + R[3 + a] * R[3 + a]) + R[6 + a] * R[6 + a]);
+        g[a] = (int)ceilf(cutoff * nrm);
+      }
+      g[3] = first;
+      g[4] = g[5] = g[6] = g[7] = g[8] = g[9] = 0;
+      g[10] = 2;
+      g[11] = 0;
+    }
+    return;
+  }
   if (pbc_mode == FNL_PBC_MINIMUM_IMAGE) {
     if (t == 0) {
       const float* R = rec + b * 9;
@@ -198,6 +213,131 @@ __device__ __forceinline__ int3 atom_cell3(const float* __restrict__ coords, int
   c.y = min(max((int)floorf(f[1]), 0), ny - 1);
   c.z = min(max((int)floorf(f[2]), 0), nz - 1);
   return c;
+}
+
+// General PBC: every atom wrapped into its cell (x + at . C, at = -floor(x . R)), the
+// integer wrap shift packed into w.
+__global__ void k_wrap(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
+                       const int32_t* __restrict__ batch_index, int n_atoms, const float* __restrict__ cells,
+                       const float* __restrict__ rec, float4* __restrict__ pos, int32_t* __restrict__ n_nbr,
+                       int32_t* __restrict__ n_nbr_ang, int32_t* __restrict__ status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_atoms) return;
+  if (species_idx[i] < 0) {
+    n_nbr[i] = 0;
+    if (n_nbr_ang) n_nbr_ang[i] = 0;
+    return;
+  }
+  FnlCell cl;
+  fnl_load_cell(cells, rec, batch_index[i], cl);
+  float xw[3];
+  int at[3];
+  fnl_wrap(coords[3 * i], coords[3 * i + 1], coords[3 * i + 2], cl, xw, at);
+  if (!fnl_shift_fits(at[0], at[1], at[2])) {  // an atom more than 511 cells away
+    at[0] = at[1] = at[2] = 0;
+    atomicOr(&status[FNL_STATUS_OVERFLOW], 4);
+    atomicOr(&status[FNL_STATUS_STICKY], 4);
+  }
+  pos[i] = make_float4(xw[0], xw[1], xw[2], __int_as_float(fnl_shift_pack(at[0], at[1], at[2])));
+}
+
+// General PBC pairs (GraphGenerator, preprocessing.py:144-257, 288-293): one wave per centre
+// i; the candidates are every atom j of i's system times every cell shift s with
+// |s_a| <= n_a, j-major, tested 64 at a time with the reference's float32 arithmetic
+// vec = (xw_j - xw_i) + s . C.  A kept slot stores j and the total shift
+// s + at_j - at_i (packed), which maps the unwrapped positions onto the same image.
+// O(n_sys_atoms * n_shifts) per centre: the convention for cells narrower than twice the
+// cutoff, where systems are small.
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
+k_pairs_general(const int32_t* __restrict__ species_idx, const int32_t* __restrict__ batch_index,
+                const int32_t* __restrict__ sys_offsets, int n_atoms, const float* __restrict__ cells,
+                const float* __restrict__ rec, const int32_t* __restrict__ grid, const float4* __restrict__ pos,
+                float rc2, float rca2, int max_nbr, int max_nbr_ang, int32_t* __restrict__ nbr,
+                int32_t* __restrict__ nbr_shift, int32_t* __restrict__ n_nbr, int32_t* __restrict__ nbr_ang,
+                int32_t* __restrict__ n_nbr_ang, int32_t* __restrict__ status) {
+  const int lane = fnl_lane();
+  const int i = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (i >= n_atoms || species_idx[i] < 0) return;
+  const int b = batch_index[i];
+  const int32_t* g = grid + b * kGridWords;
+  const int n0 = g[0], n1 = g[1], n2 = g[2];
+  const int w1 = 2 * n1 + 1, w2 = 2 * n2 + 1;
+  const int n_sh = (2 * n0 + 1) * w1 * w2;
+  const int a0 = sys_offsets[b];
+  const int total = (sys_offsets[b + 1] - a0) * n_sh;
+  FnlCell cl;
+  fnl_load_cell(cells, rec, b, cl);
+  const float4 pi = pos[i];
+  const int32_t ati = __float_as_int(pi.w);
+  const int ai0 = (ati & 1023) - 512, ai1 = ((ati >> 10) & 1023) - 512, ai2 = ((ati >> 20) & 1023) - 512;
+  int32_t* row = nbr + (size_t)i * max_nbr;
+  int32_t* row_s = nbr_shift + (size_t)i * max_nbr;
+  int32_t* row_a = nbr_ang + (size_t)i * max_nbr_ang;
+  int count = 0, count_a = 0;
+  bool range_ok = true;
+  for (int base = 0; base < total; base += 64) {
+    const int c = base + lane;
+    const bool valid = c < total;
+    const int jl = valid ? c / n_sh : 0;
+    const int sidx = valid ? c - jl * n_sh : 0;
+    const int j = a0 + jl;
+    const int s0 = sidx / (w1 * w2) - n0;
+    const int r12 = sidx % (w1 * w2);
+    const int s1 = r12 / w2 - n1, s2 = r12 % w2 - n2;
+    bool keep = false, keep_a = false;
+    int32_t code = 0;
+    if (valid && j != i && species_idx[j] >= 0) {  // no self images (the reference pairs i < j)
+      const float4 pj = pos[j];
+      const float fs0 = (float)s0, fs1 = (float)s1, fs2 = (float)s2;
+      float vx, vy, vz;
+      {
+#pragma clang fp contract(off)
+        vx = (pj.x - pi.x) + ((fs0 * cl.c[0] + fs1 * cl.c[3]) + fs2 * cl.c[6]);
+        vy = (pj.y - pi.y) + ((fs0 * cl.c[1] + fs1 * cl.c[4]) + fs2 * cl.c[7]);
+        vz = (pj.z - pi.z) + ((fs0 * cl.c[2] + fs1 * cl.c[5]) + fs2 * cl.c[8]);
+      }
+      const float d12 = fnl_d12(vx, vy, vz);
+      keep = d12 < rc2;
+      keep_a = d12 < rca2;
+      if (keep) {
+        const int32_t atj = __float_as_int(pj.w);
+        const int t0 = s0 + ((atj & 1023) - 512) - ai0;
+        const int t1 = s1 + (((atj >> 10) & 1023) - 512) - ai1;
+        const int t2 = s2 + (((atj >> 20) & 1023) - 512) - ai2;
+        range_ok = range_ok && fnl_shift_fits(t0, t1, t2);
+        code = fnl_shift_pack(t0, t1, t2);
+      }
+    }
+    const unsigned long long m = __ballot(keep);
+    const unsigned long long ma = __ballot(keep_a);
+    if (keep) {
+      const int e = count + fnl_rank_below(m);
+      if (e < max_nbr) {
+        row[e] = j;
+        row_s[e] = code;
+      }
+      if (keep_a) {
+        const int ea = count_a + fnl_rank_below(ma);
+        if (ea < max_nbr_ang && e < max_nbr) row_a[ea] = e;
+      }
+    }
+    count += __popcll(m);
+    count_a += __popcll(ma);
+  }
+  const bool bad_range = __ballot(!range_ok) != 0ull;
+  if (lane == 0) {
+    n_nbr[i] = count;
+    if (n_nbr_ang) n_nbr_ang[i] = count_a;
+    if (count > max_nbr || count_a > max_nbr_ang) {
+      atomicOr(&status[FNL_STATUS_OVERFLOW], 1);
+      atomicOr(&status[FNL_STATUS_STICKY], 1);
+    }
+    if (bad_range) {
+      atomicOr(&status[FNL_STATUS_OVERFLOW], 4);
+      atomicOr(&status[FNL_STATUS_STICKY], 4);
+    }
+  }
 }
 
 __global__ void k_bin(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
@@ -497,14 +637,18 @@ __global__ void k_status_clear(int32_t* __restrict__ status) {
 // in row order, as k_pairs does.  For the minimum-image convention the image
 // chosen at the build is the image the current positions select for every
 // pair still inside the cutoff (rc below half the lattice-plane spacing), so
-// the recomputed image reproduces the reference's stored pbc_shifts.  The
-// next pass's slot indices are loaded before the current pass is tested.
+// the recomputed image reproduces the reference's stored pbc_shifts; for
+// general PBC the stored shift of each slot is applied to the unwrapped
+// positions (the reference's update_skin formula) and carried to the output
+// rows.  The next pass's slot indices are loaded before the current pass is
+// tested.
 template <int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
 k_skin_filter(const float* __restrict__ coords, const int32_t* __restrict__ batch_index, int n_atoms,
               int pbc_mode, const float* __restrict__ cells, const float* __restrict__ rec,
-              const int32_t* __restrict__ nbr_skin, const int32_t* __restrict__ n_skin, int max_skin,
-              float rc2, float rca2, int max_nbr, int max_nbr_ang, int32_t* __restrict__ nbr,
+              const int32_t* __restrict__ nbr_skin, const int32_t* __restrict__ skin_shift,
+              const int32_t* __restrict__ n_skin, int max_skin, float rc2, float rca2, int max_nbr,
+              int max_nbr_ang, int32_t* __restrict__ nbr, int32_t* __restrict__ nbr_shift,
               int32_t* __restrict__ n_nbr, int32_t* __restrict__ nbr_ang, int32_t* __restrict__ n_nbr_ang,
               int32_t* __restrict__ status) {
   const int lane = fnl_lane();
@@ -515,7 +659,8 @@ k_skin_filter(const float* __restrict__ coords, const int32_t* __restrict__ batc
   const int32_t* srow = nbr_skin + (size_t)i * max_skin;
   FnlCell cl;
   const bool pbc = pbc_mode == FNL_PBC_MINIMUM_IMAGE;
-  if (pbc) fnl_load_cell(cells, rec, batch_index[i], cl);
+  const bool gen = pbc_mode == FNL_PBC_GENERAL;
+  if (pbc || gen) fnl_load_cell(cells, rec, batch_index[i], cl);
   const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
   int32_t* row = nbr + (size_t)i * max_nbr;
   int32_t* row_a = nbr_ang + (size_t)i * max_nbr_ang;
@@ -526,7 +671,13 @@ k_skin_filter(const float* __restrict__ coords, const int32_t* __restrict__ batc
     const int j = valid ? jr : i;  // slots past the row end hold stale values: never dereferenced
     jr = srow[min(base + 64 + lane, max_skin - 1)];
     float vx = coords[3 * j] - xi, vy = coords[3 * j + 1] - yi, vz = coords[3 * j + 2] - zi;
-    if (pbc) fnl_min_image(vx, vy, vz, cl);
+    int32_t code = 0;
+    if (gen) {
+      code = skin_shift[(size_t)i * max_skin + min(base + lane, max_skin - 1)];
+      fnl_apply_shift(vx, vy, vz, cl, code);
+    } else if (pbc) {
+      fnl_min_image(vx, vy, vz, cl);
+    }
     const float d12 = fnl_d12(vx, vy, vz);
     const bool keep = valid && d12 < rc2;
     const bool keep_a = valid && d12 < rca2;
@@ -534,7 +685,10 @@ k_skin_filter(const float* __restrict__ coords, const int32_t* __restrict__ batc
     const unsigned long long ma = __ballot(keep_a);
     if (keep) {
       const int e = count + fnl_rank_below(m);
-      if (e < max_nbr) row[e] = j;
+      if (e < max_nbr) {
+        row[e] = j;
+        if (gen) nbr_shift[(size_t)i * max_nbr + e] = code;
+      }
       if (keep_a) {
         const int ea = count_a + fnl_rank_below(ma);
         if (ea < max_nbr_ang && e < max_nbr) row_a[ea] = e;
@@ -553,10 +707,13 @@ k_skin_filter(const float* __restrict__ coords, const int32_t* __restrict__ batc
   }
 }
 
+// Flat edge list from the rows.  d12 is recomputed as the build computed it: minimum image
+// from the positions, or, for general PBC, from the wrapped positions and the cell shift
+// (total shift - at_j + at_i), so that it equals the reference's exported d12 bit for bit.
 __global__ void k_export(const float* __restrict__ coords, const int32_t* __restrict__ batch_index,
                          int n_atoms, int pbc_mode, const float* __restrict__ cells,
                          const float* __restrict__ rec, const int32_t* __restrict__ nbr,
-                         const int32_t* __restrict__ n_nbr, int max_nbr,
+                         const int32_t* __restrict__ nbr_shift, const int32_t* __restrict__ n_nbr, int max_nbr,
                          const int32_t* __restrict__ row_start, int32_t* __restrict__ edge_src,
                          int32_t* __restrict__ edge_dst, float* __restrict__ d12,
                          float* __restrict__ shifts) {
@@ -568,15 +725,31 @@ __global__ void k_export(const float* __restrict__ coords, const int32_t* __rest
   const int b = batch_index[i];
   FnlCell cl;
   const bool pbc = pbc_mode == FNL_PBC_MINIMUM_IMAGE;
-  if (pbc) fnl_load_cell(cells, rec, b, cl);
+  const bool gen = pbc_mode == FNL_PBC_GENERAL;
+  if (pbc || gen) fnl_load_cell(cells, rec, b, cl);
   const int o = row_start[i];
   for (int e = lane; e < n; e += 64) {
     const int j = nbr[(size_t)i * max_nbr + e];
-    float vx = coords[3 * j] - coords[3 * i];
-    float vy = coords[3 * j + 1] - coords[3 * i + 1];
-    float vz = coords[3 * j + 2] - coords[3 * i + 2];
+    float vx, vy, vz;
     float s[3] = {0.f, 0.f, 0.f};
-    if (pbc) fnl_min_image(vx, vy, vz, cl, s);
+    if (gen) {
+      float xwi[3], xwj[3];
+      int ati[3], atj[3];
+      fnl_wrap(coords[3 * i], coords[3 * i + 1], coords[3 * i + 2], cl, xwi, ati);
+      fnl_wrap(coords[3 * j], coords[3 * j + 1], coords[3 * j + 2], cl, xwj, atj);
+      fnl_shift_unpack(nbr_shift[(size_t)i * max_nbr + e], s[0], s[1], s[2]);
+      vx = xwj[0] - xwi[0];
+      vy = xwj[1] - xwi[1];
+      vz = xwj[2] - xwi[2];
+      fnl_apply_shift(vx, vy, vz, cl,
+                      fnl_shift_pack((int)s[0] - atj[0] + ati[0], (int)s[1] - atj[1] + ati[1],
+                                     (int)s[2] - atj[2] + ati[2]));
+    } else {
+      vx = coords[3 * j] - coords[3 * i];
+      vy = coords[3 * j + 1] - coords[3 * i + 1];
+      vz = coords[3 * j + 2] - coords[3 * i + 2];
+      if (pbc) fnl_min_image(vx, vy, vz, cl, s);
+    }
     edge_src[o + e] = i;
     edge_dst[o + e] = j;
     d12[o + e] = fnl_d12(vx, vy, vz);
@@ -631,13 +804,15 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
                                 const float* cells, const float* reciprocal_cells,
                                 const int32_t* grid, int32_t n_cells, double cutoff,
                                 double cutoff_ang, int32_t max_nbr, int32_t max_nbr_ang,
-                                void* workspace, size_t workspace_bytes, int32_t* nbr,
+                                void* workspace, size_t workspace_bytes, int32_t* nbr, int32_t* nbr_shift,
                                 int32_t* n_nbr, int32_t* nbr_ang, int32_t* n_nbr_ang,
                                 int32_t* status, fnl_stream_t stream) {
   if (n_atoms < 0 || n_sys < 1 || n_cells < 1 || max_nbr < 1 || max_nbr_ang < 1) return FNL_ERR_INVALID;
   if (!(cutoff > 0.0) || cutoff_ang > cutoff) return FNL_ERR_INVALID;
-  if (pbc_mode != FNL_PBC_NONE && pbc_mode != FNL_PBC_MINIMUM_IMAGE) return FNL_ERR_UNSUPPORTED;
-  if (pbc_mode == FNL_PBC_MINIMUM_IMAGE && (!cells || !reciprocal_cells)) return FNL_ERR_INVALID;
+  if (pbc_mode != FNL_PBC_NONE && pbc_mode != FNL_PBC_MINIMUM_IMAGE && pbc_mode != FNL_PBC_GENERAL)
+    return FNL_ERR_UNSUPPORTED;
+  if (pbc_mode != FNL_PBC_NONE && (!cells || !reciprocal_cells)) return FNL_ERR_INVALID;
+  if (pbc_mode == FNL_PBC_GENERAL && n_atoms > 0 && !nbr_shift) return FNL_ERR_INVALID;
   if (!status || !grid || !sys_offsets) return FNL_ERR_INVALID;
   if (n_atoms > 0 && (!coords || !species_idx || !batch_index || !nbr || !n_nbr)) return FNL_ERR_INVALID;
   if ((nbr_ang == nullptr) != (n_nbr_ang == nullptr)) return FNL_ERR_INVALID;
@@ -655,24 +830,34 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
                      reciprocal_cells, grid, (float)cutoff, ws.grid, status, ws.cell_count, ws.cell_fill, n_cells);
   FNL_LAUNCH_CHECK();
   if (n_atoms == 0) return FNL_OK;
-  hipLaunchKernelGGL(k_bin, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, s, coords, species_idx,
-                     batch_index, n_atoms, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_count,
-                     n_nbr, n_nbr_ang, n_cells, status);
-  FNL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, ws.cell_count, n_cells, ws.cell_start, status);
-  FNL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scatter, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, s, n_atoms, ws.atom_cell,
-                     ws.cell_start, ws.cell_fill, ws.tmp_sorted);
-  FNL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_cell_sort, dim3(fnl_blocks((long long)n_cells * 64, 256)), dim3(256), 0, s, n_cells,
-                     ws.cell_start, ws.tmp_sorted, coords, ws.sorted, ws.pos);
-  FNL_LAUNCH_CHECK();
   constexpr int W = 4;
-  hipLaunchKernelGGL(k_pairs<W>, dim3(fnl_blocks(n_atoms, W)), dim3(64 * W), 0, s, coords, batch_index,
-                     n_atoms, cells, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_start, ws.sorted,
-                     ws.pos, rc2, rca2, max_nbr, max_nbr_ang, nbr, n_nbr, nbr_ang, n_nbr_ang, status,
-                     n_cells);
-  FNL_LAUNCH_CHECK();
+  if (pbc_mode == FNL_PBC_GENERAL) {
+    hipLaunchKernelGGL(k_wrap, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, s, coords, species_idx, batch_index,
+                       n_atoms, cells, reciprocal_cells, ws.pos, n_nbr, n_nbr_ang, status);
+    FNL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pairs_general<W>, dim3(fnl_blocks(n_atoms, W)), dim3(64 * W), 0, s, species_idx,
+                       batch_index, sys_offsets, n_atoms, cells, reciprocal_cells, ws.grid, ws.pos, rc2, rca2,
+                       max_nbr, max_nbr_ang, nbr, nbr_shift, n_nbr, nbr_ang, n_nbr_ang, status);
+    FNL_LAUNCH_CHECK();
+  } else {
+    hipLaunchKernelGGL(k_bin, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, s, coords, species_idx,
+                       batch_index, n_atoms, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_count,
+                       n_nbr, n_nbr_ang, n_cells, status);
+    FNL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, ws.cell_count, n_cells, ws.cell_start, status);
+    FNL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scatter, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, s, n_atoms, ws.atom_cell,
+                       ws.cell_start, ws.cell_fill, ws.tmp_sorted);
+    FNL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_cell_sort, dim3(fnl_blocks((long long)n_cells * 64, 256)), dim3(256), 0, s, n_cells,
+                       ws.cell_start, ws.tmp_sorted, coords, ws.sorted, ws.pos);
+    FNL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pairs<W>, dim3(fnl_blocks(n_atoms, W)), dim3(64 * W), 0, s, coords, batch_index,
+                       n_atoms, cells, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_start, ws.sorted,
+                       ws.pos, rc2, rca2, max_nbr, max_nbr_ang, nbr, n_nbr, nbr_ang, n_nbr_ang, status,
+                       n_cells);
+    FNL_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(k_row_max, dim3(std::min(16, (n_atoms + 4095) / 4096)), dim3(1024), 0, s, n_nbr, n_nbr_ang,
                      n_atoms, status);
   FNL_LAUNCH_CHECK();
@@ -681,14 +866,16 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
 
 extern "C" int fnl_nblist_skin_update(const float* coords, const int32_t* batch_index, int32_t n_atoms,
                                       int32_t pbc_mode, const float* cells, const float* reciprocal_cells,
-                                      const int32_t* nbr_skin, const int32_t* n_skin, int32_t max_skin,
-                                      double cutoff, double cutoff_ang, int32_t max_nbr, int32_t max_nbr_ang,
-                                      int32_t* nbr, int32_t* n_nbr, int32_t* nbr_ang, int32_t* n_nbr_ang,
-                                      int32_t* status, fnl_stream_t stream) {
+                                      const int32_t* nbr_skin, const int32_t* skin_shift, const int32_t* n_skin,
+                                      int32_t max_skin, double cutoff, double cutoff_ang, int32_t max_nbr,
+                                      int32_t max_nbr_ang, int32_t* nbr, int32_t* nbr_shift, int32_t* n_nbr,
+                                      int32_t* nbr_ang, int32_t* n_nbr_ang, int32_t* status, fnl_stream_t stream) {
   if (n_atoms < 0 || max_skin < 1 || max_nbr < 1 || max_nbr_ang < 1) return FNL_ERR_INVALID;
   if (!(cutoff > 0.0) || cutoff_ang > cutoff || !status) return FNL_ERR_INVALID;
-  if (pbc_mode != FNL_PBC_NONE && pbc_mode != FNL_PBC_MINIMUM_IMAGE) return FNL_ERR_UNSUPPORTED;
-  if (pbc_mode == FNL_PBC_MINIMUM_IMAGE && (!cells || !reciprocal_cells)) return FNL_ERR_INVALID;
+  if (pbc_mode != FNL_PBC_NONE && pbc_mode != FNL_PBC_MINIMUM_IMAGE && pbc_mode != FNL_PBC_GENERAL)
+    return FNL_ERR_UNSUPPORTED;
+  if (pbc_mode != FNL_PBC_NONE && (!cells || !reciprocal_cells)) return FNL_ERR_INVALID;
+  if (pbc_mode == FNL_PBC_GENERAL && n_atoms > 0 && (!skin_shift || !nbr_shift)) return FNL_ERR_INVALID;
   if (n_atoms > 0 && (!coords || !batch_index || !nbr_skin || !n_skin || !nbr || !n_nbr || !nbr_ang || !n_nbr_ang))
     return FNL_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
@@ -699,8 +886,8 @@ extern "C" int fnl_nblist_skin_update(const float* coords, const int32_t* batch_
   if (n_atoms == 0) return FNL_OK;
   constexpr int W = 4;
   hipLaunchKernelGGL(k_skin_filter<W>, dim3(fnl_blocks(n_atoms, W)), dim3(64 * W), 0, s, coords, batch_index,
-                     n_atoms, pbc_mode, cells, reciprocal_cells, nbr_skin, n_skin, max_skin, rc2, rca2, max_nbr,
-                     max_nbr_ang, nbr, n_nbr, nbr_ang, n_nbr_ang, status);
+                     n_atoms, pbc_mode, cells, reciprocal_cells, nbr_skin, skin_shift, n_skin, max_skin, rc2,
+                     rca2, max_nbr, max_nbr_ang, nbr, nbr_shift, n_nbr, nbr_ang, n_nbr_ang, status);
   FNL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_row_max, dim3(std::min(16, (n_atoms + 4095) / 4096)), dim3(1024), 0, s, n_nbr, n_nbr_ang,
                      n_atoms, status);
@@ -710,17 +897,20 @@ extern "C" int fnl_nblist_skin_update(const float* coords, const int32_t* batch_
 
 extern "C" int fnl_nblist_export(const float* coords, const int32_t* batch_index, int32_t n_atoms,
                                  int32_t pbc_mode, const float* cells, const float* reciprocal_cells,
-                                 const int32_t* nbr, const int32_t* n_nbr, int32_t max_nbr,
-                                 const int32_t* row_start, int32_t* edge_src, int32_t* edge_dst,
-                                 float* d12, float* pbc_shifts, fnl_stream_t stream) {
+                                 const int32_t* nbr, const int32_t* nbr_shift, const int32_t* n_nbr,
+                                 int32_t max_nbr, const int32_t* row_start, int32_t* edge_src,
+                                 int32_t* edge_dst, float* d12, float* pbc_shifts, fnl_stream_t stream) {
   if (n_atoms < 0 || max_nbr < 1) return FNL_ERR_INVALID;
   if (n_atoms == 0) return FNL_OK;
   if (!coords || !batch_index || !nbr || !n_nbr || !row_start || !edge_src || !edge_dst || !d12)
     return FNL_ERR_INVALID;
-  if (pbc_mode == FNL_PBC_MINIMUM_IMAGE && (!cells || !reciprocal_cells)) return FNL_ERR_INVALID;
+  if (pbc_mode != FNL_PBC_NONE && pbc_mode != FNL_PBC_MINIMUM_IMAGE && pbc_mode != FNL_PBC_GENERAL)
+    return FNL_ERR_UNSUPPORTED;
+  if (pbc_mode != FNL_PBC_NONE && (!cells || !reciprocal_cells)) return FNL_ERR_INVALID;
+  if (pbc_mode == FNL_PBC_GENERAL && !nbr_shift) return FNL_ERR_INVALID;
   hipLaunchKernelGGL(k_export, dim3(fnl_blocks((long long)n_atoms * 64, 256)), dim3(256), 0,
                      (hipStream_t)stream, coords, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells,
-                     nbr, n_nbr, max_nbr, row_start, edge_src, edge_dst, d12, pbc_shifts);
+                     nbr, nbr_shift, n_nbr, max_nbr, row_start, edge_src, edge_dst, d12, pbc_shifts);
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }

@@ -171,15 +171,16 @@ class FENNIX:
         sp = np.asarray(inputs["species"], np.int32)
         nat = np.asarray(inputs["natoms"], np.int32)
         cells = inputs.get("cells")
-        if cells is not None and "minimum_image" not in inputs.get("flags", {}):
-            raise NotImplementedError("periodic systems need flags={'minimum_image': None} (general PBC is "
-                                      "not implemented on the GPU path)")
-        key = (sp.tobytes(), nat.tobytes(), None if cells is None else np.asarray(cells, np.float64).tobytes())
+        # periodic: the minimum-image convention with flags={"minimum_image": None}, else every
+        # image within the cutoff (GraphGenerator, preprocessing.py:127-257)
+        minimage = "minimum_image" in inputs.get("flags", {})
+        key = (sp.tobytes(), nat.tobytes(), None if cells is None else np.asarray(cells, np.float64).tobytes(),
+               minimage)
         sb = self._cache.get(key)
         if sb is None:
             if len(self._cache) > 8:
                 self._cache.clear()
-            sb = self.potential.setup(sp, nat, cells)
+            sb = self.potential.setup(sp, nat, cells, minimum_image=minimage)
             self._cache[key] = sb
         return sb
 

@@ -45,13 +45,14 @@ class LangevinMD:
     ``nblist_skin`` (A, > 0 enables it), ``nblist_stride`` (steps between full
     rebuilds; <= 0: from the skin, md/integrate.py:498-504) and
     ``nblist_warmup_fs`` (full rebuilds every step before that time) follow
-    fennol_md's keywords of the same names (integrate.py:477-513).
+    fennol_md's keywords of the same names (integrate.py:477-513);
+    ``minimum_image`` False selects general PBC for periodic systems.
     """
 
     def __init__(self, potential: ANIPotential, system: dict, dt_fs: float = 0.5, temperature: float = 300.0,
                  gamma_thz: float = 1.0, seed: int = 0, thermostat: str = "LGV", energy_unit: str = "eV",
                  check_every: int = 1, sb: SystemBuffers | None = None, nblist_skin: float = -1.0,
-                 nblist_stride: int = -1, nblist_warmup_fs: float = -1.0):
+                 nblist_stride: int = -1, nblist_warmup_fs: float = -1.0, minimum_image: bool = True):
         self.pot = potential
         self.lib = potential.lib
         dev = potential.device
@@ -59,7 +60,8 @@ class LangevinMD:
         natoms = system.get("natoms")
         cells = system.get("cells")
         skin = float(nblist_skin) if nblist_skin > 0 else 0.0
-        self.sb = sb if sb is not None else potential.setup(species, natoms, cells, skin=skin)
+        self.sb = sb if sb is not None else potential.setup(species, natoms, cells, skin=skin,
+                                                            minimum_image=minimum_image)
         if self.sb.skin != skin:
             raise ValueError(f"buffers built with skin {self.sb.skin}, MD asks for {skin}")
         self.nblist_stride = nblist_schedule(skin, dt_fs, nblist_stride)

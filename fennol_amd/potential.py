@@ -93,6 +93,14 @@ class ANIConfig:
         return conv
 
 
+def _status_errors(st) -> None:
+    """Raise for the status bits that regrowing the rows cannot fix."""
+    if st[_abi.FNL_STATUS_OVERFLOW] & 2:
+        raise RuntimeError("cell grid capacities inconsistent with the system")
+    if st[_abi.FNL_STATUS_OVERFLOW] & 4:
+        raise RuntimeError("an atom lies more than 511 cells from its cell's origin (general PBC shift range)")
+
+
 def _next_mult(x: int, m: int) -> int:
     return ((x + m - 1) // m) * m
 
@@ -170,6 +178,7 @@ class SystemBuffers:
     n_nbr: torch.Tensor = None
     nbr_ang: torch.Tensor = None
     n_nbr_ang: torch.Tensor = None
+    nbr_shift: torch.Tensor = None      # int32 [N, max_nbr] packed cell shifts (general PBC only)
     status: torch.Tensor = None
     workspace: torch.Tensor = None
     perm: torch.Tensor = None
@@ -193,6 +202,7 @@ class SystemBuffers:
     skin: float = 0.0
     max_skin: int = 0
     nbr_skin: torch.Tensor = None
+    skin_shift: torch.Tensor = None
     n_skin: torch.Tensor = None
     status_skin: torch.Tensor = None
     extra: dict = field(default_factory=dict)
@@ -263,10 +273,13 @@ class ANIPotential:
             first += int(np.prod(caps))
         return g, max(first, 1)
 
-    def setup(self, species, natoms=None, cells=None, skin: float = 0.0) -> SystemBuffers:
+    def setup(self, species, natoms=None, cells=None, skin: float = 0.0,
+              minimum_image: bool = True) -> SystemBuffers:
         """Device buffers of one batch.  ``skin`` > 0 (A) adds a Verlet skin list: full builds
         use cutoff + skin and :meth:`update_skin` re-filters it between builds
-        (preprocessing.py:640-696)."""
+        (preprocessing.py:640-696).  With cells, ``minimum_image`` selects the minimum-image
+        convention (the reference's ``flags={"minimum_image": None}``) or, if False, general
+        PBC with every image within the cutoff (preprocessing.py:144-257)."""
         species = np.asarray(species, np.int32)
         N = species.shape[0]
         natoms = np.array([N], np.int32) if natoms is None else np.asarray(natoms, np.int32)
@@ -292,7 +305,7 @@ class ANIPotential:
             cells = np.asarray(cells, np.float64).reshape(-1, 3, 3)
             if cells.shape[0] == 1 and B > 1:
                 cells = np.repeat(cells, B, axis=0)
-            pbc = _abi.FNL_PBC_MINIMUM_IMAGE
+            pbc = _abi.FNL_PBC_MINIMUM_IMAGE if minimum_image else _abi.FNL_PBC_GENERAL
             tc = torch.tensor(cells.astype(np.float32), device=dev).contiguous()
             tr = torch.tensor(np.linalg.inv(cells).astype(np.float32), device=dev).contiguous()
         grid, n_cells = self.grid_caps(natoms, cells)
@@ -448,6 +461,9 @@ class ANIPotential:
         sb.n_nbr = torch.zeros(max(N, 1), dtype=torch.int32, device=dev)
         sb.nbr_ang = torch.zeros((max(N, 1), sb.max_nbr_ang), dtype=torch.int32, device=dev)
         sb.n_nbr_ang = torch.zeros(max(N, 1), dtype=torch.int32, device=dev)
+        sb.nbr_shift = None
+        if sb.pbc == _abi.FNL_PBC_GENERAL:
+            sb.nbr_shift = torch.zeros((max(N, 1), sb.max_nbr), dtype=torch.int32, device=dev)
         sb.status = torch.zeros(_abi.FNL_STATUS_WORDS, dtype=torch.int32, device=dev)
         wsb = self.lib.fnl_nblist_workspace(N, max(n_sys, 1), max(n_cells, 1))
         sb.workspace = torch.empty(max(int(wsb), 256), dtype=torch.uint8, device=dev)
@@ -457,6 +473,9 @@ class ANIPotential:
     def _alloc_skin(self, sb: SystemBuffers) -> None:
         N = max(sb.extra.get("max_atoms", sb.n_atoms), 1)
         sb.nbr_skin = torch.zeros((N, sb.max_skin), dtype=torch.int32, device=self.device)
+        sb.skin_shift = None
+        if sb.pbc == _abi.FNL_PBC_GENERAL:
+            sb.skin_shift = torch.zeros((N, sb.max_skin), dtype=torch.int32, device=self.device)
         sb.n_skin = torch.zeros(N, dtype=torch.int32, device=self.device)
         sb.status_skin = torch.zeros(_abi.FNL_STATUS_WORDS, dtype=torch.int32, device=self.device)
 
@@ -492,15 +511,15 @@ class ANIPotential:
                 P(coords32), P(sb.species_idx), P(sb.batch_index), P(sb.sys_offsets), sb.n_atoms, sb.n_sys,
                 sb.pbc, P(sb.cells), P(sb.rcells), P(sb.grid), sb.n_cells, self.config.cutoff + sb.skin,
                 self.config.cutoff_angular, sb.max_skin, 1, P(sb.workspace), sb.workspace.numel(),
-                P(sb.nbr_skin), P(sb.n_skin), None, None, P(sb.status_skin), s)
+                P(sb.nbr_skin), P(sb.skin_shift), P(sb.n_skin), None, None, P(sb.status_skin), s)
             _abi.check(rc, "fnl_nblist_build")
             self.update_skin(sb, coords32, stream)
             return
         rc = self.lib.fnl_nblist_build(
             P(coords32), P(sb.species_idx), P(sb.batch_index), P(sb.sys_offsets), sb.n_atoms, sb.n_sys, sb.pbc,
             P(sb.cells), P(sb.rcells), P(sb.grid), sb.n_cells, self.config.cutoff, self.config.cutoff_angular,
-            sb.max_nbr, sb.max_nbr_ang, P(sb.workspace), sb.workspace.numel(), P(sb.nbr), P(sb.n_nbr),
-            P(sb.nbr_ang), P(sb.n_nbr_ang), P(sb.status), s)
+            sb.max_nbr, sb.max_nbr_ang, P(sb.workspace), sb.workspace.numel(), P(sb.nbr), P(sb.nbr_shift),
+            P(sb.n_nbr), P(sb.nbr_ang), P(sb.n_nbr_ang), P(sb.status), s)
         _abi.check(rc, "fnl_nblist_build")
 
     def update_skin(self, sb: SystemBuffers, coords32: torch.Tensor, stream=None) -> None:
@@ -512,9 +531,9 @@ class ANIPotential:
         P = _abi.ptr
         rc = self.lib.fnl_nblist_skin_update(
             P(coords32), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells), P(sb.nbr_skin),
-            P(sb.n_skin), sb.max_skin, self.config.cutoff, self.config.cutoff_angular, sb.max_nbr,
-            sb.max_nbr_ang, P(sb.nbr), P(sb.n_nbr), P(sb.nbr_ang), P(sb.n_nbr_ang), P(sb.status),
-            _abi.stream_handle(stream))
+            P(sb.skin_shift), P(sb.n_skin), sb.max_skin, self.config.cutoff, self.config.cutoff_angular,
+            sb.max_nbr, sb.max_nbr_ang, P(sb.nbr), P(sb.nbr_shift), P(sb.n_nbr), P(sb.nbr_ang),
+            P(sb.n_nbr_ang), P(sb.status), _abi.stream_handle(stream))
         _abi.check(rc, "fnl_nblist_skin_update")
 
     def check_reallocate(self, sb: SystemBuffers, coords32: torch.Tensor, mult: float = 1.05,
@@ -529,8 +548,7 @@ class ANIPotential:
         grown = False
         if sb.skin > 0:
             st = sb.status_skin.cpu().numpy()
-            if st[_abi.FNL_STATUS_OVERFLOW] & 2:
-                raise RuntimeError("cell grid capacities inconsistent with the system")
+            _status_errors(st)
             if st[_abi.FNL_STATUS_OVERFLOW]:
                 sb.max_skin = _next_mult(int(mult * max(st[_abi.FNL_STATUS_MAX_NBR], sb.max_skin)) + add, 8)
                 self._alloc_skin(sb)
@@ -539,16 +557,15 @@ class ANIPotential:
                     raise RuntimeError("skin list still overflows after reallocation")
                 grown = True
         st = sb.status.cpu().numpy()
-        if st[_abi.FNL_STATUS_OVERFLOW] & 2:
-            raise RuntimeError("cell grid capacities inconsistent with the system")
+        _status_errors(st)
         if st[_abi.FNL_STATUS_OVERFLOW] == 0:
             return grown
         sb.max_nbr = _next_mult(int(mult * max(st[_abi.FNL_STATUS_MAX_NBR], sb.max_nbr)) + add, 8)
         sb.max_nbr_ang = _next_mult(int(mult * max(st[_abi.FNL_STATUS_MAX_NBR_ANG], sb.max_nbr_ang)) + add, 8)
         if sb.skin > 0:
-            nbr_skin, n_skin, status_skin = sb.nbr_skin, sb.n_skin, sb.status_skin
+            kept = sb.nbr_skin, sb.skin_shift, sb.n_skin, sb.status_skin
             self._alloc_nblist(sb)
-            sb.nbr_skin, sb.n_skin, sb.status_skin = nbr_skin, n_skin, status_skin
+            sb.nbr_skin, sb.skin_shift, sb.n_skin, sb.status_skin = kept
             self.update_skin(sb, coords32)
         else:
             self._alloc_nblist(sb)
@@ -564,7 +581,7 @@ class ANIPotential:
         dt = _abi.FNL_F32 if self.precision == "fp32" else _abi.FNL_BF16
         rc = self.lib.fnl_aev_forward(
             P(coords32), P(sb.species_idx), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
-            P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(sb.nbr_ang), P(sb.n_nbr_ang), sb.max_nbr_ang,
+            P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(sb.nbr_shift), P(sb.nbr_ang), P(sb.n_nbr_ang), sb.max_nbr_ang,
             C.byref(self.cparams), P(sb.aev), sb.aev.shape[1], dt, s)
         _abi.check(rc, "fnl_aev_forward")
 
@@ -591,7 +608,7 @@ class ANIPotential:
         P = _abi.ptr
         rc = self.lib.fnl_aev_backward(
             P(coords32), P(sb.species_idx), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
-            P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(sb.nbr_ang), P(sb.n_nbr_ang), sb.max_nbr_ang,
+            P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(sb.nbr_shift), P(sb.nbr_ang), P(sb.n_nbr_ang), sb.max_nbr_ang,
             C.byref(self.cparams), P(sb.g_aev), sb.g_aev.shape[1], P(sb.grad_self), P(sb.grad_nbr),
             P(sb.vir_atom), s)
         _abi.check(rc, "fnl_aev_backward")

@@ -258,7 +258,8 @@ class PreprocessingChain:
         rst = torch.tensor(rs[:-1], device=dev)
         P = _abi.ptr
         _abi.check(lib.fnl_nblist_export(P(x32), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
-                                         P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(rst), P(es), P(ed), P(d12), P(sh),
+                                         P(sb.nbr), P(sb.nbr_shift), P(sb.n_nbr), sb.max_nbr, P(rst), P(es), P(ed),
+                                         P(d12), P(sh),
                                          _abi.stream_handle()), "fnl_nblist_export")
         es, ed, d12, sh = (t.cpu().numpy()[:E] for t in (es, ed, d12, sh))
         nat = sb.n_atoms

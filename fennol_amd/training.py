@@ -67,7 +67,7 @@ class TrainingStep:
         P = _abi.ptr
         _abi.check(self.pot.lib.fnl_aev_tangent(
             P(x32), P(disp), P(sb.species_idx), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
-            P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(sb.nbr_ang), P(sb.n_nbr_ang), sb.max_nbr_ang,
+            P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(sb.nbr_shift), P(sb.nbr_ang), P(sb.n_nbr_ang), sb.max_nbr_ang,
             ctypes.byref(self.pot.cparams), P(out), out.shape[1], _abi.stream_handle()),
             "fnl_aev_tangent")
         return out

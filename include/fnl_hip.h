@@ -40,13 +40,23 @@ typedef struct ihipStream_t* fnl_stream_t; /* == hipStream_t */
 /* periodic boundary modes */
 #define FNL_PBC_NONE 0
 #define FNL_PBC_MINIMUM_IMAGE 1
+/* every image within the cutoff (cells narrower than twice the cutoff); rows carry
+ * a packed integer cell shift per slot (nbr_shift) */
+#define FNL_PBC_GENERAL 2
+
+/* packed cell shift of a general-PBC slot: (s0+512) | (s1+512)<<10 | (s2+512)<<20,
+ * |s_a| <= 511; vec_ij = x_j - x_i + s . C */
+#define FNL_SHIFT_PACK(s0, s1, s2) \
+  ((int32_t)(((s0) + 512) | (((s1) + 512) << 10) | (((s2) + 512) << 20)))
 
 /* element types */
 #define FNL_F32 0
 #define FNL_BF16 1
 
 /* status words written by fnl_nblist_build (int32[FNL_STATUS_WORDS]) */
-#define FNL_STATUS_OVERFLOW 0       /* nonzero if any capacity was exceeded       */
+#define FNL_STATUS_OVERFLOW 0       /* bit 0: a row capacity was exceeded; bit 1:
+                                       grid capacities inconsistent; bit 2: a
+                                       general-PBC shift beyond +-511 cells        */
 #define FNL_STATUS_MAX_NBR 1        /* largest radial neighbour count seen          */
 #define FNL_STATUS_MAX_NBR_ANG 2    /* largest angular neighbour count seen         */
 #define FNL_STATUS_MAX_CELL 3       /* largest cell population seen                 */
@@ -91,12 +101,21 @@ int fnl_nblist_build(const float* coords, const int32_t* species_idx,
                      double cutoff, double cutoff_ang,
                      int32_t max_nbr, int32_t max_nbr_ang,
                      void* workspace, size_t workspace_bytes,
-                     int32_t* nbr, int32_t* n_nbr,
+                     int32_t* nbr, int32_t* nbr_shift, int32_t* n_nbr,
                      int32_t* nbr_ang, int32_t* n_nbr_ang,
                      int32_t* status, fnl_stream_t stream);
 
 /* nbr_ang and n_nbr_ang may both be NULL: radial rows only (the list built
- * with cutoff + skin for fnl_nblist_skin_update). */
+ * with cutoff + skin for fnl_nblist_skin_update).
+ *
+ * pbc_mode FNL_PBC_GENERAL replaces the general-PBC branch of the reference
+ * (preprocessing.py:144-257, pbc_shifts :288-293): atoms are wrapped into the
+ * cell (at = -floor(x . R), xw = x + at . C), every atom j != i of the system
+ * is tested at every shift s with |s_a| <= ceil(cutoff * |column a of R|)
+ * with d12 of vec = (xw_j - xw_i) + s . C, and nbr_shift[i*max_nbr + e] holds
+ * the packed total shift s + at_j - at_i.  Self images are not pairs (the
+ * reference pairs i < j).  Cost O(atoms * shifts) per centre, meant for the
+ * small cells that need it.  nbr_shift is ignored (may be NULL) otherwise. */
 
 /* Skin update: the radial and angular rows at the current positions selected
  * from the rows of a list built by fnl_nblist_build with cutoff + skin
@@ -107,15 +126,19 @@ int fnl_nblist_build(const float* coords, const int32_t* species_idx,
  * full build's float32 arithmetic; rows keep the order of the skin rows.
  * Exact as long as no atom moved more than skin/2 since the build (the
  * reference's approximation); with the minimum-image convention, cutoff +
- * skin must stay below half of every lattice-plane spacing.  status: as for
- * fnl_nblist_build (overflow also if a skin row had overflowed). */
+ * skin must stay below half of every lattice-plane spacing.  With
+ * FNL_PBC_GENERAL the skin rows' shifts (skin_shift) are applied to the
+ * current positions, vec = x_j - x_i + s . C (update_skin :648-659), and
+ * copied to nbr_shift.  status: as for fnl_nblist_build (overflow also if a
+ * skin row had overflowed). */
 int fnl_nblist_skin_update(const float* coords, const int32_t* batch_index,
                            int32_t n_atoms, int32_t pbc_mode,
                            const float* cells, const float* reciprocal_cells,
-                           const int32_t* nbr_skin, const int32_t* n_skin, int32_t max_skin,
+                           const int32_t* nbr_skin, const int32_t* skin_shift,
+                           const int32_t* n_skin, int32_t max_skin,
                            double cutoff, double cutoff_ang,
                            int32_t max_nbr, int32_t max_nbr_ang,
-                           int32_t* nbr, int32_t* n_nbr,
+                           int32_t* nbr, int32_t* nbr_shift, int32_t* n_nbr,
                            int32_t* nbr_ang, int32_t* n_nbr_ang,
                            int32_t* status, fnl_stream_t stream);
 
@@ -126,7 +149,8 @@ int fnl_nblist_skin_update(const float* coords, const int32_t* batch_index,
 int fnl_nblist_export(const float* coords, const int32_t* batch_index,
                       int32_t n_atoms, int32_t pbc_mode,
                       const float* cells, const float* reciprocal_cells,
-                      const int32_t* nbr, const int32_t* n_nbr, int32_t max_nbr,
+                      const int32_t* nbr, const int32_t* nbr_shift,
+                      const int32_t* n_nbr, int32_t max_nbr,
                       const int32_t* row_start,
                       int32_t* edge_src, int32_t* edge_dst, float* d12,
                       float* pbc_shifts, fnl_stream_t stream);
@@ -160,11 +184,16 @@ typedef struct fnl_aev_params {
 
 /* AEV length: S*nR + S(S+1)/2 * nA*nZ.  Row i of aev (stride aev_stride
  * elements, element type out_dtype) gets the channels of centre i in the
- * reference order; columns [length, aev_stride) are written as zero. */
+ * reference order; columns [length, aev_stride) are written as zero.
+ * Pair vectors: x_j - x_i, plus the minimum image (FNL_PBC_MINIMUM_IMAGE) or
+ * the slot's stored shift . C (FNL_PBC_GENERAL: nbr_shift from
+ * fnl_nblist_build, required; NULL for the other modes).  The same holds for
+ * fnl_aev_backward and fnl_aev_tangent. */
 int fnl_aev_forward(const float* coords, const int32_t* species_idx,
                     const int32_t* batch_index, int32_t n_atoms, int32_t pbc_mode,
                     const float* cells, const float* reciprocal_cells,
                     const int32_t* nbr, const int32_t* n_nbr, int32_t max_nbr,
+                    const int32_t* nbr_shift,
                     const int32_t* nbr_ang, const int32_t* n_nbr_ang, int32_t max_nbr_ang,
                     const fnl_aev_params* params,
                     void* aev, int32_t aev_stride, int32_t out_dtype,
@@ -187,6 +216,7 @@ int fnl_aev_backward(const float* coords, const int32_t* species_idx,
                      const int32_t* batch_index, int32_t n_atoms, int32_t pbc_mode,
                      const float* cells, const float* reciprocal_cells,
                      const int32_t* nbr, const int32_t* n_nbr, int32_t max_nbr,
+                     const int32_t* nbr_shift,
                      const int32_t* nbr_ang, const int32_t* n_nbr_ang, int32_t max_nbr_ang,
                      const fnl_aev_params* params,
                      const float* g, int32_t g_stride,
@@ -204,6 +234,7 @@ int fnl_aev_tangent(const float* coords, const float* disp, const int32_t* speci
                     const int32_t* batch_index, int32_t n_atoms, int32_t pbc_mode,
                     const float* cells, const float* reciprocal_cells,
                     const int32_t* nbr, const int32_t* n_nbr, int32_t max_nbr,
+                    const int32_t* nbr_shift,
                     const int32_t* nbr_ang, const int32_t* n_nbr_ang, int32_t max_nbr_ang,
                     const fnl_aev_params* params, float* daev, int32_t daev_stride,
                     fnl_stream_t stream);

@@ -252,8 +252,11 @@ def half_pairs(coords, natoms, cutoff, cells=None, reciprocal_cells=None,
                 at_shifts = -np.floor(frac)
                 xw = x + _f32_dot3(at_shifts, Cb)
                 vec0 = xw[p2] - xw[p1]
-                inv_d = np.sqrt((Rb.astype(np.float64) ** 2).sum(axis=0))
-                nrep = np.ceil((cutoff + skin) * inv_d).astype(int)
+                # float32 as the reference: (sum(R**2, axis=1))**0.5 and cutoff_skin * it
+                # (preprocessing.py:152-154)
+                Rf = Rb.astype(np.float32)
+                inv_d = np.sqrt((Rf[0] * Rf[0] + Rf[1] * Rf[1]) + Rf[2] * Rf[2]).astype(np.float32)
+                nrep = np.ceil(np.float32(cutoff + skin) * inv_d).astype(int)
                 grid = np.array(
                     np.meshgrid(*[np.arange(-k, k + 1) for k in nrep]), dtype=np.float32
                 ).T.reshape(-1, 3)

@@ -51,10 +51,10 @@ def triclinic_box(n_atoms=120, seed=0):
             "cells": cell[None], "reciprocal_cells": np.linalg.inv(cell)[None]}
 
 
-def device_half_list(pot, sb, x):
-    """Half list (src < dst, sorted) with float32 d12 from the device rows of ``sb`` at x,
-    after checking that the directed rows are symmetric and that every angular row holds
-    exactly the radial slots with d12 < rca^2."""
+def device_half_list(pot, sb, x, with_shifts=False):
+    """Half list (src < dst, sorted by src, dst and shift) with float32 d12 (and the cell
+    shifts) from the device rows of ``sb`` at x, after checking that the directed rows are
+    symmetric and that every angular row holds exactly the radial slots with d12 < rca^2."""
     import torch
 
     from fennol_amd import _abi
@@ -71,12 +71,13 @@ def device_half_list(pot, sb, x):
     rst = torch.tensor(rs[:-1], device=dev)
     P = _abi.ptr
     _abi.check(pot.lib.fnl_nblist_export(P(x), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
-                                         P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(rst), P(es), P(ed), P(d12), P(sh),
+                                         P(sb.nbr), P(sb.nbr_shift), P(sb.n_nbr), sb.max_nbr, P(rst), P(es), P(ed),
+                                         P(d12), P(sh),
                                          _abi.stream_handle()), "export")
-    es, ed, d12 = es.cpu().numpy()[:E], ed.cpu().numpy()[:E], d12.cpu().numpy()[:E]
-    i, j = rows_to_pairs(nbr, n_nbr, sb.max_nbr)
-    fwd = set(zip(i.tolist(), j.tolist()))
-    assert fwd == set(zip(j.tolist(), i.tolist())) and len(fwd) == len(i)
+    es, ed, d12, sh = es.cpu().numpy()[:E], ed.cpu().numpy()[:E], d12.cpu().numpy()[:E], sh.cpu().numpy()[:E]
+    key = [tuple(r) for r in np.concatenate([es[:, None], ed[:, None], sh], 1).astype(np.int64).tolist()]
+    rev = [(b, a, -s0, -s1, -s2) for a, b, s0, s1, s2 in key]
+    assert set(key) == set(rev) and len(set(key)) == len(key)
     na, nba = sb.n_nbr_ang.cpu().numpy(), sb.nbr_ang.cpu().numpy()
     assert np.all(na <= sb.max_nbr_ang)
     rca2 = np.float32(pot.config.cutoff_angular ** 2)
@@ -84,12 +85,21 @@ def device_half_list(pot, sb, x):
         slots = np.sort(nba[a, : na[a]])
         np.testing.assert_array_equal(slots, np.nonzero(d12[rs[a]: rs[a + 1]] < rca2)[0])
     m = es < ed
-    order = np.lexsort((ed[m], es[m]))
-    return es[m][order], ed[m][order], d12[m][order]
+    es, ed, d12, sh = es[m], ed[m], d12[m], sh[m]
+    order = np.lexsort((sh[:, 2], sh[:, 1], sh[:, 0], ed, es))
+    if with_shifts:
+        return es[order], ed[order], d12[order], sh[order]
+    return es[order], ed[order], d12[order]
 
 
-def canonical_half(src, dst, d12):
-    """Oracle half list sorted by (min, max) with its float32 d12."""
+def canonical_half(src, dst, d12, shifts=None):
+    """Oracle half list sorted by (min, max[, shift]) with its float32 d12 (and shifts, negated
+    where the pair was reversed)."""
+    src, dst = np.asarray(src), np.asarray(dst)
     lo, hi = np.minimum(src, dst), np.maximum(src, dst)
-    o = np.lexsort((hi, lo))
-    return lo[o].astype(np.int32), hi[o].astype(np.int32), np.asarray(d12, np.float32)[o]
+    if shifts is None:
+        o = np.lexsort((hi, lo))
+        return lo[o].astype(np.int32), hi[o].astype(np.int32), np.asarray(d12, np.float32)[o]
+    sh = np.where((src <= dst)[:, None], shifts, -np.asarray(shifts))
+    o = np.lexsort((sh[:, 2], sh[:, 1], sh[:, 0], hi, lo))
+    return lo[o].astype(np.int32), hi[o].astype(np.int32), np.asarray(d12, np.float32)[o], sh[o].astype(np.float32)

@@ -42,11 +42,16 @@ def test_host_side_argument_checks():
     assert lib.fnl_mlp_tile_rows(_abi.FNL_F32) == 32
     assert lib.fnl_mlp_tile_rows(_abi.FNL_BF16) == 96
     assert lib.fnl_nblist_build(None, None, None, None, 10, 1, 0, None, None, None, 1, 5.2, 3.5, 8, 8,
-                                None, 0, None, None, None, None, None, None) == _abi.FNL_ERR_INVALID if hasattr(_abi, "FNL_ERR_INVALID") else True
+                                None, 0, None, None, None, None, None, None, None) == -1
+    # general PBC without a shift output, skin update with an unknown mode
+    assert lib.fnl_nblist_build(None, None, None, None, 10, 1, _abi.FNL_PBC_GENERAL, None, None, None, 1, 5.2,
+                                3.5, 8, 8, None, 0, None, None, None, None, None, None, None) == -1
+    assert lib.fnl_nblist_skin_update(None, None, 10, 7, None, None, None, None, None, 8, 5.2, 3.5, 8, 8, None,
+                                      None, None, None, None, None, None) == -1
     p = _abi.AevParams()
     p.n_species = 9  # above the maximum
-    assert lib.fnl_aev_forward(None, None, None, 0, 0, None, None, None, None, 8, None, None, 8, p, None, 0, 0,
-                               None) == -1
+    assert lib.fnl_aev_forward(None, None, None, 0, 0, None, None, None, None, 8, None, None, None, 8, p, None, 0,
+                               0, None) == -1
     assert lib.fnl_fill_i32(None, 0, 0, None) == 0
     assert lib.fnl_fill_i32(None, 0, 5, None) == -1
     assert lib.fnl_species_mlp(None, 0, None, None, None, None, None, 0, None, None, 0, None) == -1

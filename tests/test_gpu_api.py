@@ -59,6 +59,55 @@ def test_fennix_energy_forces_virial_match_oracle(gpu):
     np.testing.assert_allclose(out2["embedding"], O.ani_aev(x32, s["species"], g, p), rtol=2e-5, atol=2e-5)
 
 
+def test_fennix_general_pbc_without_minimum_image_flag(gpu):
+    """Periodic inputs without flags={'minimum_image': None} use every image within the
+    cutoff (the reference's general branch); the 6.2 A water box needs it (rc = 5.2)."""
+    print("[api] FENNIX general PBC", flush=True)
+    s = water_box_small(8, 4)
+    model = ani_model(("H", "C", "N", "O", "S", "F", "Cl"))
+    inp = {"species": s["species"], "coordinates": s["coordinates"], "cells": s["cells"]}
+    e, f, v, _ = model.energy_and_forces_and_virial(**inp)
+    p = O.AEVParams(species_order=("H", "C", "N", "O", "S", "F", "Cl"))
+    x32 = np.asarray(s["coordinates"], np.float32).astype(np.float64)
+    Er, Fr, Vr, _ = O.energy_forces_virial(O.Model(p, model.params, minimum_image=False), x32, s["species"],
+                                           None, s["cells"], np.linalg.inv(s["cells"]))
+    np.testing.assert_allclose(e, Er, rtol=1e-5)
+    assert np.sqrt(np.mean((f - Fr) ** 2)) < 1e-4
+    np.testing.assert_allclose(v, Vr, rtol=1e-4, atol=1e-4 * np.abs(Vr).max())
+    # the minimum-image result differs on this box (images beyond the nearest are missed)
+    e_mi, _, _ = model.energy_and_forces(**inp, flags={"minimum_image": None})
+    assert abs(e_mi[0] - e[0]) > 1e-4 * abs(e[0])
+    # host-path export: pbc_shifts of the general branch (several images per pair)
+    out = model.preprocess(use_gpu=False, **inp)
+    g = out["graph"]
+    P = g["edge_src"].shape[0] // 2
+    valid = g["edge_src"][:P] < len(s["species"])
+    pairs = np.stack([g["edge_src"][:P][valid], g["edge_dst"][:P][valid]], 1)
+    assert len({tuple(r) for r in pairs.tolist()}) < len(pairs)
+
+
+def test_md_general_pbc_skin(gpu):
+    """Langevin MD in general PBC with a skin list stays finite and follows the
+    rebuild-every-step run."""
+    print("[api] MD general PBC", flush=True)
+    from fennol_amd.md import LangevinMD
+    from fennol_amd.potential import ANIConfig, ANIPotential
+
+    s = water_box_small(8, 4)
+    cfg = ANIConfig(species_order=("H", "C", "N", "O", "S", "F", "Cl"))
+    params = O.init_mlp_params(11, cfg.aev_length)
+    xs = []
+    for skin in (-1.0, 1.0):
+        pot = ANIPotential(cfg, params, precision="fp32")
+        md = LangevinMD(pot, s, dt_fs=0.5, temperature=300.0, seed=4, nblist_skin=skin, nblist_stride=8,
+                        minimum_image=False)
+        md.run(24)
+        md.verify()
+        xs.append(md.x.cpu().numpy())
+        assert np.all(np.isfinite(md.energies()))
+    np.testing.assert_allclose(xs[1], xs[0], rtol=0, atol=1e-6)
+
+
 def test_host_preprocessing_layout_matches_reference(gpu):
     print("[api] host preprocessing", flush=True)
     s = water_box_small(64, 4)

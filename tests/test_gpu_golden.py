@@ -18,7 +18,7 @@ import oracle as O
 from fennol_amd.md import LangevinMD
 from fennol_amd.potential import ANIPotential
 from tests.golden import make_golden as G
-from tests.helpers import configs, device_half_list
+from tests.helpers import canonical_half, configs, device_half_list
 
 pytestmark = pytest.mark.gpu
 
@@ -44,20 +44,23 @@ def _potential():
 def test_golden_system(gpu, name):
     _flush(f"[golden] {name}")
     f = _load(name)
-    if name == "triclinic5":
-        pytest.skip("general PBC (cell edges below twice the cutoff) is served by the oracle only; "
-                    "the device path implements the minimum-image convention")
     c, pot = _potential()
     cells = f["cells"] if "cells" in f else None
-    sb = pot.setup(f["species"], None, cells)
+    # triclinic5: cell edges below twice the cutoff, general PBC (several images per pair)
+    sb = pot.setup(f["species"], None, cells, minimum_image=name != "triclinic5")
     x = torch.tensor(f["coordinates"].astype(np.float32), device="cuda")
     pot.compute(sb, x, forces=True)
     F = pot.forces(sb)
     torch.cuda.synchronize()
-    src, dst, d12 = device_half_list(pot, sb, x)
-    np.testing.assert_array_equal(src, f["pair_src"])
-    np.testing.assert_array_equal(dst, f["pair_dst"])
-    np.testing.assert_array_equal(d12.view(np.uint32), f["pair_d12"].view(np.uint32))
+    src, dst, d12, sh = device_half_list(pot, sb, x, with_shifts=True)
+    if "pair_shifts" in f:
+        rs, rd, r12, rsh = canonical_half(f["pair_src"], f["pair_dst"], f["pair_d12"], f["pair_shifts"])
+        np.testing.assert_array_equal(sh, rsh)
+    else:
+        rs, rd, r12 = f["pair_src"], f["pair_dst"], f["pair_d12"]
+    np.testing.assert_array_equal(src, rs)
+    np.testing.assert_array_equal(dst, rd)
+    np.testing.assert_array_equal(d12.view(np.uint32), r12.view(np.uint32))
     aev = sb.aev.float().cpu().numpy()[:, : c.aev_length]
     ref = f["aev"].astype(np.float64)
     err = np.abs(aev - ref)

@@ -76,8 +76,8 @@ def test_nblist_pair_set(gpu, name):
     rst = torch.tensor(rs[:-1], device=dev)
     P = _abi.ptr
     _abi.check(pot.lib.fnl_nblist_export(P(x), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
-                                         P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(rst), P(es), P(ed), P(d12), P(sh),
-                                         _abi.stream_handle()), "export")
+                                         P(sb.nbr), P(sb.nbr_shift), P(sb.n_nbr), sb.max_nbr, P(rst), P(es), P(ed),
+                                         P(d12), P(sh), _abi.stream_handle()), "export")
     es, ed, d12 = es.cpu().numpy(), ed.cpu().numpy(), d12.cpu().numpy()
     m = es < ed
     order = np.lexsort((ed[m], es[m]))
@@ -225,6 +225,121 @@ def test_skin_update_matches_oracle(gpu, name):
     np.testing.assert_array_equal(d12.view(np.uint32), c12.view(np.uint32))
     fresh, _ = canonical_pairs_oracle(x1, s["natoms"], rc, cells)
     np.testing.assert_array_equal(np.stack([src, dst], 1), fresh)
+
+
+def _small_triclinic(n_atoms, seed, cell):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cell = np.asarray(cell, np.float64)
+    # atoms kept >= 0.9 A apart (a smooth energy for the force checks)
+    pts = []
+    while len(pts) < n_atoms:
+        x = rng.random(3) @ cell
+        ok = True
+        for y in pts:
+            d = x - y
+            for k in np.array(np.meshgrid(*[np.arange(-2, 3)] * 3)).T.reshape(-1, 3):
+                if np.linalg.norm(d + k @ cell) < 0.9:
+                    ok = False
+                    break
+            if not ok:
+                break
+        if ok:
+            pts.append(x)
+    species = rng.choice(np.array([1, 6, 7, 8], np.int32), size=n_atoms)
+    return {"species": species, "coordinates": np.array(pts), "natoms": np.array([n_atoms], np.int32),
+            "cells": cell[None], "reciprocal_cells": np.linalg.inv(cell)[None]}
+
+
+def _general_cases():
+    tri = _small_triclinic(14, 21, [[6.4, 0.0, 0.0], [1.3, 6.1, 0.0], [-0.8, 1.1, 6.6]])
+    tri2 = _small_triclinic(9, 22, [[5.2, 0.0, 0.0], [0.0, 5.6, 0.0], [1.5, -0.7, 5.9]])
+    w8 = water_box_small(8, 4)
+    both = systems.batch_systems([{k: w8[k] for k in ("species", "coordinates", "cells")},
+                                  {k: tri[k] for k in ("species", "coordinates", "cells")}])
+    # atoms moved out of the cell by different lattice vectors (unwrapped MD positions): the
+    # stored shifts carry the wrap
+    n_far = np.random.Generator(np.random.PCG64(23)).integers(-3, 4, size=(9, 3)).astype(np.float64)
+    far = {**tri2, "coordinates": tri2["coordinates"] + n_far @ tri2["cells"][0]}
+    return {"water8": w8, "triclinic14": tri, "batched": both, "unwrapped9": far}
+
+
+def _oracle_general(s, rc, skin=0.0):
+    cells = np.asarray(s["cells"])
+    return O.half_pairs(s["coordinates"], s["natoms"], rc, cells, np.linalg.inv(cells), minimum_image=False,
+                        skin=skin)
+
+
+@pytest.mark.parametrize("name", ["water8", "triclinic14", "batched", "unwrapped9"])
+def test_general_pbc_pair_set(gpu, name):
+    """Cells narrower than twice the cutoff: every image within the cutoff, with its cell shift,
+    pair set / shifts / d12 bit-exact against the oracle (general branch of the reference)."""
+    _flush(f"[nblist general] {name}")
+    s = _general_cases()[name]
+    cfg = configs()
+    _, pot = _potential(cfg, max_nbr=16, max_nbr_ang=8)
+    sb = pot.setup(s["species"], s["natoms"], s["cells"], minimum_image=False)
+    assert sb.pbc == _abi.FNL_PBC_GENERAL
+    x = torch.tensor(np.asarray(s["coordinates"], np.float32), device="cuda")
+    pot.build_nblist(sb, x)
+    pot.check_reallocate(sb, x)
+    src, dst, d12, sh = device_half_list(pot, sb, x, with_shifts=True)
+    os_, od, o12, osh = _oracle_general(s, cfg[1].cutoff)
+    cs, cd, c12, csh = canonical_half(os_, od, o12, osh)
+    assert np.any(np.bincount(cs * 10000 + cd) > 1)  # several images of one pair
+    np.testing.assert_array_equal(src, cs)
+    np.testing.assert_array_equal(dst, cd)
+    np.testing.assert_array_equal(sh, csh)
+    np.testing.assert_array_equal(d12.view(np.uint32), c12.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["water8", "triclinic14", "unwrapped9"])
+def test_general_pbc_energy_forces_virial(gpu, name):
+    _flush(f"[efv general] {name}")
+    s = _general_cases()[name]
+    cfg = configs()
+    params, pot = _potential(cfg)
+    sb = pot.setup(s["species"], s["natoms"], s["cells"], minimum_image=False)
+    E, F, V = pot.energy_forces_virial(s["species"], s["coordinates"], sb=sb)
+    model = O.Model(cfg[0], params, minimum_image=False)
+    x32 = np.asarray(s["coordinates"], np.float32).astype(np.float64)
+    Er, Fr, Vr, _ = O.energy_forces_virial(model, x32, s["species"], s["natoms"], s["cells"],
+                                           np.linalg.inv(s["cells"]))
+    np.testing.assert_allclose(E, Er, rtol=1e-5, atol=1e-5 * np.abs(Er).max())
+    rms = np.sqrt(np.mean((F - Fr) ** 2))
+    assert rms < 1e-4, rms
+    np.testing.assert_allclose(V, Vr, rtol=1e-4, atol=1e-4 * np.abs(Vr).max())
+
+
+def test_general_pbc_skin_update(gpu):
+    """Skin list in general PBC: the stored shifts are applied to the moved positions."""
+    _flush("[skin general] triclinic14")
+    s = _general_cases()["triclinic14"]
+    cfg = configs()
+    rc, skin = cfg[1].cutoff, 1.0
+    _, pot = _potential(cfg)
+    sb = pot.setup(s["species"], s["natoms"], s["cells"], skin=skin, minimum_image=False)
+    x0 = np.asarray(s["coordinates"], np.float32)
+    pot.build_nblist(sb, torch.tensor(x0, device="cuda"))
+    pot.check_reallocate(sb, torch.tensor(x0, device="cuda"))
+    rng = np.random.Generator(np.random.PCG64(31))
+    u = rng.normal(size=x0.shape)
+    x1 = (x0 + 0.45 * skin * u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+    t1 = torch.tensor(x1, device="cuda")
+    pot.update_skin(sb, t1)
+    pot.check_reallocate(sb, t1)
+    ss, sd, _, ssh = _oracle_general(s, rc, skin=skin)
+    us, ud, u12, ush = O.skin_update(ss, sd, ssh, x1, rc, s["cells"])
+    cs, cd, _, csh = canonical_half(us, ud, u12, ush)
+    # the device export recomputes d12 from wrapped positions (the build's formula): compare the
+    # pair set and shifts, and d12 against a fresh general build at x1
+    src, dst, d12, sh = device_half_list(pot, sb, t1, with_shifts=True)
+    np.testing.assert_array_equal(src, cs)
+    np.testing.assert_array_equal(dst, cd)
+    np.testing.assert_array_equal(sh, csh)
+    fs, fd, f12, fsh = canonical_half(*_oracle_general({**s, "coordinates": x1}, rc))
+    np.testing.assert_array_equal(src, fs)
+    np.testing.assert_array_equal(sh, fsh)
+    np.testing.assert_array_equal(d12.view(np.uint32), f12.view(np.uint32))
 
 
 @pytest.mark.parametrize("name", ["cluster64", "water64", "water216"])
