@@ -96,7 +96,7 @@ _SIGNATURES = {
     "fnl_species_mlp": (C.c_int, [vp, i32, C.POINTER(MlpWeights), vp, vp, vp, vp, i32, vp, vp, i32, vp]),
     "fnl_system_reduce": (C.c_int, [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "fnl_forces": (C.c_int, [vp, vp, i32, f32, vp, vp]),
-    "fnl_baoab_step": (C.c_int, [vp, vp, vp, vp, vp, f32, vp, f64, f64, vp, vp, u64, u64, vp, i32, vp]),
+    "fnl_baoab_step": (C.c_int, [vp, vp, vp, vp, vp, f32, vp, f64, f64, vp, vp, u64, u64, vp, i32, i32, vp]),
     "fnl_counter_add": (C.c_int, [vp, u64, vp]),
     "fnl_baoab_step_batched": (C.c_int, [vp, vp, vp, vp, vp, f32, vp, f64, f64, vp, vp, vp, vp, vp, u64, i32, vp]),
     "fnl_philox_normals": (C.c_int, [vp, i32, u64, u64, u32, vp]),

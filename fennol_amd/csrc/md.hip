@@ -11,14 +11,47 @@ namespace {
 
 constexpr int kRedVals = 1 + 9 + 9;  // energy, virial[9], kinetic[9]
 
-// Stage 1: one workgroup per segment (a run of at most seg_atoms atoms of one
-// system) -> FP64 partial sums.  Stage 2: one thread per system adds its
-// segments' partials in segment order.  Both use fixed summation orders, so
-// the result is bitwise reproducible.
+// Per-system sums of the partials of its segments: one wave per system, lane k
+// sums segments k, k+64, ...; then a fixed xor butterfly (deterministic order).
+__device__ __forceinline__ void reduce_system(int b, int lane, const int32_t* __restrict__ seg_offsets,
+                                              const double* partials, double* __restrict__ energy_sys,
+                                              double* __restrict__ virial_sys, double* __restrict__ kinetic_sys) {
+  double acc[kRedVals];
+#pragma unroll
+  for (int k = 0; k < kRedVals; ++k) acc[k] = 0.0;
+  for (int s = seg_offsets[b] + lane; s < seg_offsets[b + 1]; s += 64)
+#pragma unroll
+    for (int k = 0; k < kRedVals; ++k) acc[k] += partials[(size_t)s * kRedVals + k];
+#pragma unroll
+  for (int k = 0; k < kRedVals; ++k)
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+  if (lane == 0 && energy_sys) energy_sys[b] = acc[0];
+  if (lane < 9) {
+    // lane k writes component k (the values are wave-uniform after the butterfly)
+    double vk = 0.0, kk = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      if (lane == k) {
+        vk = acc[1 + k];
+        kk = acc[10 + k];
+      }
+    if (virial_sys) virial_sys[9 * b + lane] = vk;
+    if (kinetic_sys) kinetic_sys[9 * b + lane] = kk;
+  }
+}
+
+// One workgroup per segment (a run of at most seg_atoms atoms of one system)
+// -> FP64 partial sums in a fixed tree order.  The last workgroup to finish
+// (a completion count in the word before the partials) then forms the
+// per-system sums, so the result is bitwise reproducible and one launch does
+// both stages.
 __global__ void __launch_bounds__(256)
 k_reduce_segments(const int32_t* __restrict__ sys_offsets, const int32_t* __restrict__ seg_offsets, int n_sys,
                   int seg_atoms, const float* __restrict__ energy_atom, const float* __restrict__ virial_atom,
-                  const double* __restrict__ vel, const double* __restrict__ mass, double* __restrict__ partials) {
+                  const double* __restrict__ vel, const double* __restrict__ mass, double* __restrict__ partials,
+                  unsigned* __restrict__ done, double* __restrict__ energy_sys, double* __restrict__ virial_sys,
+                  double* __restrict__ kinetic_sys) {
   __shared__ double red[kRedVals][256];
   const int sgi = blockIdx.x;
   const int t = threadIdx.x;
@@ -59,40 +92,29 @@ k_reduce_segments(const int32_t* __restrict__ sys_offsets, const int32_t* __rest
     __syncthreads();
   }
   if (t < kRedVals) partials[(size_t)sgi * kRedVals + t] = red[t][0];
+  // completion count: the last workgroup sees every partial
+  __shared__ bool last;
+  __syncthreads();
+  if (t == 0) {
+    __threadfence();
+    last = atomicAdd(done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  for (int b = t >> 6; b < n_sys; b += blockDim.x >> 6)
+    reduce_system(b, t & 63, seg_offsets, partials, energy_sys, virial_sys, kinetic_sys);
+  if (t == 0) atomicExch(done, 0u);
 }
 
-// One wave per system: lane k sums the partials of segments k, k+64, ...; then a fixed
-// xor-butterfly over the wave (deterministic order).
+// Systems without segments (no atoms at all): one wave per system writes zeros.
 __global__ void __launch_bounds__(64) k_reduce_systems(const int32_t* __restrict__ seg_offsets, int n_sys,
                                                        const double* __restrict__ partials,
                                                        double* __restrict__ energy_sys, double* __restrict__ virial_sys,
                                                        double* __restrict__ kinetic_sys) {
   const int b = blockIdx.x;
-  const int lane = threadIdx.x;
   if (b >= n_sys) return;
-  double acc[kRedVals];
-#pragma unroll
-  for (int k = 0; k < kRedVals; ++k) acc[k] = 0.0;
-  for (int s = seg_offsets[b] + lane; s < seg_offsets[b + 1]; s += 64)
-#pragma unroll
-    for (int k = 0; k < kRedVals; ++k) acc[k] += partials[(size_t)s * kRedVals + k];
-#pragma unroll
-  for (int k = 0; k < kRedVals; ++k)
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
-  if (lane == 0 && energy_sys) energy_sys[b] = acc[0];
-  if (lane < 9) {
-    // lane k writes component k (the values are wave-uniform after the butterfly)
-    double vk = 0.0, kk = 0.0;
-#pragma unroll
-    for (int k = 0; k < 9; ++k)
-      if (lane == k) {
-        vk = acc[1 + k];
-        kk = acc[10 + k];
-      }
-    if (virial_sys) virial_sys[9 * b + lane] = vk;
-    if (kinetic_sys) kinetic_sys[9 * b + lane] = kk;
-  }
+  reduce_system(b, threadIdx.x, seg_offsets, partials, energy_sys, virial_sys, kinetic_sys);
 }
 
 __global__ void k_forces(const float* __restrict__ gs, const float* __restrict__ gn, int n3, float scale,
@@ -101,30 +123,49 @@ __global__ void k_forces(const float* __restrict__ gs, const float* __restrict__
   if (i < n3) f[i] = -(gs[i] + gn[i]) * scale;
 }
 
-__global__ void k_baoab(double* __restrict__ x, double* __restrict__ v, float* __restrict__ x32,
-                        const float* __restrict__ gs, const float* __restrict__ gn, float force_scale,
-                        const double* __restrict__ dtm, double half_dt, double a1,
-                        const double* __restrict__ a2, const double* __restrict__ noise, uint64_t seed,
-                        uint64_t step_arg, const uint64_t* __restrict__ step_dev, int n_atoms) {
+// step_dev (optional): {step, done}.  Every workgroup reads the step before it
+// counts itself in `done`; the last one to finish advances the step and clears
+// `done`, so a captured MD step replays with the next noise without a counter
+// kernel.  clear_gn: the neighbour-gradient rows are zeroed once read (the
+// adjoint of the same MD step accumulates into them).
+__global__ void __launch_bounds__(256)
+k_baoab(double* __restrict__ x, double* __restrict__ v, float* __restrict__ x32,
+        const float* __restrict__ gs, float* __restrict__ gn, float force_scale,
+        const double* __restrict__ dtm, double half_dt, double a1,
+        const double* __restrict__ a2, const double* __restrict__ noise, uint64_t seed,
+        uint64_t step_arg, uint64_t* __restrict__ step_dev, int n_atoms, int clear_gn) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_atoms) return;
-  const uint64_t step = step_dev ? *step_dev : step_arg;
-  double xi[3];
-  if (a2 && !noise) fnl_normals3(seed, step, 0u, (uint32_t)i, xi);
-  const double m = dtm[i];
+  const uint64_t step = step_dev ? step_dev[0] : step_arg;
+  if (i < n_atoms) {
+    double xi[3];
+    if (a2 && !noise) fnl_normals3(seed, step, 0u, (uint32_t)i, xi);
+    const double m = dtm[i];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const double f = -((double)gs[3 * i + k] + (double)gn[3 * i + k]) * (double)force_scale;
-    double vv = v[3 * i + k] + m * f;           // B (full dt: merged half kicks)
-    double xx = x[3 * i + k] + half_dt * vv;    // A
-    if (a2) {                                   // O
-      const double nz = noise ? noise[3 * i + k] : xi[k];
-      vv = a1 * vv + a2[i] * nz;
+    for (int k = 0; k < 3; ++k) {
+      const double f = -((double)gs[3 * i + k] + (double)gn[3 * i + k]) * (double)force_scale;
+      double vv = v[3 * i + k] + m * f;           // B (full dt: merged half kicks)
+      double xx = x[3 * i + k] + half_dt * vv;    // A
+      if (a2) {                                   // O
+        const double nz = noise ? noise[3 * i + k] : xi[k];
+        vv = a1 * vv + a2[i] * nz;
+      }
+      xx = xx + half_dt * vv;                     // A
+      v[3 * i + k] = vv;
+      x[3 * i + k] = xx;
+      x32[3 * i + k] = (float)xx;
+      if (clear_gn) gn[3 * i + k] = 0.0f;
     }
-    xx = xx + half_dt * vv;                     // A
-    v[3 * i + k] = vv;
-    x[3 * i + k] = xx;
-    x32[3 * i + k] = (float)xx;
+  }
+  if (step_dev) {
+    __syncthreads();  // every thread of this workgroup has read the step
+    if (threadIdx.x == 0) {
+      __threadfence();
+      const unsigned prev = atomicAdd(reinterpret_cast<unsigned*>(step_dev + 1), 1u);
+      if (prev == gridDim.x - 1) {
+        atomicExch(reinterpret_cast<unsigned long long*>(step_dev), (unsigned long long)(step + 1));
+        atomicExch(reinterpret_cast<unsigned*>(step_dev + 1), 0u);
+      }
+    }
   }
 }
 
@@ -180,18 +221,23 @@ extern "C" int fnl_system_reduce(const int32_t* sys_offsets, const int32_t* seg_
                                  fnl_stream_t stream) {
   if (n_sys < 0 || n_seg < 0 || seg_atoms < 1) return FNL_ERR_INVALID;
   if (n_sys == 0) return FNL_OK;
-  if (!sys_offsets || !seg_offsets || (n_seg > 0 && !partials)) return FNL_ERR_INVALID;
+  if (!sys_offsets || !seg_offsets || !partials) return FNL_ERR_INVALID;
   if ((energy_atom && !energy_sys) || (virial_atom && !virial_sys) || (vel && (!mass || !kinetic_sys)))
     return FNL_ERR_INVALID;
   hipStream_t st = (hipStream_t)stream;
+  // partials: [0] completion count (zero between calls), then n_seg rows of kRedVals
+  unsigned* done = reinterpret_cast<unsigned*>(partials);
+  double* rows = partials + 1;
+  double* e_out = energy_atom ? energy_sys : nullptr;
+  double* v_out = virial_atom ? virial_sys : nullptr;
+  double* k_out = vel ? kinetic_sys : nullptr;
   if (n_seg > 0) {
     hipLaunchKernelGGL(k_reduce_segments, dim3(n_seg), dim3(256), 0, st, sys_offsets, seg_offsets, n_sys,
-                       seg_atoms, energy_atom, virial_atom, vel, mass, partials);
-    FNL_LAUNCH_CHECK();
+                       seg_atoms, energy_atom, virial_atom, vel, mass, rows, done, e_out, v_out, k_out);
+  } else {
+    hipLaunchKernelGGL(k_reduce_systems, dim3(n_sys), dim3(64), 0, st, seg_offsets, n_sys, rows, e_out, v_out,
+                       k_out);
   }
-  hipLaunchKernelGGL(k_reduce_systems, dim3(n_sys), dim3(64), 0, st, seg_offsets, n_sys, partials,
-                     energy_atom ? energy_sys : nullptr, virial_atom ? virial_sys : nullptr,
-                     vel ? kinetic_sys : nullptr);
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }
@@ -207,15 +253,16 @@ extern "C" int fnl_forces(const float* grad_self, const float* grad_nbr, int32_t
   return FNL_OK;
 }
 
-extern "C" int fnl_baoab_step(double* x, double* v, float* x32, const float* grad_self, const float* grad_nbr,
+extern "C" int fnl_baoab_step(double* x, double* v, float* x32, const float* grad_self, float* grad_nbr,
                               float force_scale, const double* dtm, double half_dt, double a1, const double* a2,
-                              const double* noise, uint64_t seed, uint64_t step, const uint64_t* step_dev,
-                              int32_t n_atoms, fnl_stream_t stream) {
+                              const double* noise, uint64_t seed, uint64_t step, uint64_t* step_dev,
+                              int32_t n_atoms, int32_t clear_grad_nbr, fnl_stream_t stream) {
   if (n_atoms < 0) return FNL_ERR_INVALID;
   if (n_atoms == 0) return FNL_OK;
   if (!x || !v || !x32 || !grad_self || !grad_nbr || !dtm) return FNL_ERR_INVALID;
   hipLaunchKernelGGL(k_baoab, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, (hipStream_t)stream, x, v, x32,
-                     grad_self, grad_nbr, force_scale, dtm, half_dt, a1, a2, noise, seed, step, step_dev, n_atoms);
+                     grad_self, grad_nbr, force_scale, dtm, half_dt, a1, a2, noise, seed, step, step_dev, n_atoms,
+                     (int)(clear_grad_nbr != 0));
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }

@@ -20,11 +20,12 @@
 //               neighbouring cells (contiguous z-runs of the cell-sorted
 //               arrays) tested 64 at a time, kept pairs compacted with a wave
 //               ballot + prefix popcount into the centre's row
-//   k_row_max   a few workgroups: row maxima into the status words
+//   (row maxima go to the status words only from rows that overflowed: they
+//   size the regrown rows; a same-address atomic per centre would serialise)
 //
 // Skin update (a list built with cutoff + skin, re-filtered at the current
 // positions on the steps between builds):
-//   k_status_clear, k_skin_filter (one wave per centre), k_row_max
+//   k_status_clear, k_skin_filter (one wave per centre)
 
 #include "fnl_common.h"
 
@@ -332,6 +333,8 @@ k_pairs_general(const int32_t* __restrict__ species_idx, const int32_t* __restri
     if (count > max_nbr || count_a > max_nbr_ang) {
       atomicOr(&status[FNL_STATUS_OVERFLOW], 1);
       atomicOr(&status[FNL_STATUS_STICKY], 1);
+      atomicMax(&status[FNL_STATUS_MAX_NBR], count);
+      atomicMax(&status[FNL_STATUS_MAX_NBR_ANG], count_a);
     }
     if (bad_range) {
       atomicOr(&status[FNL_STATUS_OVERFLOW], 4);
@@ -484,9 +487,9 @@ __device__ __forceinline__ int z_runs(int cz, int nz, bool pbc, int* lo, int* hi
 // the shuffled offsets), so every pass tests 64 candidates, and the next
 // pass's positions are loaded before the current pass is compacted.  Kept
 // pairs are compacted with a ballot and a prefix popcount into the radial
-// row and, for d12 < rca^2, the angular row, in candidate order.  The row
-// maxima are reduced afterwards by k_row_max (a same-address atomic per
-// centre serialises at the device's coherence point).
+// row and, for d12 < rca^2, the angular row, in candidate order.  Only rows
+// that overflow update the maxima in the status words (a same-address
+// atomic per centre would serialise at the device's coherence point).
 template <int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
 k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_index,
@@ -586,39 +589,12 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
   if (lane == 0) {
     n_nbr[i] = count;
     if (n_nbr_ang) n_nbr_ang[i] = count_a;
-    if (count > max_nbr || count_a > max_nbr_ang) {
+    if (count > max_nbr || count_a > max_nbr_ang) {  // rare: the maxima only matter on overflow
       atomicOr(&status[FNL_STATUS_OVERFLOW], 1);
       atomicOr(&status[FNL_STATUS_STICKY], 1);
+      atomicMax(&status[FNL_STATUS_MAX_NBR], count);
+      atomicMax(&status[FNL_STATUS_MAX_NBR_ANG], count_a);
     }
-  }
-}
-
-// Row maxima (status words MAX_NBR, MAX_NBR_ANG, zeroed by k_grid): a few workgroups, one
-// atomicMax per workgroup and word.
-__global__ void __launch_bounds__(1024) k_row_max(const int32_t* __restrict__ n_nbr,
-                                                  const int32_t* __restrict__ n_nbr_ang, int n_atoms,
-                                                  int32_t* __restrict__ status) {
-  __shared__ int sm[2][16];
-  int m = 0, ma = 0;
-  for (int a = blockIdx.x * 1024 + threadIdx.x; a < n_atoms; a += gridDim.x * 1024) {
-    m = max(m, n_nbr[a]);
-    if (n_nbr_ang) ma = max(ma, n_nbr_ang[a]);
-  }
-  m = fnl_wave_max_i(m);
-  ma = fnl_wave_max_i(ma);
-  const int w = threadIdx.x >> 6;
-  if (fnl_lane() == 0) {
-    sm[0][w] = m;
-    sm[1][w] = ma;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < 16; ++k) {
-      m = max(m, sm[0][k]);
-      ma = max(ma, sm[1][k]);
-    }
-    atomicMax(&status[FNL_STATUS_MAX_NBR], max(m, sm[0][0]));
-    atomicMax(&status[FNL_STATUS_MAX_NBR_ANG], max(ma, sm[1][0]));
   }
 }
 
@@ -703,6 +679,8 @@ k_skin_filter(const float* __restrict__ coords, const int32_t* __restrict__ batc
     if (count > max_nbr || count_a > max_nbr_ang || ns > max_skin) {
       atomicOr(&status[FNL_STATUS_OVERFLOW], 1);
       atomicOr(&status[FNL_STATUS_STICKY], 1);
+      atomicMax(&status[FNL_STATUS_MAX_NBR], count);
+      atomicMax(&status[FNL_STATUS_MAX_NBR_ANG], count_a);
     }
   }
 }
@@ -858,9 +836,6 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
                        n_cells);
     FNL_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_row_max, dim3(std::min(16, (n_atoms + 4095) / 4096)), dim3(1024), 0, s, n_nbr, n_nbr_ang,
-                     n_atoms, status);
-  FNL_LAUNCH_CHECK();
   return FNL_OK;
 }
 
@@ -888,9 +863,6 @@ extern "C" int fnl_nblist_skin_update(const float* coords, const int32_t* batch_
   hipLaunchKernelGGL(k_skin_filter<W>, dim3(fnl_blocks(n_atoms, W)), dim3(64 * W), 0, s, coords, batch_index,
                      n_atoms, pbc_mode, cells, reciprocal_cells, nbr_skin, skin_shift, n_skin, max_skin, rc2,
                      rca2, max_nbr, max_nbr_ang, nbr, nbr_shift, n_nbr, nbr_ang, n_nbr_ang, status);
-  FNL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_row_max, dim3(std::min(16, (n_atoms + 4095) / 4096)), dim3(1024), 0, s, n_nbr, n_nbr_ang,
-                     n_atoms, status);
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }

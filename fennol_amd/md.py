@@ -94,7 +94,7 @@ class LangevinMD:
         else:
             raise NotImplementedError(f"thermostat {thermostat}")
         # step counter on the device: the O-step noise of a captured graph advances on replay
-        self.step_dev = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.step_dev = torch.zeros(2, dtype=torch.int64, device=dev)  # {step, completion count}
         self.graph = None
         self.graph_skin = None
         self.x = torch.tensor(np.asarray(system["coordinates"], np.float64), device=dev).contiguous()
@@ -128,9 +128,8 @@ class LangevinMD:
         rc = self.lib.fnl_baoab_step(
             self.x.data_ptr(), self.v.data_ptr(), self.x32.data_ptr(), sb.grad_self.data_ptr(),
             sb.grad_nbr.data_ptr(), self.force_scale, self.t_dtm.data_ptr(), 0.5 * self.dt, self.a1,
-            _abi.ptr(self.t_a2), None, self.seed, 0, self.step_dev.data_ptr(), self.n, s)
+            _abi.ptr(self.t_a2), None, self.seed, 0, self.step_dev.data_ptr(), self.n, 1, s)
         _abi.check(rc, "fnl_baoab_step")
-        _abi.check(self.lib.fnl_counter_add(self.step_dev.data_ptr(), 1, s), "fnl_counter_add")
 
     def _graph_update(self) -> bool | str:
         """The neighbour-list action of the next step (update_graphs, integrate.py:515-566):
@@ -148,7 +147,8 @@ class LangevinMD:
         self.istep += 1
         if check is None:
             check = self.check_every > 0 and (self.istep % self.check_every) == 0
-        self.pot.compute(self.sb, self.x32, forces=True, rebuild=rebuild, check=check, stream=stream)
+        self.pot.compute(self.sb, self.x32, forces=True, rebuild=rebuild, check=check, stream=stream,
+                         zero_grad=False)  # the integrator cleared grad_nbr
 
     def run(self, nsteps: int) -> None:
         for _ in range(nsteps):
@@ -158,7 +158,8 @@ class LangevinMD:
     def _record(self, rebuild: bool | str, stream=None) -> None:
         """One step with a fixed neighbour-list action and no host reads or counters."""
         self._integrate(stream)
-        self.pot.compute(self.sb, self.x32, forces=True, rebuild=rebuild, check=False, stream=stream)
+        self.pot.compute(self.sb, self.x32, forces=True, rebuild=rebuild, check=False, stream=stream,
+                         zero_grad=False)
 
     def capture(self, steps_per_graph: int = 1) -> None:
         """Record MD steps in HIP graphs (no host checks inside; capacities must have

@@ -196,7 +196,7 @@ class SystemBuffers:
     virial: torch.Tensor = None         # float64 [B,9]
     seg_offsets: torch.Tensor = None    # int32 [B+1] reduction segments per system
     n_seg: int = 0
-    partials: torch.Tensor = None       # float64 [n_seg, 19]
+    partials: torch.Tensor = None       # float64 [1 + n_seg * 19]: completion count, partial sums
     # Verlet skin: rows of the list built with cutoff + skin (radial only), re-filtered to the
     # radial/angular rows above on the steps between builds; skin == 0: no skin list
     skin: float = 0.0
@@ -339,7 +339,8 @@ class ANIPotential:
         segoff = np.concatenate([[0], np.cumsum(nseg)]).astype(np.int32)
         sb.seg_offsets = torch.tensor(segoff, device=dev)
         sb.n_seg = int(segoff[-1])
-        sb.partials = torch.zeros((max(sb.n_seg, 1), _abi.FNL_REDUCE_VALUES), dtype=torch.float64, device=dev)
+        # completion count + per-segment partial sums (fnl_system_reduce)
+        sb.partials = torch.zeros(1 + max(sb.n_seg, 1) * _abi.FNL_REDUCE_VALUES, dtype=torch.float64, device=dev)
         return sb
 
     # ------------------------------------------------------- capacity pools
@@ -383,7 +384,7 @@ class ANIPotential:
         sb.virial = torch.zeros((max_sys, 9), dtype=torch.float64, device=dev)
         max_seg = max_atoms // SEG_ATOMS + max_sys + 1
         sb.seg_offsets = torch.zeros(max_sys + 1, dtype=torch.int32, device=dev)
-        sb.partials = torch.zeros((max_seg, _abi.FNL_REDUCE_VALUES), dtype=torch.float64, device=dev)
+        sb.partials = torch.zeros(1 + max_seg * _abi.FNL_REDUCE_VALUES, dtype=torch.float64, device=dev)
         sb.n_atoms = 0
         sb.n_sys = 0
         return sb
@@ -622,11 +623,12 @@ class ANIPotential:
         _abi.check(rc, "fnl_system_reduce")
 
     def compute(self, sb: SystemBuffers, coords32: torch.Tensor, forces: bool = True,
-                rebuild: bool | str = True, check: bool = True, stream=None) -> None:
+                rebuild: bool | str = True, check: bool = True, stream=None, zero_grad: bool = True) -> None:
         """Energy (and gradients) of the batch at coords32 (float32 [N,3] device tensor).
 
         ``rebuild``: True = full neighbour-list build, "skin" = re-filter of the skin list
-        (:meth:`update_skin`), False = keep the current rows."""
+        (:meth:`update_skin`), False = keep the current rows.  ``zero_grad`` False: grad_nbr is
+        already zero (the MD integrator clears it when it reads the forces)."""
         if rebuild == "skin":
             with self._stage("skin"):
                 self.update_skin(sb, coords32, stream)
@@ -642,8 +644,9 @@ class ANIPotential:
         with self._stage("mlp"):
             self.mlp(sb, with_grad=forces, stream=stream)
         if forces:
-            with self._stage("zero_grad"):
-                self.zero_grad_nbr(sb, stream)
+            if zero_grad:
+                with self._stage("zero_grad"):
+                    self.zero_grad_nbr(sb, stream)
             with self._stage("aev_bwd"):
                 self.aev_backward(sb, coords32, stream, zero=False)
         with self._stage("reduce"):
@@ -665,12 +668,12 @@ class ANIPotential:
         """Size the rows from one build with headroom for an MD run without per-step host checks."""
         self.build_nblist(sb, coords32)
         self.check_reallocate(sb, coords32)
-        st = sb.status.cpu().numpy()
-        want = _next_mult(int(margin * st[_abi.FNL_STATUS_MAX_NBR]) + add, 8)
-        want_a = _next_mult(int(margin * st[_abi.FNL_STATUS_MAX_NBR_ANG]) + add, 8)
+        n = max(sb.n_atoms, 1)
+        want = _next_mult(int(margin * int(sb.n_nbr[:n].max())) + add, 8)
+        want_a = _next_mult(int(margin * int(sb.n_nbr_ang[:n].max())) + add, 8)
         want_s = sb.max_skin
         if sb.skin > 0:
-            want_s = _next_mult(int(margin * sb.status_skin.cpu().numpy()[_abi.FNL_STATUS_MAX_NBR]) + add, 8)
+            want_s = _next_mult(int(margin * int(sb.n_skin[:n].max())) + add, 8)
         if want > sb.max_nbr or want_a > sb.max_nbr_ang or want_s > sb.max_skin:
             sb.max_nbr = max(sb.max_nbr, want)
             sb.max_nbr_ang = max(sb.max_nbr_ang, want_a)

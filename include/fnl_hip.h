@@ -57,8 +57,9 @@ typedef struct ihipStream_t* fnl_stream_t; /* == hipStream_t */
 #define FNL_STATUS_OVERFLOW 0       /* bit 0: a row capacity was exceeded; bit 1:
                                        grid capacities inconsistent; bit 2: a
                                        general-PBC shift beyond +-511 cells        */
-#define FNL_STATUS_MAX_NBR 1        /* largest radial neighbour count seen          */
-#define FNL_STATUS_MAX_NBR_ANG 2    /* largest angular neighbour count seen         */
+#define FNL_STATUS_MAX_NBR 1        /* largest radial count among rows that overflowed
+                                       (0 if none): sizes the regrown rows          */
+#define FNL_STATUS_MAX_NBR_ANG 2    /* the same for the angular rows                */
 #define FNL_STATUS_MAX_CELL 3       /* largest cell population seen                 */
 #define FNL_STATUS_STICKY 4         /* overflow flags OR-ed over every build since the
                                        caller last cleared this word                */
@@ -290,10 +291,13 @@ int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_mlp_weights* 
 /* ------------------------------------------------------------------------
  * Per-system reductions in FP64 (total_energy segment_sum, fennix.py:240-246;
  * virial tensor, fennix.py:320-347; kinetic tensor, md/integrate.py:320-339).
- * Deterministic two-stage tree: each system is cut into segments of at most
- * seg_atoms atoms; seg_offsets[b] = index of the first segment of system b
- * (exclusive prefix sum of ceil(natoms_b / seg_atoms)), seg_offsets[n_sys] =
- * n_seg; partials is caller scratch of n_seg * 19 doubles.
+ * Deterministic two-stage tree in one launch: each system is cut into
+ * segments of at most seg_atoms atoms; seg_offsets[b] = index of the first
+ * segment of system b (exclusive prefix sum of ceil(natoms_b / seg_atoms)),
+ * seg_offsets[n_sys] = n_seg; partials is caller scratch of 1 + n_seg * 19
+ * doubles whose first word is a completion count: zero it once when
+ * allocating, every call leaves it at zero (the last segment workgroup forms
+ * the per-system sums).  Calls sharing one partials buffer must not overlap.
  * energy_atom may be NULL (energy_sys untouched), likewise virial_atom /
  * virial_sys, and vel (kinetic_sys untouched; mass is then ignored).
  * ---------------------------------------------------------------------- */
@@ -318,15 +322,19 @@ int fnl_forces(const float* grad_self, const float* grad_nbr, int32_t n_atoms,
  * xi ~ N(0,1) from Philox4x32-10 with counter (atom, step_lo, step_hi, 0) and
  * key (seed_lo, seed_hi), Box-Muller on (r0,r1), (r2,r3); or, if noise != NULL,
  * xi = noise (float64[N][3]).  a2 == NULL means no O-step (NVE).  If step_dev
- * is not NULL the step number is read from that device word instead of
- * `step` (so that a captured graph replays with a fresh counter; advance it
- * with fnl_counter_add).  Also writes the float32 copy of x the model reads.
+ * is not NULL it points to two zero-initialised device words {step, done}:
+ * the step number is read from the first instead of `step`, and the kernel's
+ * last workgroup advances it by one (so that a captured graph replays with a
+ * fresh counter; `done` is left at zero).  clear_grad_nbr != 0 zeroes the
+ * grad_nbr rows once read, for an adjoint of the same MD step that
+ * accumulates into them.  Also writes the float32 copy of x the model reads.
  * ---------------------------------------------------------------------- */
 int fnl_baoab_step(double* x, double* v, float* x32,
-                   const float* grad_self, const float* grad_nbr, float force_scale,
+                   const float* grad_self, float* grad_nbr, float force_scale,
                    const double* dtm, double half_dt, double a1, const double* a2,
                    const double* noise, uint64_t seed, uint64_t step,
-                   const uint64_t* step_dev, int32_t n_atoms, fnl_stream_t stream);
+                   uint64_t* step_dev, int32_t n_atoms, int32_t clear_grad_nbr,
+                   fnl_stream_t stream);
 
 /* Batched BAOAB for many independent systems (continuous-batching server):
  * atoms with dtm == 0 are left untouched (systems not integrated this step);

@@ -472,7 +472,24 @@ def test_philox_and_baoab_injected_noise(gpu):
     tgn = torch.tensor(gn, device=dev)
     _abi.check(lib.fnl_baoab_step(tx.data_ptr(), tv.data_ptr(), x32.data_ptr(), tgs.data_ptr(), tgn.data_ptr(),
                                   scale, tdtm.data_ptr(), 0.5 * dt, a1, ta2.data_ptr(), tno.data_ptr(), 0, 0, None, n,
-                                  _abi.stream_handle()), "baoab")
+                                  0, _abi.stream_handle()), "baoab")
     np.testing.assert_allclose(tx.cpu().numpy(), xr, rtol=1e-13, atol=1e-13)
     np.testing.assert_allclose(tv.cpu().numpy(), vr, rtol=1e-13, atol=1e-13)
     np.testing.assert_array_equal(x32.cpu().numpy(), xr.astype(np.float32))
+    np.testing.assert_array_equal(tgn.cpu().numpy(), gn)  # not cleared without the flag
+    # device step counter {step, done}: two calls draw the noise of steps 7 and 8 (Philox), the
+    # last workgroup advances the step; the neighbour-gradient rows are cleared once read
+    step_dev = torch.tensor([7, 0], dtype=torch.int64, device=dev)
+    xs, vs = x.copy(), v.copy()
+    for k in (7, 8):
+        Fk = -(gs.astype(np.float64) + (gn.astype(np.float64) if k == 7 else 0.0)) * np.float64(np.float32(scale))
+        xs, vs = O.baoab_step(xs, vs, Fk, mass, dt, a1, a2, O.philox_normals(99, k, n))
+    tx, tv, tgn = torch.tensor(x, device=dev), torch.tensor(v, device=dev), torch.tensor(gn, device=dev)
+    for _ in range(2):
+        _abi.check(lib.fnl_baoab_step(tx.data_ptr(), tv.data_ptr(), x32.data_ptr(), tgs.data_ptr(),
+                                      tgn.data_ptr(), scale, tdtm.data_ptr(), 0.5 * dt, a1, ta2.data_ptr(), None, 99,
+                                      0, step_dev.data_ptr(), n, 1, _abi.stream_handle()), "baoab counter")
+    assert step_dev.cpu().tolist() == [9, 0]
+    assert np.all(tgn.cpu().numpy() == 0.0)
+    np.testing.assert_allclose(tx.cpu().numpy(), xs, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(tv.cpu().numpy(), vs, rtol=1e-12, atol=1e-12)
