@@ -11,18 +11,22 @@
 // sin(theta_z) with x = 0.95 cos(angle), equal to the reference's
 // cos(arccos(x) - theta_z) without the arccos.
 //
-// Forward (per centre i):
-//   1. radial rows: neighbour j, distance, 0.25*fc, species, vector -> LDS
-//   2. angular rows: the radial slots listed in the angular row, sorted by
-//      species -> LDS
-//   3. radial and angular channel blocks as one-hot products on MFMA: the
-//      K index runs over neighbours (triplets), the rows over species
-//      (species pairs), the columns over radial shifts (d, z channels)
-//   4. the row is written from the accumulators (plus the zero padding)
-// Adjoint: the same terms are recomputed; dE/dvec per radial slot is
-// accumulated in LDS (radial part by owner lanes, angular part with LDS
-// float atomics), then scattered: grad_self[i] = -sum_e g_e (plain store),
-// grad_nbr[j] += g_e (global atomics), virial_atom[i] = sum_e vec_e (x) g_e.
+// Forward (per centre i), one wave:
+//   1. radial rows (vector, distance, species) and the angular row -> LDS
+//   2. radial slots sorted by species (ballot counting sort); each species
+//      block summed over its neighbours in registers, lanes = (neighbour
+//      group, radial shift), one store per channel
+//   3. angular slots sorted by species; the triplets of the present species
+//      pairs enumerated pair-major in chunks of 64: one lane per triplet
+//      evaluates the R_d and F_z factors, then lanes = (half, channel) sum
+//      R_d F_z over each pair's triplets in registers, one store per channel
+//   4. empty blocks and the row padding written as zeros
+// Adjoint: the same terms are recomputed.  Radial: the lane of slot e sums
+// dE_i/dvec_ie from the centre's dE/dAEV row and dE_j/dvec_ji from the
+// neighbour's row, so radial forces are gathered, not scattered.  Angular:
+// triplets in 4x4 blocks, DPP pre-sums, one LDS float add per slot and block;
+// the angular slot gradients are then scattered to grad_nbr with global
+// atomics.  grad_self[i] is written once, virial_atom[i] = sum vec (x) dE/dvec.
 
 #include <hip/hip_bfloat16.h>
 
@@ -34,6 +38,23 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;
 constexpr float kPi = 3.14159265358979323846f;
+constexpr float kLog2e = 1.44269504088896340736f;
+
+// Hardware transcendentals (v_exp_f32, v_log_f32, v_sqrt_f32, v_rcp_f32, v_cos_f32, v_sin_f32):
+// about one ulp and no special-case code.  The arguments here stay in the ranges where they are
+// exact to that level: distances and angles of atoms within the cutoff, bases of the angular
+// power clamped away from zero.
+__device__ __forceinline__ float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float hw_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// cos(pi t) and sin(pi t): the hardware takes the angle in turns
+__device__ __forceinline__ float hw_cospi(float t) { return __builtin_amdgcn_cosf(0.5f * t); }
+__device__ __forceinline__ float hw_sinpi(float t) { return __builtin_amdgcn_sinf(0.5f * t); }
+// exp(-eta x^2)
+__device__ __forceinline__ float gauss(float x, float eta) { return hw_exp2(x * x * (-eta * kLog2e)); }
+// (1 + cos(theta - theta_z))^zeta for a base b = 1 + cos(..) >= 0 up to rounding
+__device__ __forceinline__ float ang_pow(float b, float zeta) { return hw_exp2(zeta * hw_log2(fmaxf(b, 1.0e-30f))); }
 
 // Each wave owns its LDS slice, so the lanes of one wave only exchange data among
 // themselves: the LDS executes a wave's DS instructions in issue order, and wavefront-scope
@@ -46,17 +67,17 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 __device__ __forceinline__ float cos_switch(float d, float rc) {
-  return d < rc ? 0.5f * cosf(d * (kPi / rc)) + 0.5f : 0.0f;
+  return d < rc ? 0.5f * hw_cospi(d * hw_rcp(rc)) + 0.5f : 0.0f;
 }
 
 __device__ __forceinline__ float cos_switch_grad(float d, float rc) {
-  return d < rc ? -0.5f * (kPi / rc) * sinf(d * (kPi / rc)) : 0.0f;
+  return d < rc ? (-0.5f * kPi) * hw_rcp(rc) * hw_sinpi(d * hw_rcp(rc)) : 0.0f;
 }
 
 // map a linear index t < n(n-1)/2 to the pair (a, b), a < b, row-major
 __device__ __forceinline__ void tri_pair(int t, int n, int& a, int& b) {
   const float nn = (float)(2 * n - 1);
-  int r = (int)floorf(0.5f * (nn - sqrtf(nn * nn - 8.0f * (float)t)));
+  int r = (int)floorf(0.5f * (nn - hw_sqrt(nn * nn - 8.0f * (float)t)));
   r = max(0, min(r, n - 2));
   // start(r) = r*n - r(r+1)/2 ; fix rounding
   while (r > 0 && r * n - (r * (r + 1)) / 2 > t) --r;
@@ -254,7 +275,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
         fnl_apply_shift(vx, vy, vz, cl, nbr_shift[(size_t)i * max_nbr + e]);
       else if (pbc)
         fnl_min_image(vx, vy, vz, cl);
-      const float d = sqrtf(fmaxf(vx * vx + vy * vy + vz * vz, 1.0e-5f));
+      const float d = hw_sqrt(fmaxf(vx * vx + vy * vy + vz * vz, 1.0e-5f));
       nbv[e] = make_float4(vx, vy, vz, d);
       nbs[e] = species_idx[j];
     }
@@ -290,7 +311,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       for (int e = lo + q; e < hi; e += 4) {
         const float2 df = rdf[e];
         const float x = df.x - shift;
-        run += df.y * __expf(-p.radial_eta * x * x);
+        run += df.y * gauss(x, p.radial_eta);
       }
       run += __shfl_xor(run, 16, 64);
       run += __shfl_xor(run, 32, 64);
@@ -304,7 +325,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
         na, S, [&](int a) { return nbs[ae[a]]; },
         [&](int a, int pos, int sa) {
           const float4 v = nbv[ae[a]];
-          const float inv = 1.0f / fmaxf(v.w, 1.0e-5f);
+          const float inv = hw_rcp(fmaxf(v.w, 1.0e-5f));
           av[pos] = make_float4(v.x * inv, v.y * inv, v.z * inv, v.w);
           af[pos] = make_float2(cos_switch(v.w, p.cutoff_ang), __int_as_float(sa));
         },
@@ -370,58 +391,79 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       }
     }
   }
-  // present pairs, one at a time: chunks of 64 triplets (a from block 1, b from block 2,
-  // a < b); lane t evaluates triplet t into fac, then lane (h, c) sums R_d F_z over the
-  // chunk's triplets h, h+2, ... in a register
-  uint64_t pm = present;
-  while (pm) {
-    const int pr = __builtin_ctzll(pm);
-    pm &= pm - 1ull;
-    const int n1 = __builtin_amdgcn_readlane(pn1, pr), n2 = __builtin_amdgcn_readlane(pn2, pr);
-    const int o1 = __builtin_amdgcn_readlane(po1, pr), o2 = __builtin_amdgcn_readlane(po2, pr);
-    const int npp = __builtin_amdgcn_readlane(np, pr);
-    const float rn2 = 1.0f / (float)n2;
-    float run = 0.0f;
-    for (int u0 = 0; u0 < npp; u0 += 64) {
-      const int u = u0 + lane;
-      if (u < npp) {
-        int a, b;
-        if (o1 == o2) {
-          tri_pair(u, n1, a, b);
-          a += o1;
-          b += o1;
-        } else {
-          int qd = (int)((float)u * rn2);
-          if (qd * n2 > u) --qd;
-          if ((qd + 1) * n2 <= u) ++qd;
-          a = o1 + qd;
-          b = o2 + (u - qd * n2);
-        }
-        const float4 ua = av[a], ub = av[b];
-        const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
-        const float x = 0.95f * c;
-        const float st = sqrtf(fmaxf(1.0f - x * x, 0.0f));
-        const float sw = pref * af[a].x * af[b].x;
-        const float m = 0.5f * (ua.w + ub.w);
-#pragma unroll
-        for (int d = 0; d < NA; ++d) {
-          const float y = m - p.ang_shifts[d];
-          fac[d * kFacLd + lane] = __expf(-p.angular_eta * y * y);
-        }
-#pragma unroll
-        for (int z = 0; z < NZ; ++z) {
-          const float bz = 1.0f + x * p.ang_cos[z] + st * p.ang_sin[z];
-          fac[(NA + z) * kFacLd + lane] = bz > 0.0f ? sw * exp2f(p.zeta * __log2f(bz)) : 0.0f;
-        }
+  // Triplets of the present pairs, pair-major and concatenated (pair p's triplets start at
+  // tstart of lane p), in chunks of 64: lane t evaluates triplet u0 + t into fac (its pair is
+  // the last present pair starting at or before it), then lane (h, c) adds R_d F_z over the
+  // chunk's triplets h, h+2, ... of each pair segment in the chunk to a register, and stores
+  // the pair's 32 channels when its segment ends.  Pairs share chunks, so no lanes idle at a
+  // pair boundary.
+  const int t_incl = fnl_wave_incl_scan(np);
+  const int tstart = t_incl - np;
+  const int T = __builtin_amdgcn_readlane(t_incl, 63);
+  float run = 0.0f;
+  for (int u0 = 0; u0 < T; u0 += 64) {
+    const int u = u0 + lane;
+    const int c_end = min(u0 + 64, T);
+    int n1 = 1, n2 = 1, o1 = 0, o2 = 0, ts = 0;
+    for (uint64_t pm = present; pm;) {
+      const int pr = __builtin_ctzll(pm);
+      pm &= pm - 1ull;
+      const int s0 = __builtin_amdgcn_readlane(tstart, pr);
+      if (s0 >= c_end) break;
+      const int m1 = __builtin_amdgcn_readlane(pn1, pr), m2 = __builtin_amdgcn_readlane(pn2, pr);
+      const int q1 = __builtin_amdgcn_readlane(po1, pr), q2 = __builtin_amdgcn_readlane(po2, pr);
+      if (u >= s0) {
+        ts = s0;
+        n1 = m1;
+        n2 = m2;
+        o1 = q1;
+        o2 = q2;
       }
-      wave_sync();
-      const int nt = min(64, npp - u0);
-#pragma unroll 4
-      for (int tt = h; tt < nt; tt += 2) run += fR[tt] * fF[tt];
-      wave_sync();
     }
-    const float v = run + __shfl_xor(run, 32, 64);
-    if (h == 0) aout[pr * 32 + ch] = to_out<TOut>(v);
+    if (u < T) {
+      const int uu = u - ts;
+      int a, b;
+      if (o1 == o2) {
+        tri_pair(uu, n1, a, b);
+        a += o1;
+        b += o1;
+      } else {
+        int qd = (int)((float)uu * hw_rcp((float)n2));
+        if (qd * n2 > uu) --qd;
+        if ((qd + 1) * n2 <= uu) ++qd;
+        a = o1 + qd;
+        b = o2 + (uu - qd * n2);
+      }
+      const float4 ua = av[a], ub = av[b];
+      const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
+      const float x = 0.95f * c;
+      const float st = hw_sqrt(fmaxf(1.0f - x * x, 0.0f));
+      const float sw = pref * af[a].x * af[b].x;
+      const float m = 0.5f * (ua.w + ub.w);
+#pragma unroll
+      for (int d = 0; d < NA; ++d) fac[d * kFacLd + lane] = gauss(m - p.ang_shifts[d], p.angular_eta);
+#pragma unroll
+      for (int z = 0; z < NZ; ++z)
+        fac[(NA + z) * kFacLd + lane] = sw * ang_pow(1.0f + x * p.ang_cos[z] + st * p.ang_sin[z], p.zeta);
+    }
+    wave_sync();
+    for (uint64_t pm = present; pm;) {
+      const int pr = __builtin_ctzll(pm);
+      pm &= pm - 1ull;
+      const int s0 = __builtin_amdgcn_readlane(tstart, pr);
+      if (s0 >= c_end) break;
+      const int e0 = s0 + __builtin_amdgcn_readlane(np, pr);
+      if (e0 <= u0) continue;
+      const int hi = min(e0, c_end) - u0;
+#pragma unroll 4
+      for (int tt = max(s0, u0) - u0 + h; tt < hi; tt += 2) run += fR[tt] * fF[tt];
+      if (e0 <= c_end) {  // the pair's last triplet is in this chunk
+        const float v = run + __shfl_xor(run, 32, 64);
+        if (h == 0) aout[pr * 32 + ch] = to_out<TOut>(v);
+        run = 0.0f;
+      }
+    }
+    wave_sync();
   }
 }
 
@@ -504,7 +546,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       else if (pbc)
         fnl_min_image(vx, vy, vz, cl);
       const float d2 = vx * vx + vy * vy + vz * vz;
-      const float d = sqrtf(fmaxf(d2, 1.0e-5f));
+      const float d = hw_sqrt(fmaxf(d2, 1.0e-5f));
       nbv[e] = make_float4(vx, vy, vz, d);
       const float fc = cos_switch(d, p.cutoff);
       const int sj = species_idx[j];
@@ -517,12 +559,12 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       float vi = 0.0f, vj = 0.0f;
       for (int k = 0; k < dm.nR; ++k) {
         const float x = d - p.radial_shifts[k];
-        const float t = __expf(-p.radial_eta * x * x) * (dfc - 2.0f * p.radial_eta * x * fc);
+        const float t = gauss(x, p.radial_eta) * (dfc - 2.0f * p.radial_eta * x * fc);
         vi += gr[k] * t;
         vj += gj[k] * t;
       }
-      const float sc = d2 > 1.0e-5f ? 0.25f * vi / d : 0.0f;
-      const float sj_ = d2 > 1.0e-5f ? 0.25f * vj / d : 0.0f;
+      const float id = d2 > 1.0e-5f ? 0.25f * hw_rcp(d) : 0.0f;
+      const float sc = vi * id, sj_ = vj * id;
       const float gx = sc * vx, gy = sc * vy, gz = sc * vz;
       // dE/dx_i: -dE_i/dvec_ij + dE_j/dvec_ji, with vec_ji = -vec_ij
       sx -= gx + sj_ * vx;
@@ -538,7 +580,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     for (int a = lane; a < na; a += 64) {
       const int e = a < 64 ? ae0 : arow[a];
       const float4 vv = nbv[e];
-      const float inv = 1.0f / fmaxf(vv.w, 1.0e-5f);
+      const float inv = hw_rcp(fmaxf(vv.w, 1.0e-5f));
       av[a] = make_float4(vv.x * inv, vv.y * inv, vv.z * inv, vv.w);
       const bool ok = vv.x * vv.x + vv.y * vv.y + vv.z * vv.z > 1.0e-5f;
       af[a] = make_float4(cos_switch(vv.w, p.cutoff_ang), ok ? cos_switch_grad(vv.w, p.cutoff_ang) : 0.0f,
@@ -577,21 +619,17 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       const float4 fa = af[a], fb = af[b];
       const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
       const float x = 0.95f * c;
-      const float st = sqrtf(fmaxf(1.0f - x * x, 1.0e-12f));
-      const float ist = 1.0f / st;
+      const float st = hw_sqrt(fmaxf(1.0f - x * x, 1.0e-12f));
+      const float ist = hw_rcp(st);
       float F[NZ], dF[NZ];
 #pragma unroll
       for (int z = 0; z < NZ; ++z) {
-        const float bz = 1.0f + x * p.ang_cos[z] + st * p.ang_sin[z];
-        if (bz > 0.0f) {
-          const float lb = __log2f(bz);
-          F[z] = exp2f(p.zeta * lb);
-          // d/dc (1 + cos(theta - theta_z))^zeta with theta = arccos(0.95 c)
-          dF[z] = 0.95f * p.zeta * exp2f((p.zeta - 1.0f) * lb) * (p.ang_cos[z] - x * p.ang_sin[z] * ist);
-        } else {
-          F[z] = 0.0f;
-          dF[z] = 0.0f;
-        }
+        // F = b^zeta and d/dc b^zeta = 0.95 zeta b^(zeta-1) (cos_z - x sin_z / st), b = 1 + cos(theta -
+        // theta_z), theta = arccos(0.95 c); b clamped at 1e-30 (both vanish there)
+        const float lb = hw_log2(fmaxf(1.0f + x * p.ang_cos[z] + st * p.ang_sin[z], 1.0e-30f));
+        const float tz = p.zeta * lb;
+        F[z] = hw_exp2(tz);
+        dF[z] = (0.95f * p.zeta) * hw_exp2(tz - lb) * (p.ang_cos[z] - x * p.ang_sin[z] * ist);
       }
       const float m = 0.5f * (ua.w + ub.w);
       const float* gp = Gang + pair_index(__float_as_int(fa.z), __float_as_int(fb.z), dm.S) * (NA * NZ);
@@ -599,7 +637,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
 #pragma unroll
       for (int d = 0; d < NA; ++d) {
         const float y = m - p.ang_shifts[d];
-        const float R = __expf(-p.angular_eta * y * y);
+        const float R = gauss(y, p.angular_eta);
         float gz[NZ];
 #pragma unroll
         for (int q = 0; q < NZ; q += 4) *(float4*)(gz + q) = *(const float4*)(gp + d * NZ + q);
@@ -619,7 +657,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       const float dEdc = sw * S2;
       const int ea = __float_as_int(fa.w), eb = __float_as_int(fb.w);
       const float ra = (ea >> 30) ? 0.0f : dEda, rb = (eb >> 30) ? 0.0f : dEdb;
-      const float ia = 1.0f / fmaxf(ua.w, 1.0e-5f), ib = 1.0f / fmaxf(ub.w, 1.0e-5f);
+      const float ia = hw_rcp(fmaxf(ua.w, 1.0e-5f)), ib = hw_rcp(fmaxf(ub.w, 1.0e-5f));
       gax = ra * ua.x + dEdc * (ub.x - c * ua.x) * ia;
       gay = ra * ua.y + dEdc * (ub.y - c * ua.y) * ia;
       gaz = ra * ua.z + dEdc * (ub.z - c * ua.z) * ia;
@@ -738,9 +776,9 @@ k_aev_tangent(const float* __restrict__ coords, const float* __restrict__ disp,
       else if (pbc)
         fnl_min_image(vx, vy, vz, cl);
       const float d2 = vx * vx + vy * vy + vz * vz;
-      const float d = sqrtf(fmaxf(d2, 1.0e-5f));
+      const float d = hw_sqrt(fmaxf(d2, 1.0e-5f));
       const float tx = disp[3 * j] - ri0, ty = disp[3 * j + 1] - ri1, tz = disp[3 * j + 2] - ri2;
-      const float dd = d2 > 1.0e-5f ? (vx * tx + vy * ty + vz * tz) / d : 0.0f;
+      const float dd = d2 > 1.0e-5f ? (vx * tx + vy * ty + vz * tz) * hw_rcp(d) : 0.0f;
       nbv[e] = make_float4(vx, vy, vz, d);
       nbt[e] = make_float4(tx, ty, tz, dd);
       nbf[e] = make_float2(cos_switch(d, p.cutoff), __int_as_float(species_idx[j]));
@@ -757,7 +795,7 @@ k_aev_tangent(const float* __restrict__ coords, const float* __restrict__ disp,
         const float dd = nbt[e].w;
         const float2 f = nbf[e];
         const float x = v.w - shift;
-        const float ex = __expf(-p.radial_eta * x * x);
+        const float ex = gauss(x, p.radial_eta);
         const float dfc = cos_switch_grad(v.w, p.cutoff);
         lds_add(acc + __float_as_int(f.y) * dm.nR + k, 0.25f * ex * (dfc - 2.0f * p.radial_eta * x * f.x) * dd);
       }
@@ -770,7 +808,7 @@ k_aev_tangent(const float* __restrict__ coords, const float* __restrict__ disp,
       const int e = arow[a];
       const float4 v = nbv[e];
       const float4 t = nbt[e];
-      const float inv = 1.0f / fmaxf(v.w, 1.0e-5f);
+      const float inv = hw_rcp(fmaxf(v.w, 1.0e-5f));
       const float ux = v.x * inv, uy = v.y * inv, uz = v.z * inv;
       const float ut = ux * t.x + uy * t.y + uz * t.z;
       av[a] = make_float4(ux, uy, uz, v.w);
@@ -796,8 +834,8 @@ k_aev_tangent(const float* __restrict__ coords, const float* __restrict__ disp,
       const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
       const float dc = ta.x * ub.x + ta.y * ub.y + ta.z * ub.z + ua.x * tb.x + ua.y * tb.y + ua.z * tb.z;
       const float x = 0.95f * c;
-      const float st = sqrtf(fmaxf(1.0f - x * x, 1.0e-12f));
-      const float ist = 1.0f / st;
+      const float st = hw_sqrt(fmaxf(1.0f - x * x, 1.0e-12f));
+      const float ist = hw_rcp(st);
       const float sw = pref * fa.x * fb.x;
       const float dsw = pref * (fa.y * fb.x * ta.w + fa.x * fb.y * tb.w);
       const float m = 0.5f * (ua.w + ub.w);
@@ -805,7 +843,7 @@ k_aev_tangent(const float* __restrict__ coords, const float* __restrict__ disp,
 #pragma unroll
       for (int d = 0; d < NA; ++d) {
         const float y = m - p.ang_shifts[d];
-        const float R = __expf(-p.angular_eta * y * y);
+        const float R = gauss(y, p.angular_eta);
         tr[d] = dsw * R + sw * (-2.0f * p.angular_eta * y * R) * dm_;  // P_d
         tr[NA + d] = sw * R * dc;                                       // Q_d
       }
@@ -814,9 +852,10 @@ k_aev_tangent(const float* __restrict__ coords, const float* __restrict__ disp,
         const float bz = 1.0f + x * p.ang_cos[zz] + st * p.ang_sin[zz];
         float F = 0.0f, dF = 0.0f;
         if (bz > 0.0f) {
-          const float lb = __log2f(bz);
-          F = exp2f(p.zeta * lb);
-          dF = 0.95f * p.zeta * exp2f((p.zeta - 1.0f) * lb) * (p.ang_cos[zz] - x * p.ang_sin[zz] * ist);
+          const float lb = hw_log2(bz);
+          const float tz = p.zeta * lb;
+          F = hw_exp2(tz);
+          dF = (0.95f * p.zeta) * hw_exp2(tz - lb) * (p.ang_cos[zz] - x * p.ang_sin[zz] * ist);
         }
         tr[2 * NA + zz] = F;
         tr[2 * NA + NZ + zz] = dF;
