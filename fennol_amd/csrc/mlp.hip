@@ -534,7 +534,7 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
                        (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
                        energy, grad, grad_stride);
   } else if (bf && M == 96) {
-    hipLaunchKernelGGL((k_species_mlp<unsigned short, 96, 4, 4>), dim3(grid), dim3(256), 0, st,
+    hipLaunchKernelGGL((k_species_mlp<unsigned short, 96, 2, 4>), dim3(grid), dim3(256), 0, st,
                        (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
                        energy, grad, grad_stride);
   } else if (bf && M == 32) {

@@ -28,7 +28,8 @@ K_B = K_B_AU * KCALPERMOL      # us.K_B in kcal/mol/K
 FS_PER_PS = _T * FS            # us.FS = 1000: dt_internal = dt_fs / FS_PER_PS
 THZ = 1.0                      # us.THZ: gamma_internal = gamma_THz / THZ (ps^-1)
 
-ENERGY_UNITS = {"HA": 1.0, "HARTREE": 1.0, "EV": EV, "KCALPERMOL": KCALPERMOL, "MEV": 1.0e3 * EV}
+ENERGY_UNITS = {"HA": 1.0, "HARTREE": 1.0, "AU": 1.0, "EV": EV, "MEV": 1.0e3 * EV, "KCALPERMOL": KCALPERMOL,
+                "KCAL/MOL": KCALPERMOL, "KJPERMOL": 4.184 * KCALPERMOL, "KJ/MOL": 4.184 * KCALPERMOL}
 
 
 def energy_multiplier(unit: str) -> float:

@@ -351,3 +351,38 @@ def test_fnx_save_load_roundtrip(gpu, tmp_path):
     e2, f2, _ = back.energy_and_forces(species=s["species"], coordinates=s["coordinates"])
     np.testing.assert_array_equal(e, e2)
     np.testing.assert_allclose(f, f2, rtol=1e-6, atol=1e-6)
+
+
+def test_deck_runs_like_the_md_driver(gpu, tmp_path):
+    """A fennol_md-style deck (general PBC, Langevin, skin) runs through the device MD driver:
+    the trajectory file has one frame per tdump and the final positions equal a direct
+    LangevinMD run with the same parameters and seed."""
+    print("[api] deck", flush=True)
+    from fennol_amd.deck import run_deck
+    from fennol_amd.md import LangevinMD
+
+    s = water_box_small(8, 4)
+    model = ani_model(("H", "C", "N", "O", "S", "F", "Cl"))
+    model.save(tmp_path / "model.fnx")
+    sym = {1: "H", 8: "O"}
+    lines = [f"{len(s['species'])}  water8"]
+    lines += [f"{k + 1:6d}  {sym[int(z)]}  {x[0]:.8f} {x[1]:.8f} {x[2]:.8f}" for k, (z, x) in
+              enumerate(zip(s["species"], s["coordinates"]))]
+    (tmp_path / "water8.xyz").write_text("\n".join(lines) + "\n")
+    L = float(s["cells"][0][0, 0])
+    (tmp_path / "input.fnl").write_text(
+        "model_file model.fnx\nxyz_input{\n file water8.xyz\n indexed yes\n has_comment_line no\n}\n"
+        f"cell = {L} 0 0 0 {L} 0 0 0 {L}\nminimum_image no\nnsteps = 20\ndt[fs] = 0.5\ntdump[fs] = 2.5\n"
+        "temperature = 300\ngamma[THz] = 1.\nrandom_seed 12\nnblist_skin 1.\nnblist_stride 6\n"
+        "nprint 5\ntraj_format xyz\nenergy_unit eV\n")
+    res = run_deck(tmp_path / "input.fnl")
+    assert len(res["rows"]) == 4 and all(np.isfinite(r[1]) for r in res["rows"])
+    text = (tmp_path / "water8.traj.xyz").read_text().splitlines()
+    assert len(text) == 4 * (2 + len(s["species"]))
+    x_deck = res["md"].x.cpu().numpy()
+    xyz = np.array([[float(v) for v in ln.split()[2:5]] for ln in lines[1:]])
+    md = LangevinMD(model.potential, {"species": s["species"], "coordinates": xyz, "cells": s["cells"]},
+                    dt_fs=0.5, temperature=300.0, gamma_thz=1.0, seed=12, energy_unit=model.energy_unit,
+                    nblist_skin=1.0, nblist_stride=6, minimum_image=False)
+    md.run(20)
+    np.testing.assert_allclose(x_deck, md.x.cpu().numpy(), rtol=0, atol=1e-6)
