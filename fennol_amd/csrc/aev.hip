@@ -93,6 +93,24 @@ __device__ __forceinline__ float to_out<float>(float v) { return v; }
 template <>
 __device__ __forceinline__ hip_bfloat16 to_out<hip_bfloat16>(float v) { return hip_bfloat16(v); }
 
+// dE/dAEV rows: float32, or bfloat16 bits (the BF16 form)
+__device__ __forceinline__ float g2f(float v) { return v; }
+__device__ __forceinline__ float g2f(unsigned short v) { return __uint_as_float((uint32_t)v << 16); }
+
+template <typename GT>
+__device__ __forceinline__ void load_g_row(float* G, const GT* __restrict__ row, int n, int lane) {
+  if constexpr (sizeof(GT) == 4) {
+    const float4* r4 = (const float4*)row;
+    for (int c = lane; c < (n + 3) / 4; c += 64) ((float4*)G)[c] = r4[c];
+  } else {
+    const ushort4* r4 = (const ushort4*)row;
+    for (int c = lane; c < (n + 3) / 4; c += 64) {
+      const ushort4 u = r4[c];
+      ((float4*)G)[c] = make_float4(g2f(u.x), g2f(u.y), g2f(u.z), g2f(u.w));
+    }
+  }
+}
+
 struct AevDims {
   int S, P, nR, rad_len, aev_len;
 };
@@ -490,7 +508,7 @@ __host__ __device__ inline int bwd_lds_words(int max_nbr, int max_nbr_ang, int a
 // angular terms of each angular slot are scattered to its atom with global atomics.
 // Outputs: grad_self[i] = -sum_e dE_i/dvec_ie - sum_j dE_j/dvec_ji|radial;
 // grad_nbr[j] += dE_i/dvec_ij|angular; virial_atom[i] = sum_e vec_ie (x) dE_i/dvec_ie.
-template <int NA, int NZ>
+template <int NA, int NZ, typename GT>
 __global__ void __launch_bounds__(64 * kWaves)
 k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
           const int32_t* __restrict__ batch_index, int n_atoms, int pbc_mode,
@@ -498,7 +516,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           const int32_t* __restrict__ nbr, const int32_t* __restrict__ n_nbr, int max_nbr,
           const int32_t* __restrict__ nbr_shift,
           const int32_t* __restrict__ nbr_ang, const int32_t* __restrict__ n_nbr_ang, int max_nbr_ang,
-          fnl_aev_params p, AevDims dm, const float* __restrict__ g, int g_stride,
+          fnl_aev_params p, AevDims dm, const GT* __restrict__ g, int g_stride,
           float* __restrict__ grad_self, float* __restrict__ grad_nbr, float* __restrict__ virial_atom) {
   static_assert(NA * NZ == 32, "angular channel layout");
   extern __shared__ float4 lds4[];
@@ -528,10 +546,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   // the first 64 entries of the angular row, fetched with the dE/dAEV and radial rows
   const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
   const int ae0 = lane < na ? arow[lane] : 0;
-  {
-    const float4* grow = (const float4*)(g + (size_t)i * g_stride);
-    for (int c = lane; c < (dm.aev_len + 3) / 4; c += 64) ((float4*)G)[c] = grow[c];
-  }
+  load_g_row(G, g + (size_t)i * g_stride, dm.aev_len, lane);
   const int32_t* row = nbr + (size_t)i * max_nbr;
   // per-lane partial sums: dE/dx_i (self and gathered radial neighbour terms) and the virial
   float sx = 0.f, sy = 0.f, sz = 0.f;
@@ -555,13 +570,13 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       // for the centre's block of species sj (G_i) and j's block of species si (G_j)
       const float dfc = cos_switch_grad(d, p.cutoff);
       const float* gr = G + sj * dm.nR;
-      const float* gj = g + (size_t)j * g_stride + si * dm.nR;
+      const GT* gj = g + (size_t)j * g_stride + si * dm.nR;
       float vi = 0.0f, vj = 0.0f;
       for (int k = 0; k < dm.nR; ++k) {
         const float x = d - p.radial_shifts[k];
         const float t = gauss(x, p.radial_eta) * (dfc - 2.0f * p.radial_eta * x * fc);
         vi += gr[k] * t;
-        vj += gj[k] * t;
+        vj += g2f(gj[k]) * t;
       }
       const float id = d2 > 1.0e-5f ? 0.25f * hw_rcp(d) : 0.0f;
       const float sc = vi * id, sj_ = vj * id;
@@ -925,22 +940,22 @@ int launch_fwd(const float* coords, const int32_t* species_idx, const int32_t* b
   return FNL_OK;
 }
 
-template <int NA, int NZ>
+template <int NA, int NZ, typename GT>
 int launch_bwd(const float* coords, const int32_t* species_idx, const int32_t* batch_index, int n_atoms,
                int pbc_mode, const float* cells, const float* rec, const int32_t* nbr,
                const int32_t* n_nbr, int max_nbr, const int32_t* nbr_shift, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
-               int max_nbr_ang, const fnl_aev_params& p, const AevDims& dm, const float* g, int g_stride,
+               int max_nbr_ang, const fnl_aev_params& p, const AevDims& dm, const GT* g, int g_stride,
                float* grad_self, float* grad_nbr, float* virial_atom, hipStream_t s) {
   const size_t lds = (size_t)kWaves * 4 * bwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
   if (lds > 160 * 1024) return FNL_ERR_UNSUPPORTED;
   static size_t lds_set = 0;
   if (lds > lds_set) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_aev_bwd<NA, NZ>),
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_aev_bwd<NA, NZ, GT>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
       return FNL_ERR_HIP;
     lds_set = lds;
   }
-  hipLaunchKernelGGL((k_aev_bwd<NA, NZ>), dim3(fnl_blocks(n_atoms, kWaves)), dim3(64 * kWaves), lds, s,
+  hipLaunchKernelGGL((k_aev_bwd<NA, NZ, GT>), dim3(fnl_blocks(n_atoms, kWaves)), dim3(64 * kWaves), lds, s,
                      coords, species_idx, batch_index, n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr,
                      nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, p, dm, g, g_stride, grad_self, grad_nbr,
                      virial_atom);
@@ -997,27 +1012,33 @@ extern "C" int fnl_aev_backward(const float* coords, const int32_t* species_idx,
                                 const float* reciprocal_cells, const int32_t* nbr, const int32_t* n_nbr,
                                 int32_t max_nbr, const int32_t* nbr_shift, const int32_t* nbr_ang,
                                 const int32_t* n_nbr_ang,
-                                int32_t max_nbr_ang, const fnl_aev_params* params, const float* g,
-                                int32_t g_stride, float* grad_self, float* grad_nbr, float* virial_atom,
-                                fnl_stream_t stream) {
+                                int32_t max_nbr_ang, const fnl_aev_params* params, const void* g,
+                                int32_t g_stride, int32_t g_dtype, float* grad_self, float* grad_nbr,
+                                float* virial_atom, fnl_stream_t stream) {
   AevDims dm;
   if (!check_params(params, &dm)) return FNL_ERR_INVALID;
   if (!common_args_ok(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells, nbr,
                       n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang))
     return FNL_ERR_INVALID;
   if (g_stride < ((dm.aev_len + 3) & ~3) || g_stride % 4 != 0) return FNL_ERR_INVALID;
+  if (g_dtype != FNL_F32 && g_dtype != FNL_BF16) return FNL_ERR_INVALID;
   if (n_atoms == 0) return FNL_OK;
   if (!g || !grad_self || !grad_nbr || !virial_atom) return FNL_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
   const int na = params->n_ang_dist, nz = params->n_ang_sect;
-  if (na == 4 && nz == 8)
-    return launch_bwd<4, 8>(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells,
-                            nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm, g, g_stride,
-                            grad_self, grad_nbr, virial_atom, s);
-  if (na == 8 && nz == 4)
-    return launch_bwd<8, 4>(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells,
-                            nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm, g, g_stride,
-                            grad_self, grad_nbr, virial_atom, s);
+  const float* gf = (const float*)g;
+  const unsigned short* gb = (const unsigned short*)g;
+  const bool bf = g_dtype == FNL_BF16;
+#define FNL_BWD(NA_, NZ_)                                                                                         \
+  return bf ? launch_bwd<NA_, NZ_>(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells, \
+                                   nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm,  \
+                                   gb, g_stride, grad_self, grad_nbr, virial_atom, s)                            \
+            : launch_bwd<NA_, NZ_>(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells, \
+                                   nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm,  \
+                                   gf, g_stride, grad_self, grad_nbr, virial_atom, s)
+  if (na == 4 && nz == 8) FNL_BWD(4, 8);
+  if (na == 8 && nz == 4) FNL_BWD(8, 4);
+#undef FNL_BWD
   return FNL_ERR_UNSUPPORTED;
 }
 

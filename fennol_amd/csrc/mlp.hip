@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(64 * WV_, (Lds<T, M_, WV_>::BLOCKS_PER_CU))
 k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int32_t* __restrict__ perm,
               const int32_t* __restrict__ tile_start, const int32_t* __restrict__ tile_rows,
               const int32_t* __restrict__ tile_net, int n_tiles, float* __restrict__ energy,
-              float* __restrict__ grad, int grad_stride) {
+              void* __restrict__ grad, int grad_stride, int grad_bf16) {
   typedef Tr<T> tr;
   typedef Lds<T, M_, WV_> L;
   constexpr int M = M_;
@@ -461,18 +461,45 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     atom[mb] = m < nrows ? perm[r0 + m] : -1;
   }
   const int ncb_x = K / 16;
+  // BF16 rows: each pass's 256 columns of the tile go through LDS (the H2/H3 buffers, free now)
+  // so that the rows leave as contiguous 16-byte chunks
+  constexpr int GLD = 256 + 8;
+  static_assert(M * GLD * 2 <= L::OFFE - L::OFF2, "BF16 gradient staging fits the H2/H3 buffers");
+  unsigned short* gst = (unsigned short*)(smem + L::OFF2);
   for (int base = 0; base < ncb_x; base += 16) {
     f4 accx[NB1][MB];
     zero_acc(accx);
     gemm_t<T, NB1, MB, PF, WV>(accx, (const T*)P.bw[0], sizeof(T) == 2 ? H1 : K, base + wave, ncb_x, B1, L::LD1, H1);
+    if (grad_bf16) {
 #pragma unroll
-    for (int nb = 0; nb < NB1; ++nb) {
-      const int cb = base + wave + WV * nb;
-      if (cb >= ncb_x) continue;
-      const int n = cb * 16 + 4 * (l >> 4);
+      for (int nb = 0; nb < NB1; ++nb) {
+        const int cb = base + wave + WV * nb;
+        if (cb >= ncb_x) continue;
+        const int nl = (wave + WV * nb) * 16 + 4 * (l >> 4);
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb)
-        if (atom[mb] >= 0) __builtin_nontemporal_store(accx[nb][mb], (f4*)(grad + (size_t)atom[mb] * grad_stride + n));
+        for (int mb = 0; mb < MB; ++mb) Tr<unsigned short>::store4(gst + (mb * 16 + (l & 15)) * GLD + nl, accx[nb][mb]);
+      }
+      __syncthreads();
+      const int cpr = min(256, K - base * 16) / 8;  // 16-byte chunks per row in this pass
+      unsigned short* gout = (unsigned short*)grad + base * 16;
+      for (int idx = t; idx < M * cpr; idx += 64 * WV) {
+        const int m = idx / cpr, q = idx - m * cpr;
+        if (m < nrows)
+          __builtin_nontemporal_store(*(const u4v*)(gst + m * GLD + q * 8),
+                                      (u4v*)(gout + (size_t)perm[r0 + m] * grad_stride + q * 8));
+      }
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (int nb = 0; nb < NB1; ++nb) {
+        const int cb = base + wave + WV * nb;
+        if (cb >= ncb_x) continue;
+        const int n = cb * 16 + 4 * (l >> 4);
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+          if (atom[mb] >= 0)
+            __builtin_nontemporal_store(accx[nb][mb], (f4*)((float*)grad + (size_t)atom[mb] * grad_stride + n));
+      }
     }
   }
 }
@@ -487,8 +514,8 @@ extern "C" int fnl_mlp_tile_rows(int32_t dtype) {
 
 extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_mlp_weights* w,
                                const int32_t* perm, const int32_t* tile_start, const int32_t* tile_rows,
-                               const int32_t* tile_net, int32_t n_tiles, float* energy, float* grad,
-                               int32_t grad_stride, fnl_stream_t stream) {
+                               const int32_t* tile_net, int32_t n_tiles, float* energy, void* grad,
+                               int32_t grad_stride, int32_t grad_dtype, fnl_stream_t stream) {
   if (!w || n_tiles < 0) return FNL_ERR_INVALID;
   if (n_tiles == 0) return FNL_OK;
   if (!aev || !perm || !tile_start || !tile_rows || !tile_net || !energy) return FNL_ERR_INVALID;
@@ -499,7 +526,9 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
   if (w->k_pad < kc || w->k_pad % kc != 0 || aev_stride < w->k_pad ||
       aev_stride % 8 != 0)
     return FNL_ERR_INVALID;
-  if (grad && (grad_stride < w->k_pad || grad_stride % 4 != 0)) return FNL_ERR_INVALID;
+  if (grad && (grad_stride < w->k_pad || grad_stride % 8 != 0)) return FNL_ERR_INVALID;
+  if (grad_dtype != FNL_F32 && !(grad_dtype == FNL_BF16 && bf)) return FNL_ERR_INVALID;
+  const int gbf = grad_dtype == FNL_BF16 ? 1 : 0;
   MlpArgs a;
   a.k_pad = w->k_pad;
   a.alpha = w->celu_alpha;
@@ -532,23 +561,23 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
   if (bf && M == 64) {
     hipLaunchKernelGGL((k_species_mlp<unsigned short, 64, 2, 4>), dim3(grid), dim3(256), 0, st,
                        (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
-                       energy, grad, grad_stride);
+                       energy, grad, grad_stride, gbf);
   } else if (bf && M == 96) {
     hipLaunchKernelGGL((k_species_mlp<unsigned short, 96, 2, 4>), dim3(grid), dim3(256), 0, st,
                        (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
-                       energy, grad, grad_stride);
+                       energy, grad, grad_stride, gbf);
   } else if (bf && M == 32) {
     hipLaunchKernelGGL((k_species_mlp<unsigned short, 32, 2, 4>), dim3(grid), dim3(256), 0, st,
                        (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
-                       energy, grad, grad_stride);
+                       energy, grad, grad_stride, gbf);
   } else if (!bf && M == 32) {
     hipLaunchKernelGGL((k_species_mlp<float, 32, 4, 4>), dim3(grid), dim3(256), 0, st, (const float*)aev,
                        aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles, energy, grad,
-                       grad_stride);
+                       grad_stride, gbf);
   } else if (!bf && M == 16) {
     hipLaunchKernelGGL((k_species_mlp<float, 16, 2, 4>), dim3(grid), dim3(256), 0, st, (const float*)aev,
                        aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles, energy, grad,
-                       grad_stride);
+                       grad_stride, gbf);
   } else {
     return FNL_ERR_UNSUPPORTED;
   }

@@ -246,6 +246,7 @@ class ANIPotential:
         self.nets = SpeciesNetworks(params, config.aev_length, config.k_pad, precision, self.device,
                                     tile_rows=mlp_tile_rows)
         self.net_of_z = {SYMBOL_TO_Z[s]: n for n, s in enumerate(self.nets.species)}
+        self._row_dtype = _abi.FNL_F32 if precision == "fp32" else _abi.FNL_BF16  # AEV and dE/dAEV rows
         self.init_max_nbr = max_nbr
         self.init_max_nbr_ang = max_nbr_ang
         self.timer: StageTimer | None = None
@@ -328,7 +329,7 @@ class ANIPotential:
         kp = self.config.k_pad
         adt = torch.float32 if self.precision == "fp32" else torch.bfloat16
         sb.aev = torch.zeros((N, kp), dtype=adt, device=dev)
-        sb.g_aev = torch.zeros((N, kp), dtype=torch.float32, device=dev)
+        sb.g_aev = torch.zeros((N, kp), dtype=adt, device=dev)  # dE/dAEV rows: BF16 in the BF16 form
         sb.e_atom = torch.zeros(N, dtype=torch.float32, device=dev)
         sb.grad_self = torch.zeros((N, 3), dtype=torch.float32, device=dev)
         sb.grad_nbr = torch.zeros((N, 3), dtype=torch.float32, device=dev)
@@ -375,7 +376,7 @@ class ANIPotential:
         kp = self.config.k_pad
         adt = torch.float32 if self.precision == "fp32" else torch.bfloat16
         sb.aev = torch.zeros((max_atoms, kp), dtype=adt, device=dev)
-        sb.g_aev = torch.zeros((max_atoms, kp), dtype=torch.float32, device=dev)
+        sb.g_aev = torch.zeros((max_atoms, kp), dtype=adt, device=dev)
         sb.e_atom = torch.zeros(max_atoms, dtype=torch.float32, device=dev)
         sb.grad_self = torch.zeros((max_atoms, 3), dtype=torch.float32, device=dev)
         sb.grad_nbr = torch.zeros((max_atoms, 3), dtype=torch.float32, device=dev)
@@ -591,7 +592,8 @@ class ANIPotential:
         P = _abi.ptr
         rc = self.lib.fnl_species_mlp(
             P(sb.aev), sb.aev.shape[1], C.byref(self.nets.cw), P(sb.perm), P(sb.tile_start), P(sb.tile_rows),
-            P(sb.tile_net), sb.n_tiles, P(sb.e_atom), P(sb.g_aev) if with_grad else None, sb.g_aev.shape[1], s)
+            P(sb.tile_net), sb.n_tiles, P(sb.e_atom), P(sb.g_aev) if with_grad else None, sb.g_aev.shape[1],
+            self._row_dtype, s)
         _abi.check(rc, "fnl_species_mlp")
 
     def zero_grad_nbr(self, sb: SystemBuffers, stream=None) -> None:
@@ -610,7 +612,7 @@ class ANIPotential:
         rc = self.lib.fnl_aev_backward(
             P(coords32), P(sb.species_idx), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
             P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(sb.nbr_shift), P(sb.nbr_ang), P(sb.n_nbr_ang), sb.max_nbr_ang,
-            C.byref(self.cparams), P(sb.g_aev), sb.g_aev.shape[1], P(sb.grad_self), P(sb.grad_nbr),
+            C.byref(self.cparams), P(sb.g_aev), sb.g_aev.shape[1], self._row_dtype, P(sb.grad_self), P(sb.grad_nbr),
             P(sb.vir_atom), s)
         _abi.check(rc, "fnl_aev_backward")
 

@@ -63,7 +63,7 @@ class TrainingStep:
 
     def tangent(self, sb: SystemBuffers, x32: torch.Tensor, disp: torch.Tensor) -> torch.Tensor:
         """AEV tangent rows [N, k_pad] along disp [N, 3] (float32)."""
-        out = torch.empty_like(sb.g_aev)
+        out = torch.empty(sb.g_aev.shape, dtype=torch.float32, device=sb.g_aev.device)
         P = _abi.ptr
         _abi.check(self.pot.lib.fnl_aev_tangent(
             P(x32), P(disp), P(sb.species_idx), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),

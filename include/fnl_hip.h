@@ -202,7 +202,8 @@ int fnl_aev_forward(const float* coords, const int32_t* species_idx,
 
 /* Adjoint of fnl_aev_forward (the reverse pass jax.grad runs through
  * ani.py:47-142 and the graph processors).  AEV terms are recomputed rather
- * than stored.  Given g = dE/dAEV (float32 rows of stride g_stride), with
+ * than stored.  Given g = dE/dAEV (rows of stride g_stride, g_dtype FNL_F32 or
+ * FNL_BF16, read as float32), with
  * E = sum_i E_i and vec_ij = x_j - x_i:
  *   grad_self[i] = -sum_j dE_i/dvec_ij - sum_j dE_j/dvec_ji|radial
  *                  (written, float32[N][3]: the centre's own terms plus the
@@ -220,7 +221,7 @@ int fnl_aev_backward(const float* coords, const int32_t* species_idx,
                      const int32_t* nbr_shift,
                      const int32_t* nbr_ang, const int32_t* n_nbr_ang, int32_t max_nbr_ang,
                      const fnl_aev_params* params,
-                     const float* g, int32_t g_stride,
+                     const void* g, int32_t g_stride, int32_t g_dtype,
                      float* grad_self, float* grad_nbr, float* virial_atom,
                      fnl_stream_t stream);
 
@@ -255,7 +256,9 @@ int fnl_aev_tangent(const float* coords, const float* disp, const int32_t* speci
  * perm[tile_start[t] .. tile_start[t] + tile_rows[t]) with network
  * tile_net[t]; rows per tile <= the kernel's tile size (tile_rows).
  * Outputs: energy[perm[r]] and, if grad != NULL, dE/dAEV rows grad[perm[r]]
- * (float32, stride grad_stride >= k_pad; columns >= k_pad untouched).
+ * (grad_dtype FNL_F32, or FNL_BF16 with BF16 weights: halves the bytes the
+ * rows cost to write and to read back in fnl_aev_backward; stride
+ * grad_stride >= k_pad, a multiple of 8; columns >= k_pad untouched).
  * ---------------------------------------------------------------------- */
 #define FNL_MLP_H1 256
 #define FNL_MLP_H2 192
@@ -285,7 +288,7 @@ int fnl_mlp_tile_rows(int32_t dtype);
 int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_mlp_weights* w,
                     const int32_t* perm, const int32_t* tile_start,
                     const int32_t* tile_rows, const int32_t* tile_net, int32_t n_tiles,
-                    float* energy, float* grad, int32_t grad_stride,
+                    float* energy, void* grad, int32_t grad_stride, int32_t grad_dtype,
                     fnl_stream_t stream);
 
 /* ------------------------------------------------------------------------

@@ -101,7 +101,7 @@ def test_golden_mlp8(gpu):
     pot.mlp(sb, with_grad=True)
     torch.cuda.synchronize()
     e = sb.e_atom.cpu().numpy()
-    g = sb.g_aev.cpu().numpy()
+    g = sb.g_aev.float().cpu().numpy()
     np.testing.assert_allclose(e, f["energy"], rtol=1e-5, atol=1e-6)
     gr = f["grad"]
     np.testing.assert_allclose(g[:, : c.aev_length], gr, rtol=1e-4, atol=1e-5 * np.abs(gr).max())

@@ -418,7 +418,7 @@ def test_mlp_tile_sizes_agree(gpu, precision, tiles):
         x = torch.tensor(np.asarray(s["coordinates"], np.float32), device="cuda")
         pot.compute(sb, x, forces=True)
         torch.cuda.synchronize()
-        outs.append((sb.e_atom[: sb.n_atoms].cpu().numpy().copy(), sb.g_aev[: sb.n_atoms].cpu().numpy().copy()))
+        outs.append((sb.e_atom[: sb.n_atoms].cpu().numpy().copy(), sb.g_aev[: sb.n_atoms].float().cpu().numpy().copy()))
     for e, g in outs[1:]:
         np.testing.assert_allclose(e, outs[0][0], rtol=1e-6, atol=1e-7)
         np.testing.assert_allclose(g, outs[0][1], rtol=1e-5, atol=1e-6 * np.abs(outs[0][1]).max())
