@@ -389,12 +389,14 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   // zeros in 16-byte stores
   {
     constexpr int CH = 16 / (int)sizeof(TOut);  // channels per 16-byte chunk
+    // radial block of channel c: (c + 0.5) / nR rounded down, exact in float for these sizes
+    const float inv_nr = 1.0f / (float)dm.nR;
     const bool aligned = (dm.nR % CH) == 0 && (dm.rad_len % CH) == 0 && (stride % CH) == 0 &&
                          ((uintptr_t)aev & 15) == 0;
     if (aligned) {
       for (int c0 = lane * CH; c0 < stride; c0 += 64 * CH) {
         bool empty;
-        if (c0 < dm.rad_len) empty = !((rad_present >> (c0 / dm.nR)) & 1u);
+        if (c0 < dm.rad_len) empty = !((rad_present >> (int)(((float)c0 + 0.5f) * inv_nr)) & 1u);
         else if (c0 < dm.aev_len) empty = !((present >> ((c0 - dm.rad_len) >> 5)) & 1ull);
         else empty = true;
         if (empty) *(uint4*)(out + c0) = make_uint4(0u, 0u, 0u, 0u);
@@ -402,7 +404,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     } else {
       for (int c = lane; c < stride; c += 64) {
         bool empty;
-        if (c < dm.rad_len) empty = !((rad_present >> (c / dm.nR)) & 1u);
+        if (c < dm.rad_len) empty = !((rad_present >> (int)(((float)c + 0.5f) * inv_nr)) & 1u);
         else if (c < dm.aev_len) empty = !((present >> ((c - dm.rad_len) >> 5)) & 1ull);
         else empty = true;
         if (empty) out[c] = to_out<TOut>(0.0f);

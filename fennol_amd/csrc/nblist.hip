@@ -32,6 +32,7 @@
 namespace {
 
 constexpr int kGridWords = 12;
+constexpr int kCmap = 640;      // candidates per centre mapped through LDS (more: the search below)
 constexpr int kMaxRanges = 18;  // 3 x 3 columns x at most 2 z-runs  // per system: nx ny nz first | ox oy oz | iwx iwy iwz | pbc pad
 
 struct NblistWs {
@@ -536,9 +537,25 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
     roff[r] = __builtin_amdgcn_readlane(r_off, r);
     rdel[r] = __builtin_amdgcn_readlane(r_lo, r) - roff[r];
   }
-  // position in the cell-sorted arrays of candidate v < total: v + shift of the last range
-  // that starts at or before v (empty ranges start where the next one does)
+  // position in the cell-sorted arrays of candidate v < total: a per-wave LDS map filled range
+  // by range (one LDS write per candidate instead of a search per candidate and pass); beyond
+  // kCmap candidates, v + shift of the last range that starts at or before v (empty ranges
+  // start where the next one does)
+  __shared__ int cmap_all[WAVES][kCmap];
+  int* cmap = cmap_all[threadIdx.x >> 6];
+  const bool use_map = total <= kCmap;
+  if (use_map) {
+    for (int r = 0; r < nrange; ++r) {
+      const int o = __builtin_amdgcn_readlane(r_off, r), lo = __builtin_amdgcn_readlane(r_lo, r);
+      const int len = __builtin_amdgcn_readlane(r_len, r);
+      for (int k = lane; k < len; k += 64) cmap[o + k] = lo + k;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
   auto candidate = [&](int v) {
+    if (use_map) return cmap[min(v, kCmap - 1)];
     int dl = rdel[0];
 #pragma unroll
     for (int r = 1; r < kMaxRanges; ++r)
