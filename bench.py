@@ -75,8 +75,8 @@ def algorithmic_units(sb, cfg, precision: str) -> dict:
         # xyz, rows in; AEV row out
         "aev_fwd": ("hbm", float(N * (12 + 8) + 4 * nr.sum() + 4 * na.sum() + N * A * out_b)),
         "mlp": ("mfma", float(N * per_atom_mlp)),
-        # dE/dAEV in (fp32), rows in; forces (self + neighbour) and virial out
-        "aev_bwd": ("hbm", float(N * (12 + 8) + 4 * nr.sum() + 4 * na.sum() + N * A * 4 + N * (24 + 36))),
+        # dE/dAEV in (fp32, or bf16 in the BF16 form), rows in; forces (self + neighbour) and virial out
+        "aev_bwd": ("hbm", float(N * (12 + 8) + 4 * nr.sum() + 4 * na.sum() + N * A * out_b + N * (24 + 36))),
         "reduce": ("hbm", float(N * (4 + 36))),
         "baoab": ("hbm", float(N * (48 + 24 + 8 + 8 + 48 + 12))),
     }
