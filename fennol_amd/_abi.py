@@ -87,7 +87,7 @@ _SIGNATURES = {
                                          vp, vp, vp, vp, vp]),
     "fnl_nblist_export": (C.c_int, [vp, vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp]),
     "fnl_aev_forward": (C.c_int, [vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, i32,
-                                  C.POINTER(AevParams), vp, i32, i32, vp]),
+                                  C.POINTER(AevParams), vp, i32, i32, u32, vp]),
     "fnl_aev_backward": (C.c_int, [vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, i32,
                                    C.POINTER(AevParams), vp, i32, i32, vp, vp, vp, vp]),
     "fnl_aev_tangent": (C.c_int, [vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, i32,

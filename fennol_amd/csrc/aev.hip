@@ -253,7 +253,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           const int32_t* __restrict__ nbr, const int32_t* __restrict__ n_nbr, int max_nbr,
           const int32_t* __restrict__ nbr_shift,
           const int32_t* __restrict__ nbr_ang, const int32_t* __restrict__ n_nbr_ang, int max_nbr_ang,
-          fnl_aev_params p, AevDims dm, TOut* __restrict__ aev, int stride) {
+          fnl_aev_params p, AevDims dm, TOut* __restrict__ aev, int stride, uint32_t smask) {
   static_assert(NA * NZ == 32, "angular channel layout");
   extern __shared__ float4 lds4[];
   const int w = threadIdx.x >> 6;
@@ -361,6 +361,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   wave_sync();
   const int P = dm.P;
   int pn1 = 0, pn2 = 0, po1 = 0, po2 = 0, np = 0;
+  bool live = false;  // both species of the pair in smask: its block is written
   if (lane < P) {
     int s1 = 0, s2 = 0, r0 = 0;
     for (int a = 0; a < S; ++a) {
@@ -376,8 +377,10 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     po1 = tab[8 + s1];
     po2 = tab[8 + s2];
     np = s1 == s2 ? pn1 * (pn1 - 1) / 2 : pn1 * pn2;
+    live = ((smask >> s1) & (smask >> s2) & 1u) != 0u;
   }
   const uint64_t present = __ballot(np > 0);
+  const uint64_t pair_live = __ballot(live);
 
   const float pref = 2.0f * exp2f(-p.zeta);  // 2 * 0.5^zeta (ani.py:112)
   const int h = lane >> 5, ch = lane & 31;
@@ -386,8 +389,10 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const float* fF = fac + (NA + zz) * kFacLd;
   TOut* aout = out + dm.rad_len;
   // empty blocks (species without radial neighbours, pairs without triplets) and the padding:
-  // zeros in 16-byte stores
+  // zeros in 16-byte stores.  Blocks of species outside smask (and, unless smask holds every
+  // species, the padding) are not written: the caller keeps them zero.
   {
+    const bool pad_live = smask == 0xFFFFFFFFu;
     constexpr int CH = 16 / (int)sizeof(TOut);  // channels per 16-byte chunk
     // radial block of channel c: (c + 0.5) / nR rounded down, exact in float for these sizes
     const float inv_nr = 1.0f / (float)dm.nR;
@@ -395,19 +400,37 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
                          ((uintptr_t)aev & 15) == 0;
     if (aligned) {
       for (int c0 = lane * CH; c0 < stride; c0 += 64 * CH) {
-        bool empty;
-        if (c0 < dm.rad_len) empty = !((rad_present >> (int)(((float)c0 + 0.5f) * inv_nr)) & 1u);
-        else if (c0 < dm.aev_len) empty = !((present >> ((c0 - dm.rad_len) >> 5)) & 1ull);
-        else empty = true;
-        if (empty) *(uint4*)(out + c0) = make_uint4(0u, 0u, 0u, 0u);
+        bool empty, wr;
+        if (c0 < dm.rad_len) {
+          const int sb = (int)(((float)c0 + 0.5f) * inv_nr);
+          empty = !((rad_present >> sb) & 1u);
+          wr = (smask >> sb) & 1u;
+        } else if (c0 < dm.aev_len) {
+          const int pb = (c0 - dm.rad_len) >> 5;
+          empty = !((present >> pb) & 1ull);
+          wr = (pair_live >> pb) & 1ull;
+        } else {
+          empty = true;
+          wr = pad_live;
+        }
+        if (empty && wr) *(uint4*)(out + c0) = make_uint4(0u, 0u, 0u, 0u);
       }
     } else {
       for (int c = lane; c < stride; c += 64) {
-        bool empty;
-        if (c < dm.rad_len) empty = !((rad_present >> (int)(((float)c + 0.5f) * inv_nr)) & 1u);
-        else if (c < dm.aev_len) empty = !((present >> ((c - dm.rad_len) >> 5)) & 1ull);
-        else empty = true;
-        if (empty) out[c] = to_out<TOut>(0.0f);
+        bool empty, wr;
+        if (c < dm.rad_len) {
+          const int sb = (int)(((float)c + 0.5f) * inv_nr);
+          empty = !((rad_present >> sb) & 1u);
+          wr = (smask >> sb) & 1u;
+        } else if (c < dm.aev_len) {
+          const int pb = (c - dm.rad_len) >> 5;
+          empty = !((present >> pb) & 1ull);
+          wr = (pair_live >> pb) & 1ull;
+        } else {
+          empty = true;
+          wr = pad_live;
+        }
+        if (empty && wr) out[c] = to_out<TOut>(0.0f);
       }
     }
   }
@@ -915,7 +938,7 @@ int launch_fwd(const float* coords, const int32_t* species_idx, const int32_t* b
                int pbc_mode, const float* cells, const float* rec, const int32_t* nbr,
                const int32_t* n_nbr, int max_nbr, const int32_t* nbr_shift, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
                int max_nbr_ang, const fnl_aev_params& p, const AevDims& dm, void* aev, int stride,
-               int out_dtype, hipStream_t s) {
+               int out_dtype, uint32_t smask, hipStream_t s) {
   const size_t lds = (size_t)kWaves * 4 * fwd_layout(max_nbr, max_nbr_ang, p.n_ang_dist + p.n_ang_sect).words;
   if (lds > 160 * 1024) return FNL_ERR_UNSUPPORTED;
   dim3 grid(fnl_blocks(n_atoms, kWaves)), block(64 * kWaves);
@@ -932,11 +955,11 @@ int launch_fwd(const float* coords, const int32_t* species_idx, const int32_t* b
   if (out_dtype == FNL_F32) {
     hipLaunchKernelGGL((k_aev_fwd<NA, NZ, float>), grid, block, lds, s, coords, species_idx, batch_index,
                        n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang,
-                       p, dm, (float*)aev, stride);
+                       p, dm, (float*)aev, stride, smask);
   } else {
     hipLaunchKernelGGL((k_aev_fwd<NA, NZ, hip_bfloat16>), grid, block, lds, s, coords, species_idx,
                        batch_index, n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang,
-                       max_nbr_ang, p, dm, (hip_bfloat16*)aev, stride);
+                       max_nbr_ang, p, dm, (hip_bfloat16*)aev, stride, smask);
   }
   FNL_LAUNCH_CHECK();
   return FNL_OK;
@@ -987,7 +1010,8 @@ extern "C" int fnl_aev_forward(const float* coords, const int32_t* species_idx, 
                                int32_t max_nbr, const int32_t* nbr_shift, const int32_t* nbr_ang,
                                const int32_t* n_nbr_ang,
                                int32_t max_nbr_ang, const fnl_aev_params* params, void* aev,
-                               int32_t aev_stride, int32_t out_dtype, fnl_stream_t stream) {
+                               int32_t aev_stride, int32_t out_dtype, uint32_t species_mask,
+                               fnl_stream_t stream) {
   AevDims dm;
   if (!check_params(params, &dm)) return FNL_ERR_INVALID;
   if (!common_args_ok(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells, nbr,
@@ -1001,11 +1025,11 @@ extern "C" int fnl_aev_forward(const float* coords, const int32_t* species_idx, 
   if (na == 4 && nz == 8)
     return launch_fwd<4, 8>(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells,
                             nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm, aev,
-                            aev_stride, out_dtype, s);
+                            aev_stride, out_dtype, species_mask, s);
   if (na == 8 && nz == 4)
     return launch_fwd<8, 4>(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells,
                             nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm, aev,
-                            aev_stride, out_dtype, s);
+                            aev_stride, out_dtype, species_mask, s);
   return FNL_ERR_UNSUPPORTED;
 }
 

@@ -328,7 +328,11 @@ class ANIPotential:
         self._setup_tiles(sb)
         kp = self.config.k_pad
         adt = torch.float32 if self.precision == "fp32" else torch.bfloat16
+        # AEV rows zeroed once; the forward then writes only the channel blocks of the species
+        # present in this batch (the others stay zero for every atom of it)
         sb.aev = torch.zeros((N, kp), dtype=adt, device=dev)
+        sb.extra["aev_mask"] = int(np.bitwise_or.reduce(1 << np.unique(sidx[sidx >= 0]).astype(np.int64),
+                                                        initial=0)) & 0xFFFFFFFF
         sb.g_aev = torch.zeros((N, kp), dtype=adt, device=dev)  # dE/dAEV rows: BF16 in the BF16 form
         sb.e_atom = torch.zeros(N, dtype=torch.float32, device=dev)
         sb.grad_self = torch.zeros((N, 3), dtype=torch.float32, device=dev)
@@ -584,7 +588,7 @@ class ANIPotential:
         rc = self.lib.fnl_aev_forward(
             P(coords32), P(sb.species_idx), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
             P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(sb.nbr_shift), P(sb.nbr_ang), P(sb.n_nbr_ang), sb.max_nbr_ang,
-            C.byref(self.cparams), P(sb.aev), sb.aev.shape[1], dt, s)
+            C.byref(self.cparams), P(sb.aev), sb.aev.shape[1], dt, sb.extra.get("aev_mask", 0xFFFFFFFF), s)
         _abi.check(rc, "fnl_aev_forward")
 
     def mlp(self, sb: SystemBuffers, with_grad: bool = True, stream=None) -> None:

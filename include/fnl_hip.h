@@ -186,6 +186,12 @@ typedef struct fnl_aev_params {
 /* AEV length: S*nR + S(S+1)/2 * nA*nZ.  Row i of aev (stride aev_stride
  * elements, element type out_dtype) gets the channels of centre i in the
  * reference order; columns [length, aev_stride) are written as zero.
+ * species_mask: bit s set for every AEV species index s present in the batch.
+ * Channel blocks of species (radial) and species pairs (angular) outside the
+ * mask are not written, nor is the padding unless the mask is ~0u: they are
+ * zero for every atom of such a batch, and the caller keeps them zero (a
+ * buffer zeroed once and only ever used for batches with species in the
+ * mask).  ~0u writes every column.
  * Pair vectors: x_j - x_i, plus the minimum image (FNL_PBC_MINIMUM_IMAGE) or
  * the slot's stored shift . C (FNL_PBC_GENERAL: nbr_shift from
  * fnl_nblist_build, required; NULL for the other modes).  The same holds for
@@ -198,7 +204,7 @@ int fnl_aev_forward(const float* coords, const int32_t* species_idx,
                     const int32_t* nbr_ang, const int32_t* n_nbr_ang, int32_t max_nbr_ang,
                     const fnl_aev_params* params,
                     void* aev, int32_t aev_stride, int32_t out_dtype,
-                    fnl_stream_t stream);
+                    uint32_t species_mask, fnl_stream_t stream);
 
 /* Adjoint of fnl_aev_forward (the reverse pass jax.grad runs through
  * ani.py:47-142 and the graph processors).  AEV terms are recomputed rather

@@ -51,7 +51,7 @@ def test_host_side_argument_checks():
     p = _abi.AevParams()
     p.n_species = 9  # above the maximum
     assert lib.fnl_aev_forward(None, None, None, 0, 0, None, None, None, None, 8, None, None, None, 8, p, None, 0,
-                               0, None) == -1
+                               0, 0xFFFFFFFF, None) == -1
     assert lib.fnl_fill_i32(None, 0, 0, None) == 0
     assert lib.fnl_fill_i32(None, 0, 5, None) == -1
     assert lib.fnl_species_mlp(None, 0, None, None, None, None, None, 0, None, None, 0, 0, None) == -1

@@ -361,6 +361,26 @@ def test_aev_forward(gpu, name):
     assert np.all(sb.aev.float().cpu().numpy()[:, cfg[1].aev_length:] == 0)
 
 
+def test_aev_species_mask_writes_every_live_block(gpu):
+    """The forward with the batch's species mask (blocks of absent species left as the zeros of
+    the buffer) gives the rows a full write gives on a buffer filled with NaN."""
+    _flush("[aev] species mask")
+    s = water_box_small(64, 1)
+    cfg = configs(("H", "C", "N", "O", "S", "F", "Cl"))
+    _, pot = _potential(cfg)
+    sb, x = _nblist(pot, s)
+    assert sb.extra["aev_mask"] == (1 << 0) | (1 << 3)  # H and O
+    pot.aev_forward(sb, x)
+    masked = sb.aev.clone()
+    sb.aev.fill_(float("nan"))
+    sb.extra["aev_mask"] = 0xFFFFFFFF
+    pot.aev_forward(sb, x)
+    full = sb.aev.clone()
+    torch.cuda.synchronize()
+    assert not torch.isnan(full).any()
+    assert torch.equal(full, masked)
+
+
 @pytest.mark.parametrize("name", ["cluster64", "water64", "water216", "triclinic"])
 def test_energy_forces_virial_fp32(gpu, name):
     _flush(f"[efv fp32] {name}")
