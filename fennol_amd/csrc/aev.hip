@@ -121,25 +121,26 @@ __device__ __forceinline__ int pair_index(int a, int b, int S) {
   return lo * S - (lo * (lo - 1)) / 2 + (hi - lo);
 }
 
-constexpr int kFacLd = 65;     // row stride of the forward triplet-factor table (odd: no bank conflicts)
+constexpr int kFacLd = 65;     // row stride of the forward triplet-factor table (odd: the 4 R rows and
+                               // 8 F rows read at one column fall in different banks)
 constexpr int kTabWords = 16;  // per-wave table: species counts and offsets
 
 __host__ __device__ inline int round4(int n) { return (n + 3) & ~3; }
 
 // per-wave LDS layout of the forward kernel (offsets in 4-byte words, 16-byte aligned)
 struct FwdLayout {
-  int nbs, rdf, av, af, fac, tab, ae, words;
+  int rdf, av, af, fac, tab, words;
 };
+// per-wave LDS slice of the forward (4-byte words): 6 waves per SIMD at the water box's row
+// capacities (104 radial, 40 angular slots)
 __host__ __device__ inline FwdLayout fwd_layout(int max_nbr, int max_nbr_ang, int na_nz) {
   FwdLayout L;
-  L.nbs = 4 * max_nbr;
-  L.rdf = round4(L.nbs + max_nbr);
+  L.rdf = 4 * max_nbr;  // nbv at 0
   L.av = round4(L.rdf + 2 * max_nbr);
   L.af = L.av + 4 * max_nbr_ang;
   L.fac = round4(L.af + 2 * max_nbr_ang);
   L.tab = L.fac + na_nz * kFacLd;
-  L.ae = L.tab + kTabWords;
-  L.words = round4(L.ae + max_nbr_ang);
+  L.words = round4(L.tab + kTabWords);
   return L;
 }
 
@@ -226,8 +227,8 @@ __device__ __forceinline__ void species_sort(int n, int S, SpecFn spec, PlaceFn 
 }
 
 // Forward, one wave per centre.  LDS slice of the wave (4-byte words):
-//   nbv [max_nbr]      float4 (vx, vy, vz, d) per radial slot (slot order)
-//   nbs [max_nbr]      neighbour species per radial slot
+//   nbv [max_nbr]      float4 (vx, vy, vz, species bits) per radial slot (slot order);
+//                      d is recomputed from the vector where it is needed
 //   rdf [max_nbr]      float2 (d, 0.25 fc(d)), radial slots sorted by species
 //   av  [max_nbr_ang]  float4 (ux, uy, uz, d), angular slots sorted by species
 //   af  [max_nbr_ang]  float2 (fc_ang(d), species), same order
@@ -263,13 +264,11 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const FwdLayout L = fwd_layout(max_nbr, max_nbr_ang, NA + NZ);
   float* base = (float*)lds4 + (size_t)w * L.words;
   float4* nbv = (float4*)base;
-  int* nbs = (int*)(base + L.nbs);
   float2* rdf = (float2*)(base + L.rdf);
   float4* av = (float4*)(base + L.av);
   float2* af = (float2*)(base + L.af);
   float* fac = base + L.fac;
   int* tab = (int*)(base + L.tab);
-  int* ae = (int*)(base + L.ae);
 
   TOut* out = aev + (size_t)i * stride;
   const int si = species_idx[i];
@@ -293,22 +292,22 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
         fnl_apply_shift(vx, vy, vz, cl, nbr_shift[(size_t)i * max_nbr + e]);
       else if (pbc)
         fnl_min_image(vx, vy, vz, cl);
-      const float d = hw_sqrt(fmaxf(vx * vx + vy * vy + vz * vz, 1.0e-5f));
-      nbv[e] = make_float4(vx, vy, vz, d);
-      nbs[e] = species_idx[j];
+      nbv[e] = make_float4(vx, vy, vz, __int_as_float(species_idx[j]));
     }
-    // the angular row (radial slot numbers), fetched in the same round of loads
-    const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
-    for (int a = lane; a < na; a += 64) ae[a] = arow[a];
   }
+  // the angular row (radial slot numbers): entries < 64 in a register, fetched with the radial row
+  const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
+  const int ae0 = lane < na ? arow[lane] : 0;
+  auto ae_of = [&](int a) { return a < 64 ? ae0 : arow[a]; };  // a is the lane's own entry
+  auto dist = [](const float4& v) { return hw_sqrt(fmaxf(v.x * v.x + v.y * v.y + v.z * v.z, 1.0e-5f)); };
   wave_sync();
 
   int cnt[FNL_MAX_SPECIES], off[FNL_MAX_SPECIES];
   // radial slots sorted by species
   species_sort(
-      nr, S, [&](int e) { return nbs[e]; },
+      nr, S, [&](int e) { return __float_as_int(nbv[e].w); },
       [&](int e, int pos, int) {
-        const float d = nbv[e].w;
+        const float d = dist(nbv[e]);
         rdf[pos] = make_float2(d, 0.25f * cos_switch(d, p.cutoff));
       },
       cnt, off);
@@ -340,12 +339,13 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   // angular slots sorted by species: unit vector, distance, switch
   {
     species_sort(
-        na, S, [&](int a) { return nbs[ae[a]]; },
+        na, S, [&](int a) { return __float_as_int(nbv[ae_of(a)].w); },
         [&](int a, int pos, int sa) {
-          const float4 v = nbv[ae[a]];
-          const float inv = hw_rcp(fmaxf(v.w, 1.0e-5f));
-          av[pos] = make_float4(v.x * inv, v.y * inv, v.z * inv, v.w);
-          af[pos] = make_float2(cos_switch(v.w, p.cutoff_ang), __int_as_float(sa));
+          const float4 v = nbv[ae_of(a)];
+          const float d = dist(v);
+          const float inv = hw_rcp(fmaxf(d, 1.0e-5f));
+          av[pos] = make_float4(v.x * inv, v.y * inv, v.z * inv, d);
+          af[pos] = make_float2(cos_switch(d, p.cutoff_ang), __int_as_float(sa));
         },
         cnt, off);
   }
