@@ -512,7 +512,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
 
 // per-wave LDS footprint (4-byte words) of the adjoint kernel; gs starts 16-byte aligned
 __host__ __device__ inline int bwd_gs_offset(int max_nbr, int aev_len) {
-  return round4(round4(aev_len) + 6 * max_nbr);
+  return round4(round4(aev_len) + 4 * max_nbr);
 }
 __host__ __device__ inline int bwd_lds_words(int max_nbr, int max_nbr_ang, int aev_len) {
   return bwd_gs_offset(max_nbr, aev_len) + 12 * max_nbr_ang;
@@ -520,8 +520,7 @@ __host__ __device__ inline int bwd_lds_words(int max_nbr, int max_nbr_ang, int a
 
 // Adjoint, one wave per centre.  LDS slice of the wave (4-byte words):
 //   G   [aev_len]      dE/dAEV row of the centre
-//   nbv [max_nbr]      float4 (vx, vy, vz, d) per radial slot
-//   nbf [max_nbr]      float2 (fc(d), species)
+//   nbv [max_nbr]      float4 (vx, vy, vz, species bits) per radial slot
 //   gs  [max_nbr_ang]  float4 dE/dvec of the angular slot, angular terms only
 //                      (LDS float adds)
 //   av  [max_nbr_ang]  float4 (ux, uy, uz, d) per angular slot
@@ -534,7 +533,7 @@ __host__ __device__ inline int bwd_lds_words(int max_nbr, int max_nbr_ang, int a
 // Outputs: grad_self[i] = -sum_e dE_i/dvec_ie - sum_j dE_j/dvec_ji|radial;
 // grad_nbr[j] += dE_i/dvec_ij|angular; virial_atom[i] = sum_e vec_ie (x) dE_i/dvec_ie.
 template <int NA, int NZ, typename GT>
-__global__ void __launch_bounds__(64 * kWaves)
+__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(5)))
 k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
           const int32_t* __restrict__ batch_index, int n_atoms, int pbc_mode,
           const float* __restrict__ cells, const float* __restrict__ rec,
@@ -552,7 +551,6 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const int words = bwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
   float* G = (float*)lds4 + (size_t)w * words;
   float4* nbv = (float4*)(G + ((dm.aev_len + 3) & ~3));
-  float2* nbf = (float2*)(nbv + max_nbr);
   float4* gs = (float4*)(G + bwd_gs_offset(max_nbr, dm.aev_len));
   float4* av = gs + max_nbr_ang;
   float4* af = av + max_nbr_ang;
@@ -587,10 +585,9 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
         fnl_min_image(vx, vy, vz, cl);
       const float d2 = vx * vx + vy * vy + vz * vz;
       const float d = hw_sqrt(fmaxf(d2, 1.0e-5f));
-      nbv[e] = make_float4(vx, vy, vz, d);
       const float fc = cos_switch(d, p.cutoff);
       const int sj = species_idx[j];
-      nbf[e] = make_float2(fc, __int_as_float(sj));
+      nbv[e] = make_float4(vx, vy, vz, __int_as_float(sj));
       // radial: dE/dd = sum_k G[s, k] * 0.25 exp(-eta x^2) (fc' - 2 eta x fc), x = d - R_k,
       // for the centre's block of species sj (G_i) and j's block of species si (G_j)
       const float dfc = cos_switch_grad(d, p.cutoff);
@@ -620,11 +617,13 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     for (int a = lane; a < na; a += 64) {
       const int e = a < 64 ? ae0 : arow[a];
       const float4 vv = nbv[e];
-      const float inv = hw_rcp(fmaxf(vv.w, 1.0e-5f));
-      av[a] = make_float4(vv.x * inv, vv.y * inv, vv.z * inv, vv.w);
-      const bool ok = vv.x * vv.x + vv.y * vv.y + vv.z * vv.z > 1.0e-5f;
-      af[a] = make_float4(cos_switch(vv.w, p.cutoff_ang), ok ? cos_switch_grad(vv.w, p.cutoff_ang) : 0.0f,
-                          nbf[e].y, __int_as_float(row[e] | (ok ? 0 : (1 << 30))));
+      const float d2 = vv.x * vv.x + vv.y * vv.y + vv.z * vv.z;
+      const float d = hw_sqrt(fmaxf(d2, 1.0e-5f));
+      const float inv = hw_rcp(fmaxf(d, 1.0e-5f));
+      av[a] = make_float4(vv.x * inv, vv.y * inv, vv.z * inv, d);
+      const bool ok = d2 > 1.0e-5f;
+      af[a] = make_float4(cos_switch(d, p.cutoff_ang), ok ? cos_switch_grad(d, p.cutoff_ang) : 0.0f, vv.w,
+                          __int_as_float(row[e] | (ok ? 0 : (1 << 30))));
       gs[a] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
   }
