@@ -11,12 +11,13 @@
 //   forward   H_l = celu(W_l^T H_{l-1} + b_l)  -> LDS [M][N_l] (compute dtype)
 //   energy    E = H_3 . w4 + b4                 (fp32 accumulators)
 //   backward  G_3 = w4 * D_3 ; G_2 = (W_3 G_3) * D_2 ; G_1 = (W_2 G_2) * D_1
-//             dE/dAEV = W_1 G_1 -> global (fp32)
+//             dE/dAEV = W_1 G_1 -> global (fp32, or bf16 rows)
 // with D_l = celu'(Z_l) recovered from the stored activation:
 //   celu'(z) = 1 (z > 0), exp(z/alpha) = 1 + h/alpha (z <= 0).
 // G_l overwrites H_l in place.  Layer-1 inputs (AEV rows) are staged through
 // a double-buffered LDS area that aliases the H_2/H_3 buffers, which are free
-// until layer 2.  LDS stays within 80 KB so two workgroups share a CU.
+// until layer 2.  BF16 weights are stored in MFMA fragment order (one 1 KB
+// contiguous block per 16 x 32 A operand), so every weight load is whole cache lines.
 //
 // MFMA (wave64): FP32 v_mfma_f32_16x16x4_f32, BF16 v_mfma_f32_16x16x32_bf16.
 // A operand lane l: row (l%16), k-group (l/16); B operand lane l: column
@@ -88,10 +89,12 @@ struct Tr<unsigned short> {  // bf16 stored as raw bits
   static __device__ __forceinline__ f4 mma(frag a, frag b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
-  // weights: element (n, k) of a K-contiguous matrix p[n*ld + k]
+  // weights in fragment order: the (n0/16, k0/32) block of 16 x 32 elements of a matrix with ld
+  // columns is one contiguous 1 KB fragment, lane l's 8 elements at l*8 (element (n, k) at
+  // ((n/16)*(ld/32) + k/32)*512 + ((k%32)/8)*128 + (n%16)*8 + k%8): each load is 1 KB in full lines
   static __device__ __forceinline__ frag wload(const unsigned short* p, int ld, int n0, int k0) {
     const int l = fnl_lane();
-    return __builtin_bit_cast(bf8, *(const us8*)(p + (size_t)(n0 + (l & 15)) * ld + k0 + 8 * (l >> 4)));
+    return __builtin_bit_cast(bf8, *(const us8*)(p + ((size_t)(n0 >> 4) * (ld >> 5) + (k0 >> 5)) * 512 + l * 8));
   }
   static __device__ __forceinline__ frag aload(const unsigned short* buf, int ld, int m0, int k0) {
     const int l = fnl_lane();
@@ -180,6 +183,9 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
 #pragma unroll
           for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(act, lda, mb * 16, min(s + 1, nsteps - 1) * tr::KS);
         }
+        // keep the next step's LDS reads ahead of this step's MFMAs (the scheduler would otherwise
+        // sink them to their uses and expose the LDS latency with one wave per SIMD)
+        __builtin_amdgcn_sched_barrier(0);
         const int sn = min(s + PF, nsteps - 1);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
@@ -189,6 +195,7 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
           }
           wq[u][nb] = tr::wload(W, ldw, ncol[nb], sn * tr::KS);
         }
+        __builtin_amdgcn_sched_barrier(0);
         if constexpr (WV == 4) {
 #pragma unroll
           for (int mb = 0; mb < MB; ++mb) b[mb] = bn[mb];
@@ -301,8 +308,9 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     const int row = p * RPP + t / TPR;
     srow[p] = row < nrows ? perm[r0 + row] : -1;
   }
-  u4v xr[NPASS];
-  auto stage_load = [&](int kc) {
+  // two register sets: the chunk two ahead is in flight while the next one waits for its LDS buffer
+  u4v xa[NPASS], xb[NPASS];
+  auto stage_load = [&](u4v (&xr)[NPASS], int kc) {
 #pragma unroll
     for (int p = 0; p < NPASS; ++p) {
       const int rr = srow[p] >= 0 ? srow[p] : row_safe;  // a real row of the tile: the load is always issued
@@ -310,7 +318,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
       xr[p] = srow[p] >= 0 ? v : u4v{0u, 0u, 0u, 0u};
     }
   };
-  auto stage_store = [&](T* buf) {
+  auto stage_store = [&](const u4v (&xr)[NPASS], T* buf) {
 #pragma unroll
     for (int p = 0; p < NPASS; ++p) *(u4v*)(buf + (p * RPP + t / TPR) * L::LDX + scol) = xr[p];
   };
@@ -328,13 +336,17 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     for (int u = 0; u < PF; ++u)
 #pragma unroll
       for (int nb = 0; nb < NB1; ++nb) wq[u][nb] = tr::wload(W, ldw, (wave + WV * nb) * 16, u * tr::KS);
-    stage_load(0);
-    stage_store(Xs0);
-    __syncthreads();
     const int nchunks = K / KC;
-    for (int c = 0; c < nchunks; ++c) {
+    stage_load(xa, 0);
+    stage_load(xb, min(1, nchunks - 1) * KC);
+    stage_store(xa, Xs0);
+    __syncthreads();
+    // chunk c: computed from LDS buffer c&1; chunk c+1 in flight in one register set (stored at the
+    // end of the iteration), chunk c+2 issued into the other.  All loads are issued (clamped chunk
+    // index) so that the compiler can wait for the older set only.
+    auto chunk = [&](int c, u4v (&xnext)[NPASS], u4v (&xfree)[NPASS]) {
       const T* X = (c & 1) ? Xs1 : Xs0;
-      stage_load(min(c + 1, nchunks - 1) * KC);  // always issued (see gemm_t)
+      stage_load(xfree, min(c + 2, nchunks - 1) * KC);
       typename tr::frag b[MB];
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(X, L::LDX, mb * 16, 0);
@@ -349,12 +361,14 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(X, L::LDX, mb * 16, kn);
           }
+          __builtin_amdgcn_sched_barrier(0);  // next step's LDS reads stay ahead (see gemm_t)
 #pragma unroll
           for (int nb = 0; nb < NB1; ++nb) {
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb) acc1[nb][mb] = tr::mma(wq[u][nb], b[mb], acc1[nb][mb]);
             wq[u][nb] = tr::wload(W, ldw, (wave + WV * nb) * 16, min(s + PF, nsteps - 1) * tr::KS);
           }
+          __builtin_amdgcn_sched_barrier(0);
           if constexpr (WV == 4) {
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb) b[mb] = bn[mb];
@@ -364,8 +378,12 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
           }
         }
       }
-      if (c + 1 < nchunks) stage_store((c & 1) ? Xs0 : Xs1);
+      if (c + 1 < nchunks) stage_store(xnext, (c & 1) ? Xs0 : Xs1);
       __syncthreads();
+    };
+    for (int c = 0; c < nchunks; c += 2) {
+      chunk(c, xb, xa);
+      if (c + 1 < nchunks) chunk(c + 1, xa, xb);
     }
   }
   fwd_epilogue<T, NB1, MB, WV>(acc1, P.b[0], B1, L::LD1, wave, H1 / 16, alpha);

@@ -105,8 +105,20 @@ def _next_mult(x: int, m: int) -> int:
     return ((x + m - 1) // m) * m
 
 
+def fragment_order(a) -> np.ndarray:
+    """[R][C] matrix (R % 16 == 0, C % 32 == 0) in the BF16 MLP kernel's fragment order: blocks of
+    16 rows x 32 columns, block (r/16, c/32) at element offset ((r/16)*(C/32) + c/32)*512, inside
+    which element (r, c) sits at ((c%32)/8)*128 + (r%16)*8 + c%8 (one MFMA A operand per block,
+    each lane's 8 consecutive columns at lane*8)."""
+    a = np.asarray(a)
+    R, C = a.shape
+    if R % 16 or C % 32:
+        raise ValueError(f"fragment order needs rows % 16 == 0 and columns % 32 == 0, got {a.shape}")
+    return np.ascontiguousarray(a.reshape(R // 16, 16, C // 32, 4, 8).transpose(0, 2, 3, 1, 4)).reshape(R, C)
+
+
 class SpeciesNetworks:
-    """Per-species MLP weights on the device in both GEMM layouts.
+    """Per-species MLP weights on the device in both GEMM layouts (BF16: in fragment order).
 
     ``params``: {symbol: [(W [in, out], b [out]) x 4]} with widths
     A -> 256 -> 192 -> 160 -> 1 (FullyConnectedNet layers, nets.py:81-94).
@@ -142,8 +154,12 @@ class SpeciesNetworks:
             mats = [W1, np.asarray(layers[1][0]), np.asarray(layers[2][0])]
             net = self.cw.nets[n]
             for l, Wm in enumerate(mats):
-                w = torch.tensor(Wm, dtype=tdt, device=device).contiguous()
-                wt = torch.tensor(Wm.T, dtype=tdt, device=device).contiguous()
+                if dtype == "bf16":  # the kernel's MFMA operand layout
+                    w = torch.tensor(fragment_order(Wm), dtype=tdt, device=device).contiguous()
+                    wt = torch.tensor(fragment_order(Wm.T), dtype=tdt, device=device).contiguous()
+                else:
+                    w = torch.tensor(Wm, dtype=tdt, device=device).contiguous()
+                    wt = torch.tensor(Wm.T, dtype=tdt, device=device).contiguous()
                 self._keep += [w, wt]
                 setattr(net, f"w{l + 1}", w.data_ptr())
                 setattr(net, f"w{l + 1}t", wt.data_ptr())

@@ -258,6 +258,10 @@ int fnl_aev_tangent(const float* coords, const float* disp, const int32_t* speci
  *   w1t [H1][k_pad]  w2t [H2][H1]  w3t [H3][H2]   forward, K-contiguous
  *   w1  [k_pad][H1]  w2  [H1][H2]  w3  [H2][H3]   backward, N-contiguous
  *   w4 [H3], b1 [H1], b2 [H2], b3 [H3], b4 [1]   float32
+ * FP32 matrices are row-major as shown.  BF16 matrices ([R][C] as shown) are
+ * in MFMA fragment order: blocks of 16 rows x 32 columns, block (r/16, c/32)
+ * at element offset ((r/16)*(C/32) + c/32)*512, element (r, c) inside it at
+ * ((c%32)/8)*128 + (r%16)*8 + c%8 (k_pad a multiple of 128 for BF16).
  * Atoms are processed in tiles of rows of one network: tile t covers
  * perm[tile_start[t] .. tile_start[t] + tile_rows[t]) with network
  * tile_net[t]; rows per tile <= the kernel's tile size (tile_rows).

@@ -111,3 +111,21 @@ def test_nblist_schedule_matches_fennol_md():
     assert nblist_schedule(1.0, 0.5) == 40
     assert nblist_schedule(1.5, 0.7) == 42  # floor(0.75 * 40 / 0.7)
     assert nblist_schedule(2.0, 0.5, stride=10) == 10
+
+
+def test_bf16_fragment_order_layout():
+    """BF16 weights as the MLP kernel reads them: element (r, c) of an [R][C] matrix at
+    ((r/16)*(C/32) + c/32)*512 + ((c%32)/8)*128 + (r%16)*8 + c%8, element by element."""
+    from fennol_amd.potential import fragment_order
+
+    rng = np.random.default_rng(5)
+    R, C = 48, 96
+    a = rng.standard_normal((R, C))
+    flat = fragment_order(a).reshape(-1)
+    assert flat.shape == (R * C,)
+    for r in range(R):
+        for c in range(C):
+            off = ((r // 16) * (C // 32) + c // 32) * 512 + ((c % 32) // 8) * 128 + (r % 16) * 8 + c % 8
+            assert flat[off] == a[r, c]
+    with pytest.raises(ValueError):
+        fragment_order(np.zeros((20, 32)))
