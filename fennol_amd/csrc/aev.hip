@@ -91,7 +91,13 @@ __device__ __forceinline__ T to_out(float v);
 template <>
 __device__ __forceinline__ float to_out<float>(float v) { return v; }
 template <>
-__device__ __forceinline__ hip_bfloat16 to_out<hip_bfloat16>(float v) { return hip_bfloat16(v); }
+__device__ __forceinline__ hip_bfloat16 to_out<hip_bfloat16>(float v) {
+  // round to nearest even in one hardware conversion (v_cvt_pk_bf16_f32), the rounding of
+  // hip_bfloat16(float) for every finite value
+  hip_bfloat16 h;
+  h.data = __builtin_bit_cast(unsigned short, static_cast<__bf16>(v));
+  return h;
+}
 
 // dE/dAEV rows: float32, or bfloat16 bits (the BF16 form)
 __device__ __forceinline__ float g2f(float v) { return v; }

@@ -423,6 +423,27 @@ def test_energy_forces_bf16(gpu, name):
     assert f_rms <= 0.02
 
 
+def test_bf16_aev_rows_are_rounded_fp32_rows(gpu):
+    """The BF16 form's AEV rows are the FP32 rows rounded to nearest even (the same float
+    arithmetic, one conversion at the store): bit-exact against torch's rounding for normal
+    values, within the smallest normal for subnormal ones."""
+    _flush("[aev bf16 rounding] water216")
+    s = _cases()["water216"]
+    cfg = configs(("H", "C", "N", "O", "S", "F", "Cl"))
+    rows = {}
+    for prec in ("fp32", "bf16"):
+        _, pot = _potential(cfg, precision=prec)
+        sb, x = _nblist(pot, s)
+        pot.aev_forward(sb, x)
+        torch.cuda.synchronize()
+        rows[prec] = sb.aev[:, : cfg[1].aev_length].clone()
+    ref = rows["fp32"].to(torch.bfloat16)
+    got = rows["bf16"]
+    normal = rows["fp32"].abs() >= 1.1754944e-38
+    assert torch.equal(got[normal].view(torch.int16), ref[normal].view(torch.int16))
+    assert torch.all((got[~normal].float() - rows["fp32"][~normal]).abs() <= 1.1754944e-38)
+
+
 @pytest.mark.parametrize("precision,tiles", [("bf16", (32, 64, 96)), ("fp32", (16, 32))])
 def test_mlp_tile_sizes_agree(gpu, precision, tiles):
     """Every atom's MLP result depends only on its own row: all tile sizes give the same atom
