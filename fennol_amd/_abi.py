@@ -89,7 +89,7 @@ _SIGNATURES = {
     "fnl_aev_forward": (C.c_int, [vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, i32,
                                   C.POINTER(AevParams), vp, i32, i32, u32, vp]),
     "fnl_aev_backward": (C.c_int, [vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, i32,
-                                   C.POINTER(AevParams), vp, i32, i32, vp, vp, vp, vp]),
+                                   C.POINTER(AevParams), vp, i32, i32, vp, vp, vp, u32, vp]),
     "fnl_aev_tangent": (C.c_int, [vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, i32,
                                   C.POINTER(AevParams), vp, i32, vp]),
     "fnl_mlp_tile_rows": (C.c_int, [i32]),

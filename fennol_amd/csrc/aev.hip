@@ -103,23 +103,48 @@ __device__ __forceinline__ hip_bfloat16 to_out<hip_bfloat16>(float v) {
 __device__ __forceinline__ float g2f(float v) { return v; }
 __device__ __forceinline__ float g2f(unsigned short v) { return __uint_as_float((uint32_t)v << 16); }
 
+struct AevDims {
+  int S, P, nR, rad_len, aev_len;
+  uint64_t live;  // adjoint: channel blocks (radial s at bit s, angular pair p at bit S+p) read from g
+};
+
+// blocks of the species in species_mask and of the pairs of two such species (all blocks when
+// the mask is ~0u or a 4-channel chunk can straddle two radial blocks)
+inline uint64_t live_blocks(uint32_t species_mask, const AevDims& dm) {
+  if (species_mask == 0xFFFFFFFFu || dm.nR % 4 != 0) return ~0ull;
+  uint64_t live = 0;
+  int pr = 0;
+  for (int a = 0; a < dm.S; ++a) {
+    if ((species_mask >> a) & 1u) live |= 1ull << a;
+    for (int b = a; b < dm.S; ++b, ++pr)  // pair_index order: row-major upper triangle
+      if (((species_mask >> a) & 1u) && ((species_mask >> b) & 1u)) live |= 1ull << (dm.S + pr);
+  }
+  return live;
+}
+
+// 4-channel chunks of the blocks in dm.live (the others are never read: no neighbour or
+// neighbour pair of their species exists in the batch)
 template <typename GT>
-__device__ __forceinline__ void load_g_row(float* G, const GT* __restrict__ row, int n, int lane) {
+__device__ __forceinline__ void load_g_row(float* G, const GT* __restrict__ row, const AevDims& dm, int lane) {
+  const int n = dm.aev_len;
+  auto live = [&](int c) {
+    const int ch = 4 * c;
+    const int blk = ch < dm.rad_len ? ch / dm.nR : dm.S + ((ch - dm.rad_len) >> 5);
+    return ((dm.live >> blk) & 1ull) != 0;
+  };
   if constexpr (sizeof(GT) == 4) {
     const float4* r4 = (const float4*)row;
-    for (int c = lane; c < (n + 3) / 4; c += 64) ((float4*)G)[c] = r4[c];
+    for (int c = lane; c < (n + 3) / 4; c += 64)
+      if (live(c)) ((float4*)G)[c] = r4[c];
   } else {
     const ushort4* r4 = (const ushort4*)row;
     for (int c = lane; c < (n + 3) / 4; c += 64) {
+      if (!live(c)) continue;
       const ushort4 u = r4[c];
       ((float4*)G)[c] = make_float4(g2f(u.x), g2f(u.y), g2f(u.z), g2f(u.w));
     }
   }
 }
-
-struct AevDims {
-  int S, P, nR, rad_len, aev_len;
-};
 
 __device__ __forceinline__ int pair_index(int a, int b, int S) {
   // row-major upper triangle incl. diagonal of an S x S table (ani.py:124-130)
@@ -575,7 +600,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   // the first 64 entries of the angular row, fetched with the dE/dAEV and radial rows
   const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
   const int ae0 = lane < na ? arow[lane] : 0;
-  load_g_row(G, g + (size_t)i * g_stride, dm.aev_len, lane);
+  load_g_row(G, g + (size_t)i * g_stride, dm, lane);
   const int32_t* row = nbr + (size_t)i * max_nbr;
   // per-lane partial sums: dE/dx_i (self and gathered radial neighbour terms) and the virial
   float sx = 0.f, sy = 0.f, sz = 0.f;
@@ -935,6 +960,7 @@ bool check_params(const fnl_aev_params* p, AevDims* dm) {
   dm->nR = p->n_radial;
   dm->rad_len = dm->S * dm->nR;
   dm->aev_len = dm->rad_len + dm->P * p->n_ang_dist * p->n_ang_sect;
+  dm->live = ~0ull;
   return true;
 }
 
@@ -1045,9 +1071,10 @@ extern "C" int fnl_aev_backward(const float* coords, const int32_t* species_idx,
                                 const int32_t* n_nbr_ang,
                                 int32_t max_nbr_ang, const fnl_aev_params* params, const void* g,
                                 int32_t g_stride, int32_t g_dtype, float* grad_self, float* grad_nbr,
-                                float* virial_atom, fnl_stream_t stream) {
+                                float* virial_atom, uint32_t species_mask, fnl_stream_t stream) {
   AevDims dm;
   if (!check_params(params, &dm)) return FNL_ERR_INVALID;
+  dm.live = live_blocks(species_mask, dm);
   if (!common_args_ok(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells, nbr,
                       n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang))
     return FNL_ERR_INVALID;

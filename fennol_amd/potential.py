@@ -633,7 +633,7 @@ class ANIPotential:
             P(coords32), P(sb.species_idx), P(sb.batch_index), sb.n_atoms, sb.pbc, P(sb.cells), P(sb.rcells),
             P(sb.nbr), P(sb.n_nbr), sb.max_nbr, P(sb.nbr_shift), P(sb.nbr_ang), P(sb.n_nbr_ang), sb.max_nbr_ang,
             C.byref(self.cparams), P(sb.g_aev), sb.g_aev.shape[1], self._row_dtype, P(sb.grad_self), P(sb.grad_nbr),
-            P(sb.vir_atom), s)
+            P(sb.vir_atom), sb.extra.get("aev_mask", 0xFFFFFFFF), s)
         _abi.check(rc, "fnl_aev_backward")
 
     def reduce(self, sb: SystemBuffers, with_virial: bool = True, stream=None) -> None:

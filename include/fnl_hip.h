@@ -219,7 +219,11 @@ int fnl_aev_forward(const float* coords, const int32_t* species_idx,
  *   virial_atom[i] = sum_j vec_ij (x) dE_i/dvec_ij   (float32[N][9], row-major a,b)
  * so that dE/dx = grad_self + grad_nbr (only the sum is meaningful) and the
  * per-system virial dE/dstrain (fennix.py:320-347) is the per-system sum of
- * virial_atom.  The neighbour lists must be symmetric (as built). */
+ * virial_atom.  The neighbour lists must be symmetric (as built).
+ * species_mask: as for fnl_aev_forward (every species present in the batch);
+ * only the channel blocks of those species and of their pairs are read from
+ * g, the others are not used by any atom of the batch.  ~0u reads every
+ * block. */
 int fnl_aev_backward(const float* coords, const int32_t* species_idx,
                      const int32_t* batch_index, int32_t n_atoms, int32_t pbc_mode,
                      const float* cells, const float* reciprocal_cells,
@@ -229,7 +233,7 @@ int fnl_aev_backward(const float* coords, const int32_t* species_idx,
                      const fnl_aev_params* params,
                      const void* g, int32_t g_stride, int32_t g_dtype,
                      float* grad_self, float* grad_nbr, float* virial_atom,
-                     fnl_stream_t stream);
+                     uint32_t species_mask, fnl_stream_t stream);
 
 /* Tangent (forward-mode derivative) of the AEV along a displacement field
  * disp (float32[N][3]):  daev_i = sum_e dAEV_i/dvec_e . (disp_j - disp_i).

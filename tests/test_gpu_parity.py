@@ -444,6 +444,29 @@ def test_bf16_aev_rows_are_rounded_fp32_rows(gpu):
     assert torch.all((got[~normal].float() - rows["fp32"][~normal]).abs() <= 1.1754944e-38)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_aev_backward_species_mask(gpu, precision):
+    """The adjoint reading only the dE/dAEV blocks of the species present (water in a 7-species
+    model: 5 of 35 blocks) gives the result of reading every block."""
+    _flush(f"[aev bwd mask] water216 {precision}")
+    s = _cases()["water216"]
+    cfg = configs(("H", "C", "N", "O", "S", "F", "Cl"))
+    _, pot = _potential(cfg, precision=precision)
+    sb, x = _nblist(pot, s)
+    pot.aev_forward(sb, x)
+    pot.mlp(sb, with_grad=True)
+    out = {}
+    for mask in (sb.extra["aev_mask"], 0xFFFFFFFF):
+        sb.extra["aev_mask"] = mask
+        pot.aev_backward(sb, x)
+        torch.cuda.synchronize()
+        out[mask] = [t.clone() for t in (sb.grad_self, sb.grad_nbr, sb.vir_atom)]
+    part, full = out.values()
+    assert bin(list(out)[0]).count("1") == 2
+    assert torch.equal(part[0], full[0]) and torch.equal(part[2], full[2])
+    torch.testing.assert_close(part[1], full[1], rtol=1e-6, atol=1e-6)  # atomic order
+
+
 @pytest.mark.parametrize("precision,tiles", [("bf16", (32, 64, 96)), ("fp32", (16, 32))])
 def test_mlp_tile_sizes_agree(gpu, precision, tiles):
     """Every atom's MLP result depends only on its own row: all tile sizes give the same atom
