@@ -11,11 +11,12 @@
 // Pipeline (all on the caller's stream):
 //   k_grid      one workgroup per system: grid dims (+ bounding box if open);
 //               clears the cell counters
-//   k_bin       one thread per atom: cell id, population count
-//   k_scan      one workgroup: exclusive scan of cell populations
-//   k_scatter   one thread per atom: place atom in its cell segment
-//   k_cell_sort one wave per cell: order each segment by atom index
-//               (deterministic rows) and write cell-sorted float4 positions
+//   k_bin       one thread per atom: cell id, slot in the cell (population
+//               count), atom staged at its cell's slot; the last workgroup
+//               scans the populations into cell offsets
+//   k_cell_sort one wave per cell: the staged atoms ordered by index
+//               (deterministic rows) into the cell's segment, with
+//               cell-sorted float4 positions
 //   k_pairs     one wave per centre, centres in cell order: candidates of the
 //               neighbouring cells (contiguous z-runs of the cell-sorted
 //               arrays) tested 64 at a time, kept pairs compacted with a wave
@@ -32,6 +33,7 @@
 namespace {
 
 constexpr int kGridWords = 12;
+constexpr int kCellCap = 64;    // atoms per cell staged by k_bin (more: k_cell_sort gathers them)
 constexpr int kCmap = 640;      // candidates per centre mapped through LDS (more: the search below)
 constexpr int kMaxRanges = 18;  // 3 x 3 columns x at most 2 z-runs  // per system: nx ny nz first | ox oy oz | iwx iwy iwz | pbc pad
 
@@ -39,9 +41,9 @@ struct NblistWs {
   int32_t* grid;       // [n_sys][kGridWords] (floats stored as bits)
   int32_t* atom_cell;  // [N]
   int32_t* cell_count; // [n_cells]
-  int32_t* cell_fill;  // [n_cells]
   int32_t* cell_start; // [n_cells + 1]
-  int32_t* tmp_sorted; // [N]
+  int32_t* stage;      // [n_cells][kCellCap] atoms in binning order
+  unsigned* done;      // completion count of k_bin (left at zero)
   int32_t* sorted;     // [N]
   float4* pos;         // [N] cell-sorted x, y, z, atom index bits
 };
@@ -56,18 +58,18 @@ size_t ws_layout(int n_atoms, int n_sys, int n_cells, char* base, NblistWs* ws) 
   size_t o_grid = take((size_t)n_sys * kGridWords * 4);
   size_t o_ac = take((size_t)n_atoms * 4);
   size_t o_cc = take((size_t)n_cells * 4);
-  size_t o_cf = take((size_t)n_cells * 4);
   size_t o_cs = take((size_t)(n_cells + 1) * 4);
-  size_t o_ts = take((size_t)n_atoms * 4);
+  size_t o_st = take((size_t)n_cells * kCellCap * 4);
+  size_t o_dn = take(4);
   size_t o_s = take((size_t)n_atoms * 4);
   size_t o_p = take((size_t)n_atoms * 16);
   if (base && ws) {
     ws->grid = (int32_t*)(base + o_grid);
     ws->atom_cell = (int32_t*)(base + o_ac);
     ws->cell_count = (int32_t*)(base + o_cc);
-    ws->cell_fill = (int32_t*)(base + o_cf);
     ws->cell_start = (int32_t*)(base + o_cs);
-    ws->tmp_sorted = (int32_t*)(base + o_ts);
+    ws->stage = (int32_t*)(base + o_st);
+    ws->done = (unsigned*)(base + o_dn);
     ws->sorted = (int32_t*)(base + o_s);
     ws->pos = (float4*)(base + o_p);
   }
@@ -99,14 +101,12 @@ __global__ void k_grid(const float* __restrict__ coords, const int32_t* __restri
                        const int32_t* __restrict__ sys_offsets, int pbc_mode,
                        const float* __restrict__ rec, const int32_t* __restrict__ grid_cap,
                        float cutoff, int32_t* __restrict__ grid, int32_t* __restrict__ status,
-                       int32_t* __restrict__ cell_count, int32_t* __restrict__ cell_fill, int n_cells) {
+                       int32_t* __restrict__ cell_count, unsigned* __restrict__ done, int n_cells) {
   const int b = blockIdx.x;
   const int t = threadIdx.x;
   if (b == 0 && t < FNL_STATUS_STICKY) status[t] = 0;  // the sticky word persists
-  for (int c = b * blockDim.x + t; c < n_cells; c += gridDim.x * blockDim.x) {
-    cell_count[c] = 0;
-    cell_fill[c] = 0;
-  }
+  if (b == 0 && t == 0) *done = 0u;
+  for (int c = b * blockDim.x + t; c < n_cells; c += gridDim.x * blockDim.x) cell_count[c] = 0;
   const int capx = grid_cap[b * 4 + 0], capy = grid_cap[b * 4 + 1], capz = grid_cap[b * 4 + 2];
   const int first = grid_cap[b * 4 + 3];
   int32_t* g = grid + b * kGridWords;
@@ -344,96 +344,132 @@ k_pairs_general(const int32_t* __restrict__ species_idx, const int32_t* __restri
   }
 }
 
-__global__ void k_bin(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
+// One thread per atom: cell id and slot (population count), the atom index
+// staged at stage[cell * kCellCap + slot] while slot < kCellCap.  The last
+// workgroup to finish (completion count `done`, left at zero) turns the
+// populations into the exclusive prefix sum cell_start and records the
+// largest population.
+__global__ void __launch_bounds__(256) k_bin(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
                       const int32_t* __restrict__ batch_index, int n_atoms,
                       const float* __restrict__ rec, const int32_t* __restrict__ grid,
                       int32_t* __restrict__ atom_cell, int32_t* __restrict__ cell_count,
                       int32_t* __restrict__ n_nbr, int32_t* __restrict__ n_nbr_ang, int n_cells,
-                      int32_t* __restrict__ status) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_atoms) return;
-  if (species_idx[i] < 0) {
-    atom_cell[i] = -1;
-    n_nbr[i] = 0;
-    if (n_nbr_ang) n_nbr_ang[i] = 0;
-    return;
-  }
-  const int b = batch_index[i];
-  const int32_t* g = grid + b * kGridWords;
-  const int3 c = atom_cell3(coords, i, g, rec, b);
-  const int cell = g[3] + (c.x * g[1] + c.y) * g[2] + c.z;
-  if (cell < 0 || cell >= n_cells) {  // inconsistent grid capacities: drop and flag
-    atom_cell[i] = -1;
-    n_nbr[i] = 0;
-    if (n_nbr_ang) n_nbr_ang[i] = 0;
-    atomicOr(&status[FNL_STATUS_OVERFLOW], 2);
-    atomicOr(&status[FNL_STATUS_STICKY], 2);
-    return;
-  }
-  atom_cell[i] = cell;
-  atomicAdd(&cell_count[cell], 1);
-}
-
-// single-workgroup exclusive scan (1024 threads, chunked)
-__global__ void __launch_bounds__(1024) k_scan(const int32_t* __restrict__ count, int n,
-                                              int32_t* __restrict__ start, int32_t* __restrict__ status) {
-  __shared__ int wsum[16];
-  __shared__ int carry;
+                      int32_t* __restrict__ status, int32_t* __restrict__ stage, unsigned* __restrict__ done,
+                      int32_t* __restrict__ cell_start) {
   const int t = threadIdx.x;
-  const int lane = t & 63, w = t >> 6;
-  if (t == 0) carry = 0;
-  int cmax = 0;
+  const int i = blockIdx.x * blockDim.x + t;
+  if (i < n_atoms) {
+    int cell = -1;
+    if (species_idx[i] >= 0) {
+      const int b = batch_index[i];
+      const int32_t* g = grid + b * kGridWords;
+      const int3 c = atom_cell3(coords, i, g, rec, b);
+      cell = g[3] + (c.x * g[1] + c.y) * g[2] + c.z;
+      if (cell < 0 || cell >= n_cells) {  // inconsistent grid capacities: drop and flag
+        cell = -1;
+        atomicOr(&status[FNL_STATUS_OVERFLOW], 2);
+        atomicOr(&status[FNL_STATUS_STICKY], 2);
+      }
+    }
+    atom_cell[i] = cell;
+    if (cell < 0) {
+      n_nbr[i] = 0;
+      if (n_nbr_ang) n_nbr_ang[i] = 0;
+    } else {
+      const int slot = atomicAdd(&cell_count[cell], 1);
+      if (slot < kCellCap) stage[(size_t)cell * kCellCap + slot] = i;
+    }
+  }
+  // completion count: the last workgroup sees every population
+  __shared__ bool last;
+  __shared__ int wsum[4];
+  __shared__ int carry;
   __syncthreads();
-  for (int base = 0; base < n; base += 1024) {
-    const int idx = base + t;
-    const int v = idx < n ? count[idx] : 0;
-    cmax = max(cmax, v);
-    int incl = fnl_wave_incl_scan(v);
+  if (t == 0) {
+    __threadfence();
+    last = atomicAdd(done, 1u) == gridDim.x - 1;
+    carry = 0;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // exclusive scan in chunks of 1024 cells, 4 consecutive cells per thread
+  const int lane = t & 63, w = t >> 6;
+  int cmax = 0;
+  for (int base = 0; base < n_cells; base += 1024) {
+    const int idx = base + 4 * t;
+    int4 v = make_int4(0, 0, 0, 0);
+    if (idx + 3 < n_cells) {
+      v = *(const int4*)(cell_count + idx);
+    } else {
+      if (idx < n_cells) v.x = cell_count[idx];
+      if (idx + 1 < n_cells) v.y = cell_count[idx + 1];
+      if (idx + 2 < n_cells) v.z = cell_count[idx + 2];
+    }
+    cmax = max(cmax, max(max(v.x, v.y), max(v.z, v.w)));
+    const int tot = v.x + v.y + v.z + v.w;
+    const int incl = fnl_wave_incl_scan(tot);
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    if (w == 0) {
-      int s = lane < 16 ? wsum[lane] : 0;
-      int si = fnl_wave_incl_scan(s);
-      if (lane < 16) wsum[lane] = si - s;
-    }
+    int before = carry;
+    for (int q = 0; q < w; ++q) before += wsum[q];
+    const int e0 = before + incl - tot;  // exclusive start of this thread's 4 cells
+    if (idx < n_cells) cell_start[idx] = e0;
+    if (idx + 1 < n_cells) cell_start[idx + 1] = e0 + v.x;
+    if (idx + 2 < n_cells) cell_start[idx + 2] = e0 + v.x + v.y;
+    if (idx + 3 < n_cells) cell_start[idx + 3] = e0 + v.x + v.y + v.z;
     __syncthreads();
-    const int excl = carry + wsum[w] + incl - v;
-    if (idx < n) start[idx] = excl;
-    __syncthreads();
-    if (t == 1023) carry = excl + v;
+    if (t == 255) carry = before + incl;
     __syncthreads();
   }
-  if (t == 0) start[n] = carry;
+  if (t == 0) {
+    cell_start[n_cells] = carry;
+    *done = 0u;
+  }
   cmax = fnl_wave_max_i(cmax);
   if (lane == 0 && cmax > 0) atomicMax(&status[FNL_STATUS_MAX_CELL], cmax);
 }
 
-__global__ void k_scatter(int n_atoms, const int32_t* __restrict__ atom_cell,
-                          const int32_t* __restrict__ cell_start, int32_t* __restrict__ cell_fill,
-                          int32_t* __restrict__ tmp_sorted) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_atoms) return;
-  const int c = atom_cell[i];
-  if (c < 0) return;
-  const int p = cell_start[c] + atomicAdd(&cell_fill[c], 1);
-  tmp_sorted[p] = i;
-}
-
-// one wave per cell: rank sort of the segment (atom indices are distinct)
+// One wave per cell: the staged atoms of the cell ranked by index (rows come
+// out in a fixed order), written to the cell's segment of sorted[] with their
+// float4 positions.  A cell holding more than kCellCap atoms gathers its atoms
+// instead by scanning its system's atoms in index order.
 __global__ void k_cell_sort(int n_cells, const int32_t* __restrict__ cell_start,
-                            const int32_t* __restrict__ tmp_sorted, const float* __restrict__ coords,
+                            const int32_t* __restrict__ stage, const float* __restrict__ coords,
+                            const int32_t* __restrict__ atom_cell, const int32_t* __restrict__ grid,
+                            const int32_t* __restrict__ sys_offsets, int n_sys,
                             int32_t* __restrict__ sorted, float4* __restrict__ pos) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = fnl_lane();
-  if (wave >= n_cells) return;
-  const int s = cell_start[wave], e = cell_start[wave + 1];
-  const int cnt = e - s;
-  for (int a = lane; a < cnt; a += 64) {
-    const int v = tmp_sorted[s + a];
+  if (c >= n_cells) return;
+  const int s = cell_start[c], cnt = cell_start[c + 1] - s;
+  if (cnt <= kCellCap) {
+    const int v = lane < cnt ? stage[(size_t)c * kCellCap + lane] : 0x7fffffff;
     int rank = 0;
-    for (int q = 0; q < cnt; ++q) rank += (tmp_sorted[s + q] < v) ? 1 : 0;
-    sorted[s + rank] = v;
-    pos[s + rank] = make_float4(coords[3 * v], coords[3 * v + 1], coords[3 * v + 2], __int_as_float(v));
+    for (int q = 0; q < cnt; ++q) rank += __shfl(v, q, 64) < v ? 1 : 0;
+    if (lane < cnt) {
+      sorted[s + rank] = v;
+      pos[s + rank] = make_float4(coords[3 * v], coords[3 * v + 1], coords[3 * v + 2], __int_as_float(v));
+    }
+    return;
+  }
+  // the system whose cell range holds c: largest b with first cell <= c
+  int lo = 0, hi = n_sys - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (grid[mid * kGridWords + 3] <= c) lo = mid; else hi = mid - 1;
+  }
+  int k = 0;
+  for (int a0 = sys_offsets[lo]; a0 < sys_offsets[lo + 1]; a0 += 64) {
+    const int a = a0 + lane;
+    const bool mine = a < sys_offsets[lo + 1] && atom_cell[a] == c;
+    const unsigned long long m = __ballot(mine);
+    if (mine) {
+      const int r = k + fnl_rank_below(m);
+      sorted[s + r] = a;
+      pos[s + r] = make_float4(coords[3 * a], coords[3 * a + 1], coords[3 * a + 2], __int_as_float(a));
+    }
+    k += __popcll(m);
   }
 }
 
@@ -822,7 +858,7 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
   const float rca2 = nbr_ang ? (float)(cutoff_ang * cutoff_ang) : -1.0f;
 
   hipLaunchKernelGGL(k_grid, dim3(n_sys), dim3(256), 0, s, coords, species_idx, sys_offsets, pbc_mode,
-                     reciprocal_cells, grid, (float)cutoff, ws.grid, status, ws.cell_count, ws.cell_fill, n_cells);
+                     reciprocal_cells, grid, (float)cutoff, ws.grid, status, ws.cell_count, ws.done, n_cells);
   FNL_LAUNCH_CHECK();
   if (n_atoms == 0) return FNL_OK;
   constexpr int W = 4;
@@ -837,15 +873,11 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
   } else {
     hipLaunchKernelGGL(k_bin, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, s, coords, species_idx,
                        batch_index, n_atoms, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_count,
-                       n_nbr, n_nbr_ang, n_cells, status);
-    FNL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, ws.cell_count, n_cells, ws.cell_start, status);
-    FNL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scatter, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, s, n_atoms, ws.atom_cell,
-                       ws.cell_start, ws.cell_fill, ws.tmp_sorted);
+                       n_nbr, n_nbr_ang, n_cells, status, ws.stage, ws.done, ws.cell_start);
     FNL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_cell_sort, dim3(fnl_blocks((long long)n_cells * 64, 256)), dim3(256), 0, s, n_cells,
-                       ws.cell_start, ws.tmp_sorted, coords, ws.sorted, ws.pos);
+                       ws.cell_start, ws.stage, coords, ws.atom_cell, ws.grid, sys_offsets, n_sys, ws.sorted,
+                       ws.pos);
     FNL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_pairs<W>, dim3(fnl_blocks(n_atoms, W)), dim3(64 * W), 0, s, coords, batch_index,
                        n_atoms, cells, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_start, ws.sorted,

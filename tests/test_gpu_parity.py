@@ -109,6 +109,23 @@ def test_nblist_batched_clusters(gpu):
     np.testing.assert_array_equal(half, ref)
 
 
+def test_nblist_crowded_cells(gpu):
+    """Cells with more than 64 atoms (the binning kernel's staging capacity): the cell sort
+    gathers their atoms from the system's range instead; same pair set as the oracle."""
+    _flush("[nblist] crowded cells")
+    rng = np.random.Generator(np.random.PCG64(9))
+    sysl = [systems.lattice_cluster(int(n), rng) for n in (40, 1000, 64)]
+    s = systems.batch_systems(sysl)
+    cfg = configs()
+    _, pot = _potential(cfg)
+    sb, x = _nblist(pot, s)
+    assert int(sb.status[_abi.FNL_STATUS_MAX_CELL].item()) > 64
+    i, j = rows_to_pairs(sb.nbr.cpu().numpy(), sb.n_nbr.cpu().numpy(), sb.max_nbr)
+    half = np.array(sorted((a, b) for a, b in zip(i.tolist(), j.tolist()) if a < b), dtype=np.int64)
+    ref, _ = canonical_pairs_oracle(s["coordinates"], s["natoms"], cfg[1].cutoff)
+    np.testing.assert_array_equal(half, ref)
+
+
 def test_nblist_near_cutoff(gpu):
     """Pairs placed within a few ulp of the cutoff keep the reference's decision."""
     _flush("[nblist] near cutoff")
