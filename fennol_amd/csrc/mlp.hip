@@ -1,7 +1,7 @@
 // Species MLPs (SpeciesIndexNet over FullyConnectedNet, models/misc/nets.py:
 // 13-102, 301-426) as one fused forward + input-gradient kernel on MFMA.
 //
-// One workgroup (4 waves) per tile of M atoms that share a network.  Every
+// One workgroup (4 waves; 8 for 96-atom BF16 tiles) per tile of M atoms that share a network.  Every
 // GEMM is computed transposed, out^T = W^T act^T: the weights are the MFMA A
 // operand (streamed from global/L2, prefetched PF k-steps ahead in registers),
 // the activations the B operand (read from LDS), so that each lane's result
@@ -131,7 +131,8 @@ struct Lds {
   static constexpr int OFF2 = OFF1 + M * LD1 * (int)sizeof(T);
   static constexpr int OFF3 = OFF2 + M * LD2 * (int)sizeof(T);
   static constexpr int OFFE = OFF3 + M * LD3 * (int)sizeof(T);
-  static constexpr int BYTES = OFFE + WV_ * M * 4;  // + per-wave energy partials
+  static constexpr int BYTES = OFFE;
+  static_assert((H3 / 16) * M * 4 <= M * LD3 * (int)sizeof(T), "energy partials fit the H3 buffer");
   static constexpr int XBUF = M * LDX * (int)sizeof(T);  // one staging buffer, at OFF2
   static_assert(2 * XBUF <= OFFE - OFF2, "layer-1 staging must fit in the H2/H3 buffers");
   static_assert(BYTES <= 160 * 1024, "fits the LDS of one CU");
@@ -208,8 +209,8 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
   }
 }
 
-// forward epilogue: h = celu(acc + b) -> acc and LDS [m][n]
-template <typename T, int NB, int MB, int WV>
+// forward epilogue: h = celu(acc + b) -> acc and (STORE) LDS [m][n]
+template <typename T, int NB, int MB, int WV, bool STORE = true>
 __device__ __forceinline__ void fwd_epilogue(f4 (&acc)[NB][MB], const float* __restrict__ bias, T* H, int ldh,
                                              int nb0, int ncb, float alpha) {
   const int l = fnl_lane();
@@ -226,7 +227,7 @@ __device__ __forceinline__ void fwd_epilogue(f4 (&acc)[NB][MB], const float* __r
 #pragma unroll
       for (int r = 0; r < 4; ++r) v[r] = v[r] > 0.0f ? v[r] : alpha * (__expf(v[r] * ia) - 1.0f);
       acc[nb][mb] = v;
-      Tr<T>::store4(H + (mb * 16 + (l & 15)) * ldh + n, v);
+      if constexpr (STORE) Tr<T>::store4(H + (mb * 16 + (l & 15)) * ldh + n, v);
     }
   }
 }
@@ -275,7 +276,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   T* B1 = (T*)(smem + L::OFF1);
   T* B2 = (T*)(smem + L::OFF2);
   T* B3 = (T*)(smem + L::OFF3);
-  float* Epart = (float*)(smem + L::OFFE);  // [WV][M]
+  float* Epart = (float*)(smem + L::OFF3);  // [H3/16][M], in the H3 buffer (h3 is kept in registers)
   T* Xs0 = B2;
   T* Xs1 = (T*)(smem + L::OFF2 + L::XBUF);
 
@@ -402,37 +403,35 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   f4 acc3[NB3][MB];
   zero_acc(acc3);
   gemm_t<T, NB3, MB, PF, WV>(acc3, (const T*)P.fw[2], sizeof(T) == 2 ? H2 : H3, wave, H3 / 16, B2, L::LD2, H2);
-  fwd_epilogue<T, NB3, MB, WV>(acc3, P.b[2], B3, L::LD3, wave, H3 / 16, alpha);
+  // h3 stays in registers (energy and G3 are formed from acc3); B3 holds the energy partials
+  fwd_epilogue<T, NB3, MB, WV, false>(acc3, P.b[2], B3, L::LD3, wave, H3 / 16, alpha);
   {
-    float part[MB];
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb) part[mb] = 0.0f;
+    // H3 . w4 per 16-feature block into Epart[block][atom], then summed over the blocks in order:
+    // the same arithmetic for every tile size and wave count
 #pragma unroll
     for (int nb = 0; nb < NB3; ++nb) {
       const int cb = wave + WV * nb;
       if (cb >= H3 / 16) continue;
       const f4 wv = *(const f4*)(P.w4 + cb * 16 + 4 * (l >> 4));
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb)
-        part[mb] += acc3[nb][mb][0] * wv[0] + acc3[nb][mb][1] * wv[1] + acc3[nb][mb][2] * wv[2] +
-                    acc3[nb][mb][3] * wv[3];
-    }
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb) {
-      float v = part[mb];
-      v += __shfl_xor(v, 16, 64);
-      v += __shfl_xor(v, 32, 64);
-      if (l < 16) Epart[wave * M + mb * 16 + l] = v;
+      for (int mb = 0; mb < MB; ++mb) {
+        float v = acc3[nb][mb][0] * wv[0] + acc3[nb][mb][1] * wv[1] + acc3[nb][mb][2] * wv[2] +
+                  acc3[nb][mb][3] * wv[3];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (l < 16) Epart[cb * M + mb * 16 + l] = v;
+      }
     }
   }
   __syncthreads();
   if (t < nrows) {
     float e = 0.0f;
 #pragma unroll
-    for (int w2 = 0; w2 < WV; ++w2) e += Epart[w2 * M + t];
+    for (int cb = 0; cb < H3 / 16; ++cb) e += Epart[cb * M + t];
     energy[perm[r0 + t]] = e + P.b[3][0];
   }
   if (grad == nullptr) return;
+  __syncthreads();  // energy partials read: B3 takes G3
 
   // ---------------- backward ----------------
   {
@@ -580,8 +579,8 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
     hipLaunchKernelGGL((k_species_mlp<unsigned short, 64, 2, 4>), dim3(grid), dim3(256), 0, st,
                        (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
                        energy, grad, grad_stride, gbf);
-  } else if (bf && M == 96) {
-    hipLaunchKernelGGL((k_species_mlp<unsigned short, 96, 2, 4>), dim3(grid), dim3(256), 0, st,
+  } else if (bf && M == 96) {  // eight waves: two per SIMD cover each other's load latency
+    hipLaunchKernelGGL((k_species_mlp<unsigned short, 96, 2, 8>), dim3(grid), dim3(512), 0, st,
                        (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
                        energy, grad, grad_stride, gbf);
   } else if (bf && M == 32) {
