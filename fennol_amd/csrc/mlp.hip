@@ -177,15 +177,12 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
     for (int u = 0; u < PF; ++u) {
       const int s = s0 + u;
       if (s < nsteps) {
-        // activations of the next k-step, read before this step's MFMAs (with 4 waves per
-        // workgroup; with 8, the other wave on the SIMD covers the LDS latency)
+        // activations of the next k-step, read before this step's MFMAs
         typename tr::frag bn[MB];
-        if constexpr (WV == 4) {
 #pragma unroll
-          for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(act, lda, mb * 16, min(s + 1, nsteps - 1) * tr::KS);
-        }
+        for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(act, lda, mb * 16, min(s + 1, nsteps - 1) * tr::KS);
         // keep the next step's LDS reads ahead of this step's MFMAs (the scheduler would otherwise
-        // sink them to their uses and expose the LDS latency with one wave per SIMD)
+        // sink them to their uses and expose the LDS latency)
         __builtin_amdgcn_sched_barrier(0);
         const int sn = min(s + PF, nsteps - 1);
 #pragma unroll
@@ -197,13 +194,8 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
           wq[u][nb] = tr::wload(W, ldw, ncol[nb], sn * tr::KS);
         }
         __builtin_amdgcn_sched_barrier(0);
-        if constexpr (WV == 4) {
 #pragma unroll
-          for (int mb = 0; mb < MB; ++mb) b[mb] = bn[mb];
-        } else if (s + 1 < nsteps) {
-#pragma unroll
-          for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(act, lda, mb * 16, (s + 1) * tr::KS);
-        }
+        for (int mb = 0; mb < MB; ++mb) b[mb] = bn[mb];
       }
     }
   }
@@ -358,10 +350,8 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
           // activations of the next k-step of the chunk, read before this step's MFMAs
           typename tr::frag bn[MB];
           const int kn = min(s0 + u + 1, SPC - 1) * tr::KS;
-          if constexpr (WV == 4) {
 #pragma unroll
-            for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(X, L::LDX, mb * 16, kn);
-          }
+          for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(X, L::LDX, mb * 16, kn);
           __builtin_amdgcn_sched_barrier(0);  // next step's LDS reads stay ahead (see gemm_t)
 #pragma unroll
           for (int nb = 0; nb < NB1; ++nb) {
@@ -370,13 +360,8 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
             wq[u][nb] = tr::wload(W, ldw, (wave + WV * nb) * 16, min(s + PF, nsteps - 1) * tr::KS);
           }
           __builtin_amdgcn_sched_barrier(0);
-          if constexpr (WV == 4) {
 #pragma unroll
-            for (int mb = 0; mb < MB; ++mb) b[mb] = bn[mb];
-          } else {
-#pragma unroll
-            for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(X, L::LDX, mb * 16, kn);
-          }
+          for (int mb = 0; mb < MB; ++mb) b[mb] = bn[mb];
         }
       }
       if (c + 1 < nchunks) stage_store(xnext, (c & 1) ? Xs0 : Xs1);
