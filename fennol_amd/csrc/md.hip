@@ -157,9 +157,8 @@ k_baoab(double* __restrict__ x, double* __restrict__ v, float* __restrict__ x32,
     }
   }
   if (step_dev) {
-    __syncthreads();  // every thread of this workgroup has read the step
+    __syncthreads();  // every thread of this workgroup has read the step (and used it)
     if (threadIdx.x == 0) {
-      __threadfence();
       const unsigned prev = atomicAdd(reinterpret_cast<unsigned*>(step_dev + 1), 1u);
       if (prev == gridDim.x - 1) {
         atomicExch(reinterpret_cast<unsigned long long*>(step_dev), (unsigned long long)(step + 1));

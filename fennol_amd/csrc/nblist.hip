@@ -380,32 +380,31 @@ __global__ void __launch_bounds__(256) k_bin(const float* __restrict__ coords, c
       if (slot < kCellCap) stage[(size_t)cell * kCellCap + slot] = i;
     }
   }
-  // completion count: the last workgroup sees every population
+  // completion count: the last workgroup sees every population.  The populations are only ever
+  // changed by device-scope atomics whose results this workgroup's threads waited for before the
+  // barrier below, so they are final at the device's coherence point when the count is taken;
+  // the last workgroup reads them there (agent-scope atomic loads), without cache-wide fences.
   __shared__ bool last;
   __shared__ int wsum[4];
   __shared__ int carry;
   __syncthreads();
   if (t == 0) {
-    __threadfence();
     last = atomicAdd(done, 1u) == gridDim.x - 1;
     carry = 0;
   }
   __syncthreads();
   if (!last) return;
-  __threadfence();
   // exclusive scan in chunks of 1024 cells, 4 consecutive cells per thread
   const int lane = t & 63, w = t >> 6;
   int cmax = 0;
   for (int base = 0; base < n_cells; base += 1024) {
     const int idx = base + 4 * t;
     int4 v = make_int4(0, 0, 0, 0);
-    if (idx + 3 < n_cells) {
-      v = *(const int4*)(cell_count + idx);
-    } else {
-      if (idx < n_cells) v.x = cell_count[idx];
-      if (idx + 1 < n_cells) v.y = cell_count[idx + 1];
-      if (idx + 2 < n_cells) v.z = cell_count[idx + 2];
-    }
+    auto ld = [&](int q) { return __hip_atomic_load(cell_count + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
+    if (idx < n_cells) v.x = ld(idx);
+    if (idx + 1 < n_cells) v.y = ld(idx + 1);
+    if (idx + 2 < n_cells) v.z = ld(idx + 2);
+    if (idx + 3 < n_cells) v.w = ld(idx + 3);
     cmax = max(cmax, max(max(v.x, v.y), max(v.z, v.w)));
     const int tot = v.x + v.y + v.z + v.w;
     const int incl = fnl_wave_incl_scan(tot);
