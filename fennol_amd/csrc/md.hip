@@ -19,9 +19,13 @@ __device__ __forceinline__ void reduce_system(int b, int lane, const int32_t* __
   double acc[kRedVals];
 #pragma unroll
   for (int k = 0; k < kRedVals; ++k) acc[k] = 0.0;
+  // partials read at the device's coherence point (agent-scope atomic loads: see k_reduce_segments)
   for (int s = seg_offsets[b] + lane; s < seg_offsets[b + 1]; s += 64)
 #pragma unroll
-    for (int k = 0; k < kRedVals; ++k) acc[k] += partials[(size_t)s * kRedVals + k];
+    for (int k = 0; k < kRedVals; ++k)
+      acc[k] += __longlong_as_double((long long)__hip_atomic_load(
+          reinterpret_cast<const unsigned long long*>(partials + (size_t)s * kRedVals + k), __ATOMIC_RELAXED,
+          __HIP_MEMORY_SCOPE_AGENT));
 #pragma unroll
   for (int k = 0; k < kRedVals; ++k)
 #pragma unroll
@@ -91,17 +95,20 @@ k_reduce_segments(const int32_t* __restrict__ sys_offsets, const int32_t* __rest
     }
     __syncthreads();
   }
-  if (t < kRedVals) partials[(size_t)sgi * kRedVals + t] = red[t][0];
-  // completion count: the last workgroup sees every partial
+  // Each partial is written by a device-scope exchange whose result the thread waits for, so it
+  // sits at the device's coherence point before the completion count is taken; the last
+  // workgroup reads the partials there (agent-scope atomic loads), without cache-wide fences.
+  if (t < kRedVals) {
+    const unsigned long long old = atomicExch(
+        reinterpret_cast<unsigned long long*>(partials + (size_t)sgi * kRedVals + t),
+        (unsigned long long)__double_as_longlong(red[t][0]));
+    __asm__ volatile("" : : "v"(old));  // keep the returning form (wait for completion)
+  }
   __shared__ bool last;
   __syncthreads();
-  if (t == 0) {
-    __threadfence();
-    last = atomicAdd(done, 1u) == gridDim.x - 1;
-  }
+  if (t == 0) last = atomicAdd(done, 1u) == gridDim.x - 1;
   __syncthreads();
   if (!last) return;
-  __threadfence();
   for (int b = t >> 6; b < n_sys; b += blockDim.x >> 6)
     reduce_system(b, t & 63, seg_offsets, partials, energy_sys, virial_sys, kinetic_sys);
   if (t == 0) atomicExch(done, 0u);
