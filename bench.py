@@ -33,6 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # the stages that are one kernel launch each (candidates for the roofline entry)
 KERNEL_OF_STAGE = {"aev_fwd": "k_aev_fwd", "mlp": "k_species_mlp", "aev_bwd": "k_aev_bwd"}
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
+ISSUE_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "issue.json")
 MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp32": 157.3}
 SPECIES7 = ("H", "C", "N", "O", "S", "F", "Cl")
 MODEL_SEED = 11
@@ -93,6 +94,17 @@ def measured_traffic(kernel: str, workload_name: str, precision: str):
         return None
     ent = table.get(f"{workload_name}/{precision}/{kernel}")
     return None if ent is None else float(ent["bytes_per_launch"])
+
+
+def measured_issue(kernel: str, workload_name: str, precision: str):
+    """VALU busy fraction, active lanes per VALU instruction and MFMA busy fraction of ``kernel``
+    from the committed rocprofv3 issue-counter summary, or None when it was not profiled."""
+    try:
+        with open(ISSUE_FILE) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return table.get(f"{workload_name}/{precision}/{kernel}")
 
 
 def copy_calibration(mib: int) -> None:
@@ -351,6 +363,9 @@ def main() -> int:
     roofline = {"bound": bound, "achieved": round(achieved, 2), "peak": peak, "unit": unit,
                 "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": KERNEL_OF_STAGE[dom],
                 "launch_us": round(1e6 * t_s, 2), "algorithmic_per_launch": work}
+    issue = measured_issue(KERNEL_OF_STAGE[dom], args.workload, "bf16")
+    if issue is not None:  # what actually limits the kernel: its instruction issue (counters)
+        roofline["issue"] = issue
     stage_table = {}
     for prec, r in results.items():
         stage_table[prec] = {}
