@@ -16,7 +16,8 @@
 //               scans the populations into cell offsets
 //   k_cell_sort one wave per cell: the staged atoms ordered by index
 //               (deterministic rows) into the cell's segment, with
-//               cell-sorted float4 positions
+//               cell-sorted float4 positions, and the cell's table of
+//               neighbouring-cell candidate ranges
 //   k_pairs     one wave per centre, centres in cell order: candidates of the
 //               neighbouring cells (contiguous z-runs of the cell-sorted
 //               arrays) tested 64 at a time, kept pairs compacted with a wave
@@ -46,6 +47,7 @@ struct NblistWs {
   unsigned* done;      // completion count of k_bin (left at zero)
   int32_t* sorted;     // [N]
   float4* pos;         // [N] cell-sorted x, y, z, atom index bits
+  int2* ranges;        // [n_cells][kMaxRanges] candidate ranges of each cell's neighbourhood
 };
 
 size_t ws_layout(int n_atoms, int n_sys, int n_cells, char* base, NblistWs* ws) {
@@ -63,6 +65,7 @@ size_t ws_layout(int n_atoms, int n_sys, int n_cells, char* base, NblistWs* ws) 
   size_t o_dn = take(4);
   size_t o_s = take((size_t)n_atoms * 4);
   size_t o_p = take((size_t)n_atoms * 16);
+  size_t o_r = take((size_t)n_cells * kMaxRanges * 8);
   if (base && ws) {
     ws->grid = (int32_t*)(base + o_grid);
     ws->atom_cell = (int32_t*)(base + o_ac);
@@ -72,6 +75,7 @@ size_t ws_layout(int n_atoms, int n_sys, int n_cells, char* base, NblistWs* ws) 
     ws->done = (unsigned*)(base + o_dn);
     ws->sorted = (int32_t*)(base + o_s);
     ws->pos = (float4*)(base + o_p);
+    ws->ranges = (int2*)(base + o_r);
   }
   return off;
 }
@@ -429,49 +433,6 @@ __global__ void __launch_bounds__(256) k_bin(const float* __restrict__ coords, c
   if (lane == 0 && cmax > 0) atomicMax(&status[FNL_STATUS_MAX_CELL], cmax);
 }
 
-// One wave per cell: the staged atoms of the cell ranked by index (rows come
-// out in a fixed order), written to the cell's segment of sorted[] with their
-// float4 positions.  A cell holding more than kCellCap atoms gathers its atoms
-// instead by scanning its system's atoms in index order.
-__global__ void k_cell_sort(int n_cells, const int32_t* __restrict__ cell_start,
-                            const int32_t* __restrict__ stage, const float* __restrict__ coords,
-                            const int32_t* __restrict__ atom_cell, const int32_t* __restrict__ grid,
-                            const int32_t* __restrict__ sys_offsets, int n_sys,
-                            int32_t* __restrict__ sorted, float4* __restrict__ pos) {
-  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = fnl_lane();
-  if (c >= n_cells) return;
-  const int s = cell_start[c], cnt = cell_start[c + 1] - s;
-  if (cnt <= kCellCap) {
-    const int v = lane < cnt ? stage[(size_t)c * kCellCap + lane] : 0x7fffffff;
-    int rank = 0;
-    for (int q = 0; q < cnt; ++q) rank += __shfl(v, q, 64) < v ? 1 : 0;
-    if (lane < cnt) {
-      sorted[s + rank] = v;
-      pos[s + rank] = make_float4(coords[3 * v], coords[3 * v + 1], coords[3 * v + 2], __int_as_float(v));
-    }
-    return;
-  }
-  // the system whose cell range holds c: largest b with first cell <= c
-  int lo = 0, hi = n_sys - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (grid[mid * kGridWords + 3] <= c) lo = mid; else hi = mid - 1;
-  }
-  int k = 0;
-  for (int a0 = sys_offsets[lo]; a0 < sys_offsets[lo + 1]; a0 += 64) {
-    const int a = a0 + lane;
-    const bool mine = a < sys_offsets[lo + 1] && atom_cell[a] == c;
-    const unsigned long long m = __ballot(mine);
-    if (mine) {
-      const int r = k + fnl_rank_below(m);
-      sorted[s + r] = a;
-      pos[s + r] = make_float4(coords[3 * a], coords[3 * a + 1], coords[3 * a + 2], __int_as_float(a));
-    }
-    k += __popcll(m);
-  }
-}
-
 // offsets along one axis of the 3x3x3 neighbourhood, deduplicated
 __device__ __forceinline__ int axis_offsets(int c, int n, bool pbc, int* out) {
   if (pbc) {
@@ -516,11 +477,85 @@ __device__ __forceinline__ int z_runs(int cz, int nz, bool pbc, int* lo, int* hi
   return 1;
 }
 
+// One wave per cell: the staged atoms of the cell ranked by index (rows come
+// out in a fixed order), written to the cell's segment of sorted[] with their
+// float4 positions.  A cell holding more than kCellCap atoms gathers its atoms
+// instead by scanning its system's atoms in index order.  The wave also writes
+// the cell's candidate ranges for k_pairs: range r (lane r < kMaxRanges) =
+// (first position in the cell-sorted arrays, length) of one z-run of one of
+// the 3 x 3 neighbouring columns; unused ranges are (0, 0).
+__global__ void k_cell_sort(int n_cells, const int32_t* __restrict__ cell_start,
+                            const int32_t* __restrict__ stage, const float* __restrict__ coords,
+                            const int32_t* __restrict__ atom_cell, const int32_t* __restrict__ grid,
+                            const int32_t* __restrict__ sys_offsets, int n_sys,
+                            int32_t* __restrict__ sorted, float4* __restrict__ pos,
+                            int2* __restrict__ ranges) {
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = fnl_lane();
+  if (c >= n_cells) return;
+  // the system whose cell range holds c: largest b with first cell <= c
+  int sys = 0;
+  {
+    int hi = n_sys - 1;
+    while (sys < hi) {
+      const int mid = (sys + hi + 1) >> 1;
+      if (grid[mid * kGridWords + 3] <= c) sys = mid; else hi = mid - 1;
+    }
+  }
+  {
+    const int32_t* g = grid + sys * kGridWords;
+    const int nx = g[0], ny = g[1], nz = g[2], first = g[3];
+    const bool pbc = g[10] != 0;
+    const int cell = c - first;
+    int2 rr = make_int2(0, 0);
+    if (cell < nx * ny * nz) {  // cells past the system's grid (capacity slack) have no atoms
+      const int cz = cell % nz, cy = (cell / nz) % ny, cx = cell / (ny * nz);
+      int ox[3], oy[3];
+      const int nox = axis_offsets(cx, nx, pbc, ox);
+      const int noy = axis_offsets(cy, ny, pbc, oy);
+      int zlo[2], zhi[2];
+      const int nzr = z_runs(cz, nz, pbc, zlo, zhi);
+      if (lane < nox * noy * nzr) {
+        const int zr = lane % nzr, xy = lane / nzr;
+        const int a = xy / noy, bb = xy % noy;
+        const int col = first + ((cx + ox[a]) * ny + (cy + oy[bb])) * nz;
+        rr.x = cell_start[col + zlo[zr]];
+        rr.y = cell_start[col + zhi[zr] + 1] - rr.x;
+      }
+    }
+    if (lane < kMaxRanges) ranges[(size_t)c * kMaxRanges + lane] = rr;
+  }
+  const int s = cell_start[c], cnt = cell_start[c + 1] - s;
+  if (cnt <= kCellCap) {
+    const int v = lane < cnt ? stage[(size_t)c * kCellCap + lane] : 0x7fffffff;
+    int rank = 0;
+    for (int q = 0; q < cnt; ++q) rank += __shfl(v, q, 64) < v ? 1 : 0;
+    if (lane < cnt) {
+      sorted[s + rank] = v;
+      pos[s + rank] = make_float4(coords[3 * v], coords[3 * v + 1], coords[3 * v + 2], __int_as_float(v));
+    }
+    return;
+  }
+  int k = 0;
+  for (int a0 = sys_offsets[sys]; a0 < sys_offsets[sys + 1]; a0 += 64) {
+    const int a = a0 + lane;
+    const bool mine = a < sys_offsets[sys + 1] && atom_cell[a] == c;
+    const unsigned long long m = __ballot(mine);
+    if (mine) {
+      const int r = k + fnl_rank_below(m);
+      sorted[s + r] = a;
+      pos[s + r] = make_float4(coords[3 * a], coords[3 * a + 1], coords[3 * a + 2], __int_as_float(a));
+    }
+    k += __popcll(m);
+  }
+}
+
 // One wave per centre (centres in cell order).  The candidates are the atoms
 // of up to 18 contiguous ranges (3x3 columns x z-runs) of the cell-sorted
-// arrays, concatenated into one list (a wave prefix sum over the range
-// lengths held in lanes; candidate v finds its range by a binary search over
-// the shuffled offsets), so every pass tests 64 candidates, and the next
+// arrays, read from the centre cell's range table, concatenated into one list
+// (a wave prefix sum over the range lengths held in lanes; candidate v is
+// mapped through LDS, or beyond kCmap candidates found by a search over the
+// range offsets), so every pass tests 64 candidates, and the next
 // pass's positions are loaded before the current pass is compacted.  Kept
 // pairs are compacted with a ballot and a prefix popcount into the radial
 // row and, for d12 < rca^2, the angular row, in candidate order.  Only rows
@@ -531,10 +566,11 @@ __global__ void __launch_bounds__(64 * WAVES)
 k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_index,
         int n_binned_cap, const float* __restrict__ cells, const float* __restrict__ rec,
         const int32_t* __restrict__ grid, const int32_t* __restrict__ atom_cell,
-        const int32_t* __restrict__ cell_start, const int32_t* __restrict__ sorted,
-        const float4* __restrict__ pos, float rc2, float rca2, int max_nbr, int max_nbr_ang,
-        int32_t* __restrict__ nbr, int32_t* __restrict__ n_nbr, int32_t* __restrict__ nbr_ang,
-        int32_t* __restrict__ n_nbr_ang, int32_t* __restrict__ status, int n_cells) {
+        const int32_t* __restrict__ cell_start, const float4* __restrict__ pos,
+        const int32_t* __restrict__ sorted, const int2* __restrict__ ranges, float rc2, float rca2,
+        int max_nbr, int max_nbr_ang, int32_t* __restrict__ nbr, int32_t* __restrict__ n_nbr,
+        int32_t* __restrict__ nbr_ang, int32_t* __restrict__ n_nbr_ang, int32_t* __restrict__ status,
+        int n_cells) {
   const int w = threadIdx.x >> 6;
   const int lane = fnl_lane();
   const int slot = blockIdx.x * WAVES + w;
@@ -542,36 +578,18 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
   if (slot >= n_binned || slot >= n_binned_cap) return;
   const int i = sorted[slot];
   const int b = batch_index[i];
-  const int32_t* g = grid + b * kGridWords;
-  const int nx = g[0], ny = g[1], nz = g[2], first = g[3];
-  const bool pbc = g[10] != 0;
-  const int cell = atom_cell[i] - first;
-  const int cz = cell % nz, cy = (cell / nz) % ny, cx = cell / (ny * nz);
-  int ox[3], oy[3];
-  const int nox = axis_offsets(cx, nx, pbc, ox);
-  const int noy = axis_offsets(cy, ny, pbc, oy);
-  int zlo[2], zhi[2];
-  const int nzr = z_runs(cz, nz, pbc, zlo, zhi);
-  // lane r < nox*noy*nzr holds range r: start and length
+  const bool pbc = grid[b * kGridWords + 10] != 0;
+  // lane r < kMaxRanges holds range r of the centre's cell: start and length (k_cell_sort)
   int r_lo = 0, r_len = 0;
-  const int nrange = nox * noy * nzr;
-  if (lane < nrange) {
-    const int zr = lane % nzr, xy = lane / nzr;
-    const int a = xy / noy, bb = xy % noy;
-    const int col = first + ((cx + ox[a]) * ny + (cy + oy[bb])) * nz;
-    r_lo = cell_start[col + zlo[zr]];
-    r_len = cell_start[col + zhi[zr] + 1] - r_lo;
+  if (lane < kMaxRanges) {
+    const int2 rr = ranges[(size_t)atom_cell[i] * kMaxRanges + lane];
+    r_lo = rr.x;
+    r_len = rr.y;
   }
   const int r_end = fnl_wave_incl_scan(r_len);
   const int r_off = r_end - r_len;
   const int total = __builtin_amdgcn_readlane(r_end, 63);
-  // range starts (in the candidate list) and position shifts as wave-uniform scalars
-  int roff[kMaxRanges], rdel[kMaxRanges];
-#pragma unroll
-  for (int r = 0; r < kMaxRanges; ++r) {
-    roff[r] = __builtin_amdgcn_readlane(r_off, r);
-    rdel[r] = __builtin_amdgcn_readlane(r_lo, r) - roff[r];
-  }
+  const uint64_t nonempty = __ballot(r_len > 0);
   // position in the cell-sorted arrays of candidate v < total: a per-wave LDS map filled range
   // by range (one LDS write per candidate instead of a search per candidate and pass); beyond
   // kCmap candidates, v + shift of the last range that starts at or before v (empty ranges
@@ -580,7 +598,9 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
   int* cmap = cmap_all[threadIdx.x >> 6];
   const bool use_map = total <= kCmap;
   if (use_map) {
-    for (int r = 0; r < nrange; ++r) {
+    for (uint64_t rm = nonempty; rm;) {
+      const int r = __builtin_ctzll(rm);
+      rm &= rm - 1ull;
       const int o = __builtin_amdgcn_readlane(r_off, r), lo = __builtin_amdgcn_readlane(r_lo, r);
       const int len = __builtin_amdgcn_readlane(r_len, r);
       for (int k = lane; k < len; k += 64) cmap[o + k] = lo + k;
@@ -591,10 +611,11 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
   }
   auto candidate = [&](int v) {
     if (use_map) return cmap[min(v, kCmap - 1)];
-    int dl = rdel[0];
+    int dl = __builtin_amdgcn_readlane(r_lo, 0) - __builtin_amdgcn_readlane(r_off, 0);
 #pragma unroll
     for (int r = 1; r < kMaxRanges; ++r)
-      if (r < nrange && roff[r] <= v) dl = rdel[r];
+      if (__builtin_amdgcn_readlane(r_off, r) <= v)
+        dl = __builtin_amdgcn_readlane(r_lo, r) - __builtin_amdgcn_readlane(r_off, r);
     return v + dl;
   };
 
@@ -876,12 +897,12 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
     FNL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_cell_sort, dim3(fnl_blocks((long long)n_cells * 64, 256)), dim3(256), 0, s, n_cells,
                        ws.cell_start, ws.stage, coords, ws.atom_cell, ws.grid, sys_offsets, n_sys, ws.sorted,
-                       ws.pos);
+                       ws.pos, ws.ranges);
     FNL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_pairs<W>, dim3(fnl_blocks(n_atoms, W)), dim3(64 * W), 0, s, coords, batch_index,
-                       n_atoms, cells, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_start, ws.sorted,
-                       ws.pos, rc2, rca2, max_nbr, max_nbr_ang, nbr, n_nbr, nbr_ang, n_nbr_ang, status,
-                       n_cells);
+                       n_atoms, cells, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_start, ws.pos,
+                       ws.sorted, ws.ranges, rc2, rca2, max_nbr, max_nbr_ang, nbr, n_nbr, nbr_ang, n_nbr_ang,
+                       status, n_cells);
     FNL_LAUNCH_CHECK();
   }
   return FNL_OK;
