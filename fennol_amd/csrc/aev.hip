@@ -162,15 +162,17 @@ __host__ __device__ inline int round4(int n) { return (n + 3) & ~3; }
 struct FwdLayout {
   int rdf, av, af, fac, tab, words;
 };
-// per-wave LDS slice of the forward (4-byte words): 6 waves per SIMD at the water box's row
-// capacities (104 radial, 40 angular slots)
+// per-wave LDS slice of the forward (4-byte words): the triplet-factor table is only used after
+// the slot vectors (nbv) and the sorted radial slots (rdf) are dead, so it shares their space;
+// 8 waves per SIMD at the water box's row capacities (104 radial, 40 angular slots)
 __host__ __device__ inline FwdLayout fwd_layout(int max_nbr, int max_nbr_ang, int na_nz) {
   FwdLayout L;
-  L.rdf = 4 * max_nbr;  // nbv at 0
-  L.av = round4(L.rdf + 2 * max_nbr);
+  L.fac = 0;            // nbv at 0 as well
+  L.rdf = 4 * max_nbr;
+  const int lo = L.rdf + 2 * max_nbr, fw = na_nz * kFacLd;
+  L.av = round4(lo > fw ? lo : fw);
   L.af = L.av + 4 * max_nbr_ang;
-  L.fac = round4(L.af + 2 * max_nbr_ang);
-  L.tab = L.fac + na_nz * kFacLd;
+  L.tab = round4(L.af + 2 * max_nbr_ang);
   L.words = round4(L.tab + kTabWords);
   return L;
 }
