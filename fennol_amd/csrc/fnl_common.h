@@ -90,6 +90,22 @@ __device__ __forceinline__ void fnl_min_image(float& vx, float& vy, float& vz, c
   }
 }
 
+// fnl_min_image for a cell whose lattice vectors lie along the axes (c and r diagonal): the
+// off-diagonal products are exact zeros and adding a zero leaves a sum unchanged (up to the sign
+// of a zero, which neither rint nor d12 sees), so this gives the same vec bit for bit
+__device__ __forceinline__ bool fnl_cell_diagonal(const FnlCell& cl) {
+  return cl.c[1] == 0.0f && cl.c[2] == 0.0f && cl.c[3] == 0.0f && cl.c[5] == 0.0f && cl.c[6] == 0.0f &&
+         cl.c[7] == 0.0f && cl.r[1] == 0.0f && cl.r[2] == 0.0f && cl.r[3] == 0.0f && cl.r[5] == 0.0f &&
+         cl.r[6] == 0.0f && cl.r[7] == 0.0f;
+}
+
+__device__ __forceinline__ void fnl_min_image_diag(float& vx, float& vy, float& vz, const FnlCell& cl) {
+#pragma clang fp contract(off)
+  vx = vx + (-rintf(vx * cl.r[0])) * cl.c[0];
+  vy = vy + (-rintf(vy * cl.r[4])) * cl.c[4];
+  vz = vz + (-rintf(vz * cl.r[8])) * cl.c[8];
+}
+
 // Integer cell shifts (s0, s1, s2) packed into one int32 (general PBC rows):
 // 10 bits per component, offset 512, so |s| <= 511 lattice vectors.
 __device__ __host__ __forceinline__ int32_t fnl_shift_pack(int s0, int s1, int s2) {

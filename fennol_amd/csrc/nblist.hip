@@ -621,6 +621,7 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
 
   FnlCell cl;
   if (pbc) fnl_load_cell(cells, rec, b, cl);
+  const bool diag = pbc && fnl_cell_diagonal(cl);  // orthorhombic box: 3 products per candidate
   const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
   int count = 0, count_a = 0;
   int32_t* row = nbr + (size_t)i * max_nbr;
@@ -641,7 +642,10 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
     const int j = __float_as_int(pj.w);
     if (valid && j != i) {
       float vx = pj.x - xi, vy = pj.y - yi, vz = pj.z - zi;
-      if (pbc) fnl_min_image(vx, vy, vz, cl);
+      if (diag)
+        fnl_min_image_diag(vx, vy, vz, cl);
+      else if (pbc)
+        fnl_min_image(vx, vy, vz, cl);
       const float d12 = fnl_d12(vx, vy, vz);
       keep = d12 < rc2;
       keep_a = d12 < rca2;
