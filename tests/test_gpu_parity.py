@@ -126,6 +126,32 @@ def test_nblist_crowded_cells(gpu):
     np.testing.assert_array_equal(half, ref)
 
 
+def test_padding_and_isolated_atoms(gpu):
+    """Ragged batch with padding atoms (species 0), an atom beyond every cutoff and a one-atom
+    system: energies, forces and virial equal the oracle's on the real atoms; padding atoms get
+    zero force and no neighbours, isolated atoms the energy of a zero AEV and zero force."""
+    _flush("[edge] padding and isolated atoms")
+    water = np.array([[0.0, 0.0, 0.0], [0.9572, 0.0, 0.0], [-0.24, 0.927, 0.0]])
+    xa = np.concatenate([water, [[20.0, 0.0, 0.0]], [[3.0, 3.0, 3.0], [1.0, 1.0, 1.0]]])
+    sa = np.array([8, 1, 1, 8, 0, 0])
+    xb = np.array([[50.0, 50.0, 50.0]])
+    sb_ = np.array([1])
+    cfg = configs()
+    params, pot = _potential(cfg)
+    species = np.concatenate([sa, sb_])
+    coords = np.concatenate([xa, xb])
+    E, F, V = pot.energy_forces_virial(species, coords, np.array([6, 1]))
+    real = species > 0
+    model = O.Model(cfg[0], params)
+    Er, Fr, Vr, _ = O.energy_forces_virial(model, coords[real].astype(np.float32).astype(np.float64),
+                                           species[real], np.array([4, 1]))
+    np.testing.assert_allclose(E, Er, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(F[real], Fr, rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(V, Vr, rtol=1e-4, atol=1e-6)
+    assert np.all(F[~real] == 0.0)
+    assert np.all(F[3] == 0.0) and np.all(F[6] == 0.0)  # isolated atoms
+
+
 def test_nblist_near_cutoff(gpu):
     """Pairs placed within a few ulp of the cutoff keep the reference's decision."""
     _flush("[nblist] near cutoff")
