@@ -12,7 +12,9 @@ force errors of both forms against an FP64 evaluation, the roofline of the
 dominant kernel and a CPU baseline (the NumPy oracle on the host cores) are
 reported beside it, and under "skin" the same BF16 MD with a Verlet skin
 (fennol_md's nblist_skin, 2 A by default: full rebuilds every 80 steps,
-re-filters of the skin list in between).
+re-filters of the skin list in between).  "reference_box" times the setting
+the reference's only published MD rates come from (216 waters, NVE, 0.1 fs)
+next to that rate.
 """
 
 from __future__ import annotations
@@ -232,6 +234,47 @@ def bench_serving(cfg, params, n_requests: int, n_baseline: int) -> dict:
             "speedup_atom_steps": round(sat["atom_steps_per_s"] / base["atom_steps_per_s"], 2)}
 
 
+# The reference's own MD throughput figures: FP32 NVE conservation runs of the same ANI-2x
+# architecture on a 216-water box (examples/md/watersmall/conservation_tests/*/watersmall.log.bak,
+# unstated NVIDIA GPU): 689.6 steps/s median with matmul precision "highest".
+REFERENCE_BOX_FP32_ATOM_STEPS = 689.6 * 648
+
+
+def bench_reference_box(cfg, params, steps: int = 2000, warmup: int = 20) -> dict:
+    """The reference's watersmall setting: 216 waters (648 atoms) in an 18.64 A cube, NVE with a
+    0.1 fs step and the neighbour list rebuilt every step, one MD step per HIP graph replay."""
+    import torch
+
+    from fennol_amd import systems
+    from fennol_amd.md import LangevinMD
+    from fennol_amd.potential import ANIPotential
+
+    system = systems.water_box(216, 6, seed=5)
+    n = int(system["natoms"].sum())
+    out = {"workload": f"216 waters ({n} atoms), {float(system['cells'][0, 0, 0]):.2f} A cube, NVE, dt 0.1 fs, "
+                       "neighbour list rebuilt every step (the reference's watersmall conservation runs)",
+           "reference_fp32_atom_steps_per_s": round(REFERENCE_BOX_FP32_ATOM_STEPS, 1),
+           "reference_source": "examples/md/watersmall/conservation_tests/ani2x_dt0.1_highest_noskin/"
+                               "watersmall.log.bak (unstated NVIDIA GPU)"}
+    for prec in ("fp32", "bf16"):
+        pot = ANIPotential(cfg, params, precision=prec)
+        md = LangevinMD(pot, system, dt_fs=0.1, temperature=300.0, seed=MD_SEED, thermostat="NVE", check_every=0)
+        for _ in range(warmup):
+            md.step()
+        md.capture(steps_per_graph=1)
+        md.replay(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        md.replay(steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        md.verify()
+        out[prec] = {"ms_per_step": round(1000.0 * wall / steps, 4), "atom_steps_per_s": round(n * steps / wall, 1)}
+        del pot, md
+    out["fp32_vs_reference"] = round(out["fp32"]["atom_steps_per_s"] / REFERENCE_BOX_FP32_ATOM_STEPS, 2)
+    return out
+
+
 def bench_training(n_boxes: int, steps: int) -> dict:
     """cfg 4: energy + force loss and weight gradient through forces on boxes of 64 waters."""
     import torch
@@ -282,6 +325,8 @@ def main() -> int:
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="launch one plain copy kernel of this size first (for counter runs)")
     ap.add_argument("--precisions", default="fp32,bf16", help="MD forms to run (bf16 gives value)")
+    ap.add_argument("--no-reference-box", dest="reference_box", action="store_false",
+                    help="skip the 216-water NVE run that the reference's published MD rate is quoted on")
     ap.add_argument("--skin", type=float, default=2.0,
                     help="also time BF16 MD with a Verlet skin of this many A (fennol_md's nblist_skin; "
                          "rebuild stride from the skin); 0 skips")
@@ -393,6 +438,12 @@ def main() -> int:
     serving = bench_serving(cfg, params, args.serving_requests, args.serving_baseline) \
         if args.serving_requests > 0 else None
     training = bench_training(args.training_boxes, 3) if args.training_boxes > 0 else None
+    ref_box = None
+    if args.reference_box:
+        log("[bench] reference box: 216 waters, NVE, dt 0.1 fs")
+        ref_box = bench_reference_box(cfg, params)
+        log(f"[bench]   fp32 {ref_box['fp32']['ms_per_step']} ms/step ({ref_box['fp32_vs_reference']}x the "
+            f"reference's published FP32 rate), bf16 {ref_box['bf16']['ms_per_step']} ms/step")
     out = {
         "metric": "atom-steps/s and p50/p99 latency, batched BF16 vs plain FP32, at <=1 meV/atom err",
         "value": round(value, 1),
@@ -426,6 +477,7 @@ def main() -> int:
         "skin": skin,
         "serving": serving,
         "training": training,
+        "reference_box": ref_box,
     }
     print(json.dumps(out), flush=True)
     return 0
