@@ -101,6 +101,17 @@ def _status_errors(st) -> None:
         raise RuntimeError("an atom lies more than 511 cells from its cell's origin (general PBC shift range)")
 
 
+_CUS = []
+
+
+def _n_cus() -> int:
+    """Compute units of the current device (256 on an MI355X)."""
+    if not _CUS:
+        _CUS.append(torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+                    if torch.cuda.is_available() else 256)
+    return _CUS[0]
+
+
 def _next_mult(x: int, m: int) -> int:
     return ((x + m - 1) // m) * m
 
@@ -170,6 +181,11 @@ class SpeciesNetworks:
             net.b1, net.b2, net.b3, net.b4 = (x.data_ptr() for x in bs)
         self.tile_rows = tile_rows or _abi.load().fnl_mlp_tile_rows(self.cw.dtype)
         self.cw.tile_rows = self.tile_rows
+        # BF16, small systems (fewer tiles than half the CUs): 32-atom tiles, since one tile's
+        # latency is then the MLP's time (every tile size gives the same results; measured on 648
+        # atoms: 0.104 vs 0.116 ms per MD step; FP32 16-atom tiles were slower than 32); a tile
+        # size given by the caller is kept
+        self.small_tile_rows = 32 if (dtype == "bf16" and not tile_rows) else 0
 
 
 @dataclass
@@ -503,9 +519,13 @@ class ANIPotential:
 
     def _setup_tiles(self, sb: SystemBuffers) -> None:
         """Atoms grouped by network, cut into tiles of at most tile_rows rows (SpeciesIndexer role)."""
-        M = self.nets.tile_rows
         net = np.array([self.net_of_z.get(int(z), -1) if z > 0 else -1 for z in sb.species], np.int64)
         net[int(np.sum(sb.natoms)):] = -1
+        M = self.nets.tile_rows
+        counts = [int(np.sum(net == n)) for n in range(len(self.nets.species))]
+        if self.nets.small_tile_rows and sum(-(-c // M) for c in counts) < _n_cus() // 2:
+            M = self.nets.small_tile_rows
+        sb.extra["mlp_tile_rows"] = M
         perm, ts, tr, tn = [], [], [], []
         for n in range(len(self.nets.species)):
             idx = np.nonzero(net == n)[0]
@@ -610,6 +630,7 @@ class ANIPotential:
     def mlp(self, sb: SystemBuffers, with_grad: bool = True, stream=None) -> None:
         s = _abi.stream_handle(stream)
         P = _abi.ptr
+        self.nets.cw.tile_rows = sb.extra.get("mlp_tile_rows", self.nets.tile_rows)
         rc = self.lib.fnl_species_mlp(
             P(sb.aev), sb.aev.shape[1], C.byref(self.nets.cw), P(sb.perm), P(sb.tile_start), P(sb.tile_rows),
             P(sb.tile_net), sb.n_tiles, P(sb.e_atom), P(sb.g_aev) if with_grad else None, sb.g_aev.shape[1],
