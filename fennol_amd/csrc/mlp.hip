@@ -114,14 +114,6 @@ struct Tr<unsigned short> {  // bf16 stored as raw bits
   }
 };
 
-// Kernel variants: element type, atoms per tile (M) and weight prefetch depth
-// (PF k-steps, in registers).  LDS footprint bounds the workgroups per CU.
-template <typename T, int M_, int PF_>
-struct Cfg {
-  static constexpr int M = M_;
-  static constexpr int PF = PF_;
-};
-
 template <typename T, int M_, int WV_ = 4>
 struct Lds {
   static constexpr int M = M_;

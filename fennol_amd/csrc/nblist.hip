@@ -33,10 +33,12 @@
 
 namespace {
 
+// per system: nx ny nz first | ox oy oz (open: box origin) | iwx iwy iwz (cells per unit of
+// fractional or Cartesian extent, float bits) | pbc mode | pad
 constexpr int kGridWords = 12;
 constexpr int kCellCap = 64;    // atoms per cell staged by k_bin (more: k_cell_sort gathers them)
 constexpr int kCmap = 640;      // candidates per centre mapped through LDS (more: the search below)
-constexpr int kMaxRanges = 18;  // 3 x 3 columns x at most 2 z-runs  // per system: nx ny nz first | ox oy oz | iwx iwy iwz | pbc pad
+constexpr int kMaxRanges = 18;  // 3 x 3 columns x at most 2 z-runs
 
 struct NblistWs {
   int32_t* grid;       // [n_sys][kGridWords] (floats stored as bits)
