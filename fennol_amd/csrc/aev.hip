@@ -424,8 +424,11 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   // empty blocks (species without radial neighbours, pairs without triplets) and the padding:
   // zeros in 16-byte stores.  Blocks of species outside smask (and, unless smask holds every
   // species, the padding) are not written: the caller keeps them zero.
-  {
-    const bool pad_live = smask == 0xFFFFFFFFu;
+  const bool pad_live = smask == 0xFFFFFFFFu;
+  const uint64_t pmask = dm.P >= 64 ? ~0ull : ((1ull << dm.P) - 1ull);
+  const bool any_zero = ((smask & ~rad_present) & ((1u << S) - 1u)) != 0u ||
+                        ((pair_live & ~present) & pmask) != 0ull || (pad_live && stride > dm.aev_len);
+  if (any_zero) {  // (most centres have every written block non-empty)
     constexpr int CH = 16 / (int)sizeof(TOut);  // channels per 16-byte chunk
     // radial block of channel c: (c + 0.5) / nR rounded down, exact in float for these sizes
     const float inv_nr = 1.0f / (float)dm.nR;
