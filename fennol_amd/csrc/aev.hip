@@ -206,7 +206,8 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
 // (wave-uniform), and pos_of(e) is called by the lane that owns entry e with its sorted position.
 template <typename SpecFn, typename PlaceFn>
 __device__ __forceinline__ void species_sort(int n, int S, SpecFn spec, PlaceFn place,
-                                             int (&cnt)[FNL_MAX_SPECIES], int (&off)[FNL_MAX_SPECIES]) {
+                                             int (&cnt)[FNL_MAX_SPECIES], int (&off)[FNL_MAX_SPECIES],
+                                             uint32_t smask) {
   const int lane = fnl_lane();
   const uint64_t below = (1ull << lane) - 1ull;
   if (n <= 64) {  // one ballot per species serves both the counts and the ranks
@@ -216,7 +217,7 @@ __device__ __forceinline__ void species_sort(int n, int S, SpecFn spec, PlaceFn 
     for (int s = 0; s < FNL_MAX_SPECIES; ++s) {
       cnt[s] = 0;
       off[s] = acc;
-      if (s < S) {
+      if (s < S && ((smask >> s) & 1u)) {  // species outside the batch have no slot
         const uint64_t m = __ballot(sa == s);
         cnt[s] = __popcll(m);
         if (sa == s) pos = acc + __popcll(m & below);
@@ -232,7 +233,7 @@ __device__ __forceinline__ void species_sort(int n, int S, SpecFn spec, PlaceFn 
     const int sb = c + lane < n ? spec(c + lane) : S;
 #pragma unroll
     for (int s = 0; s < FNL_MAX_SPECIES; ++s)
-      if (s < S) cnt[s] += __popcll(__ballot(sb == s));
+      if (s < S && ((smask >> s) & 1u)) cnt[s] += __popcll(__ballot(sb == s));
   }
   int acc = 0;
 #pragma unroll
@@ -249,7 +250,7 @@ __device__ __forceinline__ void species_sort(int n, int S, SpecFn spec, PlaceFn 
     int pos = 0;
 #pragma unroll
     for (int s = 0; s < FNL_MAX_SPECIES; ++s) {
-      if (s < S) {
+      if (s < S && ((smask >> s) & 1u)) {  // species outside the batch have no slot
         const uint64_t m = __ballot(sa == s);
         if (sa == s) pos = run[s] + __popcll(m & below);
         run[s] += __popcll(m);
@@ -343,7 +344,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
         const float d = dist(nbv[e]);
         rdf[pos] = make_float2(d, 0.25f * cos_switch(d, p.cutoff));
       },
-      cnt, off);
+      cnt, off, smask);
   wave_sync();
 
   const int q = lane >> 4, k = lane & 15;
@@ -380,7 +381,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           av[pos] = make_float4(v.x * inv, v.y * inv, v.z * inv, d);
           af[pos] = make_float2(cos_switch(d, p.cutoff_ang), __int_as_float(sa));
         },
-        cnt, off);
+        cnt, off, smask);
   }
   // species pairs: lane p < P holds pair p's species blocks in the sorted angular row (sizes,
   // offsets) and its triplet count; the present pairs form a wave-uniform bit mask
