@@ -803,17 +803,38 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     v[3] += vy * gg.x; v[4] += vy * gg.y; v[5] += vy * gg.z;
     v[6] += vz * gg.x; v[7] += vz * gg.y; v[8] += vz * gg.z;
   }
-  sx = wave_sum_dpp(sx);
-  sy = wave_sum_dpp(sy);
-  sz = wave_sum_dpp(sz);
+  // the 12 sums (dE/dx_i, virial) as a transposing reduction: halves of the values cross between
+  // lane halves (xor 32, then 16), so that the 16 lanes of row g end up with partials of values
+  // 3g..3g+2 only, summed over the row with DPP; lane r < 3 of row g writes value 3g + r
+  {
+    const float x[12] = {sx, sy, sz, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
+    const bool hi32 = lane >= 32, hi16 = (lane & 16) != 0;
+    float y[6], z[3];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) v[k] = wave_sum_dpp(v[k]);
-  if (lane == 0) {
-    grad_self[3 * i] = sx;
-    grad_self[3 * i + 1] = sy;
-    grad_self[3 * i + 2] = sz;
+    for (int k = 0; k < 6; ++k) {
+      const float send = hi32 ? x[k] : x[6 + k];
+      y[k] = (hi32 ? x[6 + k] : x[k]) + __shfl_xor(send, 32, 64);
+    }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) virial_atom[9 * i + k] = v[k];
+    for (int k = 0; k < 3; ++k) {
+      const float send = hi16 ? y[k] : y[3 + k];
+      z[k] = (hi16 ? y[3 + k] : y[k]) + __shfl_xor(send, 16, 64);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      z[k] += dpp<0x128>(z[k]);
+      z[k] += dpp<0x124>(z[k]);
+      z[k] += dpp<0x122>(z[k]);
+      z[k] += dpp<0x121>(z[k]);
+    }
+    const int r = lane & 15, idx = 3 * (lane >> 4) + r;
+    if (r < 3) {
+      const float val = r == 0 ? z[0] : (r == 1 ? z[1] : z[2]);
+      if (idx < 3)
+        grad_self[3 * i + idx] = val;
+      else
+        virial_atom[9 * i + idx - 3] = val;
+    }
   }
 }
 
