@@ -26,6 +26,8 @@ FNL_STATUS_MAX_NBR = 1
 FNL_STATUS_MAX_NBR_ANG = 2
 FNL_STATUS_MAX_CELL = 3
 FNL_STATUS_STICKY = 4
+FNL_STATUS_STICKY_MAX_NBR = 5
+FNL_STATUS_STICKY_MAX_NBR_ANG = 6
 FNL_MAX_RADIAL = 32
 FNL_MAX_ANG_DIST = 16
 FNL_MAX_ANG_SECT = 16

@@ -1027,14 +1027,7 @@ int launch_fwd(const float* coords, const int32_t* species_idx, const int32_t* b
   dim3 grid(fnl_blocks(n_atoms, kWaves)), block(64 * kWaves);
   const void* kf = out_dtype == FNL_F32 ? reinterpret_cast<const void*>(&k_aev_fwd<NA, NZ, float>)
                                         : reinterpret_cast<const void*>(&k_aev_fwd<NA, NZ, hip_bfloat16>);
-  // raise the dynamic-LDS limit once per size (not a stream operation: kept out of graph captures
-  // that follow a first eager call)
-  static size_t lds_set[2] = {0, 0};
-  if (lds > lds_set[out_dtype == FNL_F32 ? 0 : 1]) {
-    if (hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-      return FNL_ERR_HIP;
-    lds_set[out_dtype == FNL_F32 ? 0 : 1] = lds;
-  }
+  if (const int rc = fnl_allow_full_lds(kf)) return rc;
   if (out_dtype == FNL_F32) {
     hipLaunchKernelGGL((k_aev_fwd<NA, NZ, float>), grid, block, lds, s, coords, species_idx, batch_index,
                        n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang,
@@ -1056,13 +1049,7 @@ int launch_bwd(const float* coords, const int32_t* species_idx, const int32_t* b
                float* grad_self, float* grad_nbr, float* virial_atom, hipStream_t s) {
   const size_t lds = (size_t)kWaves * 4 * bwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
   if (lds > 160 * 1024) return FNL_ERR_UNSUPPORTED;
-  static size_t lds_set = 0;
-  if (lds > lds_set) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_aev_bwd<NA, NZ, GT>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-      return FNL_ERR_HIP;
-    lds_set = lds;
-  }
+  if (const int rc = fnl_allow_full_lds(reinterpret_cast<const void*>(&k_aev_bwd<NA, NZ, GT>))) return rc;
   hipLaunchKernelGGL((k_aev_bwd<NA, NZ, GT>), dim3(fnl_blocks(n_atoms, kWaves)), dim3(64 * kWaves), lds, s,
                      coords, species_idx, batch_index, n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr,
                      nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, p, dm, g, g_stride, grad_self, grad_nbr,
@@ -1175,13 +1162,7 @@ extern "C" int fnl_aev_tangent(const float* coords, const float* disp, const int
   if (na == 4 && nz == 8) kf = reinterpret_cast<const void*>(&k_aev_tangent<4, 8>);
   else if (na == 8 && nz == 4) kf = reinterpret_cast<const void*>(&k_aev_tangent<8, 4>);
   else return FNL_ERR_UNSUPPORTED;
-  static size_t lds_set[2] = {0, 0};
-  const int slot = na == 4 ? 0 : 1;
-  if (lds > lds_set[slot]) {
-    if (hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-      return FNL_ERR_HIP;
-    lds_set[slot] = lds;
-  }
+  if (const int rc = fnl_allow_full_lds(kf)) return rc;
   dim3 grid(fnl_blocks(n_atoms, kWaves)), block(64 * kWaves);
   if (na == 4)
     hipLaunchKernelGGL((k_aev_tangent<4, 8>), grid, block, lds, s, coords, disp, species_idx, batch_index, n_atoms,

@@ -4,6 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <set>
+#include <utility>
+
 #include "../../include/fnl_hip.h"
 
 #define FNL_WAVE 64
@@ -18,6 +22,24 @@ static inline int fnl_blocks(long long n, int per_block) {
 }
 
 static inline size_t fnl_align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Raise a kernel's dynamic-LDS limit to the whole 160 KB of a CU, once per kernel and device.
+// The only state the library keeps: a mutex-protected set of the (kernel, device) pairs already
+// raised, so concurrent callers on several host threads neither race nor lower a limit another
+// thread relies on.  hipFuncSetAttribute is not a stream operation, so a call made while the
+// caller's stream is being captured into a graph records nothing.
+inline int fnl_allow_full_lds(const void* kernel) {
+  static std::mutex mu;
+  static std::set<std::pair<const void*, int>> raised;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return FNL_ERR_HIP;
+  std::lock_guard<std::mutex> lock(mu);
+  if (raised.count({kernel, dev})) return FNL_OK;
+  if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+    return FNL_ERR_HIP;
+  raised.insert({kernel, dev});
+  return FNL_OK;
+}
 
 // -------------------------------------------------------------------------
 // wave-level helpers

@@ -342,6 +342,8 @@ k_pairs_general(const int32_t* __restrict__ species_idx, const int32_t* __restri
       atomicOr(&status[FNL_STATUS_STICKY], 1);
       atomicMax(&status[FNL_STATUS_MAX_NBR], count);
       atomicMax(&status[FNL_STATUS_MAX_NBR_ANG], count_a);
+      atomicMax(&status[FNL_STATUS_STICKY_MAX_NBR], count);
+      atomicMax(&status[FNL_STATUS_STICKY_MAX_NBR_ANG], count_a);
     }
     if (bad_range) {
       atomicOr(&status[FNL_STATUS_OVERFLOW], 4);
@@ -673,6 +675,8 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
       atomicOr(&status[FNL_STATUS_STICKY], 1);
       atomicMax(&status[FNL_STATUS_MAX_NBR], count);
       atomicMax(&status[FNL_STATUS_MAX_NBR_ANG], count_a);
+      atomicMax(&status[FNL_STATUS_STICKY_MAX_NBR], count);
+      atomicMax(&status[FNL_STATUS_STICKY_MAX_NBR_ANG], count_a);
     }
   }
 }
@@ -760,6 +764,8 @@ k_skin_filter(const float* __restrict__ coords, const int32_t* __restrict__ batc
       atomicOr(&status[FNL_STATUS_STICKY], 1);
       atomicMax(&status[FNL_STATUS_MAX_NBR], count);
       atomicMax(&status[FNL_STATUS_MAX_NBR_ANG], count_a);
+      atomicMax(&status[FNL_STATUS_STICKY_MAX_NBR], count);
+      atomicMax(&status[FNL_STATUS_STICKY_MAX_NBR_ANG], count_a);
     }
   }
 }

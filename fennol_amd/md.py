@@ -5,10 +5,23 @@ merged half kicks, :286-294, LGV O-step md/thermostats.py:108-120), then the
 neighbour-list update (``update_graphs``, :515-566: a full rebuild every
 step without a skin; with ``nblist_skin`` a full rebuild every
 ``nblist_stride`` steps and re-filters of the skin list in between), then
-``update_observables`` (energy and forces, :298-339).  Positions
-and velocities are FP64 on the device; the model reads a float32 copy written
-by the integrator kernel.  Units are fennol_md's (Angstrom, ps, kcal/mol);
-the model's energies are converted with its energy unit (integrate.py:303-305).
+``update_observables`` (energy, forces, kinetic tensor and pressure,
+:298-375).  Positions and velocities are FP64 on the device; the model reads
+a float32 copy written by the integrator kernel.  Units are fennol_md's
+(Angstrom, ps, kcal/mol); the model's energies are converted with its energy
+unit (integrate.py:303-305).
+
+Capacity overflows.  ``check_every=1`` reads the neighbour-list overflow word
+after every list update and regrows the rows before the forces are used, as
+the reference's ``check_reallocate`` does (models/preprocessing.py:1859-1878,
+md/integrate.py:524-530).  Otherwise steps run without host reads (eagerly
+or as HIP-graph replays) and :meth:`LangevinMD.run` keeps a checkpoint: every
+``check_every`` steps (``checkpoint_every`` when ``check_every`` is 0) it
+reads the sticky overflow word; if a list update overflowed since the last
+checkpoint, the rows are regrown from the sticky maxima, positions,
+velocities and the step counter are restored from the checkpoint, and the
+steps are run again.  The noise of step n depends on n only, so the redone
+steps are those an ample-capacity run takes.
 """
 
 from __future__ import annotations
@@ -19,7 +32,7 @@ import numpy as np
 import torch
 
 from . import _abi, units
-from .potential import SEG_ATOMS, ANIPotential, SystemBuffers
+from .potential import SEG_ATOMS, ANIPotential, SystemBuffers, _next_mult, _status_errors
 
 
 def nblist_schedule(skin: float, dt_fs: float, stride: int = -1) -> int:
@@ -31,6 +44,13 @@ def nblist_schedule(skin: float, dt_fs: float, stride: int = -1) -> int:
     if stride <= 0:
         stride = int(math.floor(skin / 2.0 * 40.0 / dt_fs))
     return max(int(stride), 1)
+
+
+def grown_capacity(current: int, needed: int, mult: float = 1.25, add: int = 8) -> int:
+    """Row capacity after an overflow that needed ``needed`` slots (unchanged if it fits)."""
+    if needed <= current:
+        return current
+    return _next_mult(int(mult * needed) + add, 8)
 
 
 class LangevinMD:
@@ -47,12 +67,18 @@ class LangevinMD:
     ``nblist_warmup_fs`` (full rebuilds every step before that time) follow
     fennol_md's keywords of the same names (integrate.py:477-513);
     ``minimum_image`` False selects general PBC for periodic systems.
+
+    ``check_every``: 1 = the reference's per-update overflow check; K > 1 =
+    checkpointed runs checked every K steps; 0 = capacities sized with headroom
+    up front and checkpointed runs checked every ``checkpoint_every`` steps
+    (the form HIP-graph replay uses).
     """
 
     def __init__(self, potential: ANIPotential, system: dict, dt_fs: float = 0.5, temperature: float = 300.0,
                  gamma_thz: float = 1.0, seed: int = 0, thermostat: str = "LGV", energy_unit: str = "eV",
                  check_every: int = 1, sb: SystemBuffers | None = None, nblist_skin: float = -1.0,
-                 nblist_stride: int = -1, nblist_warmup_fs: float = -1.0, minimum_image: bool = True):
+                 nblist_stride: int = -1, nblist_warmup_fs: float = -1.0, minimum_image: bool = True,
+                 checkpoint_every: int = 200):
         self.pot = potential
         self.lib = potential.lib
         dev = potential.device
@@ -76,10 +102,11 @@ class LangevinMD:
         self.thermostat = thermostat.upper()
         self.seed = int(seed)
         self.istep = 0
-        # 1: read the overflow flag after every rebuild (the reference's protocol);
-        # 0: no per-step host read -- capacities are sized with headroom up front
-        # and the sticky overflow word is checked by verify()
+        if check_every < 0 or checkpoint_every < 1:
+            raise ValueError("check_every must be >= 0 and checkpoint_every >= 1")
         self.check_every = int(check_every)
+        self.checkpoint_every = int(check_every) if check_every > 1 else int(checkpoint_every)
+        self.rollbacks = 0
         self.model_energy_unit = units.energy_multiplier(energy_unit)
         self.force_scale = 1.0 / self.model_energy_unit                    # model unit -> kcal/mol
         self.t_mass = torch.tensor(self.mass, dtype=torch.float64, device=dev)
@@ -97,6 +124,7 @@ class LangevinMD:
         self.step_dev = torch.zeros(2, dtype=torch.int64, device=dev)  # {step, completion count}
         self.graph = None
         self.graph_skin = None
+        self.steps_per_graph = 1
         self.x = torch.tensor(np.asarray(system["coordinates"], np.float64), device=dev).contiguous()
         self.x32 = self.x.float().contiguous()
         self.v = torch.empty((self.n, 3), dtype=torch.float64, device=dev)
@@ -112,6 +140,8 @@ class LangevinMD:
             self.pot.fit_capacity(self.sb, self.x32)
         # initial forces (integrate.py:604-616)
         self.pot.compute(self.sb, self.x32)
+        self.pot.clear_sticky(self.sb)
+        self._ckpt = None
 
     def verify(self) -> None:
         """Raise if a neighbour-list capacity overflowed since the last verify()."""
@@ -141,18 +171,96 @@ class LangevinMD:
         return "skin"
 
     def step(self, check: bool | None = None, stream=None) -> None:
-        """integrate -> update neighbour lists -> energy and forces at the new positions."""
+        """integrate -> update neighbour lists -> energy and forces at the new positions.
+        With ``check_every=1`` the overflow word is read after the list update (the reference's
+        protocol); otherwise no host read happens here (see :meth:`run`)."""
         rebuild = self._graph_update()
         self._integrate(stream)
         self.istep += 1
         if check is None:
-            check = self.check_every > 0 and (self.istep % self.check_every) == 0
+            check = self.check_every == 1
         self.pot.compute(self.sb, self.x32, forces=True, rebuild=rebuild, check=check, stream=stream,
                          zero_grad=False)  # the integrator cleared grad_nbr
 
-    def run(self, nsteps: int) -> None:
-        for _ in range(nsteps):
-            self.step()
+    # --------------------------------------------------------- checkpoints
+    def _checkpoint(self) -> None:
+        """Device copies of x, v and the step counter, with the host-side step state."""
+        if self._ckpt is None:
+            self._ckpt = {"x": torch.empty_like(self.x), "v": torch.empty_like(self.v),
+                          "step": torch.empty_like(self.step_dev)}
+        self._ckpt["x"].copy_(self.x)
+        self._ckpt["v"].copy_(self.v)
+        self._ckpt["step"].copy_(self.step_dev)
+        self._ckpt["istep"] = self.istep
+
+    def _rollback(self) -> None:
+        """Regrow the overflowed rows from the sticky maxima, restore the checkpoint and
+        recompute the forces there with a full rebuild (which also restarts the skin schedule)."""
+        sb, pot = self.sb, self.pot
+        st = sb.status.cpu().numpy()
+        _status_errors(st)
+        sb.max_nbr = grown_capacity(sb.max_nbr, int(st[_abi.FNL_STATUS_STICKY_MAX_NBR]))
+        sb.max_nbr_ang = grown_capacity(sb.max_nbr_ang, int(st[_abi.FNL_STATUS_STICKY_MAX_NBR_ANG]))
+        if sb.skin > 0:
+            sts = sb.status_skin.cpu().numpy()
+            _status_errors(sts)
+            sb.max_skin = grown_capacity(sb.max_skin, int(sts[_abi.FNL_STATUS_STICKY_MAX_NBR]))
+        pot._alloc_nblist(sb)
+        self.x.copy_(self._ckpt["x"])
+        self.v.copy_(self._ckpt["v"])
+        self.step_dev.copy_(self._ckpt["step"])
+        self.x32.copy_(self.x)
+        self.istep = self._ckpt["istep"]
+        pot.compute(sb, self.x32, forces=True, rebuild=True, check=True)
+        self.nblist_countdown = self.nblist_stride - 1
+        pot.clear_sticky(sb)
+        self.rollbacks += 1
+        if self.graph is not None:  # the rows moved: record the steps again
+            self.capture(self.steps_per_graph, warmup=False)
+
+    def overflowed(self) -> bool:
+        """True if a list update overflowed a capacity since the sticky words were last cleared
+        (one host read: synchronises)."""
+        v = int(self.sb.status[_abi.FNL_STATUS_STICKY].item())
+        if self.sb.skin > 0:
+            v |= int(self.sb.status_skin[_abi.FNL_STATUS_STICKY].item())
+        return v != 0
+
+    def run(self, nsteps: int, graph: bool = False, max_rollbacks: int = 8) -> None:
+        """``nsteps`` MD steps.  With ``check_every=1`` (and no graph) each step checks its own
+        list update; otherwise the steps run in segments of ``checkpoint_every`` without host
+        reads (as HIP-graph replays if ``graph``: :meth:`capture` is called first if needed),
+        each segment followed by one read of the sticky overflow word and, on overflow, a
+        rollback to the segment's start with regrown rows."""
+        if self.check_every == 1 and not graph:
+            for _ in range(nsteps):
+                self.step()
+            return
+        if graph and self.graph is None:
+            self.capture(1)
+        per = self.steps_per_graph if graph else 1
+        K = max(per, (self.checkpoint_every // per) * per)
+        done = 0
+        tries = 0
+        self._checkpoint()
+        while done < nsteps:
+            n = min(K, nsteps - done)
+            if graph:
+                if n % per:
+                    raise ValueError(f"graphs hold {per} steps: nsteps must be a multiple of it")
+                self.replay(n // per)
+            else:
+                for _ in range(n):
+                    self.step(check=False)
+            if self.overflowed():
+                tries += 1
+                if tries > max_rollbacks:
+                    raise RuntimeError("neighbour-list capacity keeps overflowing after regrowth")
+                self._rollback()
+                continue
+            tries = 0
+            done += n
+            self._checkpoint()
 
     # ---------------------------------------------------------- graph replay
     def _record(self, rebuild: bool | str, stream=None) -> None:
@@ -161,24 +269,26 @@ class LangevinMD:
         self.pot.compute(self.sb, self.x32, forces=True, rebuild=rebuild, check=False, stream=stream,
                          zero_grad=False)
 
-    def capture(self, steps_per_graph: int = 1) -> None:
-        """Record MD steps in HIP graphs (no host checks inside; capacities must have
-        headroom: construct with check_every=0).  Without a skin list one graph holds
-        `steps_per_graph` steps; with one, two one-step graphs (full rebuild, skin
-        re-filter) are recorded and :meth:`replay` follows the rebuild schedule."""
-        if self.check_every != 0:
-            raise ValueError("graph capture needs check_every=0 (no host reads inside a step)")
+    def capture(self, steps_per_graph: int = 1, warmup: bool = True) -> None:
+        """Record MD steps in HIP graphs (no host checks inside; construct with check_every != 1
+        and use :meth:`run` with ``graph=True`` for checked runs).  Without a skin list one
+        graph holds `steps_per_graph` steps; with one, two one-step graphs (full rebuild, skin
+        re-filter) are recorded and :meth:`replay` follows the rebuild schedule.  ``warmup``
+        runs one (two with a skin) ordinary steps first, outside the capture."""
+        if self.check_every == 1:
+            raise ValueError("graph capture needs check_every != 1 (no host reads inside a step)")
         skin = self.nblist_stride > 1
         if skin and steps_per_graph != 1:
             raise ValueError("with a skin list every graph holds one step")
         timer, self.pot.timer = self.pot.timer, None
-        side = torch.cuda.Stream()
-        side.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(side):  # warm the allocator and kernels outside the capture
-            self.step(check=False)
-            if skin:
+        if warmup:
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):  # warm the allocator and kernels outside the capture
                 self.step(check=False)
-        torch.cuda.current_stream().wait_stream(side)
+                if skin:
+                    self.step(check=False)
+            torch.cuda.current_stream().wait_stream(side)
         kinds = (True, "skin") if skin else (True,)
         graphs = {}
         for kind in kinds:
@@ -220,6 +330,24 @@ class LangevinMD:
         """2 K / (3 N k_B) per system (md/dynamic.py printing)."""
         ek = np.trace(self.kinetic_tensor(), axis1=1, axis2=2)
         return 2.0 * ek / (3.0 * np.asarray(self.sb.natoms) * units.K_B)
+
+    def virial_kcal(self) -> np.ndarray:
+        """dE/dstrain per system [B,3,3] in kcal/mol at the current positions."""
+        return self.sb.virial.cpu().numpy().reshape(-1, 3, 3) * self.force_scale
+
+    def pressure_tensor(self) -> np.ndarray:
+        """P = 2 K / V - virial / V per system [B,3,3] in kcal/mol/A^3 (md/integrate.py:364-371,
+        the kinetic tensor of the stored mid-step velocities, pressure_o_weight = 1)."""
+        cells = self.sb.extra.get("cells64")
+        if cells is None:
+            raise ValueError("the pressure needs periodic cells")
+        vol = np.abs(np.linalg.det(cells))[:, None, None]
+        return self.kinetic_tensor() * (2.0 / vol) - self.virial_kcal() / vol
+
+    def pressure(self) -> np.ndarray:
+        """Scalar pressure trace(P)/3 per system in kcal/mol/A^3 (multiply by
+        ``units.us.get_multiplier('atm')`` for atm)."""
+        return np.trace(self.pressure_tensor(), axis1=1, axis2=2) / 3.0
 
     def total_energy_kcal(self) -> np.ndarray:
         return self.energies() / self.model_energy_unit + np.trace(self.kinetic_tensor(), axis1=1, axis2=2)

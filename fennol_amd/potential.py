@@ -333,13 +333,14 @@ class ANIPotential:
             raise ValueError(f"Species {missing} not found in networks. Handled species are "
                              f"{self.nets.species} (nets.py:394-397)")
         pbc = _abi.FNL_PBC_NONE
-        tc = tr = None
+        tc = tr = cells64 = None
         if cells is not None:
             cells = np.asarray(cells, np.float64).reshape(-1, 3, 3)
             if cells.shape[0] == 1 and B > 1:
                 cells = np.repeat(cells, B, axis=0)
             pbc = _abi.FNL_PBC_MINIMUM_IMAGE if minimum_image else _abi.FNL_PBC_GENERAL
             tc = torch.tensor(cells.astype(np.float32), device=dev).contiguous()
+            cells64 = cells
             tr = torch.tensor(np.linalg.inv(cells).astype(np.float32), device=dev).contiguous()
         grid, n_cells = self.grid_caps(natoms, cells)
         sb = SystemBuffers(
@@ -356,6 +357,7 @@ class ANIPotential:
             sb.skin = float(skin)
             grow = ((self.config.cutoff + skin) / self.config.cutoff) ** 3
             sb.max_skin = _next_mult(int(grow * self.init_max_nbr), 8)
+        sb.extra["cells64"] = cells64
         self._alloc_nblist(sb)
         self._setup_tiles(sb)
         kp = self.config.k_pad
@@ -695,15 +697,22 @@ class ANIPotential:
         with self._stage("reduce"):
             self.reduce(sb, with_virial=forces, stream=stream)
 
+    def clear_sticky(self, sb: SystemBuffers, stream=None) -> None:
+        """Zero the sticky overflow word and the sticky row maxima (status words 4-6) with a
+        fill kernel (no memset node if the caller's stream is being captured)."""
+        s = _abi.stream_handle(stream)
+        for st in (sb.status, sb.status_skin if sb.skin > 0 else None):
+            if st is not None:
+                _abi.check(self.lib.fnl_fill_i32(st.data_ptr() + 4 * _abi.FNL_STATUS_STICKY, 0, 3, s),
+                           "fnl_fill_i32")
+
     def check_sticky(self, sb: SystemBuffers, clear: bool = True) -> None:
         """Raise if any build since the last clear overflowed a capacity (synchronises)."""
         v = int(sb.status[_abi.FNL_STATUS_STICKY].item())
         if sb.skin > 0:
             v |= int(sb.status_skin[_abi.FNL_STATUS_STICKY].item())
         if clear:
-            sb.status[_abi.FNL_STATUS_STICKY:_abi.FNL_STATUS_STICKY + 1].zero_()
-            if sb.skin > 0:
-                sb.status_skin[_abi.FNL_STATUS_STICKY:_abi.FNL_STATUS_STICKY + 1].zero_()
+            self.clear_sticky(sb)
         if v:
             raise RuntimeError("neighbour-list capacity overflowed during the run")
 
@@ -723,9 +732,7 @@ class ANIPotential:
             sb.max_skin = max(sb.max_skin, want_s)
             self._alloc_nblist(sb)
             self.build_nblist(sb, coords32)
-        sb.status[_abi.FNL_STATUS_STICKY:_abi.FNL_STATUS_STICKY + 1].zero_()
-        if sb.skin > 0:
-            sb.status_skin[_abi.FNL_STATUS_STICKY:_abi.FNL_STATUS_STICKY + 1].zero_()
+        self.clear_sticky(sb)
 
     def forces(self, sb: SystemBuffers, scale: float = 1.0, stream=None) -> torch.Tensor:
         out = torch.empty((sb.n_atoms, 3), dtype=torch.float32, device=self.device)

@@ -33,6 +33,19 @@ from . import _abi, units
 from .potential import ANIPotential
 
 
+def noise_stream(rid: int) -> int:
+    """Philox stream of request rid's O-step noise (counter (local atom, step, stream)): even
+    streams, so that no request's noise repeats another's initial velocities.  Request 0 uses
+    stream 0, the stream of a single LangevinMD run."""
+    return 2 * int(rid)
+
+
+def velocity_stream(rid: int) -> tuple[int, int]:
+    """(step, stream) of request rid's initial velocities: odd streams at step 0 (request 0:
+    stream 1, as a single LangevinMD run draws them)."""
+    return 0, 2 * int(rid) + 1
+
+
 @dataclass
 class Request:
     species: np.ndarray
@@ -149,7 +162,8 @@ class ServingEngine:
             if r.md_steps > 0:
                 m = units.ATOMIC_MASSES[r.species] / units.DA
                 vv = torch.empty((r.n, 3), dtype=torch.float64, device=self.dev)
-                _abi.check(self.lib.fnl_philox_normals(vv.data_ptr(), r.n, self.seed, r.rid, 1,
+                step0, stream0 = velocity_stream(r.rid)
+                _abi.check(self.lib.fnl_philox_normals(vv.data_ptr(), r.n, self.seed, step0, stream0,
                                                        _abi.stream_handle()), "fnl_philox_normals")
                 self.v[o:o + r.n] = vv * torch.from_numpy(np.sqrt(self.kT / m)).to(self.dev)[:, None]
             else:
@@ -172,7 +186,7 @@ class ServingEngine:
         keys = np.zeros(len(self.active), np.int32)
         for b, r in enumerate(self.active):
             o = self._row0[r.rid]
-            keys[b] = r.rid
+            keys[b] = noise_stream(r.rid)
             steps[b] = r.steps_done
             if r.md_steps > 0 and r.evaluated and r.steps_done < r.md_steps:
                 m = self._masses[o:o + r.n]

@@ -3,9 +3,12 @@
  * energy / force / MD path.
  *
  * Conventions
- *   - Every buffer is device memory owned by the caller; the library keeps no
- *     global state and never allocates.  Temporaries live in a caller-provided
- *     workspace whose size is returned by the matching *_workspace() call.
+ *   - Every buffer is device memory owned by the caller; the library never
+ *     allocates.  Temporaries live in a caller-provided workspace whose size is
+ *     returned by the matching *_workspace() call.  The only process-wide state
+ *     is a mutex-protected record of the kernels whose dynamic-LDS limit has
+ *     been raised on each device, so every entry point may be called from
+ *     several host threads at once (on different buffers).
  *   - All work is enqueued on the caller's stream; no call synchronises.
  *   - Return value: 0 on success, <0 on invalid arguments or a HIP launch error
  *     (fnl_strerror() names it).  Capacity overflows are reported on the device
@@ -63,6 +66,11 @@ typedef struct ihipStream_t* fnl_stream_t; /* == hipStream_t */
 #define FNL_STATUS_MAX_CELL 3       /* largest cell population seen                 */
 #define FNL_STATUS_STICKY 4         /* overflow flags OR-ed over every build since the
                                        caller last cleared this word                */
+#define FNL_STATUS_STICKY_MAX_NBR 5     /* largest radial count of an overflowing row
+                                           over every build since the caller last
+                                           cleared words 4-6 (sizes the rows after a
+                                           rollback to a checkpoint)                 */
+#define FNL_STATUS_STICKY_MAX_NBR_ANG 6 /* the same for the angular rows              */
 #define FNL_STATUS_WORDS 8
 
 int fnl_version(void);

@@ -42,7 +42,6 @@ __all__ = [
     "ATOMIC_MASSES",
     "MDUnits",
     "md_units",
-    "check_input",
     "half_pairs",
     "skin_update",
     "symmetrize",
@@ -70,7 +69,6 @@ __all__ = [
     "philox4x32",
     "philox_normals",
     "baoab_step",
-    "kinetic_tensor",
 ]
 
 # ---------------------------------------------------------------------------
@@ -136,27 +134,6 @@ class MDUnits:
 
 
 md_units = MDUnits()
-
-
-def check_input(inputs: dict) -> dict:
-    """Default natoms / batch_index and cell shapes (preprocessing.py:1763-1791)."""
-    species = np.asarray(inputs["species"]).astype(np.int32)
-    nat = species.shape[0]
-    natoms = np.asarray(inputs.get("natoms", np.array([nat]))).astype(np.int32)
-    batch_index = np.asarray(
-        inputs.get("batch_index", np.repeat(np.arange(len(natoms), dtype=np.int32), natoms))
-    ).astype(np.int32)
-    out = {**inputs, "species": species, "natoms": natoms, "batch_index": batch_index}
-    if "cells" in inputs:
-        cells = np.asarray(inputs["cells"])
-        rec = np.asarray(inputs.get("reciprocal_cells", np.linalg.inv(cells)))
-        if cells.ndim == 2:
-            cells = cells[None]
-        if rec.ndim == 2:
-            rec = rec[None]
-        out["cells"] = cells
-        out["reciprocal_cells"] = rec
-    return out
 
 
 # ---------------------------------------------------------------------------
@@ -817,9 +794,3 @@ def baoab_step(x, v, forces, mass, dt, a1, a2, noise):
     v = a1 * v + np.asarray(a2, np.float64)[:, None] * noise
     x = x + (0.5 * dt) * v
     return x, v
-
-
-def kinetic_tensor(v, mass):
-    """ek = 0.5 sum m v v^T (integrate.py:320-339)."""
-    v = np.asarray(v, np.float64)
-    return 0.5 * np.einsum("i,ia,ib->ab", np.asarray(mass, np.float64), v, v)
