@@ -15,6 +15,8 @@ FNL_OK = 0
 FNL_ERR_INVALID = -1
 FNL_ERR_HIP = -2
 FNL_ERR_UNSUPPORTED = -3
+FNL_ERR_SCRATCH_TOO_SMALL = -4
+FNL_NBLIST_SCRATCH_BYTES = 16 << 20
 FNL_PBC_NONE = 0
 FNL_PBC_MINIMUM_IMAGE = 1
 FNL_PBC_GENERAL = 2
@@ -85,6 +87,8 @@ _SIGNATURES = {
     "fnl_nblist_workspace": (sz, [i32, i32, i32]),
     "fnl_nblist_build": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, i32, f64, f64, i32, i32,
                                    vp, sz, vp, vp, vp, vp, vp, vp, vp]),
+    "fnl_nblist_build_noalloc": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, i32, f64, f64, i32, i32,
+                                           vp, vp, vp, vp, vp, vp, vp]),
     "fnl_nblist_skin_update": (C.c_int, [vp, vp, i32, i32, vp, vp, vp, vp, vp, i32, f64, f64, i32, i32, vp, vp,
                                          vp, vp, vp, vp, vp]),
     "fnl_nblist_export": (C.c_int, [vp, vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp]),

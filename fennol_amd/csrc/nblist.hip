@@ -29,6 +29,8 @@
 // positions on the steps between builds):
 //   k_status_clear, k_skin_filter (one wave per centre)
 
+#include <vector>
+
 #include "fnl_common.h"
 
 namespace {
@@ -826,6 +828,32 @@ __global__ void k_export(const float* __restrict__ coords, const int32_t* __rest
 
 }  // namespace
 
+// Scratch of the allocation-free build (fnl_nblist_build_noalloc): a file-scope device array, one
+// copy per GPU at its own address.  Host code never passes the array's name to a kernel (in host
+// code it is a stand-in, not a device address): the address is looked up once per device with
+// hipGetSymbolAddress, with that device current, and kept in scratch_ptrs.
+static __device__ __align__(256) unsigned char scratch[FNL_NBLIST_SCRATCH_BYTES];
+
+namespace {
+
+int scratch_address(void** out) {
+  static std::mutex mu;
+  static std::vector<void*> scratch_ptrs;  // per device; nullptr until looked up
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return FNL_ERR_HIP;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((size_t)dev >= scratch_ptrs.size()) scratch_ptrs.resize((size_t)dev + 1, nullptr);
+  if (!scratch_ptrs[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FNL_ERR_HIP;
+    scratch_ptrs[dev] = p;
+  }
+  *out = scratch_ptrs[dev];
+  return FNL_OK;
+}
+
+}  // namespace
+
 extern "C" int fnl_version(void) { return 10000; }
 
 extern "C" const char* fnl_strerror(int code) {
@@ -834,6 +862,8 @@ extern "C" const char* fnl_strerror(int code) {
     case FNL_ERR_INVALID: return "invalid argument";
     case FNL_ERR_HIP: return "HIP launch error";
     case FNL_ERR_UNSUPPORTED: return "unsupported configuration";
+    case FNL_ERR_SCRATCH_TOO_SMALL:
+      return "the cell-list temporaries exceed the library's static scratch (FNL_NBLIST_SCRATCH_BYTES)";
     default: return "unknown error";
   }
 }
@@ -918,6 +948,28 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
     FNL_LAUNCH_CHECK();
   }
   return FNL_OK;
+}
+
+extern "C" int fnl_nblist_build_noalloc(const float* coords, const int32_t* species_idx,
+                                        const int32_t* batch_index, const int32_t* sys_offsets,
+                                        int32_t n_atoms, int32_t n_sys, int32_t pbc_mode,
+                                        const float* cells, const float* reciprocal_cells,
+                                        const int32_t* grid, int32_t n_cells, double cutoff,
+                                        double cutoff_ang, int32_t max_nbr, int32_t max_nbr_ang,
+                                        int32_t* nbr, int32_t* nbr_shift, int32_t* n_nbr,
+                                        int32_t* nbr_ang, int32_t* n_nbr_ang, int32_t* status,
+                                        fnl_stream_t stream) {
+  if (n_atoms < 0 || n_sys < 1 || n_cells < 1) return FNL_ERR_INVALID;
+  const size_t need = fnl_nblist_workspace(n_atoms, n_sys, n_cells);
+  if (need == 0) return FNL_ERR_INVALID;
+  if (need > FNL_NBLIST_SCRATCH_BYTES) return FNL_ERR_SCRATCH_TOO_SMALL;
+  void* ws = nullptr;
+  if (const int rc = scratch_address(&ws)) return rc;
+  // the build's first kernel (k_grid) clears the completion count and the cell counters in the
+  // workspace; everything else in it is written before it is read
+  return fnl_nblist_build(coords, species_idx, batch_index, sys_offsets, n_atoms, n_sys, pbc_mode, cells,
+                          reciprocal_cells, grid, n_cells, cutoff, cutoff_ang, max_nbr, max_nbr_ang, ws,
+                          FNL_NBLIST_SCRATCH_BYTES, nbr, nbr_shift, n_nbr, nbr_ang, n_nbr_ang, status, stream);
 }
 
 extern "C" int fnl_nblist_skin_update(const float* coords, const int32_t* batch_index, int32_t n_atoms,

@@ -126,6 +126,32 @@ int fnl_nblist_build(const float* coords, const int32_t* species_idx,
  * reference pairs i < j).  Cost O(atoms * shifts) per centre, meant for the
  * small cells that need it.  nbr_shift is ignored (may be NULL) otherwise. */
 
+/* fnl_nblist_build without a workspace argument and without any allocation:
+ * the cell-list temporaries live in a scratch array of the library
+ * (FNL_NBLIST_SCRATCH_BYTES per GPU, held until the process ends).  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP-graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Arguments and results as
+ * fnl_nblist_build.  Returns FNL_ERR_SCRATCH_TOO_SMALL, before any work, if
+ * fnl_nblist_workspace(n_atoms, n_sys, n_cells) exceeds the scratch size.
+ * Every call on one GPU shares that GPU's scratch (each call starts by
+ * clearing the counters it needs at zero): calls that overlap on one GPU --
+ * on different streams, or from several host threads -- race and must not be
+ * made. */
+#define FNL_NBLIST_SCRATCH_BYTES ((size_t)16 << 20)
+#define FNL_ERR_SCRATCH_TOO_SMALL (-4)
+
+int fnl_nblist_build_noalloc(const float* coords, const int32_t* species_idx,
+                             const int32_t* batch_index, const int32_t* sys_offsets,
+                             int32_t n_atoms, int32_t n_sys, int32_t pbc_mode,
+                             const float* cells, const float* reciprocal_cells,
+                             const int32_t* grid, int32_t n_cells,
+                             double cutoff, double cutoff_ang,
+                             int32_t max_nbr, int32_t max_nbr_ang,
+                             int32_t* nbr, int32_t* nbr_shift, int32_t* n_nbr,
+                             int32_t* nbr_ang, int32_t* n_nbr_ang,
+                             int32_t* status, fnl_stream_t stream);
+
 /* Skin update: the radial and angular rows at the current positions selected
  * from the rows of a list built by fnl_nblist_build with cutoff + skin
  * (nbr_skin / n_skin / max_skin, radial rows only).  Replaces
