@@ -2,7 +2,7 @@
 
 Mirrors FeNNol's ``FENNIX`` (models/fennix.py:26-688) for the module set of
 the ANI path: preprocessing GRAPH (+ GRAPH_FILTER, GRAPH_ANGULAR_EXTENSION,
-SPECIES_INDEXER) and modules ANI_AEV + SPECIES_INDEX_NET.  Same constructor
+SPECIES_INDEXER) and modules ANI_AEV + SPECIES_INDEX_NET (or CHEMICAL_NET).  Same constructor
 arguments (``cutoff``, ``modules``, ``preprocessing``, ``variables``,
 ``energy_terms``, ``energy_unit``), the same entry points (``preprocess``,
 ``__call__``, ``total_energy``, ``energy_and_forces``,
@@ -22,12 +22,13 @@ import numpy as np
 import torch
 
 from . import units
-from .params import from_variables, init_species_params, to_variables
+from .params import (from_chemical_variables, from_variables, init_species_params, to_chemical_variables,
+                     to_variables)
 from .potential import ANIConfig, ANIPotential
 from .preprocessing import (PREPROCESSING, GraphAngularExtension, GraphFilter, GraphGenerator,
-                            PreprocessingChain, SpeciesIndexer, check_input)
+                            PreprocessingChain, SpeciesIndexer, atom_unpadding, check_input)
 
-MODULES = {"ANI_AEV": "ANI_AEV", "SPECIES_INDEX_NET": "SPECIES_INDEX_NET"}
+MODULES = {"ANI_AEV": "ANI_AEV", "SPECIES_INDEX_NET": "SPECIES_INDEX_NET", "CHEMICAL_NET": "CHEMICAL_NET"}
 
 
 def _species_list(v):
@@ -115,12 +116,15 @@ class FENNIX:
             key = str(params.pop("module_name", params.pop("FID", name))).upper()
             if key == "ANI_AEV":
                 aev_p = (name, params)
-            elif key == "SPECIES_INDEX_NET":
+            elif key in ("SPECIES_INDEX_NET", "CHEMICAL_NET"):
+                if sin_p is not None:
+                    raise NotImplementedError("one species network module is supported")
                 sin_p = (name, params)
+                self.network_kind = key
             else:
                 raise NotImplementedError(f"module {key} is not part of the ANI path")
         if aev_p is None or sin_p is None:
-            raise ValueError("the ANI path needs an ANI_AEV and a SPECIES_INDEX_NET module")
+            raise ValueError("the ANI path needs an ANI_AEV and a SPECIES_INDEX_NET or CHEMICAL_NET module")
         aname, ap = aev_p
         self.embedding_key = ap.get("embedding_key", "embedding")
         graph_key = ap.get("graph_key", "graph")
@@ -143,10 +147,19 @@ class FENNIX:
         sname, sp = sin_p
         self.energy_module = sname
         self.energy_key = sp.get("output_key") or sname
-        hidden = sp["hidden_neurons"]
-        net_species = list(hidden.keys()) if isinstance(hidden, dict) else _species_list(sp["species_order"])
-        widths = list(next(iter(hidden.values()))) if isinstance(hidden, dict) else list(hidden)
-        if widths != [256, 192, 160] or int(sp.get("output_dim", 1)) != 1:
+        chemical = self.network_kind == "CHEMICAL_NET"
+        if chemical:
+            # ChemicalNet (nets.py:429-512): `neurons` includes the output width; every listed
+            # species has a network, atoms of other elements get 0
+            neurons = list(sp["neurons"])
+            net_species = _species_list(sp["species_order"])
+            widths, out_dim = neurons[:-1], neurons[-1]
+        else:
+            hidden = sp["hidden_neurons"]
+            net_species = list(hidden.keys()) if isinstance(hidden, dict) else _species_list(sp["species_order"])
+            widths = list(next(iter(hidden.values()))) if isinstance(hidden, dict) else list(hidden)
+            out_dim = int(sp.get("output_dim", 1))
+        if widths != [256, 192, 160] or int(out_dim) != 1:
             raise NotImplementedError("the MLP kernel implements hidden widths [256, 192, 160] -> 1")
         if not sp.get("use_bias", True):
             raise NotImplementedError("use_bias=False")
@@ -155,13 +168,15 @@ class FENNIX:
             if seed is None:
                 raise ValueError("Either variables or a seed must be provided for initialization")
             params = init_species_params(seed, self.config.aev_length, tuple(widths), 1, tuple(net_species))
-            variables = to_variables(params, sname)
+            variables = to_chemical_variables(params, sname) if chemical else to_variables(params, sname)
         self.variables = variables
-        params = from_variables(variables, sname)
-        self.potential = ANIPotential(self.config, params, precision=precision, device=device)
+        self.net_species = net_species
+        params = from_chemical_variables(variables, net_species, sname) if chemical else from_variables(variables, sname)
+        self.potential = ANIPotential(self.config, params, precision=precision, device=device,
+                                      unhandled_species="zero" if chemical else "raise")
         if self.celu_alpha != 0.1:
             self.potential.nets.cw.celu_alpha = self.celu_alpha
-        self.preprocessing = PreprocessingChain(layers, self.potential)
+        self.preprocessing = PreprocessingChain(layers, self.potential, use_atom_padding=use_atom_padding)
         self._graphs_properties = self.preprocessing.get_graphs_properties()
         self.preproc_state = self.preprocessing.init()
         self._cache = {}
@@ -200,8 +215,12 @@ class FENNIX:
         return {**out, "coordinates": x32, "_buffers": sb}
 
     # -------------------------------------------------------------- compute
-    def _run(self, unit, forces: bool, **inputs):
+    def _run(self, unit, forces: bool, embedding: bool = False, **inputs):
+        """Energies (and forces, virial) on the device; with use_atom_padding the inputs are padded
+        by the chain's AtomPadding first and the outputs unpadded (fennix.py:416-442, 471-540)."""
         inputs = check_input(inputs)
+        if self.use_atom_padding:
+            self.preproc_state, inputs = self.preprocessing.atom_padding(self.preproc_state, inputs)
         sb = self._buffers(inputs)
         x32 = torch.as_tensor(np.asarray(inputs["coordinates"], np.float32),
                               device=self.potential.device).contiguous()
@@ -219,14 +238,16 @@ class FENNIX:
             v = sb.virial.cpu().numpy().reshape(-1, 3, 3) * mult
             out["forces"] = f
             out["virial_tensor"] = v
+        if embedding:
+            out[self.embedding_key] = sb.aev.float().cpu().numpy()[:, : self.config.aev_length]
+        if self.use_atom_padding:
+            out = atom_unpadding(out)
+            e, f, v = out["total_energy"], out.get("forces"), out.get("virial_tensor")
         return e, f, v, out
 
     def __call__(self, variables=None, gpu_preprocessing=True, **inputs) -> dict:
         """Model outputs incl. the AEV (``embedding`` key) and atomic energies (fennix.py:466-478)."""
-        e, f, v, out = self._run(None, False, **inputs)
-        sb = self._buffers(check_input(inputs))
-        out[self.embedding_key] = sb.aev.float().cpu().numpy()[:, : self.config.aev_length]
-        return out
+        return self._run(None, False, embedding=True, **inputs)[3]
 
     def total_energy(self, variables=None, unit=None, gpu_preprocessing=True, **inputs):
         e, _, _, out = self._run(unit, False, **inputs)
@@ -260,6 +281,8 @@ class FENNIX:
 
     @property
     def params(self):
+        if self.network_kind == "CHEMICAL_NET":
+            return from_chemical_variables(self.variables, self.net_species, self.energy_module)
         return from_variables(self.variables, self.energy_module)
 
     # ------------------------------------------------------------- files

@@ -50,6 +50,35 @@ def to_variables(params: dict, module_name: str = "energy") -> dict:
     return {"params": {module_name: tree}}
 
 
+def to_chemical_variables(params: dict, module_name: str = "energy") -> dict:
+    """Flax-style variables tree of a ChemicalNet (nets.py:429-512): the species networks
+    vmapped over a leading species axis, {"VmapFullyConnectedNet_0": {"Layer_i": {"kernel"
+    [S, in, out], "bias" [S, out]}}}, species in the module's species_order."""
+    species = list(params.keys())
+    n = len(params[species[0]])
+    tree = {f"Layer_{i + 1}": {"kernel": np.stack([np.asarray(params[s][i][0]) for s in species]),
+                               "bias": np.stack([np.asarray(params[s][i][1]) for s in species])}
+            for i in range(n)}
+    return {"params": {module_name: {"VmapFullyConnectedNet_0": tree}}}
+
+
+def from_chemical_variables(variables: dict, species_order, module_name: str = "energy") -> dict:
+    """Per-species layers of a ChemicalNet variables tree (the inverse of to_chemical_variables;
+    the vmapped sub-tree is the one child holding Layer_* entries)."""
+    tree = variables["params"][module_name]
+    if "Layer_1" not in tree:
+        subs = [v for v in tree.values() if isinstance(v, dict) and "Layer_1" in v]
+        if len(subs) != 1:
+            raise ValueError("ChemicalNet variables: expected one vmapped FullyConnectedNet sub-tree")
+        tree = subs[0]
+    n = len([k for k in tree if k.startswith("Layer_")])
+    out = {}
+    for k, s in enumerate(species_order):
+        out[s] = [(np.asarray(tree[f"Layer_{i + 1}"]["kernel"])[k], np.asarray(tree[f"Layer_{i + 1}"]["bias"])[k])
+                  for i in range(n)]
+    return out
+
+
 def from_variables(variables: dict, module_name: str = "energy") -> dict:
     tree = variables["params"][module_name]
     out = {}

@@ -101,6 +101,30 @@ def _status_errors(st) -> None:
         raise RuntimeError("an atom lies more than 511 cells from its cell's origin (general PBC shift range)")
 
 
+def grid_caps(cutoff: float, natoms, cells) -> tuple[np.ndarray, int]:
+    """Per-system cell-grid capacities (cap_x, cap_y, cap_z, first_cell) for fnl_nblist_build and
+    the total cell count.  Systems without atoms (the padding system of AtomPadding, whose cell is
+    1000 A wide) get one cell."""
+    B = len(natoms)
+    g = np.zeros((B, 4), np.int32)
+    first = 0
+    rbin = cutoff * 1.0001 + 1.0e-4
+    for b in range(B):
+        if int(natoms[b]) == 0:
+            caps = np.array([1, 1, 1])
+        elif cells is not None:
+            R = np.linalg.inv(np.asarray(cells[b], np.float64))
+            spacing = 1.0 / np.sqrt((R ** 2).sum(axis=0))
+            caps = np.maximum(np.floor(spacing / rbin).astype(int), 1) + 1
+        else:
+            k = max(1, int(math.ceil(max(int(natoms[b]), 1) ** (1.0 / 3.0))))
+            caps = np.array([k, k, k])
+        g[b, :3] = caps
+        g[b, 3] = first
+        first += int(np.prod(caps))
+    return g, max(first, 1)
+
+
 _CUS = []
 
 
@@ -269,7 +293,8 @@ class ANIPotential:
     """
 
     def __init__(self, config: ANIConfig, params: dict, precision: str = "fp32", device="cuda",
-                 max_nbr: int = 64, max_nbr_ang: int = 32, mlp_tile_rows: int = 0):
+                 max_nbr: int = 64, max_nbr_ang: int = 32, mlp_tile_rows: int = 0,
+                 unhandled_species: str = "raise"):
         self.lib = _abi.load()
         self.config = config
         self.precision = precision
@@ -281,6 +306,12 @@ class ANIPotential:
         self._row_dtype = _abi.FNL_F32 if precision == "fp32" else _abi.FNL_BF16  # AEV and dE/dAEV rows
         self.init_max_nbr = max_nbr
         self.init_max_nbr_ang = max_nbr_ang
+        # atoms of an element without a network: "raise" (SpeciesIndexNet, nets.py:394-397) or
+        # "zero" (ChemicalNet, nets.py:480-505: their output is 0, while their AEV and their
+        # place in their neighbours' AEVs use the AEV's species map as for every atom)
+        if unhandled_species not in ("raise", "zero"):
+            raise ValueError("unhandled_species must be 'raise' or 'zero'")
+        self.unhandled_species = unhandled_species
         self.timer: StageTimer | None = None
 
     def _stage(self, name: str):
@@ -289,22 +320,7 @@ class ANIPotential:
     # ------------------------------------------------------------------ setup
     def grid_caps(self, natoms, cells) -> tuple[np.ndarray, int]:
         """Per-system cell-grid capacities (cap_x, cap_y, cap_z, first_cell)."""
-        B = len(natoms)
-        g = np.zeros((B, 4), np.int32)
-        first = 0
-        rbin = self.config.cutoff * 1.0001 + 1.0e-4
-        for b in range(B):
-            if cells is not None:
-                R = np.linalg.inv(np.asarray(cells[b], np.float64))
-                spacing = 1.0 / np.sqrt((R ** 2).sum(axis=0))
-                caps = np.maximum(np.floor(spacing / rbin).astype(int), 1) + 1
-            else:
-                k = max(1, int(math.ceil(max(int(natoms[b]), 1) ** (1.0 / 3.0))))
-                caps = np.array([k, k, k])
-            g[b, :3] = caps
-            g[b, 3] = first
-            first += int(np.prod(caps))
-        return g, max(first, 1)
+        return grid_caps(self.config.cutoff, natoms, cells)
 
     def setup(self, species, natoms=None, cells=None, skin: float = 0.0,
               minimum_image: bool = True) -> SystemBuffers:
@@ -329,7 +345,7 @@ class ANIPotential:
         real = species[: offs[-1]]
         real = real[real > 0]
         missing = sorted({PERIODIC_TABLE[z] for z in np.unique(real)} - set(self.nets.species))
-        if missing:
+        if missing and self.unhandled_species == "raise":
             raise ValueError(f"Species {missing} not found in networks. Handled species are "
                              f"{self.nets.species} (nets.py:394-397)")
         pbc = _abi.FNL_PBC_NONE
@@ -445,7 +461,7 @@ class ANIPotential:
         net = np.full(N, -1, np.int64)
         for z, n in self.net_of_z.items():
             net[species == z] = n
-        if np.any((net < 0) & (species > 0)):
+        if np.any((net < 0) & (species > 0)) and self.unhandled_species == "raise":
             missing = sorted({PERIODIC_TABLE[z] for z in np.unique(species[(net < 0) & (species > 0)])})
             raise ValueError(f"Species {missing} not found in networks")
         order = np.argsort(net, kind="stable")

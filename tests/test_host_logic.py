@@ -129,3 +129,49 @@ def test_bf16_fragment_order_layout():
             assert flat[off] == a[r, c]
     with pytest.raises(ValueError):
         fragment_order(np.zeros((20, 32)))
+
+
+def test_atom_padding_and_unpadding():
+    """AtomPadding (preprocessing.py:1654-1729): atoms to int(1.2 N) + 1 when N grows, one padding
+    system (natoms 0, cell 1000 I), masks; atom_unpadding (:1732-1760) restores the arrays."""
+    from fennol_amd.preprocessing import AtomPadding, atom_unpadding, check_input
+
+    pad = AtomPadding()
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(10, 3))
+    inp = check_input({"species": np.full(10, 8, np.int32), "coordinates": x, "cells": 9.0 * np.eye(3)})
+    st, out = pad(pad.init(), inp)
+    assert st == {"prev_nat": 13, "prev_nsys": 1}
+    assert out["species"].shape == (13,) and np.all(out["species"][10:] == -1)
+    np.testing.assert_array_equal(out["natoms"], [10, 0])
+    np.testing.assert_array_equal(out["batch_index"][10:], [1, 1, 1])
+    np.testing.assert_array_equal(out["cells"][1], 1000.0 * np.eye(3))
+    assert out["true_atoms"].sum() == 10 and list(out["true_sys"]) == [True, False]
+    back = atom_unpadding({**out, "forces": np.ones((13, 3)), "total_energy": np.array([1.0, 0.0])})
+    np.testing.assert_array_equal(back["coordinates"], x)
+    assert back["forces"].shape == (10, 3) and back["total_energy"].shape == (1,)
+    # a smaller system keeps the padded size
+    inp2 = check_input({"species": np.full(7, 1, np.int32), "coordinates": x[:7], "cells": 9.0 * np.eye(3)})
+    st2, out2 = pad(st, inp2)
+    assert st2 == st and out2["species"].shape == (13,)
+
+
+def test_serving_noise_and_velocity_streams_are_disjoint():
+    """The O-step noise of request r (counter (atom, step, 2r)) never reuses the counters of any
+    request's initial velocities (atom, 0, 2r' + 1); request 0 matches a single LangevinMD run
+    (noise stream 0, velocities stream 1)."""
+    from fennol_amd.serving import noise_stream, velocity_stream
+
+    noise = {noise_stream(r) for r in range(5000)}
+    vel = {velocity_stream(r)[1] for r in range(5000)}
+    assert not noise & vel
+    assert noise_stream(0) == 0 and velocity_stream(0) == (0, 1)
+    assert len(vel) == 5000 and len(noise) == 5000
+
+
+def test_grown_capacity():
+    from fennol_amd.md import grown_capacity
+
+    assert grown_capacity(64, 60) == 64
+    assert grown_capacity(64, 70) == 96  # int(1.25 * 70) + 8 = 95 -> multiple of 8
+    assert grown_capacity(16, 69) % 8 == 0 and grown_capacity(16, 69) >= 69
