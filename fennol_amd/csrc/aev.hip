@@ -25,8 +25,9 @@
 // dE_i/dvec_ie from the centre's dE/dAEV row and dE_j/dvec_ji from the
 // neighbour's row, so radial forces are gathered, not scattered.  Angular:
 // triplets in 4x4 blocks, DPP pre-sums, one LDS float add per slot and block;
-// the angular slot gradients are then scattered to grad_nbr with global
-// atomics.  grad_self[i] is written once, virial_atom[i] = sum vec (x) dE/dvec.
+// the angular slot gradients are then scattered to grad_nbr with global int64
+// atomics on 2^-32 fixed point (exact, order-independent sums: the same bits
+// every run).  grad_self[i] is written once, virial_atom[i] = sum vec (x) dE/dvec.
 
 #include <hip/hip_bfloat16.h>
 
@@ -566,9 +567,10 @@ __host__ __device__ inline int bwd_lds_words(int max_nbr, int max_nbr_ang, int a
 // centre's row G_i: self term and virial of i) and dE_j/dvec_ji (from j's row G_j, read from
 // global memory: the neighbour term that j's own radial block gives i), so the radial
 // neighbour terms are gathered per atom and written with the self term -- no atomics.  The
-// angular terms of each angular slot are scattered to its atom with global atomics.
+// angular terms of each angular slot are scattered to its atom with global int64 atomics on
+// fixed point (exact sums in any order).
 // Outputs: grad_self[i] = -sum_e dE_i/dvec_ie - sum_j dE_j/dvec_ji|radial;
-// grad_nbr[j] += dE_i/dvec_ij|angular; virial_atom[i] = sum_e vec_ie (x) dE_i/dvec_ie.
+// grad_nbr[j] += dE_i/dvec_ij|angular * 2^32 (int64); virial_atom[i] = sum_e vec_ie (x) dE_i/dvec_ie.
 template <int NA, int NZ, typename GT>
 __global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(5)))
 k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
@@ -578,7 +580,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           const int32_t* __restrict__ nbr_shift,
           const int32_t* __restrict__ nbr_ang, const int32_t* __restrict__ n_nbr_ang, int max_nbr_ang,
           fnl_aev_params p, AevDims dm, const GT* __restrict__ g, int g_stride,
-          float* __restrict__ grad_self, float* __restrict__ grad_nbr, float* __restrict__ virial_atom) {
+          float* __restrict__ grad_self, long long* __restrict__ grad_nbr, float* __restrict__ virial_atom) {
   static_assert(NA * NZ == 32, "angular channel layout");
   extern __shared__ float4 lds4[];
   const int w = threadIdx.x >> 6;
@@ -792,9 +794,10 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     const float4 gg = gs[a];
     const float4 ua = av[a];
     const int j = __float_as_int(af[a].w) & 0x3fffffff;
-    atomicAdd(&grad_nbr[3 * j], gg.x);
-    atomicAdd(&grad_nbr[3 * j + 1], gg.y);
-    atomicAdd(&grad_nbr[3 * j + 2], gg.z);
+    unsigned long long* gn = reinterpret_cast<unsigned long long*>(grad_nbr + 3 * j);
+    __hip_atomic_fetch_add(gn, fnl_to_fixed(gg.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(gn + 1, fnl_to_fixed(gg.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(gn + 2, fnl_to_fixed(gg.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     sx -= gg.x;
     sy -= gg.y;
     sz -= gg.z;
@@ -1046,7 +1049,7 @@ int launch_bwd(const float* coords, const int32_t* species_idx, const int32_t* b
                int pbc_mode, const float* cells, const float* rec, const int32_t* nbr,
                const int32_t* n_nbr, int max_nbr, const int32_t* nbr_shift, const int32_t* nbr_ang, const int32_t* n_nbr_ang,
                int max_nbr_ang, const fnl_aev_params& p, const AevDims& dm, const GT* g, int g_stride,
-               float* grad_self, float* grad_nbr, float* virial_atom, hipStream_t s) {
+               float* grad_self, long long* grad_nbr, float* virial_atom, hipStream_t s) {
   const size_t lds = (size_t)kWaves * 4 * bwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
   if (lds > 160 * 1024) return FNL_ERR_UNSUPPORTED;
   if (const int rc = fnl_allow_full_lds(reinterpret_cast<const void*>(&k_aev_bwd<NA, NZ, GT>))) return rc;
@@ -1109,7 +1112,7 @@ extern "C" int fnl_aev_backward(const float* coords, const int32_t* species_idx,
                                 int32_t max_nbr, const int32_t* nbr_shift, const int32_t* nbr_ang,
                                 const int32_t* n_nbr_ang,
                                 int32_t max_nbr_ang, const fnl_aev_params* params, const void* g,
-                                int32_t g_stride, int32_t g_dtype, float* grad_self, float* grad_nbr,
+                                int32_t g_stride, int32_t g_dtype, float* grad_self, int64_t* grad_nbr,
                                 float* virial_atom, uint32_t species_mask, fnl_stream_t stream) {
   AevDims dm;
   if (!check_params(params, &dm)) return FNL_ERR_INVALID;
@@ -1129,10 +1132,10 @@ extern "C" int fnl_aev_backward(const float* coords, const int32_t* species_idx,
 #define FNL_BWD(NA_, NZ_)                                                                                         \
   return bf ? launch_bwd<NA_, NZ_>(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells, \
                                    nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm,  \
-                                   gb, g_stride, grad_self, grad_nbr, virial_atom, s)                            \
+                                   gb, g_stride, grad_self, (long long*)grad_nbr, virial_atom, s)                \
             : launch_bwd<NA_, NZ_>(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells, \
                                    nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang, *params, dm,  \
-                                   gf, g_stride, grad_self, grad_nbr, virial_atom, s)
+                                   gf, g_stride, grad_self, (long long*)grad_nbr, virial_atom, s)
   if (na == 4 && nz == 8) FNL_BWD(4, 8);
   if (na == 8 && nz == 4) FNL_BWD(8, 4);
 #undef FNL_BWD

@@ -177,6 +177,20 @@ __device__ __forceinline__ float fnl_d12(float vx, float vy, float vz) {
 }
 
 // -------------------------------------------------------------------------
+// Fixed-point accumulation of the angular neighbour gradients: each float term
+// is rounded to a multiple of 2^-32 in an int64, and the int64 sums are exact,
+// so the result does not depend on the order the atomics land in (forces are
+// the same bits every run).
+// -------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long fnl_to_fixed(float v) {
+  return (unsigned long long)__double2ll_rn((double)v * (double)FNL_GRAD_FIXED_ONE);
+}
+
+__device__ __forceinline__ double fnl_from_fixed(long long v) {
+  return (double)v * (1.0 / (double)FNL_GRAD_FIXED_ONE);
+}
+
+// -------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
 // easy as 1, 2, 3", SC'11) and Box-Muller normals.
 // -------------------------------------------------------------------------

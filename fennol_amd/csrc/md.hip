@@ -124,10 +124,10 @@ __global__ void __launch_bounds__(64) k_reduce_systems(const int32_t* __restrict
   reduce_system(b, threadIdx.x, seg_offsets, partials, energy_sys, virial_sys, kinetic_sys);
 }
 
-__global__ void k_forces(const float* __restrict__ gs, const float* __restrict__ gn, int n3, float scale,
+__global__ void k_forces(const float* __restrict__ gs, const long long* __restrict__ gn, int n3, float scale,
                          float* __restrict__ f) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n3) f[i] = -(gs[i] + gn[i]) * scale;
+  if (i < n3) f[i] = (float)(-((double)gs[i] + fnl_from_fixed(gn[i])) * (double)scale);
 }
 
 // step_dev (optional): {step, done}.  Every workgroup reads the step before it
@@ -137,7 +137,7 @@ __global__ void k_forces(const float* __restrict__ gs, const float* __restrict__
 // adjoint of the same MD step accumulates into them).
 __global__ void __launch_bounds__(256)
 k_baoab(double* __restrict__ x, double* __restrict__ v, float* __restrict__ x32,
-        const float* __restrict__ gs, float* __restrict__ gn, float force_scale,
+        const float* __restrict__ gs, long long* __restrict__ gn, float force_scale,
         const double* __restrict__ dtm, double half_dt, double a1,
         const double* __restrict__ a2, const double* __restrict__ noise, uint64_t seed,
         uint64_t step_arg, uint64_t* __restrict__ step_dev, int n_atoms, int clear_gn) {
@@ -149,7 +149,7 @@ k_baoab(double* __restrict__ x, double* __restrict__ v, float* __restrict__ x32,
     const double m = dtm[i];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const double f = -((double)gs[3 * i + k] + (double)gn[3 * i + k]) * (double)force_scale;
+      const double f = -((double)gs[3 * i + k] + fnl_from_fixed(gn[3 * i + k])) * (double)force_scale;
       double vv = v[3 * i + k] + m * f;           // B (full dt: merged half kicks)
       double xx = x[3 * i + k] + half_dt * vv;    // A
       if (a2) {                                   // O
@@ -160,7 +160,7 @@ k_baoab(double* __restrict__ x, double* __restrict__ v, float* __restrict__ x32,
       v[3 * i + k] = vv;
       x[3 * i + k] = xx;
       x32[3 * i + k] = (float)xx;
-      if (clear_gn) gn[3 * i + k] = 0.0f;
+      if (clear_gn) gn[3 * i + k] = 0;
     }
   }
   if (step_dev) {
@@ -179,7 +179,7 @@ k_baoab(double* __restrict__ x, double* __restrict__ v, float* __restrict__ x32,
 // the noise of an atom depends only on (key_b, step_b, local atom index) and
 // not on where the system sits in the batch.
 __global__ void k_baoab_batched(double* __restrict__ x, double* __restrict__ v, float* __restrict__ x32,
-                                const float* __restrict__ gs, const float* __restrict__ gn, float force_scale,
+                                const float* __restrict__ gs, const long long* __restrict__ gn, float force_scale,
                                 const double* __restrict__ dtm, double half_dt, double a1,
                                 const double* __restrict__ a2, const int32_t* __restrict__ batch_index,
                                 const int32_t* __restrict__ sys_offsets, const uint64_t* __restrict__ sys_step,
@@ -193,7 +193,7 @@ __global__ void k_baoab_batched(double* __restrict__ x, double* __restrict__ v, 
   fnl_normals3(seed, sys_step[b], sys_key[b], (uint32_t)(i - sys_offsets[b]), xi);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const double f = -((double)gs[3 * i + k] + (double)gn[3 * i + k]) * (double)force_scale;
+    const double f = -((double)gs[3 * i + k] + fnl_from_fixed(gn[3 * i + k])) * (double)force_scale;
     double vv = v[3 * i + k] + m * f;
     double xx = x[3 * i + k] + half_dt * vv;
     vv = a1 * vv + a2[i] * xi[k];
@@ -248,18 +248,18 @@ extern "C" int fnl_system_reduce(const int32_t* sys_offsets, const int32_t* seg_
   return FNL_OK;
 }
 
-extern "C" int fnl_forces(const float* grad_self, const float* grad_nbr, int32_t n_atoms, float scale,
+extern "C" int fnl_forces(const float* grad_self, const int64_t* grad_nbr, int32_t n_atoms, float scale,
                           float* forces, fnl_stream_t stream) {
   if (n_atoms < 0) return FNL_ERR_INVALID;
   if (n_atoms == 0) return FNL_OK;
   if (!grad_self || !grad_nbr || !forces) return FNL_ERR_INVALID;
   hipLaunchKernelGGL(k_forces, dim3(fnl_blocks(3LL * n_atoms, 256)), dim3(256), 0, (hipStream_t)stream,
-                     grad_self, grad_nbr, 3 * n_atoms, scale, forces);
+                     grad_self, (const long long*)grad_nbr, 3 * n_atoms, scale, forces);
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }
 
-extern "C" int fnl_baoab_step(double* x, double* v, float* x32, const float* grad_self, float* grad_nbr,
+extern "C" int fnl_baoab_step(double* x, double* v, float* x32, const float* grad_self, int64_t* grad_nbr,
                               float force_scale, const double* dtm, double half_dt, double a1, const double* a2,
                               const double* noise, uint64_t seed, uint64_t step, uint64_t* step_dev,
                               int32_t n_atoms, int32_t clear_grad_nbr, fnl_stream_t stream) {
@@ -267,14 +267,14 @@ extern "C" int fnl_baoab_step(double* x, double* v, float* x32, const float* gra
   if (n_atoms == 0) return FNL_OK;
   if (!x || !v || !x32 || !grad_self || !grad_nbr || !dtm) return FNL_ERR_INVALID;
   hipLaunchKernelGGL(k_baoab, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, (hipStream_t)stream, x, v, x32,
-                     grad_self, grad_nbr, force_scale, dtm, half_dt, a1, a2, noise, seed, step, step_dev, n_atoms,
+                     grad_self, (long long*)grad_nbr, force_scale, dtm, half_dt, a1, a2, noise, seed, step, step_dev, n_atoms,
                      (int)(clear_grad_nbr != 0));
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }
 
 extern "C" int fnl_baoab_step_batched(double* x, double* v, float* x32, const float* grad_self,
-                                      const float* grad_nbr, float force_scale, const double* dtm, double half_dt,
+                                      const int64_t* grad_nbr, float force_scale, const double* dtm, double half_dt,
                                       double a1, const double* a2, const int32_t* batch_index,
                                       const int32_t* sys_offsets, const uint64_t* sys_step, const uint32_t* sys_key,
                                       uint64_t seed, int32_t n_atoms, fnl_stream_t stream) {
@@ -284,7 +284,7 @@ extern "C" int fnl_baoab_step_batched(double* x, double* v, float* x32, const fl
       !sys_key)
     return FNL_ERR_INVALID;
   hipLaunchKernelGGL(k_baoab_batched, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, (hipStream_t)stream, x, v, x32,
-                     grad_self, grad_nbr, force_scale, dtm, half_dt, a1, a2, batch_index, sys_offsets, sys_step,
+                     grad_self, (const long long*)grad_nbr, force_scale, dtm, half_dt, a1, a2, batch_index, sys_offsets, sys_step,
                      sys_key, seed, n_atoms);
   FNL_LAUNCH_CHECK();
   return FNL_OK;

@@ -246,7 +246,7 @@ class SystemBuffers:
     g_aev: torch.Tensor = None
     e_atom: torch.Tensor = None
     grad_self: torch.Tensor = None
-    grad_nbr: torch.Tensor = None
+    grad_nbr: torch.Tensor = None       # int64 [N,3] angular neighbour gradients, 2^-32 fixed point
     vir_atom: torch.Tensor = None
     energy: torch.Tensor = None         # float64 [B]
     virial: torch.Tensor = None         # float64 [B,9]
@@ -386,7 +386,7 @@ class ANIPotential:
         sb.g_aev = torch.zeros((N, kp), dtype=adt, device=dev)  # dE/dAEV rows: BF16 in the BF16 form
         sb.e_atom = torch.zeros(N, dtype=torch.float32, device=dev)
         sb.grad_self = torch.zeros((N, 3), dtype=torch.float32, device=dev)
-        sb.grad_nbr = torch.zeros((N, 3), dtype=torch.float32, device=dev)
+        sb.grad_nbr = torch.zeros((N, 3), dtype=torch.int64, device=dev)  # 2^-32 fixed point
         sb.vir_atom = torch.zeros((N, 9), dtype=torch.float32, device=dev)
         sb.energy = torch.zeros(B, dtype=torch.float64, device=dev)
         sb.virial = torch.zeros((B, 9), dtype=torch.float64, device=dev)
@@ -433,7 +433,7 @@ class ANIPotential:
         sb.g_aev = torch.zeros((max_atoms, kp), dtype=adt, device=dev)
         sb.e_atom = torch.zeros(max_atoms, dtype=torch.float32, device=dev)
         sb.grad_self = torch.zeros((max_atoms, 3), dtype=torch.float32, device=dev)
-        sb.grad_nbr = torch.zeros((max_atoms, 3), dtype=torch.float32, device=dev)
+        sb.grad_nbr = torch.zeros((max_atoms, 3), dtype=torch.int64, device=dev)  # 2^-32 fixed point
         sb.vir_atom = torch.zeros((max_atoms, 9), dtype=torch.float32, device=dev)
         sb.energy = torch.zeros(max_sys, dtype=torch.float64, device=dev)
         sb.virial = torch.zeros((max_sys, 9), dtype=torch.float64, device=dev)
@@ -656,14 +656,16 @@ class ANIPotential:
         _abi.check(rc, "fnl_species_mlp")
 
     def zero_grad_nbr(self, sb: SystemBuffers, stream=None) -> None:
-        """Clear the neighbour-gradient rows the adjoint accumulates into (a fill kernel)."""
-        _abi.check(self.lib.fnl_fill_f32(_abi.ptr(sb.grad_nbr), 0.0, sb.grad_nbr.numel(),
-                                         _abi.stream_handle(stream)), "fnl_fill_f32")
+        """Clear the neighbour-gradient rows the adjoint accumulates into (a fill kernel over the
+        int64 words)."""
+        _abi.check(self.lib.fnl_fill_i32(_abi.ptr(sb.grad_nbr), 0, 2 * sb.grad_nbr.numel(),
+                                         _abi.stream_handle(stream)), "fnl_fill_i32")
 
     def aev_backward(self, sb: SystemBuffers, coords32: torch.Tensor, stream=None, zero: bool = True) -> None:
         """dE/dx from dE/dAEV as grad_self + grad_nbr (grad_self: written per atom, the centre's
         terms and the radial terms gathered from its neighbours; grad_nbr: angular neighbour
-        terms accumulated with atomics, cleared first unless ``zero`` is False), virial per atom."""
+        terms accumulated with int64 atomics in 2^-32 fixed point -- exact sums, the same bits
+        every run -- cleared first unless ``zero`` is False), virial per atom."""
         if zero:
             self.zero_grad_nbr(sb, stream)
         s = _abi.stream_handle(stream)

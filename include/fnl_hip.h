@@ -52,6 +52,9 @@ typedef struct ihipStream_t* fnl_stream_t; /* == hipStream_t */
 #define FNL_SHIFT_PACK(s0, s1, s2) \
   ((int32_t)(((s0) + 512) | (((s1) + 512) << 10) | (((s2) + 512) << 20)))
 
+/* fixed-point unit of the angular neighbour gradients (fnl_aev_backward's grad_nbr) */
+#define FNL_GRAD_FIXED_ONE 4294967296.0
+
 /* element types */
 #define FNL_F32 0
 #define FNL_BF16 1
@@ -248,8 +251,12 @@ int fnl_aev_forward(const float* coords, const int32_t* species_idx,
  *   grad_self[i] = -sum_j dE_i/dvec_ij - sum_j dE_j/dvec_ji|radial
  *                  (written, float32[N][3]: the centre's own terms plus the
  *                  radial terms its neighbours' rows give it, gathered)
- *   grad_nbr[j] += dE_i/dvec_ij|angular for every angular edge (atomic,
- *                  float32[N][3]; the caller zeroes it first)
+ *   grad_nbr[j] += dE_i/dvec_ij|angular for every angular edge, in fixed
+ *                  point: int64[N][3] in units of 2^-32 (FNL_GRAD_FIXED_ONE per
+ *                  unit of the energy per length); each term is rounded to
+ *                  that grid and the integer sums are exact, so the result is
+ *                  the same bits whatever order the device-scope atomics land
+ *                  in; the caller zeroes it first
  *   virial_atom[i] = sum_j vec_ij (x) dE_i/dvec_ij   (float32[N][9], row-major a,b)
  * so that dE/dx = grad_self + grad_nbr (only the sum is meaningful) and the
  * per-system virial dE/dstrain (fennix.py:320-347) is the per-system sum of
@@ -266,7 +273,7 @@ int fnl_aev_backward(const float* coords, const int32_t* species_idx,
                      const int32_t* nbr_ang, const int32_t* n_nbr_ang, int32_t max_nbr_ang,
                      const fnl_aev_params* params,
                      const void* g, int32_t g_stride, int32_t g_dtype,
-                     float* grad_self, float* grad_nbr, float* virial_atom,
+                     float* grad_self, int64_t* grad_nbr, float* virial_atom,
                      uint32_t species_mask, fnl_stream_t stream);
 
 /* Tangent (forward-mode derivative) of the AEV along a displacement field
@@ -361,14 +368,15 @@ int fnl_system_reduce(const int32_t* sys_offsets, const int32_t* seg_offsets,
                       const double* vel, const double* mass, double* kinetic_sys,
                       double* partials, fnl_stream_t stream);
 
-/* forces = -(grad_self + grad_nbr) * scale  (float32[N][3]) */
-int fnl_forces(const float* grad_self, const float* grad_nbr, int32_t n_atoms,
+/* forces = -(grad_self + grad_nbr * 2^-32) * scale  (float32[N][3], formed in
+ * float64) */
+int fnl_forces(const float* grad_self, const int64_t* grad_nbr, int32_t n_atoms,
                float scale, float* forces, fnl_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * BAOAB Langevin step (md/integrate.py:286-294 with the LGV O-step of
  * md/thermostats.py:108-120), FP64 positions and velocities:
- *   F  = -(grad_self + grad_nbr) * force_scale
+ *   F  = -(grad_self + grad_nbr * 2^-32) * force_scale   (float64)
  *   v += dtm * F;  x += dt/2 v;  v = a1 v + a2 xi;  x += dt/2 v
  * xi ~ N(0,1) from Philox4x32-10 with counter (atom, step_lo, step_hi, 0) and
  * key (seed_lo, seed_hi), Box-Muller on (r0,r1), (r2,r3); or, if noise != NULL,
@@ -381,7 +389,7 @@ int fnl_forces(const float* grad_self, const float* grad_nbr, int32_t n_atoms,
  * accumulates into them.  Also writes the float32 copy of x the model reads.
  * ---------------------------------------------------------------------- */
 int fnl_baoab_step(double* x, double* v, float* x32,
-                   const float* grad_self, float* grad_nbr, float force_scale,
+                   const float* grad_self, int64_t* grad_nbr, float force_scale,
                    const double* dtm, double half_dt, double a1, const double* a2,
                    const double* noise, uint64_t seed, uint64_t step,
                    uint64_t* step_dev, int32_t n_atoms, int32_t clear_grad_nbr,
@@ -393,7 +401,7 @@ int fnl_baoab_step(double* x, double* v, float* x32,
  * step_lo, step_hi, sys_key[b]) with step = sys_step[b] and key (seed_lo,
  * seed_hi), so it does not depend on the system's place in the batch. */
 int fnl_baoab_step_batched(double* x, double* v, float* x32,
-                           const float* grad_self, const float* grad_nbr, float force_scale,
+                           const float* grad_self, const int64_t* grad_nbr, float force_scale,
                            const double* dtm, double half_dt, double a1, const double* a2,
                            const int32_t* batch_index, const int32_t* sys_offsets,
                            const uint64_t* sys_step, const uint32_t* sys_key, uint64_t seed,

@@ -507,7 +507,7 @@ def test_aev_backward_species_mask(gpu, precision):
     part, full = out.values()
     assert bin(list(out)[0]).count("1") == 2
     assert torch.equal(part[0], full[0]) and torch.equal(part[2], full[2])
-    torch.testing.assert_close(part[1], full[1], rtol=1e-6, atol=1e-6)  # atomic order
+    assert torch.equal(part[1], full[1])  # fixed-point sums: exact in any atomic order
 
 
 @pytest.mark.parametrize("precision,tiles", [("bf16", (32, 64, 96)), ("fp32", (16, 32))])
@@ -560,13 +560,15 @@ def test_philox_and_baoab_injected_noise(gpu):
     x = rng.normal(size=(n, 3))
     v = rng.normal(size=(n, 3))
     gs = rng.normal(size=(n, 3)).astype(np.float32)
-    gn = rng.normal(size=(n, 3)).astype(np.float32)
+    # the angular neighbour gradients are int64 fixed point in units of 2^-32
+    gn_fixed = np.rint(rng.normal(size=(n, 3)) * 2.0 ** 32).astype(np.int64)
+    gn = gn_fixed.astype(np.float64) * 2.0 ** -32
     mass = rng.uniform(1, 16, size=n)
     noise = rng.normal(size=(n, 3))
     dt = 5e-4
     a1, a2 = O.langevin_coefficients(dt, 1.0, 0.596, mass)
     scale = 1.0 / 0.04336409463078789
-    F = -(gs.astype(np.float64) + gn.astype(np.float64)) * np.float64(np.float32(scale))
+    F = -(gs.astype(np.float64) + gn) * np.float64(np.float32(scale))
     xr, vr = O.baoab_step(x, v, F, mass, dt, a1, a2, noise)
     dev = "cuda"
     tx = torch.tensor(x, device=dev)
@@ -576,27 +578,47 @@ def test_philox_and_baoab_injected_noise(gpu):
     ta2 = torch.tensor(a2, device=dev)
     tno = torch.tensor(noise, device=dev)
     tgs = torch.tensor(gs, device=dev)
-    tgn = torch.tensor(gn, device=dev)
+    tgn = torch.tensor(gn_fixed, device=dev)
     _abi.check(lib.fnl_baoab_step(tx.data_ptr(), tv.data_ptr(), x32.data_ptr(), tgs.data_ptr(), tgn.data_ptr(),
                                   scale, tdtm.data_ptr(), 0.5 * dt, a1, ta2.data_ptr(), tno.data_ptr(), 0, 0, None, n,
                                   0, _abi.stream_handle()), "baoab")
     np.testing.assert_allclose(tx.cpu().numpy(), xr, rtol=1e-13, atol=1e-13)
     np.testing.assert_allclose(tv.cpu().numpy(), vr, rtol=1e-13, atol=1e-13)
     np.testing.assert_array_equal(x32.cpu().numpy(), xr.astype(np.float32))
-    np.testing.assert_array_equal(tgn.cpu().numpy(), gn)  # not cleared without the flag
+    np.testing.assert_array_equal(tgn.cpu().numpy(), gn_fixed)  # not cleared without the flag
     # device step counter {step, done}: two calls draw the noise of steps 7 and 8 (Philox), the
     # last workgroup advances the step; the neighbour-gradient rows are cleared once read
     step_dev = torch.tensor([7, 0], dtype=torch.int64, device=dev)
     xs, vs = x.copy(), v.copy()
     for k in (7, 8):
-        Fk = -(gs.astype(np.float64) + (gn.astype(np.float64) if k == 7 else 0.0)) * np.float64(np.float32(scale))
+        Fk = -(gs.astype(np.float64) + (gn if k == 7 else 0.0)) * np.float64(np.float32(scale))
         xs, vs = O.baoab_step(xs, vs, Fk, mass, dt, a1, a2, O.philox_normals(99, k, n))
-    tx, tv, tgn = torch.tensor(x, device=dev), torch.tensor(v, device=dev), torch.tensor(gn, device=dev)
+    tx, tv, tgn = torch.tensor(x, device=dev), torch.tensor(v, device=dev), torch.tensor(gn_fixed, device=dev)
     for _ in range(2):
         _abi.check(lib.fnl_baoab_step(tx.data_ptr(), tv.data_ptr(), x32.data_ptr(), tgs.data_ptr(),
                                       tgn.data_ptr(), scale, tdtm.data_ptr(), 0.5 * dt, a1, ta2.data_ptr(), None, 99,
                                       0, step_dev.data_ptr(), n, 1, _abi.stream_handle()), "baoab counter")
     assert step_dev.cpu().tolist() == [9, 0]
-    assert np.all(tgn.cpu().numpy() == 0.0)
+    assert np.all(tgn.cpu().numpy() == 0)
     np.testing.assert_allclose(tx.cpu().numpy(), xs, rtol=1e-13, atol=1e-13)
     np.testing.assert_allclose(tv.cpu().numpy(), vs, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_forces_are_bitwise_reproducible(gpu, precision):
+    """The angular force scatter sums int64 fixed point (exact in any order): repeated
+    evaluations give the same bits."""
+    _flush(f"[determinism] {precision}")
+    s = water_box_small(512, 41)
+    cfg = configs(("H", "C", "N", "O", "S", "F", "Cl"))
+    _, pot = _potential(cfg, precision=precision, max_nbr=96, max_nbr_ang=40)
+    sb = pot.setup(s["species"], s["natoms"], s["cells"])
+    x = torch.tensor(np.asarray(s["coordinates"], np.float32), device="cuda")
+    outs = []
+    for _ in range(3):
+        pot.compute(sb, x)
+        outs.append((pot.forces(sb).cpu().numpy(), sb.vir_atom.cpu().numpy().copy(), sb.energy.cpu().numpy()))
+    for f, v, e in outs[1:]:
+        np.testing.assert_array_equal(f.view(np.uint32), outs[0][0].view(np.uint32))
+        np.testing.assert_array_equal(v.view(np.uint32), outs[0][1].view(np.uint32))
+        np.testing.assert_array_equal(e, outs[0][2])
