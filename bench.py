@@ -168,27 +168,24 @@ def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_ste
     return pot, md, (t1 - t0), lat, stages, eager_ms
 
 
-def accuracy(system, cfg, params, precision, fp64):
-    """Energy (meV/atom) and force-RMS (eV/A) errors against the FP64 evaluation."""
+def gpu_energy_forces(system, x, cfg, params, precision):
+    """E [B] and F [N,3] of the device potential at coordinates x (model unit, eV)."""
     from fennol_amd.potential import ANIPotential
 
     pot = ANIPotential(cfg, params, precision=precision)
-    E, F, V = pot.energy_forces_virial(system["species"], system["coordinates"], system["natoms"],
-                                       system.get("cells"))
-    Er, Fr = fp64
-    n = len(system["species"])
-    return {"energy_err_mev_per_atom": float(1000.0 * abs(E[0] - Er[0]) / n),
-            "force_rms_err_ev_per_a": float(np.sqrt(np.mean((F - Fr) ** 2)))}
+    E, F, _ = pot.energy_forces_virial(system["species"], x, system["natoms"], system.get("cells"))
+    return E, F
 
 
-def cpu_reference(system, cfg, params):
-    """The cpu_baseline leg: the FP64 NumPy oracle runs one MD step (energy, forces, BAOAB) on
-    the host cores and is timed; its energy and forces are also the FP64 accuracy reference."""
+def oracle_step(system, x, cfg, params):
+    """The cpu_baseline leg: the FP64 NumPy oracle runs one MD step at x -- the reference's
+    all-pairs neighbour list, AEV, MLP, forces and a BAOAB update -- and is timed; its energy
+    and forces are the FP64 reference the GPU results are compared with."""
     import oracle as O
 
     p = O.AEVParams(species_order=cfg.species_order)
     model = O.Model(p, params)
-    x = np.asarray(system["coordinates"], np.float64)
+    x = np.asarray(x, np.float64)
     t0 = time.perf_counter()
     g = O.build_graphs(x, system["natoms"], p, system.get("cells"), system.get("reciprocal_cells"))
     E, F, V, _ = O.energy_forces_virial(model, x, system["species"], system["natoms"], system.get("cells"),
@@ -197,8 +194,138 @@ def cpu_reference(system, cfg, params):
     a1, a2 = O.langevin_coefficients(5e-4, 1.0, O.md_units.K_B * 300.0, mass)
     noise = O.philox_normals(MD_SEED, 0, x.shape[0])
     O.baoab_step(x, np.zeros_like(x), F / O.md_units.energy_multiplier("eV"), mass, 5e-4, a1, a2, noise)
-    t1 = time.perf_counter()
-    return (E, F), t1 - t0
+    return (E, F), time.perf_counter() - t0
+
+
+def errors(E, F, Er, Fr, n):
+    return {"energy_err_mev_per_atom": float(1000.0 * np.abs(np.asarray(E) - np.asarray(Er)).max() / n),
+            "force_rms_err_ev_per_a": float(np.sqrt(np.mean((np.asarray(F) - np.asarray(Fr)) ** 2)))}
+
+
+def trajectory_checkpoints(system, cfg, params, steps: int, points) -> dict:
+    """Coordinates of the BF16 Langevin trajectory of the benchmark (graph replay) at the given
+    step numbers."""
+    import torch
+
+    from fennol_amd.md import LangevinMD
+    from fennol_amd.potential import ANIPotential
+
+    pot = ANIPotential(cfg, params, precision="bf16")
+    md = LangevinMD(pot, system, dt_fs=0.5, temperature=300.0, gamma_thz=1.0, seed=MD_SEED, check_every=0)
+    out = {0: md.x.cpu().numpy().copy()} if 0 in points else {}
+    done = 0
+    for p_ in sorted(k for k in points if k > 0):
+        md.run(p_ - done, graph=True)
+        done = p_
+        torch.cuda.synchronize()
+        out[p_] = md.x.cpu().numpy().copy()
+    md.verify()
+    del md, pot
+    torch.cuda.empty_cache()
+    return out
+
+
+def accuracy_and_cpu_baseline(system, cfg, params, precisions, steps: int = 1000,
+                              points=(0, 500, 1000)) -> tuple[dict, dict]:
+    """Errors of every form against the FP64 oracle at checkpoints of a BF16 trajectory of
+    ``steps`` steps (the oracle evaluated at the GPU trajectory's coordinates), and the CPU
+    baseline: the median time of those oracle steps (one per checkpoint)."""
+    n = len(system["species"])
+    xs = trajectory_checkpoints(system, cfg, params, steps, points)
+    rows, times = [], []
+    for k in sorted(xs):
+        log(f"[bench] FP64 oracle at trajectory step {k} (CPU)")
+        (Er, Fr), t = oracle_step(system, xs[k], cfg, params)
+        times.append(t)
+        row = {"step": k}
+        for prec in precisions:
+            E, F = gpu_energy_forces(system, xs[k], cfg, params, prec)
+            row[prec] = errors(E, F, Er, Fr, n)
+        rows.append(row)
+        log(f"[bench]   step {k}: " + ", ".join(f"{p} {row[p]['energy_err_mev_per_atom']:.4f} meV/atom "
+                                                   f"{row[p]['force_rms_err_ev_per_a']:.2e} eV/A" for p in precisions))
+    worst = {p: {key: max(r[p][key] for r in rows) for key in rows[0][p]} for p in precisions}
+    acc = {"checkpoints": rows, "worst": worst,
+           "trajectory": f"BF16 Langevin MD of the benchmark box, {steps} steps, dt 0.5 fs, 300 K"}
+    med = float(np.median(times))
+    threads = os.environ.get("OMP_NUM_THREADS")
+    cpu = {"value": round(n / med, 1), "unit": "atom-steps/s",
+           "cores": int(threads) if threads and threads.isdigit() else len(os.sched_getaffinity(0)),
+           "kind": "port",
+           "sample": (f"median of {len(times)} single MD steps (all-pairs neighbour list as the reference, AEV, "
+                      f"MLP, forces, BAOAB) of the same {n}-atom box at steps {sorted(xs)} of the trajectory, "
+                      f"FP64 NumPy oracle, {', '.join(f'{t:.1f}' for t in times)} s; element-wise NumPy on one "
+                      f"thread, BLAS products on OMP_NUM_THREADS={threads}; "
+                      f"{len(os.sched_getaffinity(0))} host cores available, CPU {cpu_model()}")}
+    return acc, cpu
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def bench_nve_drift(cfg, params, steps: int = 20000, every: int = 200) -> dict:
+    """NVE energy conservation on the reference's watersmall box (its 216-water geometry, 18.643 A
+    cube, dt 0.1 fs, initial velocities at 300 K, list rebuilt every step), FP32 and BF16:
+    total energy sampled every ``every`` steps, drift = least-squares slope, reported per ns
+    and per atom, for the reference-style total energy (kinetic energy of the stored mid-step
+    velocities, as fennol_md prints) and the synchronised one (v + dt/2 F/m)."""
+    import torch
+
+    from fennol_amd import units
+    from fennol_amd.md import LangevinMD
+    from fennol_amd.potential import ANIPotential
+
+    ws = np.load(os.path.join(ROOT, "tests", "golden", "watersmall.npz"))
+    system = {"species": ws["species"], "coordinates": ws["coordinates"], "cells": ws["cell"][None],
+              "natoms": np.array([len(ws["species"])], np.int32)}
+    n = len(ws["species"])
+    out = {"workload": f"watersmall: 216 waters ({n} atoms) of the reference example, 18.643 A cube, NVE, "
+                       f"dt 0.1 fs, {steps} steps ({steps * 1e-4:.1f} ps), seeded random-init weights",
+           "reference": "examples/md/watersmall/conservation_tests/*/watersmall.log.bak (trained ANI-2x "
+                        "weights, 1 ns): FP64 +6.7 ueV/atom/ns, FP32 highest -16 ueV/atom/ns, "
+                        "high +0.18 meV/atom/ns, default +0.39 meV/atom/ns"}
+    for prec in ("fp32", "bf16"):
+        pot = ANIPotential(cfg, params, precision=prec)
+        md = LangevinMD(pot, system, dt_fs=0.1, temperature=300.0, seed=MD_SEED, thermostat="NVE", check_every=0)
+        mass = torch.tensor(md.mass, dtype=torch.float64, device="cuda")[:, None]
+        t, e_ref, e_sync = [], [], []
+        for k in range(steps // every + 1):
+            if k:
+                md.run(every, graph=True)
+            torch.cuda.synchronize()
+            ep = float(md.energies()[0]) / md.model_energy_unit
+            v = md.v
+            F = pot.forces(md.sb).double() * md.force_scale       # kcal/mol/A
+            ek = float(0.5 * (mass * v * v).sum())
+            vs = v + (0.5 * md.dt) * F / mass
+            eks = float(0.5 * (mass * vs * vs).sum())
+            t.append(md.istep * md.dt)                             # ps
+            e_ref.append(ep + ek)
+            e_sync.append(ep + eks)
+        md.verify()
+        t = np.array(t)
+        ev = units.us.get_multiplier("eV")                         # kcal/mol -> eV
+        res = {}
+        for name, e in (("reference_style", np.array(e_ref)), ("synchronised", np.array(e_sync))):
+            slope = np.polyfit(t, e, 1)[0]                          # kcal/mol per ps
+            res[name] = {"drift_mev_per_atom_per_ns": round(float(slope * 1000.0 * ev * 1000.0 / n), 4),
+                         "first_last_mev_per_atom": round(float((e[-1] - e[0]) * ev * 1000.0 / n), 4),
+                         "fluct_rms_mev_per_atom": round(float(np.std(e - np.polyval(np.polyfit(t, e, 1), t))
+                                                                * ev * 1000.0 / n), 4)}
+        res["temperature_K_end"] = round(float(md.temperature()[0]), 1)
+        out[prec] = res
+        log(f"[bench]   NVE {prec}: drift {res['reference_style']['drift_mev_per_atom_per_ns']} (reference-style), "
+            f"{res['synchronised']['drift_mev_per_atom_per_ns']} (synchronised) meV/atom/ns")
+        del md, pot
+        torch.cuda.empty_cache()
+    return out
 
 
 def bench_serving(cfg, params, n_requests: int, n_baseline: int) -> dict:
@@ -249,10 +376,13 @@ def bench_reference_box(cfg, params, steps: int = 2000, warmup: int = 20) -> dic
     from fennol_amd.md import LangevinMD
     from fennol_amd.potential import ANIPotential
 
-    system = systems.water_box(216, 6, seed=5)
+    ws = np.load(os.path.join(ROOT, "tests", "golden", "watersmall.npz"))
+    system = {"species": ws["species"], "coordinates": ws["coordinates"], "cells": ws["cell"][None],
+              "reciprocal_cells": np.linalg.inv(ws["cell"])[None], "natoms": np.array([len(ws["species"])], np.int32)}
     n = int(system["natoms"].sum())
-    out = {"workload": f"216 waters ({n} atoms), {float(system['cells'][0, 0, 0]):.2f} A cube, NVE, dt 0.1 fs, "
-                       "neighbour list rebuilt every step (the reference's watersmall conservation runs)",
+    out = {"workload": f"the reference's watersmall box: its 216-water geometry ({n} atoms), "
+                       f"{float(system['cells'][0, 0, 0]):.3f} A cube, NVE, dt 0.1 fs, neighbour list rebuilt "
+                       "every step (the reference's watersmall conservation runs)",
            "reference_fp32_atom_steps_per_s": round(REFERENCE_BOX_FP32_ATOM_STEPS, 1),
            "reference_source": "examples/md/watersmall/conservation_tests/ani2x_dt0.1_highest_noskin/"
                                "watersmall.log.bak (unstated NVIDIA GPU)"}
@@ -270,6 +400,11 @@ def bench_reference_box(cfg, params, steps: int = 2000, warmup: int = 20) -> dic
         wall = time.perf_counter() - t0
         md.verify()
         out[prec] = {"ms_per_step": round(1000.0 * wall / steps, 4), "atom_steps_per_s": round(n * steps / wall, 1)}
+        # error against FP64 at the end of the run (the oracle at the same coordinates)
+        x_end = md.x.cpu().numpy()
+        (Er, Fr), _ = oracle_step(system, x_end, cfg, params)
+        E, F = gpu_energy_forces(system, x_end, cfg, params, prec)
+        out[prec]["vs_fp64_at_end"] = errors(E, F, Er, Fr, n)
         del pot, md
     out["fp32_vs_reference"] = round(out["fp32"]["atom_steps_per_s"] / REFERENCE_BOX_FP32_ATOM_STEPS, 2)
     return out
@@ -327,6 +462,10 @@ def main() -> int:
     ap.add_argument("--precisions", default="fp32,bf16", help="MD forms to run (bf16 gives value)")
     ap.add_argument("--no-reference-box", dest="reference_box", action="store_false",
                     help="skip the 216-water NVE run that the reference's published MD rate is quoted on")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="length of the BF16 trajectory whose start, middle and end are checked against FP64")
+    ap.add_argument("--nve-steps", type=int, default=20000,
+                    help="NVE steps (dt 0.1 fs) on the watersmall box for the energy drift; 0 skips")
     ap.add_argument("--skin", type=float, default=2.0,
                     help="also time BF16 MD with a Verlet skin of this many A (fennol_md's nblist_skin; "
                          "rebuild stride from the skin); 0 skips")
@@ -384,7 +523,16 @@ def main() -> int:
                 "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
                 "refilter_ms": round(float(stages["skin"].mean()), 4) if "skin" in stages else None,
                 "max_skin": md.sb.max_skin}
-        log(f"[bench]   skin: {skin['ms_per_step']:.3f} ms/step, re-filter {skin['refilter_ms']} ms")
+        # the re-filtered lists are the full build's lists (Verlet guarantee): the last step's
+        # energy and forces against a fresh full build at the same coordinates, so the errors
+        # against FP64 are those of the rebuild-every-step form (accuracy_vs_fp64)
+        x_end = md.x32.cpu().numpy().astype(np.float64)
+        E_skin, F_skin = md.sb.energy.cpu().numpy(), pot.forces(md.sb).cpu().numpy()
+        E_full, F_full = gpu_energy_forces(system, x_end, cfg, params, "bf16")
+        skin["vs_full_rebuild"] = {"max_abs_energy_diff_ev": float(np.abs(E_skin - E_full).max()),
+                                   "max_abs_force_diff_ev_per_a": float(np.abs(F_skin - F_full).max())}
+        log(f"[bench]   skin: {skin['ms_per_step']:.3f} ms/step, re-filter {skin['refilter_ms']} ms, "
+            f"vs full rebuild {skin['vs_full_rebuild']}")
         del pot, md
         torch.cuda.empty_cache()
 
@@ -427,14 +575,11 @@ def main() -> int:
     acc = {}
     cpu = None
     if not args.skip_cpu:
-        log("[bench] FP64 oracle on the host (accuracy reference + CPU baseline)")
-        fp64, t_cpu = cpu_reference(system, cfg, params)
-        for precision in precisions:
-            acc[precision] = accuracy(system, cfg, params, precision, fp64)
-        cores = len(os.sched_getaffinity(0))
-        cpu = {"value": round(N / t_cpu, 1), "unit": "atom-steps/s", "cores": cores, "kind": "port",
-               "sample": f"1 MD step (all-pairs nblist, AEV, MLP, forces, BAOAB) of the same {N}-atom box, "
-                         f"FP64 NumPy oracle, {t_cpu:.1f} s"}
+        log("[bench] FP64 oracle on the host at checkpoints of a 1,000-step BF16 trajectory "
+            "(accuracy reference + CPU baseline)")
+        acc, cpu = accuracy_and_cpu_baseline(system, cfg, params, precisions, steps=args.accuracy_steps,
+                                             points=(0, args.accuracy_steps // 2, args.accuracy_steps))
+    nve = bench_nve_drift(cfg, params, steps=args.nve_steps) if args.nve_steps > 0 else None
     serving = bench_serving(cfg, params, args.serving_requests, args.serving_baseline) \
         if args.serving_requests > 0 else None
     training = bench_training(args.training_boxes, 3) if args.training_boxes > 0 else None
@@ -478,6 +623,7 @@ def main() -> int:
         "serving": serving,
         "training": training,
         "reference_box": ref_box,
+        "nve_drift": nve,
     }
     print(json.dumps(out), flush=True)
     return 0
