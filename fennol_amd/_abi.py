@@ -66,7 +66,8 @@ class AevParams(C.Structure):
 
 
 class MlpNet(C.Structure):
-    _fields_ = [(n, vp) for n in ("w1t", "w2t", "w3t", "w1", "w2", "w3", "w4", "b1", "b2", "b3", "b4")]
+    _fields_ = [(n, vp) for n in ("w1t", "w2t", "w3t", "w1", "w2", "w3", "w4", "b1", "b2", "b3", "b4",
+                                  "w1t_lo", "w2t_lo", "w3t_lo", "w1_lo", "w2_lo", "w3_lo")]
 
 
 class MlpWeights(C.Structure):
@@ -99,7 +100,8 @@ _SIGNATURES = {
     "fnl_aev_tangent": (C.c_int, [vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, i32,
                                   C.POINTER(AevParams), vp, i32, vp]),
     "fnl_mlp_tile_rows": (C.c_int, [i32]),
-    "fnl_species_mlp": (C.c_int, [vp, i32, C.POINTER(MlpWeights), vp, vp, vp, vp, i32, vp, vp, i32, i32, vp]),
+    "fnl_species_mlp": (C.c_int, [vp, i32, C.POINTER(MlpWeights), vp, vp, vp, vp, i32, vp, vp, i32, i32, u64,
+                                  vp]),
     "fnl_system_reduce": (C.c_int, [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "fnl_forces": (C.c_int, [vp, vp, i32, f32, vp, vp]),
     "fnl_baoab_step": (C.c_int, [vp, vp, vp, vp, vp, f32, vp, f64, f64, vp, vp, u64, u64, vp, i32, i32, vp]),

@@ -37,16 +37,21 @@ constexpr int H1 = FNL_MLP_H1, H2 = FNL_MLP_H2, H3 = FNL_MLP_H3;
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
 struct NetPtrs {
-  const void* fw[3];  // A operand of the forward GEMMs
-  const void* bw[3];  // A operand of the backward GEMMs
+  const void* fw[3];     // A operand of the forward GEMMs
+  const void* bw[3];     // A operand of the backward GEMMs
+  const void* fw_lo[3];  // compensated BF16: the rounding remainders W - bf16(W), same layout
+  const void* bw_lo[3];
   const float* w4;
   const float* b[4];
 };
+
+constexpr int FNL_MLP_MAX_CHUNKS = 128;  // layer-1 K chunks per row: 8192 FP32 / 16384 BF16 columns
 
 struct MlpArgs {
   NetPtrs nets[FNL_MLP_MAX_NETS];
   int k_pad;
   float alpha;
+  unsigned long long k_live;  // bit q: input columns [64q, 64q + 64) may be non-zero in the batch
 };
 
 __device__ __forceinline__ unsigned short f2bf(float f) {
@@ -145,9 +150,11 @@ __device__ __forceinline__ void zero_acc(f4 (&acc)[NB][MB]) {
 
 // acc[nb][mb] += sum_k W(n, k) act(k, m) over k in [0, K) for the n-blocks
 // nb0 + WV*nb (< ncb) of this wave.  W prefetched PF steps ahead.
-template <typename T, int NB, int MB, int PF, int WV>
+// COMP: the weights are the sum of two BF16 matrices (value and rounding remainder, Wlo), each
+// multiplied on the MFMA: the product keeps ~16 significant bits of W instead of 8.
+template <typename T, int NB, int MB, int PF, int WV, bool COMP = false>
 __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ W, int ldw, int nb0, int ncb,
-                                       const T* act, int lda, int K) {
+                                       const T* act, int lda, int K, const T* __restrict__ Wlo = nullptr) {
   typedef Tr<T> tr;
   const int nsteps = K / tr::KS;
   // Every prefetch load is issued unconditionally (k-step and n-block clamped into range), so
@@ -157,10 +164,14 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) ncol[nb] = min(nb0 + WV * nb, ncb - 1) * 16;
   typename tr::frag wq[PF][NB];
+  typename tr::frag wl[COMP ? PF : 1][NB];
 #pragma unroll
   for (int u = 0; u < PF; ++u)
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) wq[u][nb] = tr::wload(W, ldw, ncol[nb], min(u, nsteps - 1) * tr::KS);
+    for (int nb = 0; nb < NB; ++nb) {
+      wq[u][nb] = tr::wload(W, ldw, ncol[nb], min(u, nsteps - 1) * tr::KS);
+      if constexpr (COMP) wl[u][nb] = tr::wload(Wlo, ldw, ncol[nb], min(u, nsteps - 1) * tr::KS);
+    }
   typename tr::frag b[MB];
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(act, lda, mb * 16, 0);
@@ -182,8 +193,13 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
           if (nb0 + WV * nb < ncb) {
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb) acc[nb][mb] = tr::mma(wq[u][nb], b[mb], acc[nb][mb]);
+            if constexpr (COMP) {
+#pragma unroll
+              for (int mb = 0; mb < MB; ++mb) acc[nb][mb] = tr::mma(wl[u][nb], b[mb], acc[nb][mb]);
+            }
           }
           wq[u][nb] = tr::wload(W, ldw, ncol[nb], sn * tr::KS);
+          if constexpr (COMP) wl[u][nb] = tr::wload(Wlo, ldw, ncol[nb], sn * tr::KS);
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -240,7 +256,7 @@ __device__ __forceinline__ void bwd_epilogue(const f4 (&acc)[NB][MB], T* HG, int
   }
 }
 
-template <typename T, int M_, int PF_, int WV_>
+template <typename T, int M_, int PF_, int WV_, bool COMP = false>
 __global__ void __launch_bounds__(64 * WV_, (Lds<T, M_, WV_>::BLOCKS_PER_CU))
 k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int32_t* __restrict__ perm,
               const int32_t* __restrict__ tile_start, const int32_t* __restrict__ tile_rows,
@@ -308,22 +324,46 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     for (int p = 0; p < NPASS; ++p) *(u4v*)(buf + (p * RPP + t / TPR) * L::LDX + scol) = xr[p];
   };
 
+  // Input chunks of KC columns that hold a non-zero column for some atom of the batch (k_live):
+  // only those are multiplied (the others contribute exact zeros).  live_chunk[v] is the v-th of
+  // them; the loop below runs over v, and a k-step s of that loop is column block
+  // live_chunk[s / SPC] * KC + (s % SPC) * KS.
+  __shared__ int live_chunk[FNL_MLP_MAX_CHUNKS];
+  __shared__ int n_live_sh;
+  if (t == 0) {
+    int n = 0;
+    for (int c = 0; c < K / KC; ++c) {
+      const unsigned long long bits = KC == 128 ? (3ull << (2 * c)) : (1ull << c);
+      if (c >= 64 / (KC / 64) || (args.k_live & bits)) live_chunk[n++] = c;
+    }
+    if (n == 0) live_chunk[n++] = 0;  // at least one chunk: the bias-only result stays correct
+    n_live_sh = n;
+  }
+  __syncthreads();
+  const int nlive = n_live_sh;
+
   f4 acc1[NB1][MB];
   zero_acc(acc1);
   {
     constexpr int SPC = KC / tr::KS;  // k-steps per chunk
     static_assert(SPC % PF == 0, "prefetch ring aligned to chunks");
     const T* W = (const T*)P.fw[0];
+    const T* Wl = (const T*)P.fw_lo[0];
     const int ldw = sizeof(T) == 2 ? K : H1;
-    const int nsteps = K / tr::KS;
+    const int nsteps = nlive * SPC;
+    auto kstep = [&](int s) { return (live_chunk[s / SPC] * SPC + s % SPC) * tr::KS; };
     typename tr::frag wq[PF][NB1];
+    typename tr::frag wl[COMP ? PF : 1][NB1];
 #pragma unroll
     for (int u = 0; u < PF; ++u)
 #pragma unroll
-      for (int nb = 0; nb < NB1; ++nb) wq[u][nb] = tr::wload(W, ldw, (wave + WV * nb) * 16, u * tr::KS);
-    const int nchunks = K / KC;
-    stage_load(xa, 0);
-    stage_load(xb, min(1, nchunks - 1) * KC);
+      for (int nb = 0; nb < NB1; ++nb) {
+        wq[u][nb] = tr::wload(W, ldw, (wave + WV * nb) * 16, kstep(min(u, nsteps - 1)));
+        if constexpr (COMP) wl[u][nb] = tr::wload(Wl, ldw, (wave + WV * nb) * 16, kstep(min(u, nsteps - 1)));
+      }
+    const int nchunks = nlive;
+    stage_load(xa, live_chunk[0] * KC);
+    stage_load(xb, live_chunk[min(1, nchunks - 1)] * KC);
     stage_store(xa, Xs0);
     __syncthreads();
     // chunk c: computed from LDS buffer c&1; chunk c+1 in flight in one register set (stored at the
@@ -331,7 +371,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     // index) so that the compiler can wait for the older set only.
     auto chunk = [&](int c, u4v (&xnext)[NPASS], u4v (&xfree)[NPASS]) {
       const T* X = (c & 1) ? Xs1 : Xs0;
-      stage_load(xfree, min(c + 2, nchunks - 1) * KC);
+      stage_load(xfree, live_chunk[min(c + 2, nchunks - 1)] * KC);
       typename tr::frag b[MB];
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(X, L::LDX, mb * 16, 0);
@@ -345,11 +385,17 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
 #pragma unroll
           for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(X, L::LDX, mb * 16, kn);
           __builtin_amdgcn_sched_barrier(0);  // next step's LDS reads stay ahead (see gemm_t)
+          const int kpf = kstep(min(s + PF, nsteps - 1));
 #pragma unroll
           for (int nb = 0; nb < NB1; ++nb) {
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb) acc1[nb][mb] = tr::mma(wq[u][nb], b[mb], acc1[nb][mb]);
-            wq[u][nb] = tr::wload(W, ldw, (wave + WV * nb) * 16, min(s + PF, nsteps - 1) * tr::KS);
+            if constexpr (COMP) {
+#pragma unroll
+              for (int mb = 0; mb < MB; ++mb) acc1[nb][mb] = tr::mma(wl[u][nb], b[mb], acc1[nb][mb]);
+            }
+            wq[u][nb] = tr::wload(W, ldw, (wave + WV * nb) * 16, kpf);
+            if constexpr (COMP) wl[u][nb] = tr::wload(Wl, ldw, (wave + WV * nb) * 16, kpf);
           }
           __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -371,7 +417,8 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   {
     f4 acc2[NB2][MB];
     zero_acc(acc2);
-    gemm_t<T, NB2, MB, PF, WV>(acc2, (const T*)P.fw[1], sizeof(T) == 2 ? H1 : H2, wave, H2 / 16, B1, L::LD1, H1);
+    gemm_t<T, NB2, MB, PF, WV, COMP>(acc2, (const T*)P.fw[1], sizeof(T) == 2 ? H1 : H2, wave, H2 / 16, B1, L::LD1, H1,
+                                     (const T*)P.fw_lo[1]);
     fwd_epilogue<T, NB2, MB, WV>(acc2, P.b[1], B2, L::LD2, wave, H2 / 16, alpha);
   }
   __syncthreads();
@@ -379,7 +426,8 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   // ---------------- layer 3 + energy ----------------
   f4 acc3[NB3][MB];
   zero_acc(acc3);
-  gemm_t<T, NB3, MB, PF, WV>(acc3, (const T*)P.fw[2], sizeof(T) == 2 ? H2 : H3, wave, H3 / 16, B2, L::LD2, H2);
+  gemm_t<T, NB3, MB, PF, WV, COMP>(acc3, (const T*)P.fw[2], sizeof(T) == 2 ? H2 : H3, wave, H3 / 16, B2, L::LD2, H2,
+                                   (const T*)P.fw_lo[2]);
   // h3 stays in registers (energy and G3 are formed from acc3); B3 holds the energy partials
   fwd_epilogue<T, NB3, MB, WV, false>(acc3, P.b[2], B3, L::LD3, wave, H3 / 16, alpha);
   {
@@ -436,14 +484,16 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   {
     f4 acc2[NB2][MB];
     zero_acc(acc2);
-    gemm_t<T, NB2, MB, PF, WV>(acc2, (const T*)P.bw[2], sizeof(T) == 2 ? H3 : H2, wave, H2 / 16, B3, L::LD3, H3);
+    gemm_t<T, NB2, MB, PF, WV, COMP>(acc2, (const T*)P.bw[2], sizeof(T) == 2 ? H3 : H2, wave, H2 / 16, B3, L::LD3, H3,
+                                     (const T*)P.bw_lo[2]);
     bwd_epilogue<T, NB2, MB, WV>(acc2, B2, L::LD2, wave, H2 / 16, alpha);
   }
   __syncthreads();
   {
     f4 accg[NB1][MB];
     zero_acc(accg);
-    gemm_t<T, NB1, MB, PF, WV>(accg, (const T*)P.bw[1], sizeof(T) == 2 ? H2 : H1, wave, H1 / 16, B2, L::LD2, H2);
+    gemm_t<T, NB1, MB, PF, WV, COMP>(accg, (const T*)P.bw[1], sizeof(T) == 2 ? H2 : H1, wave, H1 / 16, B2, L::LD2, H2,
+                                     (const T*)P.bw_lo[1]);
     bwd_epilogue<T, NB1, MB, WV>(accg, B1, L::LD1, wave, H1 / 16, alpha);
   }
   __syncthreads();
@@ -461,9 +511,14 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   static_assert(M * GLD * 2 <= L::OFFE - L::OFF2, "BF16 gradient staging fits the H2/H3 buffers");
   unsigned short* gst = (unsigned short*)(smem + L::OFF2);
   for (int base = 0; base < ncb_x; base += 16) {
+    // input columns [16 base, 16 base + 256): skipped when no atom of the batch has a non-zero
+    // there (their dE/dAEV is never read: the adjoint reads the live blocks only)
+    const int q0 = base / 4;
+    if (q0 < 64 && ((args.k_live >> q0) & 0xFull) == 0ull) continue;
     f4 accx[NB1][MB];
     zero_acc(accx);
-    gemm_t<T, NB1, MB, PF, WV>(accx, (const T*)P.bw[0], sizeof(T) == 2 ? H1 : K, base + wave, ncb_x, B1, L::LD1, H1);
+    gemm_t<T, NB1, MB, PF, WV, COMP>(accx, (const T*)P.bw[0], sizeof(T) == 2 ? H1 : K, base + wave, ncb_x, B1,
+                                     L::LD1, H1, (const T*)P.bw_lo[0]);
     if (grad_bf16) {
 #pragma unroll
       for (int nb = 0; nb < NB1; ++nb) {
@@ -509,7 +564,7 @@ extern "C" int fnl_mlp_tile_rows(int32_t dtype) {
 extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_mlp_weights* w,
                                const int32_t* perm, const int32_t* tile_start, const int32_t* tile_rows,
                                const int32_t* tile_net, int32_t n_tiles, float* energy, void* grad,
-                               int32_t grad_stride, int32_t grad_dtype, fnl_stream_t stream) {
+                               int32_t grad_stride, int32_t grad_dtype, uint64_t k_live, fnl_stream_t stream) {
   if (!w || n_tiles < 0) return FNL_ERR_INVALID;
   if (n_tiles == 0) return FNL_OK;
   if (!aev || !perm || !tile_start || !tile_rows || !tile_net || !energy) return FNL_ERR_INVALID;
@@ -520,12 +575,15 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
   if (w->k_pad < kc || w->k_pad % kc != 0 || aev_stride < w->k_pad ||
       aev_stride % 8 != 0)
     return FNL_ERR_INVALID;
+  if (w->k_pad > FNL_MLP_MAX_CHUNKS * kc) return FNL_ERR_UNSUPPORTED;  // the kernel's live-chunk table
   if (grad && (grad_stride < w->k_pad || grad_stride % 8 != 0)) return FNL_ERR_INVALID;
   if (grad_dtype != FNL_F32 && !(grad_dtype == FNL_BF16 && bf)) return FNL_ERR_INVALID;
   const int gbf = grad_dtype == FNL_BF16 ? 1 : 0;
   MlpArgs a;
   a.k_pad = w->k_pad;
   a.alpha = w->celu_alpha;
+  a.k_live = k_live ? (unsigned long long)k_live : ~0ull;
+  bool comp = bf;  // compensated BF16 when every network carries the remainder matrices
   for (int n = 0; n < FNL_MLP_MAX_NETS; ++n) {
     NetPtrs& p = a.nets[n];
     if (n >= w->n_nets) {
@@ -535,6 +593,16 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
     const fnl_mlp_net& s = w->nets[n];
     if (!s.w1t || !s.w2t || !s.w3t || !s.w1 || !s.w2 || !s.w3 || !s.w4 || !s.b1 || !s.b2 || !s.b3 || !s.b4)
       return FNL_ERR_INVALID;
+    const bool has_lo = s.w1t_lo && s.w2t_lo && s.w3t_lo && s.w1_lo && s.w2_lo && s.w3_lo;
+    const bool any_lo = s.w1t_lo || s.w2t_lo || s.w3t_lo || s.w1_lo || s.w2_lo || s.w3_lo;
+    if (any_lo && (!has_lo || !bf)) return FNL_ERR_INVALID;  // all six remainders, BF16 only
+    comp = comp && has_lo;
+    p.fw_lo[0] = s.w1t_lo;
+    p.fw_lo[1] = s.w2t_lo;
+    p.fw_lo[2] = s.w3t_lo;
+    p.bw_lo[0] = s.w1_lo;
+    p.bw_lo[1] = s.w2_lo;
+    p.bw_lo[2] = s.w3_lo;
     // A operands: K-contiguous for BF16 (8 k per lane), N-contiguous for FP32
     // (16 lanes read 16 consecutive n); forward uses W^T, backward W.
     p.fw[0] = bf ? s.w1t : s.w1;
@@ -552,18 +620,23 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
   hipStream_t st = (hipStream_t)stream;
   const int M = w->tile_rows > 0 ? w->tile_rows : fnl_mlp_tile_rows(w->dtype);
   const int grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
+#define FNL_MLP_BF16(M_, WV_)                                                                                  \
+  do {                                                                                                         \
+    if (comp)                                                                                                  \
+      hipLaunchKernelGGL((k_species_mlp<unsigned short, M_, 2, WV_, true>), dim3(grid), dim3(64 * WV_), 0, st, \
+                         (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net,     \
+                         n_tiles, energy, grad, grad_stride, gbf);                                             \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_species_mlp<unsigned short, M_, 2, WV_, false>), dim3(grid), dim3(64 * WV_), 0,   \
+                         st, (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, \
+                         n_tiles, energy, grad, grad_stride, gbf);                                             \
+  } while (0)
   if (bf && M == 64) {
-    hipLaunchKernelGGL((k_species_mlp<unsigned short, 64, 2, 4>), dim3(grid), dim3(256), 0, st,
-                       (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
-                       energy, grad, grad_stride, gbf);
+    FNL_MLP_BF16(64, 4);
   } else if (bf && M == 96) {  // eight waves: two per SIMD cover each other's load latency
-    hipLaunchKernelGGL((k_species_mlp<unsigned short, 96, 2, 8>), dim3(grid), dim3(512), 0, st,
-                       (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
-                       energy, grad, grad_stride, gbf);
+    FNL_MLP_BF16(96, 8);
   } else if (bf && M == 32) {
-    hipLaunchKernelGGL((k_species_mlp<unsigned short, 32, 2, 4>), dim3(grid), dim3(256), 0, st,
-                       (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles,
-                       energy, grad, grad_stride, gbf);
+    FNL_MLP_BF16(32, 4);
   } else if (!bf && M == 32) {
     hipLaunchKernelGGL((k_species_mlp<float, 32, 4, 4>), dim3(grid), dim3(256), 0, st, (const float*)aev,
                        aev_stride, a, perm, tile_start, tile_rows, tile_net, n_tiles, energy, grad,
@@ -575,6 +648,7 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
   } else {
     return FNL_ERR_UNSUPPORTED;
   }
+#undef FNL_MLP_BF16
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }

@@ -13,6 +13,7 @@ GraphAngularExtension / SpeciesIndexer (models/preprocessing.py), ANIAEV
 from __future__ import annotations
 
 import contextlib
+import functools
 import ctypes as C
 import math
 from dataclasses import dataclass, field
@@ -55,6 +56,34 @@ class ANIConfig:
     def aev_length(self) -> int:
         return (self.n_species * self.radial_dist_divisions
                 + self.n_pairs * self.angular_dist_divisions * self.angle_sections)
+
+    @functools.lru_cache(maxsize=256)
+    def live_column_blocks(self, species_mask: int) -> int:
+        """Bit q set if AEV columns [64q, 64q + 64) hold a channel block written for a batch whose
+        AEV species indices are the set bits of ``species_mask`` (radial blocks of those species,
+        angular blocks of pairs of them); 0 (= every column) for the full mask.  The MLP skips
+        the products of the other columns, which are zero in every row of such a batch."""
+        if species_mask == 0xFFFFFFFF or species_mask == 0:
+            return 0
+        R = self.radial_dist_divisions
+        A = self.angular_dist_divisions * self.angle_sections
+        spans = []
+        pr = 0
+        for a in range(self.n_species):
+            if (species_mask >> a) & 1:
+                spans.append((a * R, (a + 1) * R))
+        for a in range(self.n_species):
+            for b in range(a, self.n_species):  # pair_index order: row-major upper triangle
+                if (species_mask >> a) & 1 and (species_mask >> b) & 1:
+                    c0 = self.n_species * R + pr * A
+                    spans.append((c0, c0 + A))
+                pr += 1
+        live = 0
+        for c0, c1 in spans:
+            for q in range(c0 // 64, (c1 - 1) // 64 + 1):
+                if q < 64:
+                    live |= 1 << q
+        return live or 1
 
     @property
     def k_pad(self) -> int:
@@ -157,16 +186,22 @@ class SpeciesNetworks:
 
     ``params``: {symbol: [(W [in, out], b [out]) x 4]} with widths
     A -> 256 -> 192 -> 160 -> 1 (FullyConnectedNet layers, nets.py:81-94).
+
+    BF16 with ``compensated``: each hidden-layer matrix is stored as hi = bf16(W) and
+    lo = bf16(W - hi), and every product takes both on the MFMA, so the weights keep about
+    16 significant bits (plain BF16 weights bias every atom of a similar environment the same
+    way, an energy error that grows with the system's energy scale).
     """
 
     def __init__(self, params: dict, k_in: int, k_pad: int, dtype: str = "fp32", device="cuda",
-                 celu_alpha: float = CELU_ALPHA, tile_rows: int = 0):
+                 celu_alpha: float = CELU_ALPHA, tile_rows: int = 0, compensated: bool = True):
         if dtype not in ("fp32", "bf16"):
             raise ValueError("dtype must be 'fp32' or 'bf16'")
         if len(params) > _abi.FNL_MLP_MAX_NETS:
             raise ValueError("too many networks")
         self.species = list(params.keys())
         self.dtype = dtype
+        self.compensated = bool(compensated) and dtype == "bf16"
         self.k_pad = k_pad
         tdt = torch.float32 if dtype == "fp32" else torch.bfloat16
         self._keep = []
@@ -192,6 +227,14 @@ class SpeciesNetworks:
                 if dtype == "bf16":  # the kernel's MFMA operand layout
                     w = torch.tensor(fragment_order(Wm), dtype=tdt, device=device).contiguous()
                     wt = torch.tensor(fragment_order(Wm.T), dtype=tdt, device=device).contiguous()
+                    if self.compensated:
+                        W64 = torch.tensor(Wm, dtype=torch.float64)
+                        lo = (W64 - W64.to(tdt).to(torch.float64)).numpy()  # exact in float64
+                        w_lo = torch.tensor(fragment_order(lo), dtype=tdt, device=device).contiguous()
+                        wt_lo = torch.tensor(fragment_order(lo.T), dtype=tdt, device=device).contiguous()
+                        self._keep += [w_lo, wt_lo]
+                        setattr(net, f"w{l + 1}_lo", w_lo.data_ptr())
+                        setattr(net, f"w{l + 1}t_lo", wt_lo.data_ptr())
                 else:
                     w = torch.tensor(Wm, dtype=tdt, device=device).contiguous()
                     wt = torch.tensor(Wm.T, dtype=tdt, device=device).contiguous()
@@ -289,19 +332,20 @@ class ANIPotential:
 
     ``precision``: "fp32" (AEV, GEMMs and activations in FP32) or "bf16" (AEV
     computed in FP32 and stored in BF16, MLP GEMMs in BF16 on MFMA with FP32
-    accumulation).  Per-system energies and virials are reduced in FP64.
+    accumulation; hidden-layer weights as BF16 hi + lo pairs unless
+    ``bf16_compensated`` is False).  Per-system energies and virials are reduced in FP64.
     """
 
     def __init__(self, config: ANIConfig, params: dict, precision: str = "fp32", device="cuda",
                  max_nbr: int = 64, max_nbr_ang: int = 32, mlp_tile_rows: int = 0,
-                 unhandled_species: str = "raise"):
+                 unhandled_species: str = "raise", bf16_compensated: bool = True):
         self.lib = _abi.load()
         self.config = config
         self.precision = precision
         self.device = torch.device(device)
         self.cparams = config.c_params()
         self.nets = SpeciesNetworks(params, config.aev_length, config.k_pad, precision, self.device,
-                                    tile_rows=mlp_tile_rows)
+                                    tile_rows=mlp_tile_rows, compensated=bf16_compensated)
         self.net_of_z = {SYMBOL_TO_Z[s]: n for n, s in enumerate(self.nets.species)}
         self._row_dtype = _abi.FNL_F32 if precision == "fp32" else _abi.FNL_BF16  # AEV and dE/dAEV rows
         self.init_max_nbr = max_nbr
@@ -652,7 +696,7 @@ class ANIPotential:
         rc = self.lib.fnl_species_mlp(
             P(sb.aev), sb.aev.shape[1], C.byref(self.nets.cw), P(sb.perm), P(sb.tile_start), P(sb.tile_rows),
             P(sb.tile_net), sb.n_tiles, P(sb.e_atom), P(sb.g_aev) if with_grad else None, sb.g_aev.shape[1],
-            self._row_dtype, s)
+            self._row_dtype, self.config.live_column_blocks(sb.extra.get("aev_mask", 0xFFFFFFFF)), s)
         _abi.check(rc, "fnl_species_mlp")
 
     def zero_grad_nbr(self, sb: SystemBuffers, stream=None) -> None:

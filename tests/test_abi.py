@@ -54,4 +54,4 @@ def test_host_side_argument_checks():
                                0, 0xFFFFFFFF, None) == -1
     assert lib.fnl_fill_i32(None, 0, 0, None) == 0
     assert lib.fnl_fill_i32(None, 0, 5, None) == -1
-    assert lib.fnl_species_mlp(None, 0, None, None, None, None, None, 0, None, None, 0, 0, None) == -1
+    assert lib.fnl_species_mlp(None, 0, None, None, None, None, None, 0, None, None, 0, 0, 0, None) == -1
