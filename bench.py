@@ -328,10 +328,13 @@ def bench_nve_drift(cfg, params, steps: int = 20000, every: int = 200) -> dict:
     return out
 
 
-def bench_serving(cfg, params, n_requests: int, n_baseline: int) -> dict:
-    """cfg 3: continuous batching (BF16 MLP) vs plain FP32 per-request launches."""
+def bench_serving(cfg, params, n_requests: int, n_baseline: int, n_check: int = 24) -> dict:
+    """Continuous batching of cluster requests (BF16 MLP) vs plain FP32 per-request launches:
+    saturated (every request waiting at t=0) and Poisson arrivals at 70 % of the saturated rate;
+    latency per request class; FP64 error of a sample of the served results."""
     import torch
 
+    import oracle as O
     from fennol_amd.potential import ANIConfig, ANIPotential
     from fennol_amd.serving import make_requests, per_request_baseline, poisson_arrivals, serve_stream
 
@@ -341,22 +344,49 @@ def bench_serving(cfg, params, n_requests: int, n_baseline: int) -> dict:
     p4 = init_species_params(MODEL_SEED, cfg4.aev_length)
     reqs = make_requests(n_requests, seed=3)
     pot = ANIPotential(cfg4, p4, precision="bf16")
+    pool = dict(max_atoms=65536, max_systems=2048)
     log(f"[bench] serving: {n_requests} requests, saturated")
-    sat = serve_stream(pot, reqs, None, max_atoms=65536, max_systems=2048)
+    sat = serve_stream(pot, reqs, None, **pool)
     rate = 0.7 * sat["requests_per_s"]
-    log(f"[bench] serving: Poisson arrivals at {rate:.1f} req/s")
-    poi = serve_stream(pot, reqs, poisson_arrivals(len(reqs), rate, seed=1), max_atoms=65536, max_systems=2048)
+    log(f"[bench] serving: {sat['requests_per_s']:.0f} req/s, {sat['atom_steps_per_s'] / 1e6:.1f} M atom-steps/s; "
+        f"Poisson arrivals at {rate:.1f} req/s")
+    poi = serve_stream(pot, reqs, poisson_arrivals(len(reqs), rate, seed=1), **pool)
+    # FP64 error of served results: energy + forces requests at their inputs, MD requests at the
+    # positions they returned
+    served = poi["_requests"]
+    model = O.Model(O.AEVParams(species_order=cfg4.species_order), p4)
+    rng = np.random.Generator(np.random.PCG64(7))
+    pick = rng.choice(len(served), size=min(n_check, len(served)), replace=False)
+    e_err, f_err = {"ef": [], "md": []}, {"ef": [], "md": []}
+    for k in pick:
+        r = served[int(k)]
+        x = r.positions if r.md_steps else np.asarray(r.coordinates)
+        x64 = np.asarray(x, np.float32).astype(np.float64)
+        Er, Fr, _, _ = O.energy_forces_virial(model, x64, r.species, np.array([r.n]))
+        e_err[r.kind].append(1000.0 * abs(r.energy - Er[0]) / r.n)
+        f_err[r.kind].append(float(np.sqrt(np.mean((r.forces - Fr) ** 2))))
+    accuracy = {kind: {"requests": len(e_err[kind]),
+                       "max_energy_err_mev_per_atom": round(max(e_err[kind]), 5) if e_err[kind] else None,
+                       "max_force_rms_err_ev_per_a": round(max(f_err[kind]), 5) if f_err[kind] else None}
+                for kind in ("ef", "md")}
     del pot
     torch.cuda.empty_cache()
     log(f"[bench] serving: FP32 per-request baseline on {n_baseline} requests")
     pot32 = ANIPotential(cfg4, p4, precision="fp32")
     base = per_request_baseline(pot32, reqs[:n_baseline])
-    keep = ("wall_s", "requests", "atom_steps", "atom_steps_per_s", "requests_per_s", "iterations", "p50_ms",
-            "p99_ms", "ef_p50_ms", "ef_p99_ms", "md_p50_ms", "md_p99_ms", "service_p50_ms", "service_p99_ms")
+    keep = ("wall_s", "requests", "atom_steps", "atom_steps_per_s", "requests_per_s", "iterations", "layouts",
+            "restarts", "regrowths", "graphs", "p50_ms", "p99_ms", "ef_requests", "ef_p50_ms", "ef_p99_ms",
+            "md_requests", "md_p50_ms", "md_p99_ms", "service_p50_ms", "service_p99_ms")
     rnd = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items() if k in keep}
     return {"workload": f"{n_requests} cluster requests, 8-512 atoms log-uniform, 30% 200-step Langevin, "
-                        f"70% single energy+forces", "bf16_saturated": rnd(sat),
+                        f"70% single energy+forces",
+            "pool": {"max_atoms": sat["max_atoms"], "max_systems": sat["max_systems"],
+                     "hbm_bytes": int(sat["pool_bytes"]), "bytes_per_atom_row": round(sat["row_bytes"], 1),
+                     "bytes_per_request_slot": round(sat["sys_bytes"], 1),
+                     "admission": "FIFO while the request's rows and slot fit the pool's HBM bytes"},
+            "bf16_saturated": rnd(sat),
             "bf16_poisson": {**rnd(poi), "arrival_rate_per_s": round(rate, 2)},
+            "accuracy_vs_fp64": {**accuracy, "sample": f"{len(pick)} random served requests of the Poisson run"},
             "fp32_per_request": rnd(base),
             "speedup_atom_steps": round(sat["atom_steps_per_s"] / base["atom_steps_per_s"], 2)}
 
@@ -452,7 +482,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="water23k", choices=["water23k", "water7k"])
     ap.add_argument("--skip-cpu", action="store_true", help="skip the FP64 oracle (accuracy and CPU baseline)")
-    ap.add_argument("--serving-requests", type=int, default=1024, help="0 skips the serving measurement")
+    ap.add_argument("--serving-requests", type=int, default=8192, help="0 skips the serving measurement")
     ap.add_argument("--serving-baseline", type=int, default=64)
     ap.add_argument("--training-boxes", type=int, default=256, help="0 skips the training measurement")
     ap.add_argument("--tile-bf16", type=int, default=0, help="MLP atoms per tile, BF16 form (0: default)")

@@ -81,6 +81,42 @@ class MlpWeights(C.Structure):
     ]
 
 
+FNL_POOL_N_ATOMS = 0
+FNL_POOL_N_SYS = 1
+FNL_POOL_N_TILES = 2
+FNL_POOL_ERROR = 3
+FNL_POOL_COUNTS = 8
+FNL_POOL_HEADER_WORDS = 16
+
+
+class PoolRequest(C.Structure):
+    _fields_ = [
+        ("n_atoms", i32),
+        ("src_row", i32),
+        ("stage_row", i32),
+        ("md_left", i32),
+        ("evaluated", i32),
+        ("noise_key", u32),
+        ("vel_key", u32),
+        ("reserved", u32),
+        ("step", u64),
+    ]
+
+
+POOL_TABLE_FIELDS = ("z_species", "z_net", "z_dtm", "z_a2", "z_vscale")
+POOL_ROW_FIELDS = ("x", "v", "x32", "z", "grad_self", "grad_nbr", "e_atom", "virial_atom", "dtm", "a2",
+                   "old_x", "old_v", "old_grad_self", "old_grad_nbr", "old_z", "batch_index", "species_idx", "perm")
+POOL_SYS_FIELDS = ("sys_offsets", "seg_offsets", "grid_cap", "tile_start", "tile_rows_tab", "tile_net",
+                   "sys_left", "sys_eval", "sys_step", "sys_key", "sys_fin", "sys_out_row", "counts")
+
+
+class Pool(C.Structure):
+    _fields_ = ([(n, i32) for n in ("max_atoms", "max_sys", "max_tiles", "n_nets", "tile_rows", "seg_atoms",
+                                    "n_z", "stage_rows")]
+                + [("seed", u64)]
+                + [(n, vp) for n in POOL_TABLE_FIELDS + POOL_ROW_FIELDS + POOL_SYS_FIELDS])
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     "fnl_version": (C.c_int, []),
@@ -101,13 +137,18 @@ _SIGNATURES = {
                                   C.POINTER(AevParams), vp, i32, vp]),
     "fnl_mlp_tile_rows": (C.c_int, [i32]),
     "fnl_species_mlp": (C.c_int, [vp, i32, C.POINTER(MlpWeights), vp, vp, vp, vp, i32, vp, vp, i32, i32, u64,
-                                  vp]),
+                                  vp, vp]),
     "fnl_system_reduce": (C.c_int, [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "fnl_forces": (C.c_int, [vp, vp, i32, f32, vp, vp]),
     "fnl_baoab_step": (C.c_int, [vp, vp, vp, vp, vp, f32, vp, f64, f64, vp, vp, u64, u64, vp, i32, i32, vp]),
     "fnl_counter_add": (C.c_int, [vp, u64, vp]),
     "fnl_baoab_step_batched": (C.c_int, [vp, vp, vp, vp, vp, f32, vp, f64, f64, vp, vp, vp, vp, vp, u64, i32, vp]),
     "fnl_philox_normals": (C.c_int, [vp, i32, u64, u64, u32, vp]),
+    "fnl_pool_out_bytes": (sz, [i32, i32, i32]),
+    "fnl_pool_save_rows": (C.c_int, [C.POINTER(Pool), i32, vp]),
+    "fnl_pool_layout": (C.c_int, [C.POINTER(Pool), vp, i32, vp, vp, vp]),
+    "fnl_pool_integrate": (C.c_int, [C.POINTER(Pool), f32, f64, f64, i32, vp]),
+    "fnl_pool_finish": (C.c_int, [C.POINTER(Pool), vp, vp, vp, i32, i32, vp, f32, i32, vp, sz, vp]),
     "fnl_fill_i32": (C.c_int, [vp, i32, sz, vp]),
     "fnl_fill_f32": (C.c_int, [vp, f32, sz, vp]),
 }

@@ -237,3 +237,24 @@ __device__ __forceinline__ void fnl_normals3(uint64_t seed, uint64_t step, uint3
   out[1] = rad0 * s0;
   out[2] = rad1 * c1;
 }
+
+// One atom's BAOAB step with merged half kicks (md/integrate.py:286-294, LGV O-step of
+// md/thermostats.py:108-120): F = -(grad_self + grad_nbr 2^-32) * force_scale in float64,
+// v += dtm F; x += dt/2 v; v = a1 v + a2 xi; x += dt/2 v.  Shared by the batched kernels so
+// that a request integrates to the same bits wherever it is served.
+__device__ __forceinline__ void fnl_baoab_row(double* __restrict__ x, double* __restrict__ v,
+                                              float* __restrict__ x32, const float* __restrict__ gs,
+                                              const long long* __restrict__ gn, float force_scale, double dtm,
+                                              double half_dt, double a1, double a2, const double xi[3], int i) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double f = -((double)gs[3 * i + k] + fnl_from_fixed(gn[3 * i + k])) * (double)force_scale;
+    double vv = v[3 * i + k] + dtm * f;
+    double xx = x[3 * i + k] + half_dt * vv;
+    vv = a1 * vv + a2 * xi[k];
+    xx = xx + half_dt * vv;
+    v[3 * i + k] = vv;
+    x[3 * i + k] = xx;
+    x32[3 * i + k] = (float)xx;
+  }
+}

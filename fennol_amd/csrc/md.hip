@@ -10,6 +10,7 @@
 namespace {
 
 constexpr int kRedVals = 1 + 9 + 9;  // energy, virial[9], kinetic[9]
+constexpr int kFuseMaxSystems = 16;  // systems summed by the last segment workgroup itself
 
 // Per-system sums of the partials of its segments: one wave per system, lane k
 // sums segments k, k+64, ...; then a fixed xor butterfly (deterministic order).
@@ -55,7 +56,7 @@ k_reduce_segments(const int32_t* __restrict__ sys_offsets, const int32_t* __rest
                   int seg_atoms, const float* __restrict__ energy_atom, const float* __restrict__ virial_atom,
                   const double* __restrict__ vel, const double* __restrict__ mass, double* __restrict__ partials,
                   unsigned* __restrict__ done, double* __restrict__ energy_sys, double* __restrict__ virial_sys,
-                  double* __restrict__ kinetic_sys) {
+                  double* __restrict__ kinetic_sys, int fuse_final) {
   __shared__ double red[kRedVals][256];
   const int sgi = blockIdx.x;
   const int t = threadIdx.x;
@@ -98,6 +99,10 @@ k_reduce_segments(const int32_t* __restrict__ sys_offsets, const int32_t* __rest
   // Each partial is written by a device-scope exchange whose result the thread waits for, so it
   // sits at the device's coherence point before the completion count is taken; the last
   // workgroup reads the partials there (agent-scope atomic loads), without cache-wide fences.
+  if (!fuse_final) {  // k_reduce_systems forms the per-system sums in the next launch
+    if (t < kRedVals) partials[(size_t)sgi * kRedVals + t] = red[t][0];
+    return;
+  }
   if (t < kRedVals) {
     const unsigned long long old = atomicExch(
         reinterpret_cast<unsigned long long*>(partials + (size_t)sgi * kRedVals + t),
@@ -191,17 +196,7 @@ __global__ void k_baoab_batched(double* __restrict__ x, double* __restrict__ v, 
   const int b = batch_index[i];
   double xi[3];
   fnl_normals3(seed, sys_step[b], sys_key[b], (uint32_t)(i - sys_offsets[b]), xi);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const double f = -((double)gs[3 * i + k] + fnl_from_fixed(gn[3 * i + k])) * (double)force_scale;
-    double vv = v[3 * i + k] + m * f;
-    double xx = x[3 * i + k] + half_dt * vv;
-    vv = a1 * vv + a2[i] * xi[k];
-    xx = xx + half_dt * vv;
-    v[3 * i + k] = vv;
-    x[3 * i + k] = xx;
-    x32[3 * i + k] = (float)xx;
-  }
+  fnl_baoab_row(x, v, x32, gs, gn, force_scale, m, half_dt, a1, a2[i], xi, i);
 }
 
 __global__ void k_counter_add(uint64_t* __restrict__ p, uint64_t v) {
@@ -237,10 +232,15 @@ extern "C" int fnl_system_reduce(const int32_t* sys_offsets, const int32_t* seg_
   double* e_out = energy_atom ? energy_sys : nullptr;
   double* v_out = virial_atom ? virial_sys : nullptr;
   double* k_out = vel ? kinetic_sys : nullptr;
+  // few systems: the last segment workgroup forms the per-system sums (one launch); many: one
+  // wave per system in a second launch (the last workgroup would loop over all of them)
+  const bool fuse = n_sys <= kFuseMaxSystems;
   if (n_seg > 0) {
     hipLaunchKernelGGL(k_reduce_segments, dim3(n_seg), dim3(256), 0, st, sys_offsets, seg_offsets, n_sys,
-                       seg_atoms, energy_atom, virial_atom, vel, mass, rows, done, e_out, v_out, k_out);
-  } else {
+                       seg_atoms, energy_atom, virial_atom, vel, mass, rows, done, e_out, v_out, k_out,
+                       (int)fuse);
+  }
+  if (n_seg == 0 || !fuse) {
     hipLaunchKernelGGL(k_reduce_systems, dim3(n_sys), dim3(64), 0, st, seg_offsets, n_sys, rows, e_out, v_out,
                        k_out);
   }

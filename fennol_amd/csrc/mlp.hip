@@ -52,6 +52,7 @@ struct MlpArgs {
   int k_pad;
   float alpha;
   unsigned long long k_live;  // bit q: input columns [64q, 64q + 64) may be non-zero in the batch
+  const int32_t* n_tiles_dev;  // tiles in use, read on the device (the grid covers n_tiles); or null
 };
 
 __device__ __forceinline__ unsigned short f2bf(float f) {
@@ -285,11 +286,13 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   const int l = fnl_lane();
   // Workgroups go to the XCDs round-robin (workgroup b -> XCD b % 8).  Each XCD takes a contiguous
   // range of the network-sorted tile list, so its L2 holds the weights of one or two networks.
-  const int per_xcd = (n_tiles + kXcds - 1) / kXcds;
+  const int nt = args.n_tiles_dev ? min(*args.n_tiles_dev, n_tiles) : n_tiles;
+  const int per_xcd = (nt + kXcds - 1) / kXcds;
   const int tile = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
-  if (tile >= n_tiles) return;
+  if (tile >= nt) return;
   const int net = tile_net[tile];
   const int nrows = tile_rows[tile];
+  if (nrows <= 0) return;
   const int r0 = tile_start[tile];
   const NetPtrs& P = args.nets[net];
   const int K = args.k_pad;
@@ -564,7 +567,8 @@ extern "C" int fnl_mlp_tile_rows(int32_t dtype) {
 extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_mlp_weights* w,
                                const int32_t* perm, const int32_t* tile_start, const int32_t* tile_rows,
                                const int32_t* tile_net, int32_t n_tiles, float* energy, void* grad,
-                               int32_t grad_stride, int32_t grad_dtype, uint64_t k_live, fnl_stream_t stream) {
+                               int32_t grad_stride, int32_t grad_dtype, uint64_t k_live,
+                               const int32_t* n_tiles_dev, fnl_stream_t stream) {
   if (!w || n_tiles < 0) return FNL_ERR_INVALID;
   if (n_tiles == 0) return FNL_OK;
   if (!aev || !perm || !tile_start || !tile_rows || !tile_net || !energy) return FNL_ERR_INVALID;
@@ -583,6 +587,7 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
   a.k_pad = w->k_pad;
   a.alpha = w->celu_alpha;
   a.k_live = k_live ? (unsigned long long)k_live : ~0ull;
+  a.n_tiles_dev = n_tiles_dev;
   bool comp = bf;  // compensated BF16 when every network carries the remainder matrices
   for (int n = 0; n < FNL_MLP_MAX_NETS; ++n) {
     NetPtrs& p = a.nets[n];

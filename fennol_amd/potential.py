@@ -696,7 +696,8 @@ class ANIPotential:
         rc = self.lib.fnl_species_mlp(
             P(sb.aev), sb.aev.shape[1], C.byref(self.nets.cw), P(sb.perm), P(sb.tile_start), P(sb.tile_rows),
             P(sb.tile_net), sb.n_tiles, P(sb.e_atom), P(sb.g_aev) if with_grad else None, sb.g_aev.shape[1],
-            self._row_dtype, self.config.live_column_blocks(sb.extra.get("aev_mask", 0xFFFFFFFF)), s)
+            self._row_dtype, self.config.live_column_blocks(sb.extra.get("aev_mask", 0xFFFFFFFF)),
+            sb.extra.get("n_tiles_dev"), s)
         _abi.check(rc, "fnl_species_mlp")
 
     def zero_grad_nbr(self, sb: SystemBuffers, stream=None) -> None:

@@ -353,7 +353,10 @@ int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_mlp_weights* 
                     const int32_t* perm, const int32_t* tile_start,
                     const int32_t* tile_rows, const int32_t* tile_net, int32_t n_tiles,
                     float* energy, void* grad, int32_t grad_stride, int32_t grad_dtype,
-                    uint64_t k_live, fnl_stream_t stream);
+                    uint64_t k_live, const int32_t* n_tiles_dev, fnl_stream_t stream);
+/* n_tiles_dev: NULL, or a device word holding the number of tiles in use
+ * (<= n_tiles, which then sizes the grid only): tables built on the device
+ * (fnl_pool_layout) and launches a HIP graph replays with changing tiles. */
 /* k_live: bit q set if input columns [64q, 64q + 64) may be non-zero for some atom
  * of the batch (0 = all).  The other columns must be zero in every AEV row: the
  * kernel skips their products (exact zeros) and writes dE/dAEV only in the
@@ -429,6 +432,125 @@ int fnl_counter_add(uint64_t* counter, uint64_t value, fnl_stream_t stream);
  * use stream = 1; the O-step uses stream = 0). */
 int fnl_philox_normals(double* out, int32_t n_atoms, uint64_t seed, uint64_t step,
                        uint32_t stream_id, fnl_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Serving pool: continuous batching of independent open-boundary systems
+ * (energy + forces requests and short Langevin runs) in one set of
+ * capacity-static device buffers.  No FeNNol function corresponds: the
+ * reference batches systems on the host (models/preprocessing.py:1763-1791,
+ * analyze.py:108-200) and builds its per-species index on the host
+ * (SpeciesIndexer, preprocessing.py:1583-1651).  Here the caller uploads one
+ * fnl_pool_request per active request (and the atoms of new ones) when the set
+ * changes; fnl_pool_layout builds every per-atom and per-system table on the
+ * device, and one serving iteration is a fixed sequence of launches over
+ * capacity-sized grids whose live sizes are read from device words (a HIP
+ * graph can hold it):
+ *   fnl_pool_integrate -> fnl_nblist_build -> fnl_aev_forward ->
+ *   fnl_species_mlp (n_tiles_dev = counts + FNL_POOL_N_TILES) -> clear of
+ *   grad_nbr -> fnl_aev_backward -> fnl_system_reduce -> fnl_pool_finish.
+ * A request's results do not depend on the other requests or on its rows:
+ * its neighbour rows, AEV rows, tree reductions (segments counted from its
+ * first row) and noise (counter = its local atom index, its step, its key)
+ * are its own.
+ * ---------------------------------------------------------------------- */
+typedef struct fnl_pool_request {
+  int32_t n_atoms;      /* rows of the request (>= 1)                               */
+  int32_t src_row;      /* first row of its atoms in the saved previous layout
+                           (fnl_pool_save_rows), or -1 for a new request           */
+  int32_t stage_row;    /* new request: first row of its atoms in stage_z/stage_x  */
+  int32_t md_left;      /* Langevin steps still to do (0: energy + forces only)     */
+  int32_t evaluated;    /* 1 once it has been evaluated (its forces are known)      */
+  uint32_t noise_key;   /* Philox stream of its O-step noise                        */
+  uint32_t vel_key;     /* new MD request: Philox stream of its initial velocities,
+                           drawn at step 0                                          */
+  uint32_t reserved;
+  uint64_t step;        /* its O-step counter (Langevin steps done)                 */
+} fnl_pool_request;
+
+/* device words of fnl_pool.counts */
+#define FNL_POOL_N_ATOMS 0   /* rows in use (the requests are packed from row 0)   */
+#define FNL_POOL_N_SYS 1     /* active requests                                     */
+#define FNL_POOL_N_TILES 2   /* MLP tiles of the layout                             */
+#define FNL_POOL_ERROR 3     /* non-zero: the last layout was refused (sizes beyond
+                                the capacities; the pool is then empty)             */
+#define FNL_POOL_COUNTS 8
+#define FNL_POOL_HEADER_WORDS 16  /* int32 words of fnl_pool_finish's record header */
+
+typedef struct fnl_pool {
+  int32_t max_atoms;    /* row capacity                                             */
+  int32_t max_sys;      /* request capacity                                         */
+  int32_t max_tiles;    /* MLP tile capacity: >= max_atoms / tile_rows + n_nets     */
+  int32_t n_nets;       /* MLP networks (<= FNL_MLP_MAX_NETS)                       */
+  int32_t tile_rows;    /* atoms per MLP tile                                       */
+  int32_t seg_atoms;    /* atoms per reduction segment (fnl_system_reduce)          */
+  int32_t n_z;          /* entries of the per-element tables (atomic numbers)       */
+  int32_t stage_rows;   /* rows of stage_z / stage_x                                */
+  uint64_t seed;        /* Philox key of the velocities and the O-step noise        */
+  /* per-element tables [n_z], indexed by atomic number (0 = padding) */
+  const int32_t* z_species;  /* AEV species index, -1 = padding                     */
+  const int32_t* z_net;      /* MLP network, -1 = none (energy 0, ChemicalNet rule) */
+  const double* z_dtm;       /* dt / m                                              */
+  const double* z_a2;        /* sqrt((1 - a1^2) kT / m)                             */
+  const double* z_vscale;    /* sqrt(kT / m)                                        */
+  /* per-row state [max_atoms] */
+  double* x; double* v; float* x32; int32_t* z;
+  float* grad_self; int64_t* grad_nbr; float* e_atom; float* virial_atom;
+  double* dtm; double* a2;
+  /* the previous layout's rows, written by fnl_pool_save_rows [max_atoms] */
+  double* old_x; double* old_v; float* old_grad_self; int64_t* old_grad_nbr; int32_t* old_z;
+  /* layout tables */
+  int32_t* batch_index; int32_t* species_idx; int32_t* perm;   /* [max_atoms]       */
+  int32_t* sys_offsets; int32_t* seg_offsets;                  /* [max_sys + 1]     */
+  int32_t* grid_cap;                                           /* [max_sys][4]      */
+  int32_t* tile_start; int32_t* tile_rows_tab; int32_t* tile_net; /* [max_tiles]    */
+  /* per-request serving state [max_sys] */
+  int32_t* sys_left; int32_t* sys_eval; uint64_t* sys_step; uint32_t* sys_key;
+  int32_t* sys_fin;     /* slot among this iteration's finished requests, or -1     */
+  int32_t* sys_out_row; /* first result row of a finished request                   */
+  int32_t* counts;      /* [FNL_POOL_COUNTS] device words                           */
+} fnl_pool;
+
+/* Copy rows [0, n_rows) of x, v, grad_self, grad_nbr and z to the old_* arrays
+ * (before a layout that moves continuing requests). */
+int fnl_pool_save_rows(const fnl_pool* pool, int32_t n_rows, fnl_stream_t stream);
+
+/* Layout of n_req requests (req: device array, in pool order) packed from row
+ * 0.  Builds sys_offsets, seg_offsets, grid_cap (open-boundary cell-grid
+ * capacities k^3 >= n), batch_index, species_idx, the network-sorted perm and
+ * tile tables (the order of pool_layout_arrays on the host), the per-request
+ * state, and the rows: continuing requests from the old_* arrays, new ones
+ * from stage_z / stage_x (device) with velocities v = N(0,1) * z_vscale from
+ * (seed, step 0, vel_key, local atom) for MD requests; rows beyond the last
+ * request are cleared to padding.  Rows and requests beyond the capacities
+ * are refused on the device (FNL_POOL_ERROR, empty pool). */
+int fnl_pool_layout(const fnl_pool* pool, const fnl_pool_request* req, int32_t n_req,
+                    const int32_t* stage_z, const double* stage_x, fnl_stream_t stream);
+
+/* BAOAB step (fnl_baoab_step_batched's arithmetic) of the rows of requests
+ * with md_left > 0 that have been evaluated; grid over n_rows rows (>= the
+ * rows in use; the rest exit). */
+int fnl_pool_integrate(const fnl_pool* pool, float force_scale, double half_dt, double a1,
+                       int32_t n_rows, fnl_stream_t stream);
+
+/* End of an iteration: every evaluated request is marked evaluated, an
+ * integrated one advances its step and md_left; requests whose md_left is 0
+ * finish.  Their energies (energy_sys), final positions (x) and forces
+ * (-(grad_self + grad_nbr 2^-32) * force_scale) are packed into `out`
+ * (device, out_bytes):
+ *   int32 header[FNL_POOL_HEADER_WORDS]: n_sys, n_fin, n_fin_atoms, n_atoms,
+ *     status[0..3] (FNL_STATUS_OVERFLOW, _MAX_NBR, _MAX_NBR_ANG, _MAX_CELL of
+ *     this iteration's build), error (non-zero: the record did not fit
+ *     out_bytes, or the layout was refused), then zeros;
+ *   int32 flags[n_sys] (bit 0: finished now; bit 1: a row of this request
+ *     exceeded max_nbr / max_nbr_ang, its result is not valid), padded to 8 B;
+ *   double energy[n_fin] (finished requests in pool order);
+ *   double x[n_fin_atoms][3]; float forces[n_fin_atoms][3].
+ * fnl_pool_out_bytes gives the size of that record. */
+int fnl_pool_finish(const fnl_pool* pool, const double* energy_sys, const int32_t* n_nbr,
+                    const int32_t* n_nbr_ang, int32_t max_nbr, int32_t max_nbr_ang,
+                    const int32_t* status, float force_scale, int32_t n_rows, void* out,
+                    size_t out_bytes, fnl_stream_t stream);
+size_t fnl_pool_out_bytes(int32_t n_sys, int32_t n_fin, int32_t n_fin_atoms);
 
 /* Device fill helpers (graph-capture safe replacements for memset). */
 int fnl_fill_i32(int32_t* p, int32_t value, size_t count, fnl_stream_t stream);

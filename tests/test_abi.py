@@ -54,4 +54,52 @@ def test_host_side_argument_checks():
                                0, 0xFFFFFFFF, None) == -1
     assert lib.fnl_fill_i32(None, 0, 0, None) == 0
     assert lib.fnl_fill_i32(None, 0, 5, None) == -1
-    assert lib.fnl_species_mlp(None, 0, None, None, None, None, None, 0, None, None, 0, 0, 0, None) == -1
+    assert lib.fnl_species_mlp(None, 0, None, None, None, None, None, 0, None, None, 0, 0, 0, None, None) == -1
+
+
+def _c_layout(struct: str, fields: list[str], tmp_path) -> dict:
+    """sizeof and offsetof of a header struct, from the host C++ compiler."""
+    import shutil
+    import subprocess
+
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if cxx is None:
+        pytest.skip("no host C++ compiler")
+    lines = [f'#include "{HEADER}"', "#include <cstddef>", "#include <cstdio>", "int main() {",
+             f'  std::printf("size %zu\\n", sizeof({struct}));']
+    lines += [f'  std::printf("{f} %zu\\n", offsetof({struct}, {f}));' for f in fields]
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.cpp"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cxx, "-o", str(exe), str(src)], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    return {k: int(v) for k, v in (ln.split() for ln in out.splitlines())}
+
+
+@pytest.mark.parametrize("name", ["Pool", "PoolRequest", "MlpWeights", "MlpNet", "AevParams"])
+def test_ctypes_structs_match_the_header(name, tmp_path):
+    """Every ctypes structure has the size and field offsets of its C declaration."""
+    import ctypes as C
+
+    cls = getattr(_abi, name)
+    cname = {"Pool": "fnl_pool", "PoolRequest": "fnl_pool_request", "MlpWeights": "fnl_mlp_weights",
+             "MlpNet": "fnl_mlp_net", "AevParams": "fnl_aev_params"}[name]
+    fields = [f for f, _ in cls._fields_]
+    lay = _c_layout(cname, fields, tmp_path)
+    assert lay["size"] == C.sizeof(cls)
+    for f in fields:
+        assert lay[f] == getattr(cls, f).offset, f
+
+
+def test_serving_record_size_matches_the_library():
+    if not _abi.LIB_PATH.exists():
+        pytest.skip("HIP library not built")
+    from fennol_amd.serving import REQ_DTYPE, out_bytes
+
+    lib = _abi.load()
+    for args in [(0, 0, 0), (1, 1, 9), (7, 3, 100), (2048, 2048, 65536)]:
+        assert lib.fnl_pool_out_bytes(*args) == out_bytes(*args)
+    assert REQ_DTYPE.itemsize == __import__("ctypes").sizeof(_abi.PoolRequest)
+    assert [REQ_DTYPE.fields[f][1] for f, _ in _abi.PoolRequest._fields_] == \
+        [getattr(_abi.PoolRequest, f).offset for f, _ in _abi.PoolRequest._fields_]

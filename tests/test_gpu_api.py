@@ -297,49 +297,6 @@ def test_md_skin_graph_replay_matches_eager(gpu):
         np.testing.assert_allclose(a, b, rtol=0, atol=1e-6)
 
 
-def test_serving_results_same_alone_and_in_mix(gpu):
-    """Continuous batching: each request's result equals its result when served alone."""
-    print("[api] serving mix", flush=True)
-    from fennol_amd.potential import ANIConfig, ANIPotential
-    from fennol_amd.serving import Request, ServingEngine
-
-    cfg = ANIConfig()
-    params = O.init_mlp_params(11, O.aev_length(O.AEVParams()))
-    rng = np.random.Generator(np.random.PCG64(21))
-    reqs = []
-    for k, n in enumerate((9, 40, 77, 130, 256, 17, 64, 300)):
-        c = systems.lattice_cluster(n, rng)
-        reqs.append(Request(c["species"], c["coordinates"], md_steps=(12 if k % 3 == 0 else 0),
-                            arrival=0.0, rid=k))
-
-    def serve(batch, max_atoms):
-        pot = ANIPotential(cfg, params, precision="fp32")
-        eng = ServingEngine(pot, max_atoms=max_atoms, max_systems=16, seed=4)
-        for r in batch:
-            r2 = Request(r.species, r.coordinates, r.md_steps, 0.0, r.rid)
-            eng.submit(r2)
-        out = {}
-        while len(out) < len(batch):
-            for r in eng.iterate(float("inf")):
-                out[r.rid] = r
-        return out
-
-    mix = serve(reqs, max_atoms=700)   # capacity forces staggered admission
-    for r in reqs:
-        alone = serve([r], max_atoms=512)[r.rid]
-        m = mix[r.rid]
-        assert m.steps_done == r.md_steps
-        np.testing.assert_allclose(m.energy, alone.energy, rtol=1e-5, atol=1e-5)
-        np.testing.assert_allclose(m.forces, alone.forces, rtol=1e-4, atol=1e-4)
-        np.testing.assert_allclose(m.positions, alone.positions, rtol=0, atol=1e-6)
-    # single evaluations equal the direct potential (oracle-checked elsewhere)
-    pot = ANIPotential(cfg, params, precision="fp32")
-    for r in reqs:
-        if r.md_steps == 0:
-            e, f, _ = pot.energy_forces_virial(r.species, r.coordinates)
-            np.testing.assert_allclose(mix[r.rid].energy, e[0], rtol=1e-5, atol=1e-5)
-
-
 def test_fnx_save_load_roundtrip(gpu, tmp_path):
     print("[api] fnx roundtrip", flush=True)
     s = systems.cluster_config0(6)
