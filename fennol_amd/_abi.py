@@ -81,6 +81,10 @@ class MlpWeights(C.Structure):
     ]
 
 
+class TrainNet(C.Structure):
+    _fields_ = [(n, vp) for n in ("w1", "b1", "w2", "b2", "w3", "b3", "w4", "b4")]
+
+
 FNL_POOL_N_ATOMS = 0
 FNL_POOL_N_SYS = 1
 FNL_POOL_N_TILES = 2
@@ -144,6 +148,10 @@ _SIGNATURES = {
     "fnl_counter_add": (C.c_int, [vp, u64, vp]),
     "fnl_baoab_step_batched": (C.c_int, [vp, vp, vp, vp, vp, f32, vp, f64, f64, vp, vp, vp, vp, vp, u64, i32, vp]),
     "fnl_philox_normals": (C.c_int, [vp, i32, u64, u64, u32, vp]),
+    "fnl_train_workspace": (sz, [i32, i32, i32]),
+    "fnl_train_grad_stride": (sz, [i32]),
+    "fnl_train_mlp_grad": (C.c_int, [vp, i32, vp, i32, i32, C.POINTER(TrainNet), i32, C.POINTER(i32), vp, vp, f32,
+                                     f32, u64, vp, sz, vp, vp]),
     "fnl_pool_out_bytes": (sz, [i32, i32, i32]),
     "fnl_pool_save_rows": (C.c_int, [C.POINTER(Pool), i32, vp]),
     "fnl_pool_layout": (C.c_int, [C.POINTER(Pool), vp, i32, vp, vp, vp]),

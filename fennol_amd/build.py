@@ -17,7 +17,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIBDIR = PKG / "_lib"
 LIBNAME = "libfnl_hip.so"
-SOURCES = ["nblist.hip", "aev.hip", "mlp.hip", "md.hip", "serving.hip"]
+SOURCES = ["nblist.hip", "aev.hip", "mlp.hip", "md.hip", "serving.hip", "train.hip"]
 ARCH = os.environ.get("FNL_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

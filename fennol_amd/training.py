@@ -18,8 +18,11 @@ respect to the weights (:508).  Here the gradient is assembled by hand:
   differentiated twice: forward primal and tangent chains, then the reverse of
   both (CELU'' enters through the tangent chain).
 
-The MLP part runs as plain FP32 library GEMMs (torch.mm on rocBLAS/hipBLASLt);
-the AEV, its tangent, the forces and the energies come from the HIP kernels.
+The MLP part is ``fnl_train_mlp_grad``: FP32 MFMA kernels that run both chains
+on shared weight operands, fuse the CELU parts and the start of the reverse
+pass into their epilogues, and reduce the weight gradients over atoms in a
+fixed order (the same bits every run); the AEV, its tangent, the forces and the
+energies come from the other HIP kernels.  No library GEMM runs.
 """
 
 from __future__ import annotations
@@ -31,15 +34,6 @@ import torch
 
 from . import _abi
 from .potential import ANIPotential, SystemBuffers
-
-
-def _celu_parts(z: torch.Tensor, alpha: float):
-    e = torch.exp(torch.clamp(z, max=0.0) / alpha)
-    pos = z > 0
-    h = torch.where(pos, z, alpha * (e - 1.0))
-    d1 = torch.where(pos, torch.ones_like(z), e)
-    d2 = torch.where(pos, torch.zeros_like(z), e / alpha)
-    return h, d1, d2
 
 
 class TrainingStep:
@@ -55,11 +49,21 @@ class TrainingStep:
         self.alpha = celu_alpha
         dev = potential.device
         self.species = list(params.keys())
+        if self.species != list(potential.nets.species):
+            raise ValueError("params must list the potential's networks in its order")
         self.weights = {}
-        for s, layers in params.items():
-            self.weights[s] = [(torch.tensor(np.asarray(W), dtype=torch.float32, device=dev),
-                                torch.tensor(np.asarray(b), dtype=torch.float32, device=dev)) for W, b in layers]
+        self.cnets = (_abi.TrainNet * len(self.species))()
+        for n, (s, layers) in enumerate(params.items()):
+            self.weights[s] = [(torch.tensor(np.asarray(W), dtype=torch.float32, device=dev).contiguous(),
+                                torch.tensor(np.asarray(b), dtype=torch.float32, device=dev).contiguous())
+                               for W, b in layers]
+            (W1, b1), (W2, b2), (W3, b3), (W4, b4) = self.weights[s]
+            c = self.cnets[n]
+            c.w1, c.b1, c.w2, c.b2 = W1.data_ptr(), b1.data_ptr(), W2.data_ptr(), b2.data_ptr()
+            c.w3, c.b3, c.w4, c.b4 = W3.data_ptr(), b3.data_ptr(), W4.data_ptr(), b4.data_ptr()
         self.k_in = potential.config.aev_length
+        self.gstride = int(potential.lib.fnl_train_grad_stride(self.k_in))
+        self._ws = None
 
     def tangent(self, sb: SystemBuffers, x32: torch.Tensor, disp: torch.Tensor) -> torch.Tensor:
         """AEV tangent rows [N, k_pad] along disp [N, 3] (float32)."""
@@ -86,57 +90,41 @@ class TrainingStep:
         rF = (F - f_ref) * real[:, None]
         loss = self.w_e * float((rE ** 2).sum()) / n_e + self.w_f * float((rF.double() ** 2).sum()) / n_f
         dA = self.tangent(sb, x32, rF.contiguous())
-        cE_atom = ((2.0 * self.w_e / n_e) * rE)[sb.batch_index[:N].long()].float()
+        cE_atom = ((2.0 * self.w_e / n_e) * rE)[sb.batch_index[:N].long()].float().contiguous()
         cD = -2.0 * self.w_f / n_f
-        grads = {}
-        species_z = torch.as_tensor(sb.species, device=pot.device)
-        aev = sb.aev[:, : self.k_in].float()
-        for n, s in enumerate(self.species):
-            idx = torch.nonzero(species_z == _z(s)).flatten()
-            grads[s] = self._species_grad(s, aev[idx], dA[idx, : self.k_in], cE_atom[idx], cD)
+        grads = self.mlp_grad(sb, dA, cE_atom, cD)
         return loss, grads, E, F
 
-    def _species_grad(self, s: str, A: torch.Tensor, dA: torch.Tensor, cE: torch.Tensor, cD: float):
-        (W1, b1), (W2, b2), (W3, b3), (W4, b4) = self.weights[s]
-        a = self.alpha
-        if A.shape[0] == 0:
-            return [(torch.zeros_like(W), torch.zeros_like(b)) for W, b in self.weights[s]]
-        # forward primal and tangent chains
-        z1 = A @ W1 + b1
-        h1, s1, t1 = _celu_parts(z1, a)
-        dz1 = dA @ W1
-        dh1 = s1 * dz1
-        z2 = h1 @ W2 + b2
-        h2, s2, t2 = _celu_parts(z2, a)
-        dz2 = dh1 @ W2
-        dh2 = s2 * dz2
-        z3 = h2 @ W3 + b3
-        h3, s3, t3 = _celu_parts(z3, a)
-        dz3 = dh2 @ W3
-        dh3 = s3 * dz3
-        w4 = W4[:, 0]
-        # reverse through E = h3 w4 + b4 (adjoint cE) and D = dh3 w4 (adjoint cD)
-        gW4 = (h3 * cE[:, None]).sum(0) + cD * dh3.sum(0)
-        gb4 = cE.sum().reshape(1)
-        bh3 = cE[:, None] * w4[None, :]
-        bdh3 = torch.full_like(dh3, cD) * w4[None, :]
-        bdz3 = s3 * bdh3
-        bz3 = t3 * dz3 * bdh3 + s3 * bh3
-        gW3 = dh2.T @ bdz3 + h2.T @ bz3
-        gb3 = bz3.sum(0)
-        bdh2 = bdz3 @ W3.T
-        bh2 = bz3 @ W3.T
-        bdz2 = s2 * bdh2
-        bz2 = t2 * dz2 * bdh2 + s2 * bh2
-        gW2 = dh1.T @ bdz2 + h1.T @ bz2
-        gb2 = bz2.sum(0)
-        bdh1 = bdz2 @ W2.T
-        bh1 = bz2 @ W2.T
-        bdz1 = s1 * bdh1
-        bz1 = t1 * dz1 * bdh1 + s1 * bh1
-        gW1 = dA.T @ bdz1 + A.T @ bz1
-        gb1 = bz1.sum(0)
-        return [(gW1, gb1), (gW2, gb2), (gW3, gb3), (gW4[:, None], gb4)]
+    def mlp_grad(self, sb: SystemBuffers, dA: torch.Tensor, c_energy: torch.Tensor, c_force: float) -> dict:
+        """dL/dW of every network from the AEV rows of ``sb``, the tangent rows dA [N, k_pad],
+        per-atom energy adjoints (atom order) and the force adjoint (fnl_train_mlp_grad)."""
+        pot = self.pot
+        # atoms grouped by network in network order: the potential's tile permutation
+        counts = [int(np.sum(sb.species[: int(np.sum(sb.natoms))] == _z(s))) for s in self.species]
+        net_rows = (ctypes.c_int32 * (len(counts) + 1))(*np.concatenate([[0], np.cumsum(counts)]).astype(int))
+        n_rows = int(net_rows[len(counts)])
+        need = int(pot.lib.fnl_train_workspace(n_rows, len(counts), self.k_in))
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(need, dtype=torch.uint8, device=pot.device)
+        grad = torch.empty(len(counts) * self.gstride, dtype=torch.float32, device=pot.device)
+        k_live = pot.config.live_column_blocks(sb.extra.get("aev_mask", 0xFFFFFFFF))
+        P = _abi.ptr
+        _abi.check(pot.lib.fnl_train_mlp_grad(
+            P(sb.aev), sb.aev.shape[1], P(dA), dA.shape[1], self.k_in, self.cnets, len(counts), net_rows,
+            P(sb.perm), P(c_energy), float(c_force), float(self.alpha), k_live, P(self._ws), self._ws.numel(),
+            P(grad), _abi.stream_handle()), "fnl_train_mlp_grad")
+        K, H1, H2, H3 = self.k_in, 256, 192, 160
+        shapes = [(K, H1), (H1,), (H1, H2), (H2,), (H2, H3), (H3,), (H3, 1), (1,)]
+        out = {}
+        for n, s in enumerate(self.species):
+            g = grad[n * self.gstride: (n + 1) * self.gstride]
+            parts, o = [], 0
+            for shp in shapes:
+                size = int(np.prod(shp))
+                parts.append(g[o: o + size].view(*shp))
+                o += size
+            out[s] = [(parts[0], parts[1]), (parts[2], parts[3]), (parts[4], parts[5]), (parts[6], parts[7])]
+        return out
 
 
 def _z(symbol: str) -> int:

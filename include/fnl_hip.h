@@ -434,6 +434,49 @@ int fnl_philox_normals(double* out, int32_t n_atoms, uint64_t seed, uint64_t ste
                        uint32_t stream_id, fnl_stream_t stream);
 
 /* ------------------------------------------------------------------------
+ * Weight gradient of the energy + force loss through the species MLPs.
+ *
+ * Replaces the MLP part of jax.grad of the force-carrying loss in FeNNol's
+ * train_step (training/utils.py:140-525, :508): for each network, with input
+ * rows A (AEV), tangent rows dA = dAEV/dx . r (fnl_aev_tangent along
+ * r = F - F_ref), per-atom energy adjoints c_energy = dL/dE_atom and the force
+ * adjoint c_force = dL/dD (D = sum_i g_i . dA_i):
+ *   forward   z_l = h_{l-1} W_l + b_l, dz_l = dh_{l-1} W_l,
+ *             h_l = celu(z_l), dh_l = celu'(z_l) dz_l            (h_0 = A, dh_0 = dA)
+ *   reverse   bh3 = c_energy w4, bdh3 = c_force w4,
+ *             bdz_l = s_l bdh_l, bz_l = t_l dz_l bdh_l + s_l bh_l  (s = celu', t = celu'')
+ *             bdh_{l-1} = bdz_l W_l^T, bh_{l-1} = bz_l W_l^T
+ *   gradient  dW_l = dh_{l-1}^T bdz_l + h_{l-1}^T bz_l, db_l = sum bz_l,
+ *             dw4 = sum (h3 c_energy + c_force dh3), db4 = sum c_energy
+ * FP32 on MFMA; the reductions over atoms are split in chunks and summed in a
+ * fixed order (the gradient is the same bits every run).
+ *   weights   fnl_train_net per network, row-major W [in][out] (w1: k_in x 256)
+ *   rows      perm lists the atoms grouped by network in network order;
+ *             net_rows (host, [n_nets+1]) are the offsets of the groups
+ *   aev/daev  rows in atom order (float32, strides >= k_in, multiples of 4)
+ *   k_live    bit q: AEV columns [64q, 64q+64) may be non-zero (0 = all);
+ *             the gradient rows of the others are zero
+ *   grad      float32 [n_nets][fnl_train_grad_stride(k_in)]: dW1 [k_in][256],
+ *             db1 [256], dW2 [256][192], db2 [192], dW3 [192][160], db3 [160],
+ *             dw4 [160], db4 [1]
+ *   workspace fnl_train_workspace(rows, n_nets, k_in) bytes
+ * ---------------------------------------------------------------------- */
+typedef struct fnl_train_net {
+  const float* w1; const float* b1;
+  const float* w2; const float* b2;
+  const float* w3; const float* b3;
+  const float* w4; const float* b4;
+} fnl_train_net;
+
+size_t fnl_train_workspace(int32_t n_rows, int32_t n_nets, int32_t k_in);
+size_t fnl_train_grad_stride(int32_t k_in);
+int fnl_train_mlp_grad(const float* aev, int32_t aev_stride, const float* daev, int32_t daev_stride,
+                       int32_t k_in, const fnl_train_net* nets, int32_t n_nets,
+                       const int32_t* net_rows, const int32_t* perm, const float* c_energy,
+                       float c_force, float celu_alpha, uint64_t k_live, void* workspace,
+                       size_t workspace_bytes, float* grad, fnl_stream_t stream);
+
+/* ------------------------------------------------------------------------
  * Serving pool: continuous batching of independent open-boundary systems
  * (energy + forces requests and short Langevin runs) in one set of
  * capacity-static device buffers.  No FeNNol function corresponds: the

@@ -77,14 +77,14 @@ def _c_layout(struct: str, fields: list[str], tmp_path) -> dict:
     return {k: int(v) for k, v in (ln.split() for ln in out.splitlines())}
 
 
-@pytest.mark.parametrize("name", ["Pool", "PoolRequest", "MlpWeights", "MlpNet", "AevParams"])
+@pytest.mark.parametrize("name", ["Pool", "PoolRequest", "MlpWeights", "MlpNet", "AevParams", "TrainNet"])
 def test_ctypes_structs_match_the_header(name, tmp_path):
     """Every ctypes structure has the size and field offsets of its C declaration."""
     import ctypes as C
 
     cls = getattr(_abi, name)
     cname = {"Pool": "fnl_pool", "PoolRequest": "fnl_pool_request", "MlpWeights": "fnl_mlp_weights",
-             "MlpNet": "fnl_mlp_net", "AevParams": "fnl_aev_params"}[name]
+             "MlpNet": "fnl_mlp_net", "AevParams": "fnl_aev_params", "TrainNet": "fnl_train_net"}[name]
     fields = [f for f, _ in cls._fields_]
     lay = _c_layout(cname, fields, tmp_path)
     assert lay["size"] == C.sizeof(cls)

@@ -51,12 +51,50 @@ def test_aev_tangent_matches_finite_differences(gpu):
     assert np.all(err <= 2e-3 * np.maximum(1.0, np.abs(num))), err.max()
 
 
-def test_training_gradient_finite_differences(gpu):
-    print("[train] weight gradient", flush=True)
+def _celu_parts(z, alpha):
+    e = torch.exp(torch.clamp(z, max=0.0) / alpha)
+    pos = z > 0
+    return (torch.where(pos, z, alpha * (e - 1.0)), torch.where(pos, torch.ones_like(z), e),
+            torch.where(pos, torch.zeros_like(z), e / alpha))
+
+
+def _torch_species_grad(weights, A, dA, cE, cD, a=0.1):
+    """Plain PyTorch FP32 restatement of the MLP part of the gradient (forward primal and
+    tangent chains, reverse of both), the reference the HIP kernels are compared with."""
+    (W1, b1), (W2, b2), (W3, b3), (W4, b4) = weights
+    z1 = A @ W1 + b1
+    h1, s1, t1 = _celu_parts(z1, a)
+    dz1 = dA @ W1
+    dh1 = s1 * dz1
+    z2 = h1 @ W2 + b2
+    h2, s2, t2 = _celu_parts(z2, a)
+    dz2 = dh1 @ W2
+    dh2 = s2 * dz2
+    z3 = h2 @ W3 + b3
+    h3, s3, t3 = _celu_parts(z3, a)
+    dz3 = dh2 @ W3
+    dh3 = s3 * dz3
+    w4 = W4[:, 0]
+    gW4 = (h3 * cE[:, None]).sum(0) + cD * dh3.sum(0)
+    bh3 = cE[:, None] * w4[None, :]
+    bdh3 = torch.full_like(dh3, cD) * w4[None, :]
+    bdz3 = s3 * bdh3
+    bz3 = t3 * dz3 * bdh3 + s3 * bh3
+    bdh2, bh2 = bdz3 @ W3.T, bz3 @ W3.T
+    bdz2 = s2 * bdh2
+    bz2 = t2 * dz2 * bdh2 + s2 * bh2
+    bdh1, bh1 = bdz2 @ W2.T, bz2 @ W2.T
+    bdz1 = s1 * bdh1
+    bz1 = t1 * dz1 * bdh1 + s1 * bh1
+    return [(dA.T @ bdz1 + A.T @ bz1, bz1.sum(0)), (dh1.T @ bdz2 + h1.T @ bz2, bz2.sum(0)),
+            (dh2.T @ bdz3 + h2.T @ bz3, bz3.sum(0)), (gW4[:, None], cE.sum().reshape(1))]
+
+
+def _training_setup(n_boxes=4, seed=5):
     from fennol_amd.potential import ANIConfig, ANIPotential
     from fennol_amd.training import TrainingStep
 
-    batch = systems.training_boxes(n_boxes=4, seed=5)
+    batch = systems.training_boxes(n_boxes=n_boxes, seed=seed)
     cfg = ANIConfig(species_order=SPECIES7)
     p = O.AEVParams(species_order=SPECIES7)
     params = O.init_mlp_params(11, cfg.aev_length)
@@ -69,19 +107,69 @@ def test_training_gradient_finite_differences(gpu):
     sb = pot.setup(batch["species"], batch["natoms"], batch["cells"])
     x32 = torch.tensor(np.asarray(batch["coordinates"], np.float32), device="cuda")
     ts = TrainingStep(pot, params)
+    return batch, p, params, g, e_ref, f_ref, pot, sb, x32, ts
+
+
+def test_training_kernels_match_a_torch_fp32_reference(gpu):
+    """fnl_train_mlp_grad against the same chains written with torch FP32 products, on every
+    weight and bias of both networks present (H and O); dead layer-1 rows are exact zeros; a
+    second call gives the same bits (fixed-order reductions)."""
+    print("[train] kernels vs torch fp32", flush=True)
+    batch, p, params, g, e_ref, f_ref, pot, sb, x32, ts = _training_setup(n_boxes=8, seed=6)
+    pot.compute(sb, x32)
+    rng = np.random.Generator(np.random.PCG64(3))
+    r = torch.tensor(rng.normal(size=x32.shape).astype(np.float32), device="cuda")
+    dA = ts.tangent(sb, x32, r)
+    N = sb.n_atoms
+    cE = torch.tensor(rng.normal(size=N).astype(np.float32), device="cuda")
+    cD = -0.37
+    grads = ts.mlp_grad(sb, dA, cE, cD)
+    again = ts.mlp_grad(sb, dA, cE, cD)
+    z = torch.as_tensor(batch["species"], device="cuda")
+    K = ts.k_in
+    live = pot.config.live_column_blocks(sb.extra["aev_mask"])
+    dead = torch.tensor([((live >> (c // 64)) & 1) == 0 for c in range(K)], device="cuda")
+    from fennol_amd.units import SYMBOL_TO_Z
+    for s in ts.species:
+        idx = torch.nonzero(z == SYMBOL_TO_Z[s]).flatten()
+        ref = _torch_species_grad(ts.weights[s], sb.aev[idx, :K].float(), dA[idx, :K], cE[idx], cD)
+        for l in range(4):
+            for k in range(2):
+                got, want = grads[s][l][k], ref[l][k]
+                assert torch.equal(got, again[s][l][k])
+                if idx.numel() == 0:
+                    assert not got.any()
+                    continue
+                scale = float(want.abs().max()) + 1e-30
+                err = float((got - want).abs().max())
+                assert err <= 2e-4 * scale, (s, l, k, err, scale)
+        if idx.numel():
+            assert not grads[s][0][0][dead].any()
+
+
+def test_training_gradient_finite_differences(gpu):
+    """Loss and 64 sampled weights (plus 8 biases) against central differences of the FP64
+    oracle loss (h = 1e-4).  Layer-1 samples come from AEV channels the batch's atoms use (H and
+    O blocks), so zeroing a layer-1 gradient fails the test."""
+    print("[train] weight gradient", flush=True)
+    batch, p, params, g, e_ref, f_ref, pot, sb, x32, ts = _training_setup()
     loss, grads, E, F = ts(sb, x32, torch.tensor(e_ref, device="cuda"),
                            torch.tensor(f_ref, dtype=torch.float32, device="cuda"))
     model = O.Model(p, params)
     loss_ref = _oracle_loss(model, batch, e_ref, f_ref, g)
     assert loss == pytest.approx(loss_ref, rel=1e-4)
+    live = pot.config.live_column_blocks(sb.extra["aev_mask"])
+    aev = sb.aev.float().cpu().numpy()
     rng = np.random.Generator(np.random.PCG64(7))
     h = 1e-4
     checked = 0
-    for s in ("H", "O"):
+    for s, zz in (("H", 1), ("O", 8)):
+        used = np.nonzero(np.abs(aev[batch["species"] == zz]).max(0) > 1e-3)[0]
+        used = used[[((live >> (c // 64)) & 1) == 1 for c in used]]
         for layer in range(4):
             W = params[s][layer][0]
-            for _ in range(2):
-                i = int(rng.integers(0, W.shape[0] if layer else 128))  # layer 1: channels that are non-zero
+            for _ in range(8):
+                i = int(rng.choice(used)) if layer == 0 else int(rng.integers(0, W.shape[0]))
                 j = int(rng.integers(0, W.shape[1]))
                 old = W[i, j]
                 W[i, j] = old + h
@@ -92,6 +180,8 @@ def test_training_gradient_finite_differences(gpu):
                 num = (lp - lm) / (2 * h)
                 got = float(grads[s][layer][0][i, j])
                 assert got == pytest.approx(num, rel=5e-3, abs=1e-6 * max(1.0, abs(loss_ref))), (s, layer, i, j)
+                if layer == 0:
+                    assert abs(num) > 1e-8 * max(1.0, abs(loss_ref)), (s, i, j)  # a live channel
                 checked += 1
             b = params[s][layer][1]
             k = int(rng.integers(0, b.shape[0]))
@@ -104,5 +194,4 @@ def test_training_gradient_finite_differences(gpu):
             num = (lp - lm) / (2 * h)
             got = float(grads[s][layer][1][k])
             assert got == pytest.approx(num, rel=5e-3, abs=1e-6 * max(1.0, abs(loss_ref))), (s, "bias", layer, k)
-            checked += 1
-    assert checked == 24
+    assert checked == 64
