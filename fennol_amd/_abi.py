@@ -152,6 +152,8 @@ _SIGNATURES = {
     "fnl_train_grad_stride": (sz, [i32]),
     "fnl_train_mlp_grad": (C.c_int, [vp, i32, vp, i32, i32, C.POINTER(TrainNet), i32, C.POINTER(i32), vp, vp, f32,
                                      f32, u64, vp, sz, vp, vp]),
+    "fnl_train_residuals": (C.c_int, [vp, vp, i32, vp, vp, f32, vp, vp, vp, i32, f64, f64, f64, f64, vp, vp, vp, vp,
+                                      vp, vp]),
     "fnl_pool_out_bytes": (sz, [i32, i32, i32]),
     "fnl_pool_save_rows": (C.c_int, [C.POINTER(Pool), i32, vp]),
     "fnl_pool_layout": (C.c_int, [C.POINTER(Pool), vp, i32, vp, vp, vp]),

@@ -161,19 +161,24 @@ __host__ __device__ inline int round4(int n) { return (n + 3) & ~3; }
 
 // per-wave LDS layout of the forward kernel (offsets in 4-byte words, 16-byte aligned)
 struct FwdLayout {
-  int rdf, av, af, fac, tab, words;
+  int nbt, rdf, av, at, af, ag, fac, tab, words;
 };
 // per-wave LDS slice of the forward (4-byte words): the triplet-factor table is only used after
 // the slot vectors (nbv) and the sorted radial slots (rdf) are dead, so it shares their space;
-// 8 waves per SIMD at the water box's row capacities (104 radial, 40 angular slots)
-__host__ __device__ inline FwdLayout fwd_layout(int max_nbr, int max_nbr_ang, int na_nz) {
+// 8 waves per SIMD at the water box's row capacities (104 radial, 40 angular slots).  The
+// tangent (tan) adds the slot displacements (nbt), the angular slots' unit-vector and distance
+// tangents (at) and switch derivatives (ag), float4 radial entries and a table twice as tall.
+__host__ __device__ inline FwdLayout fwd_layout(int max_nbr, int max_nbr_ang, int na_nz, bool tan = false) {
   FwdLayout L;
   L.fac = 0;            // nbv at 0 as well
-  L.rdf = 4 * max_nbr;
-  const int lo = L.rdf + 2 * max_nbr, fw = na_nz * kFacLd;
+  L.nbt = 4 * max_nbr;  // tangent only
+  L.rdf = (tan ? 8 : 4) * max_nbr;
+  const int lo = L.rdf + (tan ? 4 : 2) * max_nbr, fw = (tan ? 2 : 1) * na_nz * kFacLd;
   L.av = round4(lo > fw ? lo : fw);
-  L.af = L.av + 4 * max_nbr_ang;
-  L.tab = round4(L.af + 2 * max_nbr_ang);
+  L.at = L.av + 4 * max_nbr_ang;                   // tangent only
+  L.af = L.at + (tan ? 4 * max_nbr_ang : 0);
+  L.ag = L.af + 2 * max_nbr_ang;                   // tangent only
+  L.tab = round4(L.ag + (tan ? 2 * max_nbr_ang : 0));
   L.words = round4(L.tab + kTabWords);
   return L;
 }
@@ -281,7 +286,14 @@ __device__ __forceinline__ void species_sort(int n, int S, SpecFn spec, PlaceFn 
 // R_d F_z of the triplets h, h+2, ... of each pair segment in the chunk to a
 // register; when a segment ends, the halves combine (one shuffle) and the 32
 // channels of the pair are stored.  Every channel is written exactly once.
-template <int NA, int NZ, typename TOut>
+//
+// TAN: the tangent (forward-mode JVP) of the AEV along a displacement field r instead,
+//   dA_i = sum_e dA_i/dvec_e . (r_j - r_i)
+// (the same edge vectors and terms; used for weight gradients through forces, where
+// F . r = -sum_i dE/dA_i . dA_i).  Radial: d/dd (0.25 fc exp(-eta x^2)) dd.  Angular, per triplet
+// the tangent of sw F_z(c) R_d(m) is F_z (dsw R_d + sw R_d' dm) + F_z' (sw R_d dc) =
+// F_z P_d + F_z' Q_d: the table holds P_d, Q_d, F_z, F_z' and each channel adds two products.
+template <int NA, int NZ, typename TOut, bool TAN = false>
 __global__ void __launch_bounds__(64 * kWaves)
 k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
           const int32_t* __restrict__ batch_index, int n_atoms, int pbc_mode,
@@ -289,19 +301,24 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           const int32_t* __restrict__ nbr, const int32_t* __restrict__ n_nbr, int max_nbr,
           const int32_t* __restrict__ nbr_shift,
           const int32_t* __restrict__ nbr_ang, const int32_t* __restrict__ n_nbr_ang, int max_nbr_ang,
-          fnl_aev_params p, AevDims dm, TOut* __restrict__ aev, int stride, uint32_t smask) {
+          fnl_aev_params p, AevDims dm, TOut* __restrict__ aev, int stride, uint32_t smask,
+          const float* __restrict__ disp) {
   static_assert(NA * NZ == 32, "angular channel layout");
   extern __shared__ float4 lds4[];
   const int w = threadIdx.x >> 6;
   const int lane = fnl_lane();
   const int i = blockIdx.x * kWaves + w;
   if (i >= n_atoms) return;
-  const FwdLayout L = fwd_layout(max_nbr, max_nbr_ang, NA + NZ);
+  const FwdLayout L = fwd_layout(max_nbr, max_nbr_ang, NA + NZ, TAN);
   float* base = (float*)lds4 + (size_t)w * L.words;
   float4* nbv = (float4*)base;
+  float4* nbt = (float4*)(base + L.nbt);
   float2* rdf = (float2*)(base + L.rdf);
+  float4* rdf4 = (float4*)(base + L.rdf);
   float4* av = (float4*)(base + L.av);
+  float4* at = (float4*)(base + L.at);
   float2* af = (float2*)(base + L.af);
+  float2* ag = (float2*)(base + L.ag);
   float* fac = base + L.fac;
   int* tab = (int*)(base + L.tab);
 
@@ -320,6 +337,12 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   {
     const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
     const int32_t* row = nbr + (size_t)i * max_nbr;
+    float ri0 = 0.0f, ri1 = 0.0f, ri2 = 0.0f;
+    if constexpr (TAN) {
+      ri0 = disp[3 * i];
+      ri1 = disp[3 * i + 1];
+      ri2 = disp[3 * i + 2];
+    }
     for (int e = lane; e < nr; e += 64) {
       const int j = row[e];
       float vx = coords[3 * j] - xi, vy = coords[3 * j + 1] - yi, vz = coords[3 * j + 2] - zi;
@@ -328,6 +351,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       else if (pbc)
         fnl_min_image(vx, vy, vz, cl);
       nbv[e] = make_float4(vx, vy, vz, __int_as_float(species_idx[j]));
+      if constexpr (TAN) nbt[e] = make_float4(disp[3 * j] - ri0, disp[3 * j + 1] - ri1, disp[3 * j + 2] - ri2, 0.0f);
     }
   }
   // the angular row (radial slot numbers): entries < 64 in a register, fetched with the radial row
@@ -335,6 +359,11 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const int ae0 = lane < na ? arow[lane] : 0;
   auto ae_of = [&](int a) { return a < 64 ? ae0 : arow[a]; };  // a is the lane's own entry
   auto dist = [](const float4& v) { return hw_sqrt(fmaxf(v.x * v.x + v.y * v.y + v.z * v.z, 1.0e-5f)); };
+  // tangent of the distance along the slot displacement t (0 for coincident atoms)
+  auto dist_dot = [](const float4& v, const float4& t, float d) {
+    const float d2 = v.x * v.x + v.y * v.y + v.z * v.z;
+    return d2 > 1.0e-5f ? (v.x * t.x + v.y * t.y + v.z * t.z) * hw_rcp(d) : 0.0f;
+  };
   wave_sync();
 
   int cnt[FNL_MAX_SPECIES], off[FNL_MAX_SPECIES];
@@ -343,7 +372,11 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       nr, S, [&](int e) { return __float_as_int(nbv[e].w); },
       [&](int e, int pos, int) {
         const float d = dist(nbv[e]);
-        rdf[pos] = make_float2(d, 0.25f * cos_switch(d, p.cutoff));
+        if constexpr (TAN)
+          rdf4[pos] = make_float4(d, 0.25f * cos_switch(d, p.cutoff), 0.25f * cos_switch_grad(d, p.cutoff),
+                                  dist_dot(nbv[e], nbt[e], d));
+        else
+          rdf[pos] = make_float2(d, 0.25f * cos_switch(d, p.cutoff));
       },
       cnt, off, smask);
   wave_sync();
@@ -361,9 +394,15 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       float run = 0.0f;
 #pragma unroll 4
       for (int e = lo + q; e < hi; e += 4) {
-        const float2 df = rdf[e];
-        const float x = df.x - shift;
-        run += df.y * gauss(x, p.radial_eta);
+        if constexpr (TAN) {  // (0.25 fc' - 2 eta x 0.25 fc) exp(-eta x^2) dd
+          const float4 df = rdf4[e];
+          const float x = df.x - shift;
+          run += (df.z - 2.0f * p.radial_eta * x * df.y) * gauss(x, p.radial_eta) * df.w;
+        } else {
+          const float2 df = rdf[e];
+          const float x = df.x - shift;
+          run += df.y * gauss(x, p.radial_eta);
+        }
       }
       run += __shfl_xor(run, 16, 64);
       run += __shfl_xor(run, 32, 64);
@@ -376,11 +415,20 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     species_sort(
         na, S, [&](int a) { return __float_as_int(nbv[ae_of(a)].w); },
         [&](int a, int pos, int sa) {
-          const float4 v = nbv[ae_of(a)];
+          const int e = ae_of(a);
+          const float4 v = nbv[e];
           const float d = dist(v);
           const float inv = hw_rcp(fmaxf(d, 1.0e-5f));
-          av[pos] = make_float4(v.x * inv, v.y * inv, v.z * inv, d);
+          const float ux = v.x * inv, uy = v.y * inv, uz = v.z * inv;
+          av[pos] = make_float4(ux, uy, uz, d);
           af[pos] = make_float2(cos_switch(d, p.cutoff_ang), __int_as_float(sa));
+          if constexpr (TAN) {  // du = (t - u (u . t)) / d, dd
+            const float4 t = nbt[e];
+            const float ut = ux * t.x + uy * t.y + uz * t.z;
+            at[pos] = make_float4((t.x - ux * ut) * inv, (t.y - uy * ut) * inv, (t.z - uz * ut) * inv,
+                                  dist_dot(v, t, d));
+            ag[pos] = make_float2(cos_switch_grad(d, p.cutoff_ang), 0.0f);
+          }
         },
         cnt, off, smask);
   }
@@ -420,8 +468,11 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const float pref = 2.0f * exp2f(-p.zeta);  // 2 * 0.5^zeta (ani.py:112)
   const int h = lane >> 5, ch = lane & 31;
   const int dd = ch / NZ, zz = ch % NZ;
+  // table rows: forward R_d, sw F_z; tangent P_d, Q_d, F_z, F_z'
   const float* fR = fac + dd * kFacLd;
-  const float* fF = fac + (NA + zz) * kFacLd;
+  const float* fF = fac + ((TAN ? 2 * NA : NA) + zz) * kFacLd;
+  const float* fQ = fac + (NA + dd) * kFacLd;
+  const float* fG = fac + (2 * NA + NZ + zz) * kFacLd;
   TOut* aout = out + dm.rad_len;
   // empty blocks (species without radial neighbours, pairs without triplets) and the padding:
   // zeros in 16-byte stores.  Blocks of species outside smask (and, unless smask holds every
@@ -518,14 +569,43 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       const float4 ua = av[a], ub = av[b];
       const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
       const float x = 0.95f * c;
-      const float st = hw_sqrt(fmaxf(1.0f - x * x, 0.0f));
       const float sw = pref * af[a].x * af[b].x;
       const float m = 0.5f * (ua.w + ub.w);
+      if constexpr (TAN) {
+        const float4 ta = at[a], tb = at[b];
+        const float dc = ta.x * ub.x + ta.y * ub.y + ta.z * ub.z + ua.x * tb.x + ua.y * tb.y + ua.z * tb.z;
+        const float st = hw_sqrt(fmaxf(1.0f - x * x, 1.0e-12f));
+        const float ist = hw_rcp(st);
+        const float dsw = pref * (ag[a].x * af[b].x * ta.w + af[a].x * ag[b].x * tb.w);
+        const float dm_ = 0.5f * (ta.w + tb.w);
 #pragma unroll
-      for (int d = 0; d < NA; ++d) fac[d * kFacLd + lane] = gauss(m - p.ang_shifts[d], p.angular_eta);
+        for (int d = 0; d < NA; ++d) {
+          const float y = m - p.ang_shifts[d];
+          const float R = gauss(y, p.angular_eta);
+          fac[d * kFacLd + lane] = dsw * R + sw * (-2.0f * p.angular_eta * y * R) * dm_;  // P_d
+          fac[(NA + d) * kFacLd + lane] = sw * R * dc;                                       // Q_d
+        }
 #pragma unroll
-      for (int z = 0; z < NZ; ++z)
-        fac[(NA + z) * kFacLd + lane] = sw * ang_pow(1.0f + x * p.ang_cos[z] + st * p.ang_sin[z], p.zeta);
+        for (int z = 0; z < NZ; ++z) {
+          const float bz = 1.0f + x * p.ang_cos[z] + st * p.ang_sin[z];
+          float F = 0.0f, dF = 0.0f;
+          if (bz > 0.0f) {
+            const float lb = hw_log2(bz);
+            const float tz = p.zeta * lb;
+            F = hw_exp2(tz);
+            dF = (0.95f * p.zeta) * hw_exp2(tz - lb) * (p.ang_cos[z] - x * p.ang_sin[z] * ist);
+          }
+          fac[(2 * NA + z) * kFacLd + lane] = F;
+          fac[(2 * NA + NZ + z) * kFacLd + lane] = dF;
+        }
+      } else {
+        const float st = hw_sqrt(fmaxf(1.0f - x * x, 0.0f));
+#pragma unroll
+        for (int d = 0; d < NA; ++d) fac[d * kFacLd + lane] = gauss(m - p.ang_shifts[d], p.angular_eta);
+#pragma unroll
+        for (int z = 0; z < NZ; ++z)
+          fac[(NA + z) * kFacLd + lane] = sw * ang_pow(1.0f + x * p.ang_cos[z] + st * p.ang_sin[z], p.zeta);
+      }
     }
     wave_sync();
     for (uint64_t pm = present; pm;) {
@@ -537,7 +617,12 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       if (e0 <= u0) continue;
       const int hi = min(e0, c_end) - u0;
 #pragma unroll 4
-      for (int tt = max(s0, u0) - u0 + h; tt < hi; tt += 2) run += fR[tt] * fF[tt];
+      for (int tt = max(s0, u0) - u0 + h; tt < hi; tt += 2) {
+        if constexpr (TAN)
+          run += fR[tt] * fF[tt] + fQ[tt] * fG[tt];
+        else
+          run += fR[tt] * fF[tt];
+      }
       if (e0 <= c_end) {  // the pair's last triplet is in this chunk
         const float v = run + __shfl_xor(run, 32, 64);
         if (h == 0) aout[pr * 32 + ch] = to_out<TOut>(v);
@@ -841,168 +926,6 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   }
 }
 
-// Tangent (forward-mode JVP) of the AEV along a displacement field r:
-//   dA_i = sum_e dA_i/dvec_e . (r_j - r_i)
-// (the same edge vectors and terms as the forward; used for weight gradients
-// through forces, where F . r = -dE/dx . r = -sum_i dE/dA_i . dA_i).
-// Per triplet the tangent of K fc_a fc_b F_z(c) R_d(m) is
-//   F_z (dsw R_d + sw R_d' dm) + F_z' (sw R_d dc) = F_z P_d + F_z' Q_d.
-__host__ __device__ inline int tan_lds_words(int max_nbr, int max_nbr_ang, int aev_len) {
-  return round4(10 * max_nbr) + 12 * max_nbr_ang + round4(aev_len) + 64 * 32;
-}
-
-template <int NA, int NZ>
-__global__ void __launch_bounds__(64 * kWaves)
-k_aev_tangent(const float* __restrict__ coords, const float* __restrict__ disp,
-              const int32_t* __restrict__ species_idx, const int32_t* __restrict__ batch_index, int n_atoms,
-              int pbc_mode, const float* __restrict__ cells, const float* __restrict__ rec,
-              const int32_t* __restrict__ nbr, const int32_t* __restrict__ n_nbr, int max_nbr,
-          const int32_t* __restrict__ nbr_shift,
-              const int32_t* __restrict__ nbr_ang, const int32_t* __restrict__ n_nbr_ang, int max_nbr_ang,
-              fnl_aev_params p, AevDims dm, float* __restrict__ out_rows, int stride) {
-  static_assert(NA * NZ == 32, "angular channel layout");
-  extern __shared__ float4 lds4[];
-  const int w = threadIdx.x >> 6;
-  const int lane = fnl_lane();
-  const int i = blockIdx.x * kWaves + w;
-  if (i >= n_atoms) return;
-  const int words = tan_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
-  float* base = (float*)lds4 + (size_t)w * words;
-  float4* nbv = (float4*)base;                        // vx vy vz d
-  float4* nbt = nbv + max_nbr;                        // dvec (x y z), dd
-  float2* nbf = (float2*)(nbt + max_nbr);             // fc, species
-  float4* av = (float4*)(base + round4(10 * max_nbr));  // u, d
-  float4* at = av + max_nbr_ang;                      // du, dd
-  float4* af = at + max_nbr_ang;                      // fc_a, fc_a', species
-  float* acc = (float*)(af + max_nbr_ang);
-  float* trip = acc + ((dm.aev_len + 3) & ~3);
-
-  float* out = out_rows + (size_t)i * stride;
-  const int si = species_idx[i];
-  if (si < 0) {
-    for (int c = lane; c < stride; c += 64) out[c] = 0.0f;
-    return;
-  }
-  const bool pbc = pbc_mode == FNL_PBC_MINIMUM_IMAGE;
-  FnlCell cl;
-  if (pbc || nbr_shift) fnl_load_cell(cells, rec, batch_index[i], cl);
-  const int nr = min(n_nbr[i], max_nbr);
-  const int na = min(n_nbr_ang[i], max_nbr_ang);
-  for (int c = lane; c < dm.aev_len; c += 64) acc[c] = 0.0f;
-  {
-    const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
-    const float ri0 = disp[3 * i], ri1 = disp[3 * i + 1], ri2 = disp[3 * i + 2];
-    const int32_t* row = nbr + (size_t)i * max_nbr;
-    for (int e = lane; e < nr; e += 64) {
-      const int j = row[e];
-      float vx = coords[3 * j] - xi, vy = coords[3 * j + 1] - yi, vz = coords[3 * j + 2] - zi;
-      if (nbr_shift)  // general PBC: the stored image (GraphProcessor, preprocessing.py:727-735)
-        fnl_apply_shift(vx, vy, vz, cl, nbr_shift[(size_t)i * max_nbr + e]);
-      else if (pbc)
-        fnl_min_image(vx, vy, vz, cl);
-      const float d2 = vx * vx + vy * vy + vz * vz;
-      const float d = hw_sqrt(fmaxf(d2, 1.0e-5f));
-      const float tx = disp[3 * j] - ri0, ty = disp[3 * j + 1] - ri1, tz = disp[3 * j + 2] - ri2;
-      const float dd = d2 > 1.0e-5f ? (vx * tx + vy * ty + vz * tz) * hw_rcp(d) : 0.0f;
-      nbv[e] = make_float4(vx, vy, vz, d);
-      nbt[e] = make_float4(tx, ty, tz, dd);
-      nbf[e] = make_float2(cos_switch(d, p.cutoff), __int_as_float(species_idx[j]));
-    }
-  }
-  wave_sync();
-  // radial: d/dd (0.25 fc exp(-eta x^2)) * dd
-  {
-    const int q = lane >> 4, k = lane & 15;
-    if (k < dm.nR) {
-      const float shift = p.radial_shifts[k];
-      for (int e = q; e < nr; e += 4) {
-        const float4 v = nbv[e];
-        const float dd = nbt[e].w;
-        const float2 f = nbf[e];
-        const float x = v.w - shift;
-        const float ex = gauss(x, p.radial_eta);
-        const float dfc = cos_switch_grad(v.w, p.cutoff);
-        lds_add(acc + __float_as_int(f.y) * dm.nR + k, 0.25f * ex * (dfc - 2.0f * p.radial_eta * x * f.x) * dd);
-      }
-    }
-  }
-  // angular rows: u, du = (dvec - u (u . dvec)) / d, dd, fc, fc'
-  {
-    const int32_t* arow = nbr_ang + (size_t)i * max_nbr_ang;
-    for (int a = lane; a < na; a += 64) {
-      const int e = arow[a];
-      const float4 v = nbv[e];
-      const float4 t = nbt[e];
-      const float inv = hw_rcp(fmaxf(v.w, 1.0e-5f));
-      const float ux = v.x * inv, uy = v.y * inv, uz = v.z * inv;
-      const float ut = ux * t.x + uy * t.y + uz * t.z;
-      av[a] = make_float4(ux, uy, uz, v.w);
-      at[a] = make_float4((t.x - ux * ut) * inv, (t.y - uy * ut) * inv, (t.z - uz * ut) * inv, t.w);
-      af[a] = make_float4(cos_switch(v.w, p.cutoff_ang), cos_switch_grad(v.w, p.cutoff_ang), nbf[e].y, 0.0f);
-    }
-  }
-  wave_sync();
-  const float pref = 2.0f * exp2f(-p.zeta);
-  const int T = na * (na - 1) / 2;
-  float* aacc = acc + dm.rad_len;
-  constexpr int G = 64 / NZ;
-  const int g = lane / NZ, z = lane % NZ;
-  for (int base_t = 0; base_t < T; base_t += 64) {
-    const int t = base_t + lane;
-    float* tr = trip + lane * 32;
-    if (t < T) {
-      int a, b;
-      tri_pair(t, na, a, b);
-      const float4 ua = av[a], ub = av[b], ta = at[a], tb = at[b];
-      const float4 fa = af[a], fb = af[b];
-      const int sa = __float_as_int(fa.z), sb = __float_as_int(fb.z);
-      const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
-      const float dc = ta.x * ub.x + ta.y * ub.y + ta.z * ub.z + ua.x * tb.x + ua.y * tb.y + ua.z * tb.z;
-      const float x = 0.95f * c;
-      const float st = hw_sqrt(fmaxf(1.0f - x * x, 1.0e-12f));
-      const float ist = hw_rcp(st);
-      const float sw = pref * fa.x * fb.x;
-      const float dsw = pref * (fa.y * fb.x * ta.w + fa.x * fb.y * tb.w);
-      const float m = 0.5f * (ua.w + ub.w);
-      const float dm_ = 0.5f * (ta.w + tb.w);
-#pragma unroll
-      for (int d = 0; d < NA; ++d) {
-        const float y = m - p.ang_shifts[d];
-        const float R = gauss(y, p.angular_eta);
-        tr[d] = dsw * R + sw * (-2.0f * p.angular_eta * y * R) * dm_;  // P_d
-        tr[NA + d] = sw * R * dc;                                       // Q_d
-      }
-#pragma unroll
-      for (int zz = 0; zz < NZ; ++zz) {
-        const float bz = 1.0f + x * p.ang_cos[zz] + st * p.ang_sin[zz];
-        float F = 0.0f, dF = 0.0f;
-        if (bz > 0.0f) {
-          const float lb = hw_log2(bz);
-          const float tz = p.zeta * lb;
-          F = hw_exp2(tz);
-          dF = (0.95f * p.zeta) * hw_exp2(tz - lb) * (p.ang_cos[zz] - x * p.ang_sin[zz] * ist);
-        }
-        tr[2 * NA + zz] = F;
-        tr[2 * NA + NZ + zz] = dF;
-      }
-      tr[31] = __int_as_float(pair_index(sa, sb, dm.S) * 32);
-    } else {
-      tr[31] = __int_as_float(-1);
-    }
-    wave_sync();
-    const int nt = min(64, T - base_t);
-    for (int tt = g; tt < nt; tt += G) {
-      const float* q = trip + tt * 32;
-      const int po = __float_as_int(q[31]);
-      const float F = q[2 * NA + z], dF = q[2 * NA + NZ + z];
-#pragma unroll
-      for (int d = 0; d < NA; ++d) lds_add(aacc + po + d * NZ + z, F * q[d] + dF * q[NA + d]);
-    }
-    wave_sync();
-  }
-  for (int c = lane; c < stride; c += 64) out[c] = c < dm.aev_len ? acc[c] : 0.0f;
-}
-
 bool check_params(const fnl_aev_params* p, AevDims* dm) {
   if (!p) return false;
   if (p->n_species < 1 || p->n_species > FNL_MAX_SPECIES) return false;
@@ -1034,11 +957,11 @@ int launch_fwd(const float* coords, const int32_t* species_idx, const int32_t* b
   if (out_dtype == FNL_F32) {
     hipLaunchKernelGGL((k_aev_fwd<NA, NZ, float>), grid, block, lds, s, coords, species_idx, batch_index,
                        n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang,
-                       p, dm, (float*)aev, stride, smask);
+                       p, dm, (float*)aev, stride, smask, nullptr);
   } else {
     hipLaunchKernelGGL((k_aev_fwd<NA, NZ, hip_bfloat16>), grid, block, lds, s, coords, species_idx,
                        batch_index, n_atoms, pbc_mode, cells, rec, nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang,
-                       max_nbr_ang, p, dm, (hip_bfloat16*)aev, stride, smask);
+                       max_nbr_ang, p, dm, (hip_bfloat16*)aev, stride, smask, nullptr);
   }
   FNL_LAUNCH_CHECK();
   return FNL_OK;
@@ -1157,24 +1080,25 @@ extern "C" int fnl_aev_tangent(const float* coords, const float* disp, const int
   if (daev_stride < dm.aev_len) return FNL_ERR_INVALID;
   if (n_atoms == 0) return FNL_OK;
   if (!disp || !daev) return FNL_ERR_INVALID;
-  const size_t lds = (size_t)kWaves * 4 * tan_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
+  const int na = params->n_ang_dist, nz = params->n_ang_sect;
+  const size_t lds = (size_t)kWaves * 4 * fwd_layout(max_nbr, max_nbr_ang, na + nz, true).words;
   if (lds > 160 * 1024) return FNL_ERR_UNSUPPORTED;
   hipStream_t s = (hipStream_t)stream;
-  const int na = params->n_ang_dist, nz = params->n_ang_sect;
   const void* kf = nullptr;
-  if (na == 4 && nz == 8) kf = reinterpret_cast<const void*>(&k_aev_tangent<4, 8>);
-  else if (na == 8 && nz == 4) kf = reinterpret_cast<const void*>(&k_aev_tangent<8, 4>);
+  if (na == 4 && nz == 8) kf = reinterpret_cast<const void*>(&k_aev_fwd<4, 8, float, true>);
+  else if (na == 8 && nz == 4) kf = reinterpret_cast<const void*>(&k_aev_fwd<8, 4, float, true>);
   else return FNL_ERR_UNSUPPORTED;
   if (const int rc = fnl_allow_full_lds(kf)) return rc;
   dim3 grid(fnl_blocks(n_atoms, kWaves)), block(64 * kWaves);
+  // every channel block written (zeros where no neighbour or pair contributes)
   if (na == 4)
-    hipLaunchKernelGGL((k_aev_tangent<4, 8>), grid, block, lds, s, coords, disp, species_idx, batch_index, n_atoms,
-                       pbc_mode, cells, reciprocal_cells, nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang,
-                       *params, dm, daev, daev_stride);
+    hipLaunchKernelGGL((k_aev_fwd<4, 8, float, true>), grid, block, lds, s, coords, species_idx, batch_index,
+                       n_atoms, pbc_mode, cells, reciprocal_cells, nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang,
+                       max_nbr_ang, *params, dm, daev, daev_stride, 0xFFFFFFFFu, disp);
   else
-    hipLaunchKernelGGL((k_aev_tangent<8, 4>), grid, block, lds, s, coords, disp, species_idx, batch_index, n_atoms,
-                       pbc_mode, cells, reciprocal_cells, nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang,
-                       *params, dm, daev, daev_stride);
+    hipLaunchKernelGGL((k_aev_fwd<8, 4, float, true>), grid, block, lds, s, coords, species_idx, batch_index,
+                       n_atoms, pbc_mode, cells, reciprocal_cells, nbr, n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang,
+                       max_nbr_ang, *params, dm, daev, daev_stride, 0xFFFFFFFFu, disp);
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }

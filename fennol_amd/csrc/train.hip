@@ -415,26 +415,118 @@ __global__ void __launch_bounds__(256) k_train_reduce(const TrainArgs a, const T
   const int c0 = a.chunks[net], c1 = a.chunks[net + 1];
   const int t0 = a.tiles[net], t1 = a.tiles[net + 1];
   float s = 0.0f;
+  // chunk partials: eight independent loads in flight, summed in chunk order
+  auto sum_chunks = [&](const float* part, size_t cstride, size_t off) {
+    float acc = 0.0f;
+    int c = c0;
+    for (; c + 8 <= c1; c += 8) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = part[(size_t)(c + u) * cstride + off];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; c < c1; ++c) acc += part[(size_t)c * cstride + off];
+    return acc;
+  };
+  (void)t0;
+  (void)t1;
   if (e < s1) {
     const int i = (int)(e / H1);
-    if (chunk_live(a.k_live, i & ~63))
-      for (int c = c0; c < c1; ++c) s += bf.part1[(size_t)c * s1 + e];
-  } else if (e < s2) {
-    for (int t = t0; t < t1; ++t) s += bf.tile_part[(size_t)t * TP_STRIDE + TP_GB1 + (e - s1)];
-  } else if (e < s3) {
-    for (int c = c0; c < c1; ++c) s += bf.part2[(size_t)c * H1 * H2 + (e - s2)];
-  } else if (e < s4) {
-    for (int t = t0; t < t1; ++t) s += bf.tile_part[(size_t)t * TP_STRIDE + TP_GB2 + (e - s3)];
-  } else if (e < s5) {
-    for (int c = c0; c < c1; ++c) s += bf.part3[(size_t)c * H2 * H3 + (e - s4)];
-  } else if (e < s6) {
-    for (int t = t0; t < t1; ++t) s += bf.tile_part[(size_t)t * TP_STRIDE + TP_GB3 + (e - s5)];
-  } else if (e < s7) {
-    for (int t = t0; t < t1; ++t) s += bf.tile_part[(size_t)t * TP_STRIDE + TP_GW4 + (e - s6)];
+    if (chunk_live(a.k_live, i & ~63)) s = sum_chunks(bf.part1, s1, e);
+  } else if (e >= s2 && e < s3) {
+    s = sum_chunks(bf.part2, (size_t)H1 * H2, e - s2);
+  } else if (e >= s4 && e < s5) {
+    s = sum_chunks(bf.part3, (size_t)H2 * H3, e - s4);
   } else {
-    for (int t = t0; t < t1; ++t) s += bf.tile_part[(size_t)t * TP_STRIDE + TP_GB4];
+    return;  // biases and w4: k_train_reduce_tiles
   }
   grad[(size_t)net * gstride + e] = s;
+}
+
+// Biases and w4 of each network: one workgroup per value sums the network's tile partials
+// (thread t: tiles t, t+256, ..., then a fixed tree).
+__global__ void __launch_bounds__(256) k_train_reduce_tiles(const TrainArgs a, const TrainBufs bf,
+                                                            float* __restrict__ grad, size_t gstride) {
+  __shared__ float red[256];
+  const int v = blockIdx.x, net = blockIdx.y, t = threadIdx.x;
+  const int t0 = a.tiles[net], t1 = a.tiles[net + 1];
+  float s = 0.0f;
+  for (int k = t0 + t; k < t1; k += 256) s += bf.tile_part[(size_t)k * TP_STRIDE + v];
+  red[t] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) red[t] += red[t + o];
+    __syncthreads();
+  }
+  if (t != 0) return;
+  const size_t s1 = (size_t)a.k_in * H1, s3 = s1 + H1 + (size_t)H1 * H2, s5 = s3 + H2 + (size_t)H2 * H3;
+  size_t e;
+  if (v < TP_GB2) e = s1 + (v - TP_GB1);
+  else if (v < TP_GB3) e = s3 + (v - TP_GB2);
+  else if (v < TP_GW4) e = s5 + (v - TP_GB3);
+  else if (v < TP_GB4) e = s5 + H3 + (v - TP_GW4);
+  else e = s5 + 2 * H3;
+  grad[(size_t)net * gstride + e] = red[0];
+}
+
+// Residuals of the loss, per atom: forces, masked force residuals, energy adjoints and the
+// workgroup's FP64 sum of squared force residuals (fixed tree).
+__global__ void __launch_bounds__(256)
+k_train_residuals(const double* __restrict__ energy, const double* __restrict__ e_ref,
+                  const float* __restrict__ grad_self, const long long* __restrict__ grad_nbr, float force_scale,
+                  const float* __restrict__ f_ref, const int32_t* __restrict__ species_idx,
+                  const int32_t* __restrict__ batch_index, int n_atoms, double ce_scale, float* __restrict__ forces,
+                  float* __restrict__ r_forces, float* __restrict__ c_energy, double* __restrict__ partials) {
+  __shared__ double red[256];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double sq = 0.0;
+  if (i < n_atoms) {
+    const bool real = species_idx[i] >= 0;
+    const int b = batch_index[i];
+    c_energy[i] = (float)(ce_scale * (energy[b] - e_ref[b]));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float f = (float)(-((double)grad_self[3 * i + k] + fnl_from_fixed(grad_nbr[3 * i + k])) * (double)force_scale);
+      const float r = real ? f - f_ref[3 * i + k] : 0.0f;
+      forces[3 * i + k] = f;
+      r_forces[3 * i + k] = r;
+      sq += (double)r * (double)r;
+    }
+  }
+  red[threadIdx.x] = sq;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// loss = w_e sum_b rE_b^2 / n_e + w_f sum rF^2 / n_f (fixed order)
+__global__ void __launch_bounds__(256)
+k_train_loss(const double* __restrict__ energy, const double* __restrict__ e_ref, int n_sys,
+             const double* __restrict__ partials, int n_part, double w_e, double w_f, double n_e, double n_f,
+             double* __restrict__ loss) {
+  __shared__ double re[256], rf[256];
+  const int t = threadIdx.x;
+  double se = 0.0, sf = 0.0;
+  for (int b = t; b < n_sys; b += 256) {
+    const double r = energy[b] - e_ref[b];
+    se += r * r;
+  }
+  for (int k = t; k < n_part; k += 256) sf += partials[k];
+  re[t] = se;
+  rf[t] = sf;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) {
+      re[t] += re[t + o];
+      rf[t] += rf[t + o];
+    }
+    __syncthreads();
+  }
+  if (t == 0) loss[0] = w_e * re[0] / n_e + w_f * rf[0] / n_f;
 }
 
 struct WsPlan {
@@ -533,6 +625,29 @@ extern "C" int fnl_train_mlp_grad(const float* aev, int32_t aev_stride, const fl
   }
   hipLaunchKernelGGL(k_train_reduce, dim3((unsigned)((gstride + 255) / 256), n_nets), dim3(256), 0, st, a, b, grad,
                      gstride);
+  hipLaunchKernelGGL(k_train_reduce_tiles, dim3(TP_GB4 + 1, n_nets), dim3(256), 0, st, a, b, grad, gstride);
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
+}
+
+extern "C" int fnl_train_residuals(const double* energy, const double* e_ref, int32_t n_sys, const float* grad_self,
+                                   const int64_t* grad_nbr, float force_scale, const float* f_ref,
+                                   const int32_t* species_idx, const int32_t* batch_index, int32_t n_atoms,
+                                   double w_e, double w_f, double n_e, double n_f, float* forces, float* r_forces,
+                                   float* c_energy, double* loss, double* partials, fnl_stream_t stream) {
+  if (n_sys < 1 || n_atoms < 0 || !(n_e > 0.0) || !(n_f > 0.0)) return FNL_ERR_INVALID;
+  if (!energy || !e_ref || !loss || !partials) return FNL_ERR_INVALID;
+  if (n_atoms > 0 && (!grad_self || !grad_nbr || !f_ref || !species_idx || !batch_index || !forces || !r_forces ||
+                      !c_energy))
+    return FNL_ERR_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = fnl_blocks(n_atoms, 256);
+  if (nb > 0)
+    hipLaunchKernelGGL(k_train_residuals, dim3(nb), dim3(256), 0, st, energy, e_ref, grad_self,
+                       (const long long*)grad_nbr, force_scale, f_ref, species_idx, batch_index, n_atoms,
+                       2.0 * w_e / n_e, forces, r_forces, c_energy, partials);
+  hipLaunchKernelGGL(k_train_loss, dim3(1), dim3(256), 0, st, energy, e_ref, n_sys, partials, nb, w_e, w_f, n_e, n_f,
+                     loss);
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }

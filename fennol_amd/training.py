@@ -81,19 +81,28 @@ class TrainingStep:
         pot = self.pot
         pot.compute(sb, x32, forces=True, rebuild=True, check=True)
         N = sb.n_atoms
-        real = torch.as_tensor(sb.species > 0, device=pot.device)
+        dev = pot.device
         E = sb.energy[: sb.n_sys]
-        F = pot.forces(sb)
         n_e = float(sb.n_sys)
-        n_f = 3.0 * float(real.sum().item())
-        rE = E - e_ref.to(torch.float64)
-        rF = (F - f_ref) * real[:, None]
-        loss = self.w_e * float((rE ** 2).sum()) / n_e + self.w_f * float((rF.double() ** 2).sum()) / n_f
-        dA = self.tangent(sb, x32, rF.contiguous())
-        cE_atom = ((2.0 * self.w_e / n_e) * rE)[sb.batch_index[:N].long()].float().contiguous()
+        n_f = 3.0 * float(np.count_nonzero(np.asarray(sb.species[:N]) > 0))
+        F = torch.empty((N, 3), dtype=torch.float32, device=dev)
+        rF = torch.empty((N, 3), dtype=torch.float32, device=dev)
+        cE_atom = torch.empty(N, dtype=torch.float32, device=dev)
+        loss = torch.empty(1, dtype=torch.float64, device=dev)
+        part = torch.empty(max(1, (N + 255) // 256), dtype=torch.float64, device=dev)
+        P = _abi.ptr
+        e_ref64 = e_ref.to(torch.float64).contiguous()
+        f_ref32 = f_ref.to(torch.float32).contiguous()
+        # forces, residuals, energy adjoints and the loss in two launches; the loss stays on the
+        # device until every launch of the step is queued (one wait)
+        _abi.check(pot.lib.fnl_train_residuals(
+            P(E), P(e_ref64), sb.n_sys, P(sb.grad_self), P(sb.grad_nbr), 1.0, P(f_ref32), P(sb.species_idx),
+            P(sb.batch_index), N, self.w_e, self.w_f, n_e, n_f, P(F), P(rF), P(cE_atom), P(loss), P(part),
+            _abi.stream_handle()), "fnl_train_residuals")
+        dA = self.tangent(sb, x32, rF)
         cD = -2.0 * self.w_f / n_f
         grads = self.mlp_grad(sb, dA, cE_atom, cD)
-        return loss, grads, E, F
+        return float(loss), grads, E, F
 
     def mlp_grad(self, sb: SystemBuffers, dA: torch.Tensor, c_energy: torch.Tensor, c_force: float) -> dict:
         """dL/dW of every network from the AEV rows of ``sb``, the tangent rows dA [N, k_pad],

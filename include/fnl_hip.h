@@ -476,6 +476,20 @@ int fnl_train_mlp_grad(const float* aev, int32_t aev_stride, const float* daev, 
                        float c_force, float celu_alpha, uint64_t k_live, void* workspace,
                        size_t workspace_bytes, float* grad, fnl_stream_t stream);
 
+/* Residuals of the energy + force loss (the mse terms of training/utils.py:
+ * 329-343 with weights :496-501):
+ *   F_i = -(grad_self + grad_nbr 2^-32) * force_scale       (float32, as fnl_forces)
+ *   rF_i = (F_i - F_ref,i) for real atoms (species index >= 0), else 0
+ *   c_energy_i = 2 w_e / n_e * (E_b - E_ref,b), b the system of atom i
+ *   loss = w_e sum_b (E_b - E_ref,b)^2 / n_e + w_f sum rF^2 / n_f   (float64)
+ * Every sum runs in a fixed order.  partials: float64[ceil(n_atoms/256)]. */
+int fnl_train_residuals(const double* energy, const double* e_ref, int32_t n_sys,
+                        const float* grad_self, const int64_t* grad_nbr, float force_scale,
+                        const float* f_ref, const int32_t* species_idx, const int32_t* batch_index,
+                        int32_t n_atoms, double w_e, double w_f, double n_e, double n_f,
+                        float* forces, float* r_forces, float* c_energy, double* loss,
+                        double* partials, fnl_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * Serving pool: continuous batching of independent open-boundary systems
  * (energy + forces requests and short Langevin runs) in one set of
