@@ -63,25 +63,52 @@ def model_params(cfg):
     return init_species_params(MODEL_SEED, cfg.aev_length)
 
 
+# Issue model of the AEV kernels: the VALU lane-operations one triplet and one radial slot need
+# with every lane busy and no bookkeeping (per triplet: cosine, sine, switch product and mean
+# distance, NA Gaussians, NZ powers, and the 32 channel products -- twice that with the
+# derivatives and the dE/dAEV contraction in the adjoint; per radial slot: distance, switch and
+# 16 shifted Gaussians, for the centre's and the neighbour's row in the adjoint)
+VALU_MODEL = {"k_aev_fwd": (100, 90), "k_aev_bwd": (205, 180)}
+
+
+def live_channels(cfg, species_mask: int) -> tuple[int, int]:
+    """(radial, angular) AEV channels written for a batch whose species are the bits of the mask."""
+    S = sum(1 for s in range(cfg.n_species) if (species_mask >> s) & 1)
+    return S * cfg.radial_dist_divisions, S * (S + 1) // 2 * cfg.angular_dist_divisions * cfg.angle_sections
+
+
 def algorithmic_units(sb, cfg, precision: str) -> dict:
-    """Per-launch algorithmic bytes / flops of each stage, from the measured neighbour counts."""
+    """Per-launch algorithmic bytes / flops of each stage, from the measured neighbour counts and
+    the channel blocks the batch's species use (the other blocks are never read or written)."""
     nr = sb.n_nbr.cpu().numpy().astype(np.float64)
     na = sb.n_nbr_ang.cpu().numpy().astype(np.float64)
     N = sb.n_atoms
-    A = cfg.aev_length
+    rad, ang = live_channels(cfg, sb.extra.get("aev_mask", (1 << cfg.n_species) - 1))
+    L = rad + ang
     out_b = 2 if precision == "bf16" else 4
-    per_atom_mlp = 2.0 * (A * 256 + 256 * 192 + 192 * 160 + 160)   # forward
-    per_atom_mlp += 2.0 * (160 * 192 + 192 * 256 + 256 * A)         # input gradient
+    H = 256 * 192 + 192 * 160
+    per_atom_mlp = 2.0 * (L * 256 + H + 160) + 2.0 * (H + 256 * L)  # forward and input gradient
+    # executed on the MFMA: the 128 (BF16) / 64 (FP32) column chunks holding a live column, hi and lo
+    kc = 128 if precision == "bf16" else 64
+    live = cfg.live_column_blocks(sb.extra.get("aev_mask", 0xFFFFFFFF))
+    k_exec = cfg.k_pad if not live else kc * len({q // (kc // 64) for q in range(64) if (live >> q) & 1})
+    comp = 2.0 if precision == "bf16" else 1.0
+    exec_mlp = comp * (2.0 * (k_exec * 256 + H + 160) + 2.0 * (H + 256 * k_exec))
     return {
         # xyz + species in, neighbour rows (radial + angular) and counts out
         "nblist": ("hbm", float(N * (12 + 4 + 8) + 4 * nr.sum() + 4 * na.sum())),
-        # xyz, rows in; AEV row out
-        "aev_fwd": ("hbm", float(N * (12 + 8) + 4 * nr.sum() + 4 * na.sum() + N * A * out_b)),
+        # xyz, species, rows in; the live channel blocks of the AEV row out
+        "aev_fwd": ("hbm", float(N * (12 + 4 + 8) + 4 * nr.sum() + 4 * na.sum() + N * L * out_b)),
         "mlp": ("mfma", float(N * per_atom_mlp)),
-        # dE/dAEV in (fp32, or bf16 in the BF16 form), rows in; forces (self + neighbour) and virial out
-        "aev_bwd": ("hbm", float(N * (12 + 8) + 4 * nr.sum() + 4 * na.sum() + N * A * out_b + N * (24 + 36))),
+        # xyz, species, rows and the live dE/dAEV blocks in (plus each atom's radial blocks once,
+        # which its neighbours gather); forces (self + int64 neighbour sums) and virial out
+        "aev_bwd": ("hbm", float(N * (12 + 4 + 8) + 4 * nr.sum() + 4 * na.sum() + N * (L + rad) * out_b
+                                 + N * (12 + 24 + 36))),
         "reduce": ("hbm", float(N * (4 + 36))),
         "baoab": ("hbm", float(N * (48 + 24 + 8 + 8 + 48 + 12))),
+        "_mlp_executed": float(N * exec_mlp),
+        "_counts": {"n_r": float(nr[:N].mean()), "n_a": float(na[:N].mean()),
+                    "triplets": float((na[:N] * (na[:N] - 1) / 2).mean()), "live_channels": L},
     }
 
 
@@ -588,7 +615,18 @@ def main() -> int:
                 "launch_us": round(1e6 * t_s, 2), "algorithmic_per_launch": work}
     issue = measured_issue(KERNEL_OF_STAGE[dom], args.workload, "bf16")
     if issue is not None:  # what actually limits the kernel: its instruction issue (counters)
-        roofline["issue"] = issue
+        roofline["issue"] = dict(issue)
+        kern = KERNEL_OF_STAGE[dom]
+        if kern in VALU_MODEL and "valu_insts" in issue:
+            c = bf["units"]["_counts"]
+            per_t, per_slot = VALU_MODEL[kern]
+            model = (per_t * c["triplets"] + per_slot * c["n_r"]) / 64.0
+            measured = issue["valu_insts"] / N
+            roofline["issue"].update({
+                "valu_insts_per_centre": round(measured, 1), "model_insts_per_centre": round(model, 1),
+                "model_fraction": round(model / measured, 3),
+                "model": f"{per_t} lane-ops per triplet ({c['triplets']:.1f} per centre) + {per_slot} per radial "
+                         f"slot ({c['n_r']:.1f}), 64 lanes per instruction"})
     stage_table = {}
     for prec, r in results.items():
         stage_table[prec] = {}
@@ -600,6 +638,8 @@ def main() -> int:
                     ent["GB/s"] = round(w / (v.mean() / 1000.0) / 1e9, 1)
                 else:
                     ent["TFLOP/s"] = round(w / (v.mean() / 1000.0) / 1e12, 2)
+                    if k == "mlp":  # the MFMA work actually run (live chunks; hi and lo in BF16)
+                        ent["executed_TFLOP/s"] = round(r["units"]["_mlp_executed"] / (v.mean() / 1000.0) / 1e12, 2)
             stage_table[prec][k] = ent
 
     acc = {}
