@@ -57,7 +57,7 @@ k_reduce_segments(const int32_t* __restrict__ sys_offsets, const int32_t* __rest
                   const double* __restrict__ vel, const double* __restrict__ mass, double* __restrict__ partials,
                   unsigned* __restrict__ done, double* __restrict__ energy_sys, double* __restrict__ virial_sys,
                   double* __restrict__ kinetic_sys, int fuse_final) {
-  __shared__ double red[kRedVals][256];
+  __shared__ double wsum[4][kRedVals];
   const int sgi = blockIdx.x;
   const int t = threadIdx.x;
   // system of this segment: largest b with seg_offsets[b] <= sgi
@@ -86,27 +86,33 @@ k_reduce_segments(const int32_t* __restrict__ sys_offsets, const int32_t* __rest
       acc[16] += m * vz * vx; acc[17] += m * vz * vy; acc[18] += m * vz * vz;
     }
   }
+  // wave sums by a fixed xor butterfly, then the four wave sums in order (the same tree every
+  // launch: bitwise reproducible)
+  const bool need_v = virial_atom != nullptr, need_k = vel != nullptr;
 #pragma unroll
-  for (int k = 0; k < kRedVals; ++k) red[k][t] = acc[k];
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) {
+  for (int k = 0; k < kRedVals; ++k) {
+    if ((k >= 1 && k < 10 && !need_v) || (k >= 10 && !need_k)) continue;
 #pragma unroll
-      for (int k = 0; k < kRedVals; ++k) red[k][t] += red[k][t + s];
-    }
-    __syncthreads();
+    for (int o = 32; o >= 1; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
   }
+  if ((t & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < kRedVals; ++k) wsum[t >> 6][k] = acc[k];
+  }
+  __syncthreads();
+  double seg = 0.0;
+  if (t < kRedVals) seg = ((wsum[0][t] + wsum[1][t]) + wsum[2][t]) + wsum[3][t];
   // Each partial is written by a device-scope exchange whose result the thread waits for, so it
   // sits at the device's coherence point before the completion count is taken; the last
   // workgroup reads the partials there (agent-scope atomic loads), without cache-wide fences.
   if (!fuse_final) {  // k_reduce_systems forms the per-system sums in the next launch
-    if (t < kRedVals) partials[(size_t)sgi * kRedVals + t] = red[t][0];
+    if (t < kRedVals) partials[(size_t)sgi * kRedVals + t] = seg;
     return;
   }
   if (t < kRedVals) {
     const unsigned long long old = atomicExch(
         reinterpret_cast<unsigned long long*>(partials + (size_t)sgi * kRedVals + t),
-        (unsigned long long)__double_as_longlong(red[t][0]));
+        (unsigned long long)__double_as_longlong(seg));
     __asm__ volatile("" : : "v"(old));  // keep the returning form (wait for completion)
   }
   __shared__ bool last;
