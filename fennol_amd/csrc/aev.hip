@@ -104,9 +104,13 @@ __device__ __forceinline__ hip_bfloat16 to_out<hip_bfloat16>(float v) {
 __device__ __forceinline__ float g2f(float v) { return v; }
 __device__ __forceinline__ float g2f(unsigned short v) { return __uint_as_float((uint32_t)v << 16); }
 
+constexpr int kMaxGroups = 192;  // 8-channel groups of the longest AEV (8 species: 176)
+
 struct AevDims {
   int S, P, nR, rad_len, aev_len;
   uint64_t live;  // adjoint: channel blocks (radial s at bit s, angular pair p at bit S+p) read from g
+  int n_groups;   // adjoint: the live blocks as 8-channel groups (first channel / 8), or 0
+  uint16_t groups[kMaxGroups];
 };
 
 // blocks of the species in species_mask and of the pairs of two such species (all blocks when
@@ -127,6 +131,22 @@ inline uint64_t live_blocks(uint32_t species_mask, const AevDims& dm) {
 // neighbour pair of their species exists in the batch)
 template <typename GT>
 __device__ __forceinline__ void load_g_row(float* G, const GT* __restrict__ row, const AevDims& dm, int lane) {
+  if (dm.n_groups > 0) {  // one 16-byte (BF16) or two (FP32) loads per live 8-channel group
+    for (int q = lane; q < dm.n_groups; q += 64) {
+      const int c = 8 * (int)dm.groups[q];
+      if constexpr (sizeof(GT) == 4) {
+        ((float4*)(G + c))[0] = ((const float4*)(row + c))[0];
+        ((float4*)(G + c))[1] = ((const float4*)(row + c))[1];
+      } else {
+        const uint4 u = *(const uint4*)(row + c);
+        ((float4*)(G + c))[0] = make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
+                                            __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
+        ((float4*)(G + c))[1] = make_float4(__uint_as_float(u.z << 16), __uint_as_float(u.z & 0xffff0000u),
+                                            __uint_as_float(u.w << 16), __uint_as_float(u.w & 0xffff0000u));
+      }
+    }
+    return;
+  }
   const int n = dm.aev_len;
   auto live = [&](int c) {
     const int ch = 4 * c;
@@ -939,6 +959,7 @@ bool check_params(const fnl_aev_params* p, AevDims* dm) {
   dm->rad_len = dm->S * dm->nR;
   dm->aev_len = dm->rad_len + dm->P * p->n_ang_dist * p->n_ang_sect;
   dm->live = ~0ull;
+  dm->n_groups = 0;
   return true;
 }
 
@@ -1040,6 +1061,20 @@ extern "C" int fnl_aev_backward(const float* coords, const int32_t* species_idx,
   AevDims dm;
   if (!check_params(params, &dm)) return FNL_ERR_INVALID;
   dm.live = live_blocks(species_mask, dm);
+  // the live blocks as 8-channel groups (blocks are whole groups when nR % 8 == 0)
+  if (dm.nR % 8 == 0 && g_stride % 8 == 0) {
+    int ng = 0;
+    for (int blk = 0; blk < dm.S + dm.P && ng >= 0; ++blk) {
+      if (!((dm.live >> blk) & 1ull)) continue;
+      const int c0 = blk < dm.S ? blk * dm.nR : dm.rad_len + (blk - dm.S) * 32;
+      const int c1 = blk < dm.S ? c0 + dm.nR : c0 + 32;
+      for (int c = c0; c < c1; c += 8) {
+        if (ng >= kMaxGroups) { ng = -1; break; }
+        dm.groups[ng++] = (uint16_t)(c / 8);
+      }
+    }
+    dm.n_groups = ng > 0 ? ng : 0;
+  }
   if (!common_args_ok(coords, species_idx, batch_index, n_atoms, pbc_mode, cells, reciprocal_cells, nbr,
                       n_nbr, max_nbr, nbr_shift, nbr_ang, n_nbr_ang, max_nbr_ang))
     return FNL_ERR_INVALID;
