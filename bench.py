@@ -91,7 +91,8 @@ def algorithmic_units(sb, cfg, precision: str) -> dict:
     # executed on the MFMA: the 128 (BF16) / 64 (FP32) column chunks holding a live column, hi and lo
     kc = 128 if precision == "bf16" else 64
     live = cfg.live_column_blocks(sb.extra.get("aev_mask", 0xFFFFFFFF))
-    k_exec = cfg.k_pad if not live else kc * len({q // (kc // 64) for q in range(64) if (live >> q) & 1})
+    pieces = sum(1 for q in range(min(64, cfg.k_pad // 32)) if (live >> q) & 1) + max(0, cfg.k_pad // 32 - 64)
+    k_exec = cfg.k_pad if not live else kc * -(-pieces // (kc // 32))
     comp = 2.0 if precision == "bf16" else 1.0
     exec_mlp = comp * (2.0 * (k_exec * 256 + H + 160) + 2.0 * (H + 256 * k_exec))
     return {

@@ -51,7 +51,7 @@ struct MlpArgs {
   NetPtrs nets[FNL_MLP_MAX_NETS];
   int k_pad;
   float alpha;
-  unsigned long long k_live;  // bit q: input columns [64q, 64q + 64) may be non-zero in the batch
+  unsigned long long k_live;  // bit q: input columns [32q, 32q + 32) may be non-zero in the batch
   const int32_t* n_tiles_dev;  // tiles in use, read on the device (the grid covers n_tiles); or null
 };
 
@@ -312,13 +312,39 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     const int row = p * RPP + t / TPR;
     srow[p] = row < nrows ? perm[r0 + row] : -1;
   }
+  // Input pieces of 32 columns that hold a non-zero column for some atom of the batch (k_live,
+  // bit q: columns [32q, 32q + 32); columns from 2048 on are always live): only those are
+  // multiplied (the others contribute exact zeros).  live_piece[v] is the v-th of them, padded
+  // to whole chunks with a dead piece (its columns are zero in every row); chunk c stages pieces
+  // c*PPC .. c*PPC + PPC - 1, and k-step s of the loop is column
+  // live_piece[s / SPP] * PW + (s % SPP) * KS.
+  constexpr int PW = 32;            // piece width (columns)
+  constexpr int PPC = KC / PW;      // pieces per chunk
+  constexpr int SPP = PW / tr::KS;  // k-steps per piece
+  static_assert(KC % PW == 0 && PW % tr::KS == 0 && PW % VEC == 0, "pieces tile the chunks");
+  __shared__ int live_piece[FNL_MLP_MAX_CHUNKS * 4];
+  __shared__ int n_chunks_sh;
+  if (t == 0) {
+    int n = 0, dead = -1;
+    for (int q = 0; q < K / PW; ++q) {
+      if (q >= 64 || ((args.k_live >> q) & 1ull)) live_piece[n++] = q;
+      else if (dead < 0) dead = q;
+    }
+    if (n == 0) live_piece[n++] = dead;  // no live column: one zero piece, the bias-only result
+    while (n % PPC) live_piece[n++] = dead;  // a dead piece exists unless every piece is live
+    n_chunks_sh = n / PPC;
+  }
+  __syncthreads();
+  const int nlive = n_chunks_sh;
   // two register sets: the chunk two ahead is in flight while the next one waits for its LDS buffer
   u4v xa[NPASS], xb[NPASS];
-  auto stage_load = [&](u4v (&xr)[NPASS], int kc) {
+  // chunk c of the live pieces: this thread's 16 bytes lie in piece c * PPC + scol / 32
+  auto stage_load = [&](u4v (&xr)[NPASS], int c) {
+    const int col = live_piece[c * (KC / 32) + scol / 32] * 32 + scol % 32;
 #pragma unroll
     for (int p = 0; p < NPASS; ++p) {
       const int rr = srow[p] >= 0 ? srow[p] : row_safe;  // a real row of the tile: the load is always issued
-      const u4v v = __builtin_nontemporal_load((const u4v*)(aev + (size_t)rr * aev_stride + kc + scol));
+      const u4v v = __builtin_nontemporal_load((const u4v*)(aev + (size_t)rr * aev_stride + col));
       xr[p] = srow[p] >= 0 ? v : u4v{0u, 0u, 0u, 0u};
     }
   };
@@ -327,23 +353,6 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     for (int p = 0; p < NPASS; ++p) *(u4v*)(buf + (p * RPP + t / TPR) * L::LDX + scol) = xr[p];
   };
 
-  // Input chunks of KC columns that hold a non-zero column for some atom of the batch (k_live):
-  // only those are multiplied (the others contribute exact zeros).  live_chunk[v] is the v-th of
-  // them; the loop below runs over v, and a k-step s of that loop is column block
-  // live_chunk[s / SPC] * KC + (s % SPC) * KS.
-  __shared__ int live_chunk[FNL_MLP_MAX_CHUNKS];
-  __shared__ int n_live_sh;
-  if (t == 0) {
-    int n = 0;
-    for (int c = 0; c < K / KC; ++c) {
-      const unsigned long long bits = KC == 128 ? (3ull << (2 * c)) : (1ull << c);
-      if (c >= 64 / (KC / 64) || (args.k_live & bits)) live_chunk[n++] = c;
-    }
-    if (n == 0) live_chunk[n++] = 0;  // at least one chunk: the bias-only result stays correct
-    n_live_sh = n;
-  }
-  __syncthreads();
-  const int nlive = n_live_sh;
 
   f4 acc1[NB1][MB];
   zero_acc(acc1);
@@ -354,7 +363,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     const T* Wl = (const T*)P.fw_lo[0];
     const int ldw = sizeof(T) == 2 ? K : H1;
     const int nsteps = nlive * SPC;
-    auto kstep = [&](int s) { return (live_chunk[s / SPC] * SPC + s % SPC) * tr::KS; };
+    auto kstep = [&](int s) { return live_piece[s / SPP] * PW + (s % SPP) * tr::KS; };
     typename tr::frag wq[PF][NB1];
     typename tr::frag wl[COMP ? PF : 1][NB1];
 #pragma unroll
@@ -365,8 +374,8 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
         if constexpr (COMP) wl[u][nb] = tr::wload(Wl, ldw, (wave + WV * nb) * 16, kstep(min(u, nsteps - 1)));
       }
     const int nchunks = nlive;
-    stage_load(xa, live_chunk[0] * KC);
-    stage_load(xb, live_chunk[min(1, nchunks - 1)] * KC);
+    stage_load(xa, 0);
+    stage_load(xb, min(1, nchunks - 1));
     stage_store(xa, Xs0);
     __syncthreads();
     // chunk c: computed from LDS buffer c&1; chunk c+1 in flight in one register set (stored at the
@@ -374,7 +383,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     // index) so that the compiler can wait for the older set only.
     auto chunk = [&](int c, u4v (&xnext)[NPASS], u4v (&xfree)[NPASS]) {
       const T* X = (c & 1) ? Xs1 : Xs0;
-      stage_load(xfree, live_chunk[min(c + 2, nchunks - 1)] * KC);
+      stage_load(xfree, min(c + 2, nchunks - 1));
       typename tr::frag b[MB];
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(X, L::LDX, mb * 16, 0);
@@ -516,8 +525,8 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   for (int base = 0; base < ncb_x; base += 16) {
     // input columns [16 base, 16 base + 256): skipped when no atom of the batch has a non-zero
     // there (their dE/dAEV is never read: the adjoint reads the live blocks only)
-    const int q0 = base / 4;
-    if (q0 < 64 && ((args.k_live >> q0) & 0xFull) == 0ull) continue;
+    const int q0 = base / 2;  // first 32-column piece of the pass
+    if (q0 + 8 <= 64 && ((args.k_live >> q0) & 0xFFull) == 0ull) continue;
     f4 accx[NB1][MB];
     zero_acc(accx);
     gemm_t<T, NB1, MB, PF, WV, COMP>(accx, (const T*)P.bw[0], sizeof(T) == 2 ? H1 : K, base + wave, ncb_x, B1,

@@ -18,7 +18,7 @@
 //                  over a chunk of RC atoms (the reduction runs over atoms);
 //   k_train_reduce the chunk and tile partials of each network summed in a fixed order, so the
 //                  gradient is the same bits every run.
-// Layer-1 work covers only the 64-column AEV blocks in k_live (the other columns are zero).
+// Layer-1 work covers only the 32-column AEV pieces in k_live (the other columns are zero).
 
 #include "fnl_common.h"
 
@@ -90,9 +90,16 @@ __device__ __forceinline__ int find_net(const int* off, int n_nets, int idx) {
   return n;
 }
 
+// k_live bit q: AEV columns [32q, 32q + 32) may be non-zero (columns from 2048 on: always)
 __device__ __forceinline__ bool chunk_live(unsigned long long live, int k0) {
-  const int q = k0 >> 6;
+  const int q = k0 >> 5;
   return q >= 64 || ((live >> q) & 1ull);
+}
+
+// the 64 columns from k0 (a multiple of 64): live if either of their pieces is
+__device__ __forceinline__ bool block64_live(unsigned long long live, int k0) {
+  const int q = k0 >> 5;
+  return q >= 64 || ((live >> q) & 3ull);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -330,7 +337,7 @@ __global__ void __launch_bounds__(256) k_train_wgrad(const TrainArgs a, const Tr
   const int NI = LAYER == 1 ? a.k_in : NI_;
   const int i0 = blockIdx.y * 64, j0 = blockIdx.z * 64;
   if (i0 >= NI || j0 >= N) return;
-  if (LAYER == 1 && !chunk_live(a.k_live, i0)) return;
+  if (LAYER == 1 && !block64_live(a.k_live, i0)) return;
   const int net = find_net(a.chunks, a.n_nets, chunk);
   const int rb = a.rows[net] + (chunk - a.chunks[net]) * RC;
   const int re = min(rb + RC, a.rows[net + 1]);
@@ -433,7 +440,7 @@ __global__ void __launch_bounds__(256) k_train_reduce(const TrainArgs a, const T
   (void)t1;
   if (e < s1) {
     const int i = (int)(e / H1);
-    if (chunk_live(a.k_live, i & ~63)) s = sum_chunks(bf.part1, s1, e);
+    if (block64_live(a.k_live, i & ~63)) s = sum_chunks(bf.part1, s1, e);
   } else if (e >= s2 && e < s3) {
     s = sum_chunks(bf.part2, (size_t)H1 * H2, e - s2);
   } else if (e >= s4 && e < s5) {

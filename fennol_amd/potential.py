@@ -59,10 +59,11 @@ class ANIConfig:
 
     @functools.lru_cache(maxsize=256)
     def live_column_blocks(self, species_mask: int) -> int:
-        """Bit q set if AEV columns [64q, 64q + 64) hold a channel block written for a batch whose
+        """Bit q set if AEV columns [32q, 32q + 32) hold a channel block written for a batch whose
         AEV species indices are the set bits of ``species_mask`` (radial blocks of those species,
-        angular blocks of pairs of them); 0 (= every column) for the full mask.  The MLP skips
-        the products of the other columns, which are zero in every row of such a batch."""
+        angular blocks of pairs of them); 0 (= every column) for the full mask.  The MLP and the
+        training kernels skip the products of the other columns, which are zero in every row of
+        such a batch (columns from 2048 on are always treated as live)."""
         if species_mask == 0xFFFFFFFF or species_mask == 0:
             return 0
         R = self.radial_dist_divisions
@@ -80,7 +81,7 @@ class ANIConfig:
                 pr += 1
         live = 0
         for c0, c1 in spans:
-            for q in range(c0 // 64, (c1 - 1) // 64 + 1):
+            for q in range(c0 // 32, (c1 - 1) // 32 + 1):
                 if q < 64:
                     live |= 1 << q
         return live or 1

@@ -357,12 +357,12 @@ int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_mlp_weights* 
 /* n_tiles_dev: NULL, or a device word holding the number of tiles in use
  * (<= n_tiles, which then sizes the grid only): tables built on the device
  * (fnl_pool_layout) and launches a HIP graph replays with changing tiles. */
-/* k_live: bit q set if input columns [64q, 64q + 64) may be non-zero for some atom
+/* k_live: bit q set if input columns [32q, 32q + 32) may be non-zero for some atom
  * of the batch (0 = all).  The other columns must be zero in every AEV row: the
  * kernel skips their products (exact zeros) and writes dE/dAEV only in the
  * 256-column passes that hold a live column (fnl_aev_backward with the matching
  * species_mask never reads the others).
- * Columns at and beyond 4096 are always treated as live. */
+ * Columns at and beyond 2048 are always treated as live. */
 
 /* ------------------------------------------------------------------------
  * Per-system reductions in FP64 (total_energy segment_sum, fennix.py:240-246;
@@ -454,7 +454,7 @@ int fnl_philox_normals(double* out, int32_t n_atoms, uint64_t seed, uint64_t ste
  *   rows      perm lists the atoms grouped by network in network order;
  *             net_rows (host, [n_nets+1]) are the offsets of the groups
  *   aev/daev  rows in atom order (float32, strides >= k_in, multiples of 4)
- *   k_live    bit q: AEV columns [64q, 64q+64) may be non-zero (0 = all);
+ *   k_live    bit q: AEV columns [32q, 32q+32) may be non-zero (0 = all);
  *             the gradient rows of the others are zero
  *   grad      float32 [n_nets][fnl_train_grad_stride(k_in)]: dW1 [k_in][256],
  *             db1 [256], dW2 [256][192], db2 [192], dW3 [192][160], db3 [160],

@@ -537,8 +537,8 @@ def test_aev_backward_species_mask(gpu, precision):
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 def test_mlp_skips_dead_columns_exactly(gpu, precision):
-    """Layer 1 over only the 64-column blocks that hold the batch's species (water in a
-    7-species model: 6 of 16 blocks) gives, bit for bit, the atom energies and the live
+    """Layer 1 over only the 32-column pieces that hold the batch's species (water in a
+    7-species model: 8 of 32 pieces) gives, bit for bit, the atom energies and the live
     dE/dAEV columns of the product over all 1024 columns (the skipped products are exact
     zeros); dE/dAEV columns are written in passes of 256, and the passes without a live
     column are skipped (their columns keep the buffer's zeros)."""
@@ -549,7 +549,8 @@ def test_mlp_skips_dead_columns_exactly(gpu, precision):
     sb, x = _nblist(pot, s)
     pot.aev_forward(sb, x)
     live = cfg[1].live_column_blocks(sb.extra["aev_mask"])
-    assert live == 0b110000001111  # H, O radial; HH, HO, OO angular
+    # 32-column pieces: H, O radial (0, 1); HH (3, 4), HO (6, 7), OO (21, 22) angular
+    assert live == sum(1 << q for q in (0, 1, 3, 4, 6, 7, 21, 22))
     out = []
     for mask in (sb.extra["aev_mask"], 0xFFFFFFFF):
         sb.extra["aev_mask"] = mask
@@ -559,9 +560,9 @@ def test_mlp_skips_dead_columns_exactly(gpu, precision):
         out.append((sb.e_atom.clone(), sb.g_aev.float().clone()))
     (e_p, g_p), (e_f, g_f) = out
     assert torch.equal(e_p, e_f)
-    on = torch.tensor([(live >> (c // 64)) & 1 == 1 for c in range(g_p.shape[1])], device=g_p.device)
+    on = torch.tensor([(live >> (c // 32)) & 1 == 1 for c in range(g_p.shape[1])], device=g_p.device)
     assert torch.equal(g_p[:, on], g_f[:, on])
-    skipped = torch.tensor([(live >> (c // 256 * 4)) & 0xF == 0 for c in range(g_p.shape[1])], device=g_p.device)
+    skipped = torch.tensor([(live >> (c // 256 * 8)) & 0xFF == 0 for c in range(g_p.shape[1])], device=g_p.device)
     assert skipped.sum() == 512
     assert not g_p[:, skipped].any() and g_f[:, skipped].any()
 

@@ -128,7 +128,7 @@ def test_training_kernels_match_a_torch_fp32_reference(gpu):
     z = torch.as_tensor(batch["species"], device="cuda")
     K = ts.k_in
     live = pot.config.live_column_blocks(sb.extra["aev_mask"])
-    dead = torch.tensor([((live >> (c // 64)) & 1) == 0 for c in range(K)], device="cuda")
+    dead = torch.tensor([((live >> (c // 64 * 2)) & 3) == 0 for c in range(K)], device="cuda")
     from fennol_amd.units import SYMBOL_TO_Z
     for s in ts.species:
         idx = torch.nonzero(z == SYMBOL_TO_Z[s]).flatten()
@@ -165,7 +165,7 @@ def test_training_gradient_finite_differences(gpu):
     checked = 0
     for s, zz in (("H", 1), ("O", 8)):
         used = np.nonzero(np.abs(aev[batch["species"] == zz]).max(0) > 1e-3)[0]
-        used = used[[((live >> (c // 64)) & 1) == 1 for c in used]]
+        used = used[[((live >> (c // 32)) & 1) == 1 for c in used]]
         for layer in range(4):
             W = params[s][layer][0]
             for _ in range(8):
