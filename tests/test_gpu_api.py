@@ -180,7 +180,8 @@ def test_langevin_temperature(gpu):
 
 
 def test_md_graph_replay_matches_eager(gpu):
-    """A captured MD step replays like the eager step (several replays, eager work in between)."""
+    """A captured MD step replays like the eager step (several replays, eager work in between):
+    the same positions, energies and virials bit for bit."""
     print("[api] graph replay", flush=True)
     from fennol_amd.md import LangevinMD
     from fennol_amd.potential import ANIConfig, ANIPotential
@@ -211,13 +212,14 @@ def test_md_graph_replay_matches_eager(gpu):
                 torch.cuda.synchronize()
                 traj.append(md.x.cpu().numpy().copy())
         md.verify()
-        runs.append((traj, md.istep))
-    (te, ne), (tg, ng) = runs
+        runs.append((traj, md.istep, md.energies().copy(), md.virial_kcal().copy()))
+    (te, ne, ee, ve), (tg, ng, eg, vg) = runs
     assert ne == ng == 8
-    # the neighbour force scatter uses float atomics, so runs agree to rounding (~1e-9 A after
-    # 8 steps); a repeated or skipped noise draw would move atoms by ~1e-3 A
+    # fixed-order sums throughout (int64 neighbour-force sums, ordered reductions): the same bits
     for a, b in zip(te, tg):
-        np.testing.assert_allclose(a, b, rtol=0, atol=1e-6)
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(ee, eg)
+    np.testing.assert_array_equal(ve, vg)
 
 
 def test_md_skin_matches_rebuild_every_step(gpu):
