@@ -77,9 +77,10 @@ def live_channels(cfg, species_mask: int) -> tuple[int, int]:
     return S * cfg.radial_dist_divisions, S * (S + 1) // 2 * cfg.angular_dist_divisions * cfg.angle_sections
 
 
-def algorithmic_units(sb, cfg, precision: str) -> dict:
+def algorithmic_units(sb, cfg, precision: str, compensated: bool = False) -> dict:
     """Per-launch algorithmic bytes / flops of each stage, from the measured neighbour counts and
-    the channel blocks the batch's species use (the other blocks are never read or written)."""
+    the channel blocks the batch's species use (the other blocks are never read or written).
+    ``compensated``: the BF16 weights are hi + lo pairs (two MFMA products each)."""
     nr = sb.n_nbr.cpu().numpy().astype(np.float64)
     na = sb.n_nbr_ang.cpu().numpy().astype(np.float64)
     N = sb.n_atoms
@@ -88,12 +89,13 @@ def algorithmic_units(sb, cfg, precision: str) -> dict:
     out_b = 2 if precision == "bf16" else 4
     H = 256 * 192 + 192 * 160
     per_atom_mlp = 2.0 * (L * 256 + H + 160) + 2.0 * (H + 256 * L)  # forward and input gradient
-    # executed on the MFMA: the 128 (BF16) / 64 (FP32) column chunks holding a live column, hi and lo
+    # executed on the MFMA: the 128 (BF16) / 64 (FP32) column chunks holding a live column (hi and
+    # lo with compensated weights)
     kc = 128 if precision == "bf16" else 64
     live = cfg.live_column_blocks(sb.extra.get("aev_mask", 0xFFFFFFFF))
     pieces = sum(1 for q in range(min(64, cfg.k_pad // 32)) if (live >> q) & 1) + max(0, cfg.k_pad // 32 - 64)
     k_exec = cfg.k_pad if not live else kc * -(-pieces // (kc // 32))
-    comp = 2.0 if precision == "bf16" else 1.0
+    comp = 2.0 if precision == "bf16" and compensated else 1.0
     exec_mlp = comp * (2.0 * (k_exec * 256 + H + 160) + 2.0 * (H + 256 * k_exec))
     return {
         # xyz + species in, neighbour rows (radial + angular) and counts out
@@ -151,7 +153,8 @@ def copy_calibration(mib: int) -> None:
     del a, b
 
 
-def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_steps=20, skin=-1.0):
+def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_steps=20, skin=-1.0,
+           compensated=False):
     """Warm-up, then a per-stage breakdown (eager steps, HIP events around each stage on the
     launch stream), then the timed run: one MD step per HIP graph, replayed ``steps`` times back
     to back (the host enqueues one graph launch per step).  Returns the potential, the MD state,
@@ -162,7 +165,7 @@ def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_ste
     from fennol_amd.md import LangevinMD
     from fennol_amd.potential import ANIPotential, StageTimer
 
-    pot = ANIPotential(cfg, params, precision=precision, mlp_tile_rows=tile_rows)
+    pot = ANIPotential(cfg, params, precision=precision, mlp_tile_rows=tile_rows, bf16_compensated=compensated)
     md = LangevinMD(pot, system, dt_fs=0.5, temperature=300.0, gamma_thz=1.0, seed=MD_SEED, check_every=0,
                     nblist_skin=skin)
     for _ in range(warmup):
@@ -196,11 +199,17 @@ def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_ste
     return pot, md, (t1 - t0), lat, stages, eager_ms
 
 
-def gpu_energy_forces(system, x, cfg, params, precision):
+def form_precision(form: str) -> tuple[str, bool]:
+    """(precision, compensated weights) of a benchmarked form: fp32, bf16 or bf16_compensated."""
+    return ("bf16", True) if form == "bf16_compensated" else (form, False)
+
+
+def gpu_energy_forces(system, x, cfg, params, form):
     """E [B] and F [N,3] of the device potential at coordinates x (model unit, eV)."""
     from fennol_amd.potential import ANIPotential
 
-    pot = ANIPotential(cfg, params, precision=precision)
+    precision, comp = form_precision(form)
+    pot = ANIPotential(cfg, params, precision=precision, bf16_compensated=comp)
     E, F, _ = pot.energy_forces_virial(system["species"], x, system["natoms"], system.get("cells"))
     return E, F
 
@@ -517,7 +526,8 @@ def main() -> int:
     ap.add_argument("--tile-fp32", type=int, default=0, help="MLP atoms per tile, FP32 form (0: default)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="launch one plain copy kernel of this size first (for counter runs)")
-    ap.add_argument("--precisions", default="fp32,bf16", help="MD forms to run (bf16 gives value)")
+    ap.add_argument("--precisions", default="fp32,bf16,bf16_compensated",
+                    help="MD forms to run (bf16 gives value; bf16_compensated: hi + lo BF16 weights)")
     ap.add_argument("--no-reference-box", dest="reference_box", action="store_false",
                     help="skip the 216-water NVE run that the reference's published MD rate is quoted on")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
@@ -548,20 +558,21 @@ def main() -> int:
 
     if args.calibrate_copy_mib > 0:
         copy_calibration(args.calibrate_copy_mib)
-    precisions = [p for p in ("fp32", "bf16") if p in args.precisions.split(",")]
+    precisions = [p for p in ("fp32", "bf16", "bf16_compensated") if p in args.precisions.split(",")]
     if "bf16" not in precisions:
         print("the bf16 form gives the headline value; --precisions must include it", file=sys.stderr)
         return 2
     results = {}
-    for precision in precisions:
-        log(f"[bench] MD {precision}: warmup {args.warmup}, steps {args.steps}")
+    for form in precisions:
+        log(f"[bench] MD {form}: warmup {args.warmup}, steps {args.steps}")
+        precision, comp = form_precision(form)
         tile = args.tile_bf16 if precision == "bf16" else args.tile_fp32
         pot, md, wall, lat, stages, eager_ms = run_md(system, cfg, params, precision, args.steps, args.warmup,
-                                                      tile_rows=tile)
-        units = algorithmic_units(md.sb, cfg, precision)
-        results[precision] = dict(wall=wall, lat=lat, stages=stages, units=units, eager_ms=eager_ms,
-                                  max_nbr=md.sb.max_nbr, max_nbr_ang=md.sb.max_nbr_ang)
-        log(f"[bench]   {precision}: {1000 * wall / args.steps:.3f} ms/step (graph replay), eager "
+                                                      tile_rows=tile, compensated=comp)
+        units = algorithmic_units(md.sb, cfg, precision, pot.nets.compensated)
+        results[form] = dict(wall=wall, lat=lat, stages=stages, units=units, eager_ms=eager_ms,
+                             max_nbr=md.sb.max_nbr, max_nbr_ang=md.sb.max_nbr_ang)
+        log(f"[bench]   {form}: {1000 * wall / args.steps:.3f} ms/step (graph replay), eager "
             f"{eager_ms:.3f} ms/step; stages (ms): " + ", ".join(f"{k} {v.mean():.3f}" for k, v in stages.items()))
         del pot, md
         torch.cuda.empty_cache()
@@ -596,6 +607,7 @@ def main() -> int:
 
     bf = results["bf16"]
     fp = results.get("fp32")
+    cp = results.get("bf16_compensated")
     ms_bf = 1000.0 * bf["wall"] / args.steps
     value = N * args.steps / bf["wall"]
 
@@ -676,7 +688,8 @@ def main() -> int:
         "config": {"workload": f"periodic water box, {N // 3} waters ({N} atoms), min-image, neighbour list "
                                f"rebuilt every step, BAOAB Langevin dt=0.5 fs 300 K gamma=1/ps",
                    "aev_length": cfg.aev_length, "mlp": "256-192-160-1 CELU(0.1), 4 species networks",
-                   "precision": "BF16 MLP GEMMs (FP32 accumulate), FP32 AEV, FP64 reductions and integrator"},
+                   "precision": "BF16 MLP weights and activations on MFMA (FP32 accumulate), FP32 AEV, "
+                                "FP64 reductions and integrator"},
         "eager_ms_per_step": round(bf["eager_ms"], 4),
         "fp32": None if fp is None else {
             "value": round(N * args.steps / fp["wall"], 1), "ms_per_step": round(1000.0 * fp["wall"] / args.steps, 4),
@@ -684,6 +697,11 @@ def main() -> int:
             "p50_ms": round(float(np.percentile(fp["lat"], 50)), 4),
             "p99_ms": round(float(np.percentile(fp["lat"], 99)), 4)},
         "bf16_over_fp32": None if fp is None else round(fp["wall"] / bf["wall"], 3),
+        "bf16_compensated": None if cp is None else {
+            "weights": "hidden-layer weights as BF16 hi + lo pairs (two MFMA products each)",
+            "value": round(N * args.steps / cp["wall"], 1), "ms_per_step": round(1000.0 * cp["wall"] / args.steps, 4),
+            "p50_ms": round(float(np.percentile(cp["lat"], 50)), 4),
+            "p99_ms": round(float(np.percentile(cp["lat"], 99)), 4)},
         "p50_ms": round(float(np.percentile(bf["lat"], 50)), 4),
         "p99_ms": round(float(np.percentile(bf["lat"], 99)), 4),
         "accuracy_vs_fp64": acc,
