@@ -154,11 +154,12 @@ def copy_calibration(mib: int) -> None:
 
 
 def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_steps=20, skin=-1.0,
-           compensated=False):
+           compensated=False, steps_per_graph=1):
     """Warm-up, then a per-stage breakdown (eager steps, HIP events around each stage on the
-    launch stream), then the timed run: one MD step per HIP graph, replayed ``steps`` times back
-    to back (the host enqueues one graph launch per step).  Returns the potential, the MD state,
-    the wall time of the timed steps, the per-step latencies (ms), the stage times and the eager
+    launch stream), then the timed run: ``steps_per_graph`` MD steps per HIP graph (one with a
+    skin list), replayed back to back until ``steps`` steps ran (the host enqueues one graph
+    launch per replay).  Returns the potential, the MD state, the wall time of the timed steps,
+    the per-step latencies (ms; a replay's time over its steps), the stage times and the eager
     wall time per step."""
     import torch
 
@@ -182,20 +183,22 @@ def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_ste
     eager_ms = 1000.0 * (time.perf_counter() - te0) / stage_steps
     pot.timer = None
     stages = timer.times_ms()
-    md.capture(steps_per_graph=1)
+    spg = steps_per_graph if skin <= 0 and steps % steps_per_graph == 0 else 1
+    md.capture(steps_per_graph=spg)
     md.replay(1)
     torch.cuda.synchronize()
-    step_ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    n_rep = steps // spg
+    step_ev = [torch.cuda.Event(enable_timing=True) for _ in range(n_rep + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     step_ev[0].record()
-    for k in range(steps):
+    for k in range(n_rep):
         md.replay(1)
         step_ev[k + 1].record()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     md.verify()
-    lat = np.array([step_ev[k].elapsed_time(step_ev[k + 1]) for k in range(steps)])
+    lat = np.array([step_ev[k].elapsed_time(step_ev[k + 1]) / spg for k in range(n_rep)])
     return pot, md, (t1 - t0), lat, stages, eager_ms
 
 
@@ -517,6 +520,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps-per-graph", type=int, default=10,
+                    help="MD steps captured in one HIP graph (1 if it does not divide --steps)")
     ap.add_argument("--workload", default="water23k", choices=["water23k", "water7k"])
     ap.add_argument("--skip-cpu", action="store_true", help="skip the FP64 oracle (accuracy and CPU baseline)")
     ap.add_argument("--serving-requests", type=int, default=8192, help="0 skips the serving measurement")
@@ -568,7 +573,8 @@ def main() -> int:
         precision, comp = form_precision(form)
         tile = args.tile_bf16 if precision == "bf16" else args.tile_fp32
         pot, md, wall, lat, stages, eager_ms = run_md(system, cfg, params, precision, args.steps, args.warmup,
-                                                      tile_rows=tile, compensated=comp)
+                                                      tile_rows=tile, compensated=comp,
+                                                      steps_per_graph=args.steps_per_graph)
         units = algorithmic_units(md.sb, cfg, precision, pot.nets.compensated)
         results[form] = dict(wall=wall, lat=lat, stages=stages, units=units, eager_ms=eager_ms,
                              max_nbr=md.sb.max_nbr, max_nbr_ang=md.sb.max_nbr_ang)
@@ -596,7 +602,7 @@ def main() -> int:
         # energy and forces against a fresh full build at the same coordinates, so the errors
         # against FP64 are those of the rebuild-every-step form (accuracy_vs_fp64)
         x_end = md.x32.cpu().numpy().astype(np.float64)
-        E_skin, F_skin = md.sb.energy.cpu().numpy(), pot.forces(md.sb).cpu().numpy()
+        E_skin, F_skin = md.energies(), pot.forces(md.sb).cpu().numpy()
         E_full, F_full = gpu_energy_forces(system, x_end, cfg, params, "bf16")
         skin["vs_full_rebuild"] = {"max_abs_energy_diff_ev": float(np.abs(E_skin - E_full).max()),
                                    "max_abs_force_diff_ev_per_a": float(np.abs(F_skin - F_full).max())}
@@ -689,7 +695,8 @@ def main() -> int:
                                f"rebuilt every step, BAOAB Langevin dt=0.5 fs 300 K gamma=1/ps",
                    "aev_length": cfg.aev_length, "mlp": "256-192-160-1 CELU(0.1), 4 species networks",
                    "precision": "BF16 MLP weights and activations on MFMA (FP32 accumulate), FP32 AEV, "
-                                "FP64 reductions and integrator"},
+                                "FP64 reductions and integrator",
+                   "steps_per_graph": args.steps_per_graph if args.steps % args.steps_per_graph == 0 else 1},
         "eager_ms_per_step": round(bf["eager_ms"], 4),
         "fp32": None if fp is None else {
             "value": round(N * args.steps / fp["wall"], 1), "ms_per_step": round(1000.0 * fp["wall"] / args.steps, 4),

@@ -15,6 +15,7 @@ FNL_OK = 0
 FNL_ERR_INVALID = -1
 FNL_ERR_HIP = -2
 FNL_ERR_UNSUPPORTED = -3
+FNL_REDUCE_FUSED_MAX_SYSTEMS = 16  # systems fnl_baoab_step_sums sums in its own launch
 FNL_ERR_SCRATCH_TOO_SMALL = -4
 FNL_NBLIST_SCRATCH_BYTES = 16 << 20
 FNL_PBC_NONE = 0
@@ -145,6 +146,8 @@ _SIGNATURES = {
     "fnl_system_reduce": (C.c_int, [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "fnl_forces": (C.c_int, [vp, vp, i32, f32, vp, vp]),
     "fnl_baoab_step": (C.c_int, [vp, vp, vp, vp, vp, f32, vp, f64, f64, vp, vp, u64, u64, vp, i32, i32, vp]),
+    "fnl_baoab_step_sums": (C.c_int, [vp, vp, vp, vp, vp, f32, vp, f64, f64, vp, vp, u64, u64, vp, i32, i32,
+                                      vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
     "fnl_counter_add": (C.c_int, [vp, u64, vp]),
     "fnl_baoab_step_batched": (C.c_int, [vp, vp, vp, vp, vp, f32, vp, f64, f64, vp, vp, vp, vp, vp, u64, i32, vp]),
     "fnl_philox_normals": (C.c_int, [vp, i32, u64, u64, u32, vp]),

@@ -10,7 +10,7 @@
 namespace {
 
 constexpr int kRedVals = 1 + 9 + 9;  // energy, virial[9], kinetic[9]
-constexpr int kFuseMaxSystems = 16;  // systems summed by the last segment workgroup itself
+constexpr int kFuseMaxSystems = FNL_REDUCE_FUSED_MAX_SYSTEMS;  // summed by the last segment workgroup
 
 // Per-system sums of the partials of its segments: one wave per system, lane k
 // sums segments k, k+64, ...; then a fixed xor butterfly (deterministic order).
@@ -46,19 +46,18 @@ __device__ __forceinline__ void reduce_system(int b, int lane, const int32_t* __
   }
 }
 
-// One workgroup per segment (a run of at most seg_atoms atoms of one system)
-// -> FP64 partial sums in a fixed tree order.  The last workgroup to finish
-// (a completion count in the word before the partials) then forms the
-// per-system sums, so the result is bitwise reproducible and one launch does
-// both stages.
-__global__ void __launch_bounds__(256)
-k_reduce_segments(const int32_t* __restrict__ sys_offsets, const int32_t* __restrict__ seg_offsets, int n_sys,
-                  int seg_atoms, const float* __restrict__ energy_atom, const float* __restrict__ virial_atom,
-                  const double* __restrict__ vel, const double* __restrict__ mass, double* __restrict__ partials,
-                  unsigned* __restrict__ done, double* __restrict__ energy_sys, double* __restrict__ virial_sys,
-                  double* __restrict__ kinetic_sys, int fuse_final) {
+// One workgroup (256 threads) per segment (a run of at most seg_atoms atoms of
+// one system) -> FP64 partial sums in a fixed tree order.  The last of the
+// n_seg segment workgroups to finish (a completion count in the word before the
+// partials) then forms the per-system sums, so the result is bitwise
+// reproducible and one launch does both stages.
+__device__ __forceinline__ void reduce_segment_block(
+    int sgi, int n_seg, const int32_t* __restrict__ sys_offsets, const int32_t* __restrict__ seg_offsets,
+    int n_sys, int seg_atoms, const float* __restrict__ energy_atom, const float* __restrict__ virial_atom,
+    const double* __restrict__ vel, const double* __restrict__ mass, double* __restrict__ partials,
+    unsigned* __restrict__ done, double* __restrict__ energy_sys, double* __restrict__ virial_sys,
+    double* __restrict__ kinetic_sys, int fuse_final) {
   __shared__ double wsum[4][kRedVals];
-  const int sgi = blockIdx.x;
   const int t = threadIdx.x;
   // system of this segment: largest b with seg_offsets[b] <= sgi
   int lo = 0, hi = n_sys - 1;
@@ -117,12 +116,22 @@ k_reduce_segments(const int32_t* __restrict__ sys_offsets, const int32_t* __rest
   }
   __shared__ bool last;
   __syncthreads();
-  if (t == 0) last = atomicAdd(done, 1u) == gridDim.x - 1;
+  if (t == 0) last = atomicAdd(done, 1u) == (unsigned)(n_seg - 1);
   __syncthreads();
   if (!last) return;
   for (int b = t >> 6; b < n_sys; b += blockDim.x >> 6)
     reduce_system(b, t & 63, seg_offsets, partials, energy_sys, virial_sys, kinetic_sys);
   if (t == 0) atomicExch(done, 0u);
+}
+
+__global__ void __launch_bounds__(256)
+k_reduce_segments(const int32_t* __restrict__ sys_offsets, const int32_t* __restrict__ seg_offsets, int n_sys,
+                  int seg_atoms, const float* __restrict__ energy_atom, const float* __restrict__ virial_atom,
+                  const double* __restrict__ vel, const double* __restrict__ mass, double* __restrict__ partials,
+                  unsigned* __restrict__ done, double* __restrict__ energy_sys, double* __restrict__ virial_sys,
+                  double* __restrict__ kinetic_sys, int fuse_final) {
+  reduce_segment_block(blockIdx.x, gridDim.x, sys_offsets, seg_offsets, n_sys, seg_atoms, energy_atom, virial_atom,
+                       vel, mass, partials, done, energy_sys, virial_sys, kinetic_sys, fuse_final);
 }
 
 // Systems without segments (no atoms at all): one wave per system writes zeros.
@@ -141,18 +150,21 @@ __global__ void k_forces(const float* __restrict__ gs, const long long* __restri
   if (i < n3) f[i] = (float)(-((double)gs[i] + fnl_from_fixed(gn[i])) * (double)scale);
 }
 
-// step_dev (optional): {step, done}.  Every workgroup reads the step before it
-// counts itself in `done`; the last one to finish advances the step and clears
-// `done`, so a captured MD step replays with the next noise without a counter
-// kernel.  clear_gn: the neighbour-gradient rows are zeroed once read (the
-// adjoint of the same MD step accumulates into them).
-__global__ void __launch_bounds__(256)
-k_baoab(double* __restrict__ x, double* __restrict__ v, float* __restrict__ x32,
-        const float* __restrict__ gs, long long* __restrict__ gn, float force_scale,
-        const double* __restrict__ dtm, double half_dt, double a1,
-        const double* __restrict__ a2, const double* __restrict__ noise, uint64_t seed,
-        uint64_t step_arg, uint64_t* __restrict__ step_dev, int n_atoms, int clear_gn) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One workgroup (256 threads) of the BAOAB step: block blk of n_blocks covers
+// atoms [256 blk, 256 blk + 256).  step_dev (optional): {step, done}.  Every
+// workgroup reads the step before it counts itself in `done`; the last one to
+// finish advances the step and clears `done`, so a captured MD step replays
+// with the next noise without a counter kernel.  clear_gn: the
+// neighbour-gradient rows are zeroed once read (the adjoint of the same MD step
+// accumulates into them).
+__device__ __forceinline__ void baoab_block(int blk, int n_blocks, double* __restrict__ x, double* __restrict__ v,
+                                            float* __restrict__ x32, const float* __restrict__ gs,
+                                            long long* __restrict__ gn, float force_scale,
+                                            const double* __restrict__ dtm, double half_dt, double a1,
+                                            const double* __restrict__ a2, const double* __restrict__ noise,
+                                            uint64_t seed, uint64_t step_arg, uint64_t* __restrict__ step_dev,
+                                            int n_atoms, int clear_gn) {
+  const int i = blk * 256 + (int)threadIdx.x;
   const uint64_t step = step_dev ? step_dev[0] : step_arg;
   if (i < n_atoms) {
     double xi[3];
@@ -178,11 +190,65 @@ k_baoab(double* __restrict__ x, double* __restrict__ v, float* __restrict__ x32,
     __syncthreads();  // every thread of this workgroup has read the step (and used it)
     if (threadIdx.x == 0) {
       const unsigned prev = atomicAdd(reinterpret_cast<unsigned*>(step_dev + 1), 1u);
-      if (prev == gridDim.x - 1) {
+      if (prev == (unsigned)(n_blocks - 1)) {
         atomicExch(reinterpret_cast<unsigned long long*>(step_dev), (unsigned long long)(step + 1));
         atomicExch(reinterpret_cast<unsigned*>(step_dev + 1), 0u);
       }
     }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_baoab(double* __restrict__ x, double* __restrict__ v, float* __restrict__ x32,
+        const float* __restrict__ gs, long long* __restrict__ gn, float force_scale,
+        const double* __restrict__ dtm, double half_dt, double a1,
+        const double* __restrict__ a2, const double* __restrict__ noise, uint64_t seed,
+        uint64_t step_arg, uint64_t* __restrict__ step_dev, int n_atoms, int clear_gn) {
+  baoab_block(blockIdx.x, gridDim.x, x, v, x32, gs, gn, force_scale, dtm, half_dt, a1, a2, noise, seed, step_arg,
+              step_dev, n_atoms, clear_gn);
+}
+
+// The BAOAB step and the per-system energy and virial sums of the last force
+// evaluation in one launch: workgroups [0, n_seg) reduce the segments (the
+// last of them forms the per-system sums), workgroups [n_seg, n_seg +
+// n_blocks) integrate.  The two parts touch disjoint memory (the sums read
+// the atom energies and virials, which the step does not write), so they run
+// side by side and the launch takes the longer of the two.
+struct BaoabArgs {
+  double* x;
+  double* v;
+  float* x32;
+  const float* gs;
+  long long* gn;
+  float force_scale;
+  const double* dtm;
+  double half_dt, a1;
+  const double* a2;
+  const double* noise;
+  uint64_t seed, step;
+  uint64_t* step_dev;
+  int n_atoms, clear_gn;
+};
+
+struct SumArgs {
+  const int32_t* sys_offsets;
+  const int32_t* seg_offsets;
+  int n_sys, n_seg, seg_atoms;
+  const float* energy_atom;
+  const float* virial_atom;
+  double* rows;
+  unsigned* done;
+  double* energy_sys;
+  double* virial_sys;
+};
+
+__global__ void __launch_bounds__(256) k_baoab_sums(BaoabArgs b, SumArgs r) {
+  if ((int)blockIdx.x < r.n_seg) {
+    reduce_segment_block(blockIdx.x, r.n_seg, r.sys_offsets, r.seg_offsets, r.n_sys, r.seg_atoms, r.energy_atom,
+                         r.virial_atom, nullptr, nullptr, r.rows, r.done, r.energy_sys, r.virial_sys, nullptr, 1);
+  } else {
+    baoab_block(blockIdx.x - r.n_seg, gridDim.x - r.n_seg, b.x, b.v, b.x32, b.gs, b.gn, b.force_scale, b.dtm,
+                b.half_dt, b.a1, b.a2, b.noise, b.seed, b.step, b.step_dev, b.n_atoms, b.clear_gn);
   }
 }
 
@@ -275,6 +341,30 @@ extern "C" int fnl_baoab_step(double* x, double* v, float* x32, const float* gra
   hipLaunchKernelGGL(k_baoab, dim3(fnl_blocks(n_atoms, 256)), dim3(256), 0, (hipStream_t)stream, x, v, x32,
                      grad_self, (long long*)grad_nbr, force_scale, dtm, half_dt, a1, a2, noise, seed, step, step_dev, n_atoms,
                      (int)(clear_grad_nbr != 0));
+  FNL_LAUNCH_CHECK();
+  return FNL_OK;
+}
+
+extern "C" int fnl_baoab_step_sums(double* x, double* v, float* x32, const float* grad_self, int64_t* grad_nbr,
+                                   float force_scale, const double* dtm, double half_dt, double a1, const double* a2,
+                                   const double* noise, uint64_t seed, uint64_t step, uint64_t* step_dev,
+                                   int32_t n_atoms, int32_t clear_grad_nbr, const int32_t* sys_offsets,
+                                   const int32_t* seg_offsets, int32_t n_sys, int32_t n_seg, int32_t seg_atoms,
+                                   const float* energy_atom, double* energy_sys, const float* virial_atom,
+                                   double* virial_sys, double* partials, fnl_stream_t stream) {
+  if (n_atoms < 0 || n_sys < 0 || n_seg < 0 || seg_atoms < 1) return FNL_ERR_INVALID;
+  if (!x || !v || !x32 || !grad_self || !grad_nbr || !dtm) return FNL_ERR_INVALID;
+  if (!sys_offsets || !seg_offsets || !partials || !energy_atom || !energy_sys) return FNL_ERR_INVALID;
+  if (virial_atom && !virial_sys) return FNL_ERR_INVALID;
+  // the last segment workgroup forms the per-system sums itself only for a few systems
+  if (n_sys > kFuseMaxSystems) return FNL_ERR_UNSUPPORTED;
+  if (n_atoms == 0 || n_sys == 0 || n_seg == 0) return FNL_ERR_INVALID;
+  BaoabArgs b{x, v, x32, grad_self, (long long*)grad_nbr, force_scale, dtm, half_dt, a1, a2, noise, seed, step,
+              step_dev, n_atoms, (int)(clear_grad_nbr != 0)};
+  SumArgs r{sys_offsets, seg_offsets, n_sys, n_seg, seg_atoms, energy_atom, virial_atom, partials + 1,
+            reinterpret_cast<unsigned*>(partials), energy_sys, virial_atom ? virial_sys : nullptr};
+  const int n_blocks = fnl_blocks(n_atoms, 256);
+  hipLaunchKernelGGL(k_baoab_sums, dim3(n_seg + n_blocks), dim3(256), 0, (hipStream_t)stream, b, r);
   FNL_LAUNCH_CHECK();
   return FNL_OK;
 }

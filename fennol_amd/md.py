@@ -122,6 +122,7 @@ class LangevinMD:
             raise NotImplementedError(f"thermostat {thermostat}")
         # step counter on the device: the O-step noise of a captured graph advances on replay
         self.step_dev = torch.zeros(2, dtype=torch.int64, device=dev)  # {step, completion count}
+        self._sums_stale = False  # a replayed step leaves its per-system sums to the next one
         self.graph = None
         self.graph_skin = None
         self.steps_per_graph = 1
@@ -161,6 +162,18 @@ class LangevinMD:
             _abi.ptr(self.t_a2), None, self.seed, 0, self.step_dev.data_ptr(), self.n, 1, s)
         _abi.check(rc, "fnl_baoab_step")
 
+    def _baoab_sums(self) -> None:
+        """The BAOAB step with, in the same launch, the per-system energy and virial sums of the
+        atom energies and virials the last force evaluation left (fnl_baoab_step_sums)."""
+        sb, P = self.sb, _abi.ptr
+        rc = self.lib.fnl_baoab_step_sums(
+            self.x.data_ptr(), self.v.data_ptr(), self.x32.data_ptr(), sb.grad_self.data_ptr(),
+            sb.grad_nbr.data_ptr(), self.force_scale, self.t_dtm.data_ptr(), 0.5 * self.dt, self.a1,
+            P(self.t_a2), None, self.seed, 0, self.step_dev.data_ptr(), self.n, 1, P(sb.sys_offsets),
+            P(sb.seg_offsets), sb.n_sys, sb.n_seg, SEG_ATOMS, P(sb.e_atom), P(sb.energy), P(sb.vir_atom),
+            P(sb.virial), P(sb.partials), _abi.stream_handle())
+        _abi.check(rc, "fnl_baoab_step_sums")
+
     def _graph_update(self) -> bool | str:
         """The neighbour-list action of the next step (update_graphs, integrate.py:515-566):
         True = full rebuild, "skin" = skin re-filter; advances the countdown."""
@@ -181,6 +194,7 @@ class LangevinMD:
             check = self.check_every == 1
         self.pot.compute(self.sb, self.x32, forces=True, rebuild=rebuild, check=check, stream=stream,
                          zero_grad=False)  # the integrator cleared grad_nbr
+        self._sums_stale = False
 
     # --------------------------------------------------------- checkpoints
     def _checkpoint(self) -> None:
@@ -212,6 +226,7 @@ class LangevinMD:
         self.x32.copy_(self.x)
         self.istep = self._ckpt["istep"]
         pot.compute(sb, self.x32, forces=True, rebuild=True, check=True)
+        self._sums_stale = False
         self.nblist_countdown = self.nblist_stride - 1
         pot.clear_sticky(sb)
         self.rollbacks += 1
@@ -263,11 +278,25 @@ class LangevinMD:
             self._checkpoint()
 
     # ---------------------------------------------------------- graph replay
-    def _record(self, rebuild: bool | str, stream=None) -> None:
-        """One step with a fixed neighbour-list action and no host reads or counters."""
-        self._integrate(stream)
-        self.pot.compute(self.sb, self.x32, forces=True, rebuild=rebuild, check=False, stream=stream,
-                         zero_grad=False)
+    def _record(self, rebuild: bool | str) -> None:
+        """One step with a fixed neighbour-list action and no host reads or counters, for
+        capture.  With at most 16 systems the per-system sums (energy, virial) of the previous
+        step are formed by the integrator's launch (:meth:`_baoab_sums`) rather than by a launch
+        of their own at the end of the step; :meth:`energies` and :meth:`virial_kcal` form the
+        last replayed step's sums when they are read."""
+        pot, sb = self.pot, self.sb
+        if sb.n_sys > _abi.FNL_REDUCE_FUSED_MAX_SYSTEMS:
+            self._integrate()
+            pot.compute(sb, self.x32, forces=True, rebuild=rebuild, check=False, zero_grad=False)
+            return
+        self._baoab_sums()
+        if rebuild == "skin":
+            pot.update_skin(sb, self.x32)
+        else:
+            pot.build_nblist(sb, self.x32)
+        pot.aev_forward(sb, self.x32)
+        pot.mlp(sb, with_grad=True)
+        pot.aev_backward(sb, self.x32, zero=False)  # the integrator cleared grad_nbr
 
     def capture(self, steps_per_graph: int = 1, warmup: bool = True) -> None:
         """Record MD steps in HIP graphs (no host checks inside; construct with check_every != 1
@@ -310,10 +339,20 @@ class LangevinMD:
             else:
                 (self.graph if self._graph_update() is True else self.graph_skin).replay()
                 self.istep += 1
+        if n_graphs > 0 and self.sb.n_sys <= _abi.FNL_REDUCE_FUSED_MAX_SYSTEMS:
+            self._sums_stale = True
+
+    def _current_sums(self) -> None:
+        """Form the per-system sums of the last replayed step (its graph leaves them to the next
+        step's integrator launch)."""
+        if self._sums_stale:
+            self.pot.reduce(self.sb, with_virial=True)
+            self._sums_stale = False
 
     # -------------------------------------------------------------- observables
     def energies(self) -> np.ndarray:
         """Potential energy per system in the model's energy unit."""
+        self._current_sums()
         return self.sb.energy.cpu().numpy()
 
     def kinetic_tensor(self) -> np.ndarray:
@@ -333,6 +372,7 @@ class LangevinMD:
 
     def virial_kcal(self) -> np.ndarray:
         """dE/dstrain per system [B,3,3] in kcal/mol at the current positions."""
+        self._current_sums()
         return self.sb.virial.cpu().numpy().reshape(-1, 3, 3) * self.force_scale
 
     def pressure_tensor(self) -> np.ndarray:

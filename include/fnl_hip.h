@@ -413,6 +413,30 @@ int fnl_baoab_step(double* x, double* v, float* x32,
                    uint64_t* step_dev, int32_t n_atoms, int32_t clear_grad_nbr,
                    fnl_stream_t stream);
 
+/* fnl_baoab_step and the per-system energy (and virial) sums of
+ * fnl_system_reduce over the atom energies and virials of the last force
+ * evaluation, in one launch (the sums of models/fennix.py:240-246 and
+ * :320-347, the step of md/integrate.py:286-294): the segment workgroups and
+ * the integrator's workgroups run side by side, as the two touch disjoint
+ * memory.  For MD loops that read the sums of step k while integrating step
+ * k+1.  Same arguments and results as the two calls; energy_atom and
+ * energy_sys are required, virial_atom may be NULL (virial_sys untouched), no
+ * kinetic sums.  At most FNL_REDUCE_FUSED_MAX_SYSTEMS systems
+ * (FNL_ERR_UNSUPPORTED beyond: use the two calls), at least one atom and one
+ * segment. */
+#define FNL_REDUCE_FUSED_MAX_SYSTEMS 16
+
+int fnl_baoab_step_sums(double* x, double* v, float* x32,
+                        const float* grad_self, int64_t* grad_nbr, float force_scale,
+                        const double* dtm, double half_dt, double a1, const double* a2,
+                        const double* noise, uint64_t seed, uint64_t step,
+                        uint64_t* step_dev, int32_t n_atoms, int32_t clear_grad_nbr,
+                        const int32_t* sys_offsets, const int32_t* seg_offsets,
+                        int32_t n_sys, int32_t n_seg, int32_t seg_atoms,
+                        const float* energy_atom, double* energy_sys,
+                        const float* virial_atom, double* virial_sys,
+                        double* partials, fnl_stream_t stream);
+
 /* Batched BAOAB for many independent systems (continuous-batching server):
  * atoms with dtm == 0 are left untouched (systems not integrated this step);
  * the O-step noise of atom i in system b uses counter (i - sys_offsets[b],

@@ -55,6 +55,21 @@ def test_host_side_argument_checks():
     assert lib.fnl_fill_i32(None, 0, 0, None) == 0
     assert lib.fnl_fill_i32(None, 0, 5, None) == -1
     assert lib.fnl_species_mlp(None, 0, None, None, None, None, None, 0, None, None, 0, 0, 0, None, None) == -1
+    # the fused step + sums: more systems than its last workgroup sums, no segments (the pointers
+    # are never dereferenced: the host checks return first)
+    dummy = 64
+    args = [dummy] * 5 + [1.0, dummy, 0.5, 1.0, dummy, None, 1, 0, dummy, 10, 1, dummy, dummy]
+    tail = [512, dummy, dummy, dummy, dummy, dummy, None]
+    assert lib.fnl_baoab_step_sums(*args, 17, 17, *tail) == _abi.FNL_ERR_UNSUPPORTED
+    assert lib.fnl_baoab_step_sums(*args, 1, 0, *tail) == -1
+    assert lib.fnl_baoab_step_sums(*args[:-1], None, 1, 1, *tail) == -1  # no seg_offsets
+
+
+def test_header_constants_match_the_bindings():
+    text = HEADER.read_text()
+    for name in ("FNL_REDUCE_VALUES", "FNL_REDUCE_FUSED_MAX_SYSTEMS", "FNL_POOL_HEADER_WORDS"):
+        m = re.search(rf"#define {name} (\d+)", text)
+        assert m and int(m.group(1)) == getattr(_abi, name), name
 
 
 def _c_layout(struct: str, fields: list[str], tmp_path) -> dict:

@@ -181,7 +181,9 @@ def test_langevin_temperature(gpu):
 
 def test_md_graph_replay_matches_eager(gpu):
     """A captured MD step replays like the eager step (several replays, eager work in between):
-    the same positions, energies and virials bit for bit."""
+    the same positions, energies and virials bit for bit.  Each replayed step's integrator launch
+    forms the previous step's per-system sums, so after the replays the stored energies are one
+    step old until energies() forms the last step's."""
     print("[api] graph replay", flush=True)
     from fennol_amd.md import LangevinMD
     from fennol_amd.potential import ANIConfig, ANIPotential
@@ -202,12 +204,15 @@ def test_md_graph_replay_matches_eager(gpu):
                 torch.cuda.synchronize()
                 _ = md.x.sum().item()      # eager work between replays
                 traj.append(md.x.cpu().numpy().copy())
+            stale = md.sb.energy.cpu().numpy().copy()  # the sums of step 7
         else:
             traj = []
             md.step(check=False)
             md.step(check=False)
-            for _ in range(3):
+            for k in range(3):
                 md.step(check=False)
+                if k == 2:
+                    e7 = md.energies().copy()
                 md.step(check=False)
                 torch.cuda.synchronize()
                 traj.append(md.x.cpu().numpy().copy())
@@ -220,6 +225,8 @@ def test_md_graph_replay_matches_eager(gpu):
         np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(ee, eg)
     np.testing.assert_array_equal(ve, vg)
+    np.testing.assert_array_equal(stale, e7)
+    assert not np.array_equal(stale, eg)
 
 
 def test_md_skin_matches_rebuild_every_step(gpu):
