@@ -75,16 +75,31 @@ __device__ __forceinline__ float cos_switch_grad(float d, float rc) {
   return d < rc ? (-0.5f * kPi) * hw_rcp(rc) * hw_sinpi(d * hw_rcp(rc)) : 0.0f;
 }
 
-// map a linear index t < n(n-1)/2 to the pair (a, b), a < b, row-major
+// map a linear index t < n(n-1)/2 to the pair (a, b), a < b, row-major.  Row r starts at
+// start(r) = r n - r(r+1)/2; r is the root of start(r) = t rounded down, from a float square
+// root whose operand (2n-1)^2 - 8t is exact (n < 2^11) and whose result is within a few ulp, so
+// the estimate is off by at most one row and one correction each way is exact.
 __device__ __forceinline__ void tri_pair(int t, int n, int& a, int& b) {
   const float nn = (float)(2 * n - 1);
   int r = (int)floorf(0.5f * (nn - hw_sqrt(nn * nn - 8.0f * (float)t)));
   r = max(0, min(r, n - 2));
-  // start(r) = r*n - r(r+1)/2 ; fix rounding
-  while (r > 0 && r * n - (r * (r + 1)) / 2 > t) --r;
-  while (r < n - 2 && (r + 1) * n - ((r + 1) * (r + 2)) / 2 <= t) ++r;
+  if (r > 0 && r * n - (r * (r + 1)) / 2 > t) --r;
+  if (r < n - 2 && (r + 1) * n - ((r + 1) * (r + 2)) / 2 <= t) ++r;
   a = r;
   b = t - (r * n - (r * (r + 1)) / 2) + r + 1;
+}
+
+// map a linear index k < m(m+1)/2 to the block (A, B), A <= B, of the upper triangle of an
+// m x m table with its diagonal, row-major: row A starts at A m - A(A-1)/2 (same rounding
+// argument as tri_pair, m < 2^11).
+__device__ __forceinline__ void tri_block(int k, int m, int& A, int& B) {
+  const float mm = (float)(2 * m + 1);
+  int r = (int)floorf(0.5f * (mm - hw_sqrt(fmaxf(mm * mm - 8.0f * (float)k, 0.0f))));
+  r = max(0, min(r, m - 1));
+  if (r > 0 && r * m - (r * (r - 1)) / 2 > k) --r;
+  if (r < m - 1 && (r + 1) * m - ((r + 1) * r) / 2 <= k) ++r;
+  A = r;
+  B = r + (k - (r * m - (r * (r - 1)) / 2));
 }
 
 template <typename T>
@@ -811,14 +826,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   for (int kb0 = 0; kb0 < nblk; kb0 += 4) {
     const int kb = kb0 + (lane >> 4);
     int A = 0, B = 0;
-    if (kb < nblk) {
-      int r0 = 0;
-      while (kb >= r0 + (nb - A)) {
-        r0 += nb - A;
-        ++A;
-      }
-      B = A + (kb - r0);
-    }
+    if (kb < nblk) tri_block(kb, nb, A, B);
     const int a = 4 * A + (cc & 3), b = 4 * B + (cc >> 2);
     const bool valid = kb < nblk && a < b && b < na;
     float gax = 0.f, gay = 0.f, gaz = 0.f, gbx = 0.f, gby = 0.f, gbz = 0.f;
