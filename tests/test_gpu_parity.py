@@ -467,7 +467,7 @@ def test_energy_forces_bf16(gpu, name):
 
 
 def test_bf16_compensated_weights_remove_the_rounding_bias(gpu):
-    """BF16 weights stored as hi + lo pairs (bf16_compensated): the energy of the 7-species water216
+    """BF16 weights stored as hi + lo pairs (the default): the energy of the 7-species water216
     box is within 0.02 meV/atom of the FP64 oracle and at least 5 times closer than with plain
     BF16 weights (whose rounding shifts every similar atom the same way: measured -0.081 vs
     -0.004 meV/atom); forces keep the BF16 bar."""
