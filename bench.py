@@ -183,22 +183,25 @@ def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_ste
     eager_ms = 1000.0 * (time.perf_counter() - te0) / stage_steps
     pot.timer = None
     stages = timer.times_ms()
-    spg = steps_per_graph if skin <= 0 and steps % steps_per_graph == 0 else 1
+    spg = steps_per_graph if skin > 0 or steps % steps_per_graph == 0 else 1
     md.capture(steps_per_graph=spg)
     md.replay(1)
     torch.cuda.synchronize()
-    n_rep = steps // spg
-    step_ev = [torch.cuda.Event(enable_timing=True) for _ in range(n_rep + 1)]
+    step_ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    sizes = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     step_ev[0].record()
-    for k in range(n_rep):
-        md.replay(1)
-        step_ev[k + 1].record()
+    done = 0
+    while done < steps:  # one graph per replay: spg steps, or one rebuild / re-filter step
+        k = md.replay(1, max_steps=steps - done)
+        done += k
+        sizes.append(k)
+        step_ev[len(sizes)].record()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     md.verify()
-    lat = np.array([step_ev[k].elapsed_time(step_ev[k + 1]) / spg for k in range(n_rep)])
+    lat = np.array([step_ev[r].elapsed_time(step_ev[r + 1]) / k for r, k in enumerate(sizes)])
     return pot, md, (t1 - t0), lat, stages, eager_ms
 
 
@@ -595,8 +598,10 @@ def main() -> int:
         k_skin = max(args.steps, 2 * stride)
         log(f"[bench] MD bf16 with a {args.skin} A skin (rebuild every {stride} steps): {k_skin} steps")
         pot, md, wall, lat, stages, eager_ms = run_md(system, cfg, params, "bf16", k_skin, args.warmup,
-                                                      tile_rows=args.tile_bf16, skin=args.skin)
+                                                      tile_rows=args.tile_bf16, skin=args.skin,
+                                                      steps_per_graph=args.steps_per_graph)
         skin = {"nblist_skin_A": args.skin, "rebuild_stride": stride, "steps": k_skin,
+                "steps_per_graph": args.steps_per_graph,
                 "value": round(N * k_skin / wall, 1), "unit": "atom-steps/s",
                 "ms_per_step": round(1000.0 * wall / k_skin, 4),
                 "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),

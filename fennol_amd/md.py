@@ -125,6 +125,7 @@ class LangevinMD:
         self._sums_stale = False  # a replayed step leaves its per-system sums to the next one
         self.graph = None
         self.graph_skin = None
+        self.graph_skin_run = None
         self.steps_per_graph = 1
         self.x = torch.tensor(np.asarray(system["coordinates"], np.float64), device=dev).contiguous()
         self.x32 = self.x.float().contiguous()
@@ -261,9 +262,9 @@ class LangevinMD:
         while done < nsteps:
             n = min(K, nsteps - done)
             if graph:
-                if n % per:
+                if self.graph_skin is None and n % per:
                     raise ValueError(f"graphs hold {per} steps: nsteps must be a multiple of it")
-                self.replay(n // per)
+                self.replay(n, max_steps=n)
             else:
                 for _ in range(n):
                     self.step(check=False)
@@ -301,14 +302,16 @@ class LangevinMD:
     def capture(self, steps_per_graph: int = 1, warmup: bool = True) -> None:
         """Record MD steps in HIP graphs (no host checks inside; construct with check_every != 1
         and use :meth:`run` with ``graph=True`` for checked runs).  Without a skin list one
-        graph holds `steps_per_graph` steps; with one, two one-step graphs (full rebuild, skin
-        re-filter) are recorded and :meth:`replay` follows the rebuild schedule.  ``warmup``
-        runs one (two with a skin) ordinary steps first, outside the capture."""
+        graph holds `steps_per_graph` steps; with one, one-step graphs of a full rebuild and of
+        a skin re-filter are recorded, and for steps_per_graph > 1 also a graph of that many
+        re-filter steps, which :meth:`replay` uses wherever the rebuild schedule has that many
+        re-filters ahead.  ``warmup`` runs one (two with a skin) ordinary steps first, outside
+        the capture."""
         if self.check_every == 1:
             raise ValueError("graph capture needs check_every != 1 (no host reads inside a step)")
+        if steps_per_graph < 1:
+            raise ValueError("steps_per_graph must be at least 1")
         skin = self.nblist_stride > 1
-        if skin and steps_per_graph != 1:
-            raise ValueError("with a skin list every graph holds one step")
         timer, self.pot.timer = self.pot.timer, None
         if warmup:
             side = torch.cuda.Stream()
@@ -318,29 +321,51 @@ class LangevinMD:
                 if skin:
                     self.step(check=False)
             torch.cuda.current_stream().wait_stream(side)
-        kinds = (True, "skin") if skin else (True,)
+        # (name, neighbour-list action, steps)
+        kinds = [(True, True, 1), ("skin", "skin", 1)] if skin else [(True, True, steps_per_graph)]
+        if skin and steps_per_graph > 1:
+            kinds.append(("skin_run", "skin", steps_per_graph))
         graphs = {}
-        for kind in kinds:
+        for name, action, steps in kinds:
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                for _ in range(steps_per_graph):
-                    self._record(kind)
-            graphs[kind] = g
+                for _ in range(steps):
+                    self._record(action)
+            graphs[name] = g
         self.graph = graphs[True]
         self.graph_skin = graphs.get("skin")
+        self.graph_skin_run = graphs.get("skin_run")
         self.steps_per_graph = steps_per_graph
         self.pot.timer = timer
 
-    def replay(self, n_graphs: int = 1) -> None:
+    def replay(self, n_graphs: int = 1, max_steps: int | None = None) -> int:
+        """Replay ``n_graphs`` captured graphs, or fewer once ``max_steps`` steps ran; returns the
+        number of MD steps advanced.  With a skin list each graph follows the rebuild schedule
+        (:meth:`_graph_update`): a full rebuild or a re-filter step, or the graph of
+        steps_per_graph re-filters where the schedule has that many ahead and as many steps are
+        left."""
+        done = 0
+        S = self.steps_per_graph
         for _ in range(n_graphs):
+            left = None if max_steps is None else max_steps - done
+            if left is not None and left <= 0:
+                break
             if self.graph_skin is None:
                 self.graph.replay()
-                self.istep += self.steps_per_graph
+                k = S
+            elif (self.graph_skin_run is not None and self.istep >= self.nblist_warmup
+                  and self.nblist_countdown >= S and (left is None or left >= S)):
+                self.graph_skin_run.replay()  # S re-filters: what S calls of _graph_update give
+                self.nblist_countdown -= S
+                k = S
             else:
                 (self.graph if self._graph_update() is True else self.graph_skin).replay()
-                self.istep += 1
-        if n_graphs > 0 and self.sb.n_sys <= _abi.FNL_REDUCE_FUSED_MAX_SYSTEMS:
+                k = 1
+            self.istep += k
+            done += k
+        if done > 0 and self.sb.n_sys <= _abi.FNL_REDUCE_FUSED_MAX_SYSTEMS:
             self._sums_stale = True
+        return done
 
     def _current_sums(self) -> None:
         """Form the per-system sums of the last replayed step (its graph leaves them to the next

@@ -306,6 +306,46 @@ def test_md_skin_graph_replay_matches_eager(gpu):
         np.testing.assert_allclose(a, b, rtol=0, atol=1e-6)
 
 
+def test_md_skin_multi_step_graphs_match_eager(gpu):
+    """With a skin list and steps_per_graph=3, replay() uses the 3-re-filter graph where the
+    rebuild schedule (stride 5) has three re-filters ahead and one-step graphs elsewhere: the
+    positions after every replay and the final energies and virials equal the eager steps bit
+    for bit, and the step count and schedule end where the eager run's do."""
+    print("[api] skin multi-step graphs", flush=True)
+    from fennol_amd.md import LangevinMD
+    from fennol_amd.potential import ANIConfig, ANIPotential
+
+    s = water_box_small(64, 11)
+    cfg = ANIConfig(species_order=("H", "C", "N", "O", "S", "F", "Cl"))
+    params = O.init_mlp_params(11, cfg.aev_length)
+
+    def make():
+        pot = ANIPotential(cfg, params, precision="fp32")
+        return LangevinMD(pot, s, dt_fs=0.5, temperature=300.0, seed=7, check_every=0, nblist_skin=1.0,
+                          nblist_stride=5)
+
+    md = make()
+    eager = {}
+    for k in range(2 + 23):
+        md.step(check=False)
+        eager[k + 1] = md.x.cpu().numpy().copy()
+    e_ref, v_ref, c_ref = md.energies().copy(), md.virial_kcal().copy(), md.nblist_countdown
+
+    md = make()
+    md.capture(steps_per_graph=3)  # two eager warm-up steps
+    done, sizes = 0, []
+    while done < 23:
+        k = md.replay(1, max_steps=23 - done)
+        done += k
+        sizes.append(k)
+        np.testing.assert_array_equal(md.x.cpu().numpy(), eager[2 + done])
+    md.verify()
+    assert md.istep == 25 and md.nblist_countdown == c_ref
+    assert 3 in sizes and 1 in sizes
+    np.testing.assert_array_equal(md.energies(), e_ref)
+    np.testing.assert_array_equal(md.virial_kcal(), v_ref)
+
+
 def test_fnx_save_load_roundtrip(gpu, tmp_path):
     print("[api] fnx roundtrip", flush=True)
     s = systems.cluster_config0(6)
