@@ -669,11 +669,18 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
 }
 
 // per-wave LDS footprint (4-byte words) of the adjoint kernel; gs starts 16-byte aligned
-__host__ __device__ inline int bwd_gs_offset(int max_nbr, int aev_len) {
-  return round4(round4(aev_len) + 4 * max_nbr);
+// The region after G holds the radial slot vectors (4 words a slot) and, once the angular slots
+// are set up, the triplet terms of one pass of rounds (at most 128 slot entries of 3 words for
+// rounds of up to 32 pairs, one round of n2 <= max_nbr_ang + 1 slots beyond).
+__host__ __device__ inline int bwd_nbv_words(int max_nbr, int max_nbr_ang) {
+  const int t = 3 * (max_nbr_ang + 1) > 384 ? 3 * (max_nbr_ang + 1) : 384;
+  return round4(4 * max_nbr > t ? 4 * max_nbr : t);
+}
+__host__ __device__ inline int bwd_gs_offset(int max_nbr, int max_nbr_ang, int aev_len) {
+  return round4(aev_len) + bwd_nbv_words(max_nbr, max_nbr_ang);
 }
 __host__ __device__ inline int bwd_lds_words(int max_nbr, int max_nbr_ang, int aev_len) {
-  return bwd_gs_offset(max_nbr, aev_len) + 12 * max_nbr_ang;
+  return bwd_gs_offset(max_nbr, max_nbr_ang, aev_len) + 12 * max_nbr_ang;
 }
 
 // Adjoint, one wave per centre.  LDS slice of the wave (4-byte words):
@@ -710,7 +717,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const int words = bwd_lds_words(max_nbr, max_nbr_ang, dm.aev_len);
   float* G = (float*)lds4 + (size_t)w * words;
   float4* nbv = (float4*)(G + ((dm.aev_len + 3) & ~3));
-  float4* gs = (float4*)(G + bwd_gs_offset(max_nbr, dm.aev_len));
+  float4* gs = (float4*)(G + bwd_gs_offset(max_nbr, max_nbr_ang, dm.aev_len));
   float4* av = gs + max_nbr_ang;
   float4* af = av + max_nbr_ang;
 
@@ -814,93 +821,111 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   wave_sync();
 
   const float pref = 2.0f * exp2f(-p.zeta);
-  const int T = na * (na - 1) / 2;
   const float* Gang = G + dm.rad_len;
-  // Triplets in 4x4 blocks of the (a, b) table: a row of 16 lanes holds block (A, B), lane c
-  // the pair a = 4A + c%4, b = 4B + c/4 (a < b).  The lanes that share a (c%4 equal) and the
-  // lanes that share b (c/4 equal) pre-sum their dE/dvec terms with DPP moves, so that only one
-  // lane per slot and block adds to gs (LDS float adds cost in proportion to the active lanes).
-  const int nb = (na + 3) >> 2;
-  const int nblk = nb * (nb + 1) / 2;
-  const int cc = lane & 15;
-  for (int kb0 = 0; kb0 < nblk; kb0 += 4) {
-    const int kb = kb0 + (lane >> 4);
-    int A = 0, B = 0;
-    if (kb < nblk) tri_block(kb, nb, A, B);
-    const int a = 4 * A + (cc & 3), b = 4 * B + (cc >> 2);
-    const bool valid = kb < nblk && a < b && b < na;
-    float gax = 0.f, gay = 0.f, gaz = 0.f, gbx = 0.f, gby = 0.f, gbz = 0.f;
-    if (valid) {
-      const float4 ua = av[a], ub = av[b];
-      const float4 fa = af[a], fb = af[b];
-      const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
-      const float x = 0.95f * c;
-      const float st = hw_sqrt(fmaxf(1.0f - x * x, 1.0e-12f));
-      const float ist = hw_rcp(st);
-      float F[NZ], dF[NZ];
-#pragma unroll
-      for (int z = 0; z < NZ; ++z) {
-        // F = b^zeta and d/dc b^zeta = 0.95 zeta b^(zeta-1) (cos_z - x sin_z / st), b = 1 + cos(theta -
-        // theta_z), theta = arccos(0.95 c); b clamped at 1e-30 (both vanish there)
-        const float lb = hw_log2(fmaxf(1.0f + x * p.ang_cos[z] + st * p.ang_sin[z], 1.0e-30f));
-        const float tz = p.zeta * lb;
-        F[z] = hw_exp2(tz);
-        dF[z] = (0.95f * p.zeta) * hw_exp2(tz - lb) * (p.ang_cos[z] - x * p.ang_sin[z] * ist);
+  // Triplets in the rounds of the circle method: the slots, padded to an even count n2 (slot na
+  // is a dummy when na is odd), are paired n2/2 at a time so that each slot is in exactly one
+  // pair of a round, and the n2 - 1 rounds hold every pair once.  Round r pairs slot n2 - 1 with
+  // r and slots (r + k) mod (n2 - 1), (r - k) mod (n2 - 1) for k = 1 .. n2/2 - 1.  A pass holds
+  // rpc = 64 / (n2/2) whole rounds, so all lanes but 64 mod (n2/2) evaluate a triplet.  Each lane
+  // stores the dE/dvec terms of its two slots at tmp[round][slot] (plain stores: a slot occurs
+  // once in a round), then lanes (slot, component) add the pass's rounds to gs in round order --
+  // no LDS atomics and the same order every run.  With more than 64 pairs in a round (na > 128)
+  // a round is split over passes.  tmp reuses the radial slot vectors' space (dead here).
+  float* tmp = (float*)nbv;
+  const int n2 = (na + 1) & ~1;
+  const int P = n2 >> 1;
+  const int n_rounds = n2 - 1;
+  const int rpc = P > 0 ? max(1, 64 / P) : 1;
+  const int lane_round = P > 0 ? lane / P : 0, lane_pair = lane - lane_round * P;
+  for (int r0 = 0; r0 < n_rounds; r0 += rpc) {
+    for (int k0 = 0; k0 < (rpc > 1 ? 1 : P); k0 += 64) {
+      const int rl = rpc > 1 ? lane_round : 0;
+      const int kk = rpc > 1 ? lane_pair : k0 + lane;
+      const int r = r0 + rl;
+      const bool live = rl < rpc && r < n_rounds && kk < P;
+      int q1 = 0, q2 = 0;
+      if (live) {
+        if (kk == 0) {
+          q1 = n2 - 1;
+          q2 = r;
+        } else {
+          q1 = r + kk;
+          if (q1 >= n_rounds) q1 -= n_rounds;
+          q2 = r - kk;
+          if (q2 < 0) q2 += n_rounds;
+        }
       }
-      const float m = 0.5f * (ua.w + ub.w);
-      const float* gp = Gang + pair_index(__float_as_int(fa.z), __float_as_int(fb.z), dm.S) * (NA * NZ);
-      float S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
-#pragma unroll
-      for (int d = 0; d < NA; ++d) {
-        const float y = m - p.ang_shifts[d];
-        const float R = gauss(y, p.angular_eta);
-        float gz[NZ];
-#pragma unroll
-        for (int q = 0; q < NZ; q += 4) *(float4*)(gz + q) = *(const float4*)(gp + d * NZ + q);
-        float gf = 0.0f, gdf = 0.0f;
+      const int a = min(q1, q2), b = max(q1, q2);
+      const bool valid = live && b < na;
+      float gax = 0.f, gay = 0.f, gaz = 0.f, gbx = 0.f, gby = 0.f, gbz = 0.f;
+      if (valid) {
+        const float4 ua = av[a], ub = av[b];
+        const float4 fa = af[a], fb = af[b];
+        const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
+        const float x = 0.95f * c;
+        const float st = hw_sqrt(fmaxf(1.0f - x * x, 1.0e-12f));
+        const float ist = hw_rcp(st);
+        float F[NZ], dF[NZ];
 #pragma unroll
         for (int z = 0; z < NZ; ++z) {
-          gf += gz[z] * F[z];
-          gdf += gz[z] * dF[z];
+          // F = b^zeta and d/dc b^zeta = 0.95 zeta b^(zeta-1) (cos_z - x sin_z / st), b = 1 + cos(theta -
+          // theta_z), theta = arccos(0.95 c); b clamped at 1e-30 (both vanish there)
+          const float lb = hw_log2(fmaxf(1.0f + x * p.ang_cos[z] + st * p.ang_sin[z], 1.0e-30f));
+          const float tz = p.zeta * lb;
+          F[z] = hw_exp2(tz);
+          dF[z] = (0.95f * p.zeta) * hw_exp2(tz - lb) * (p.ang_cos[z] - x * p.ang_sin[z] * ist);
         }
-        S0 += gf * R;
-        S1 += gf * R * (-2.0f * p.angular_eta * y);
-        S2 += gdf * R;
+        const float m = 0.5f * (ua.w + ub.w);
+        const float* gp = Gang + pair_index(__float_as_int(fa.z), __float_as_int(fb.z), dm.S) * (NA * NZ);
+        float S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d < NA; ++d) {
+          const float y = m - p.ang_shifts[d];
+          const float R = gauss(y, p.angular_eta);
+          float gz[NZ];
+#pragma unroll
+          for (int q = 0; q < NZ; q += 4) *(float4*)(gz + q) = *(const float4*)(gp + d * NZ + q);
+          float gf = 0.0f, gdf = 0.0f;
+#pragma unroll
+          for (int z = 0; z < NZ; ++z) {
+            gf += gz[z] * F[z];
+            gdf += gz[z] * dF[z];
+          }
+          S0 += gf * R;
+          S1 += gf * R * (-2.0f * p.angular_eta * y);
+          S2 += gdf * R;
+        }
+        const float sw = pref * fa.x * fb.x;
+        const float dEda = pref * fa.y * fb.x * S0 + 0.5f * sw * S1;
+        const float dEdb = pref * fa.x * fb.y * S0 + 0.5f * sw * S1;
+        const float dEdc = sw * S2;
+        const int ea = __float_as_int(fa.w), eb = __float_as_int(fb.w);
+        const float ra = (ea >> 30) ? 0.0f : dEda, rb = (eb >> 30) ? 0.0f : dEdb;
+        const float ia = hw_rcp(fmaxf(ua.w, 1.0e-5f)), ib = hw_rcp(fmaxf(ub.w, 1.0e-5f));
+        gax = ra * ua.x + dEdc * (ub.x - c * ua.x) * ia;
+        gay = ra * ua.y + dEdc * (ub.y - c * ua.y) * ia;
+        gaz = ra * ua.z + dEdc * (ub.z - c * ua.z) * ia;
+        gbx = rb * ub.x + dEdc * (ua.x - c * ub.x) * ib;
+        gby = rb * ub.y + dEdc * (ua.y - c * ub.y) * ib;
+        gbz = rb * ub.z + dEdc * (ua.z - c * ub.z) * ib;
       }
-      const float sw = pref * fa.x * fb.x;
-      const float dEda = pref * fa.y * fb.x * S0 + 0.5f * sw * S1;
-      const float dEdb = pref * fa.x * fb.y * S0 + 0.5f * sw * S1;
-      const float dEdc = sw * S2;
-      const int ea = __float_as_int(fa.w), eb = __float_as_int(fb.w);
-      const float ra = (ea >> 30) ? 0.0f : dEda, rb = (eb >> 30) ? 0.0f : dEdb;
-      const float ia = hw_rcp(fmaxf(ua.w, 1.0e-5f)), ib = hw_rcp(fmaxf(ub.w, 1.0e-5f));
-      gax = ra * ua.x + dEdc * (ub.x - c * ua.x) * ia;
-      gay = ra * ua.y + dEdc * (ub.y - c * ua.y) * ia;
-      gaz = ra * ua.z + dEdc * (ub.z - c * ua.z) * ia;
-      gbx = rb * ub.x + dEdc * (ua.x - c * ub.x) * ib;
-      gby = rb * ub.y + dEdc * (ua.y - c * ub.y) * ib;
-      gbz = rb * ub.z + dEdc * (ua.z - c * ub.z) * ib;
+      if (live) {  // every slot of the round, the dummy included (zeros)
+        float* ta = tmp + (rl * n2 + a) * 3;
+        float* tb = tmp + (rl * n2 + b) * 3;
+        ta[0] = gax; ta[1] = gay; ta[2] = gaz;
+        tb[0] = gbx; tb[1] = gby; tb[2] = gbz;
+      }
     }
-    // lanes c, c+4, c+8, c+12 share a; lanes 4k..4k+3 share b
-    gax += dpp<0x124>(gax); gay += dpp<0x124>(gay); gaz += dpp<0x124>(gaz);
-    gax += dpp<0x128>(gax); gay += dpp<0x128>(gay); gaz += dpp<0x128>(gaz);
-    gbx += dpp<0xB1>(gbx); gby += dpp<0xB1>(gby); gbz += dpp<0xB1>(gbz);
-    gbx += dpp<0x4E>(gbx); gby += dpp<0x4E>(gby); gbz += dpp<0x4E>(gbz);
-    // one lane per slot and block, and only where the block holds a valid triplet of the slot
-    if (kb < nblk && cc < 4 && max(a + 1, 4 * B) <= min(4 * B + 3, na - 1)) {
-      float* pa = (float*)(gs + a);
-      lds_add(pa + 0, gax);
-      lds_add(pa + 1, gay);
-      lds_add(pa + 2, gaz);
+    wave_sync();
+    const int n_pass = min(rpc, n_rounds - r0);
+    for (int q = lane; q < 3 * na; q += 64) {
+      const int sl = q / 3, c = q - 3 * sl;
+      float acc = 0.0f;
+      for (int rl = 0; rl < n_pass; ++rl) acc += tmp[(rl * n2 + sl) * 3 + c];
+      ((float*)(gs + sl))[c] += acc;
     }
-    if (kb < nblk && (cc & 3) == 0 && b < na && b > 4 * A) {
-      float* pb = (float*)(gs + b);
-      lds_add(pb + 0, gbx);
-      lds_add(pb + 1, gby);
-      lds_add(pb + 2, gbz);
-    }
+    wave_sync();
   }
-  wave_sync();
 
   // angular terms: scatter to the neighbour atoms, add to the centre's sums and virial
   for (int a = lane; a < na; a += 64) {
