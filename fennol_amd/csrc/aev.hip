@@ -920,8 +920,10 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
     const int n_pass = min(rpc, n_rounds - r0);
     for (int q = lane; q < 3 * na; q += 64) {
       const int sl = q / 3, c = q - 3 * sl;
+      const float* tq = tmp + q;  // entry (round 0, slot sl, component c); rounds 3 n2 apart
       float acc = 0.0f;
-      for (int rl = 0; rl < n_pass; ++rl) acc += tmp[(rl * n2 + sl) * 3 + c];
+#pragma unroll 4
+      for (int rl = 0; rl < n_pass; ++rl, tq += 3 * n2) acc += *tq;
       ((float*)(gs + sl))[c] += acc;
     }
     wave_sync();
