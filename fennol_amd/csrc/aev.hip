@@ -686,10 +686,10 @@ __host__ __device__ inline int bwd_lds_words(int max_nbr, int max_nbr_ang, int a
 // Adjoint, one wave per centre.  LDS slice of the wave (4-byte words):
 //   G   [aev_len]      dE/dAEV row of the centre
 //   nbv [max_nbr]      float4 (vx, vy, vz, species bits) per radial slot
-//   gs  [max_nbr_ang]  float4 dE/dvec of the angular slot, angular terms only
-//                      (LDS float adds)
+//   gs  [max_nbr_ang]  float4 dE/dvec of the angular slot, angular terms only (xyz), neighbour atom (w)
+//                      (per-pass round sums)
 //   av  [max_nbr_ang]  float4 (ux, uy, uz, d) per angular slot
-//   af  [max_nbr_ang]  float4 (fc_ang, fc_ang', species, neighbour atom | degenerate bit)
+//   af  [max_nbr_ang]  float4 (fc_ang, fc_ang', species, 1/d; negative for a degenerate slot)
 // Radial terms: the lane of slot e (neighbour j) evaluates both dE_i/dvec_ij (from the
 // centre's row G_i: self term and virial of i) and dE_j/dvec_ji (from j's row G_j, read from
 // global memory: the neighbour term that j's own radial block gives i), so the radial
@@ -813,14 +813,17 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       const float inv = hw_rcp(fmaxf(d, 1.0e-5f));
       av[a] = make_float4(vv.x * inv, vv.y * inv, vv.z * inv, d);
       const bool ok = d2 > 1.0e-5f;
+      // af.w: 1/d, negative for a degenerate slot (coincident atoms: no radial term)
       af[a] = make_float4(cos_switch(d, p.cutoff_ang), ok ? cos_switch_grad(d, p.cutoff_ang) : 0.0f, vv.w,
-                          __int_as_float(row[e] | (ok ? 0 : (1 << 30))));
-      gs[a] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                          ok ? inv : -inv);
+      gs[a] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(row[e]));  // .w: the neighbour atom
     }
   }
   wave_sync();
 
   const float pref = 2.0f * exp2f(-p.zeta);
+  const float zm1 = p.zeta - 1.0f;
+  const float dpref = (0.95f * p.zeta) * pref;  // d/dc of (1 + cos(theta - theta_z))^zeta: 0.95 zeta b^(zeta-1) (..)
   const float* Gang = G + dm.rad_len;
   // Triplets in the rounds of the circle method: the slots, padded to an even count n2 (slot na
   // is a dummy when na is odd), are paired n2/2 at a time so that each slot is in exactly one
@@ -864,19 +867,22 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
         const float c = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
         const float x = 0.95f * c;
         const float st = hw_sqrt(fmaxf(1.0f - x * x, 1.0e-12f));
-        const float ist = hw_rcp(st);
+        const float xi = x * hw_rcp(st);
+        // b = 1 + cos(theta - theta_z), theta = arccos(0.95 c), clamped at 1e-30 (where every term
+        // vanishes): F = b^zeta = b^(zeta-1) b, and d/dc b^zeta = 0.95 zeta b^(zeta-1) (cos_z - x sin_z
+        // / st), whose factor 0.95 zeta is applied once per triplet (dpref); one log2 and one exp2 per
+        // section
         float F[NZ], dF[NZ];
 #pragma unroll
         for (int z = 0; z < NZ; ++z) {
-          // F = b^zeta and d/dc b^zeta = 0.95 zeta b^(zeta-1) (cos_z - x sin_z / st), b = 1 + cos(theta -
-          // theta_z), theta = arccos(0.95 c); b clamped at 1e-30 (both vanish there)
-          const float lb = hw_log2(fmaxf(1.0f + x * p.ang_cos[z] + st * p.ang_sin[z], 1.0e-30f));
-          const float tz = p.zeta * lb;
-          F[z] = hw_exp2(tz);
-          dF[z] = (0.95f * p.zeta) * hw_exp2(tz - lb) * (p.ang_cos[z] - x * p.ang_sin[z] * ist);
+          const float bz = fmaxf(1.0f + x * p.ang_cos[z] + st * p.ang_sin[z], 1.0e-30f);
+          const float pm1 = hw_exp2(zm1 * hw_log2(bz));
+          F[z] = pm1 * bz;
+          dF[z] = pm1 * (p.ang_cos[z] - xi * p.ang_sin[z]);
         }
         const float m = 0.5f * (ua.w + ub.w);
         const float* gp = Gang + pair_index(__float_as_int(fa.z), __float_as_int(fb.z), dm.S) * (NA * NZ);
+        // S0 = sum_d R_d g.F, S1 = sum_d y_d R_d g.F (dR_d/dm = -2 eta y_d R_d), S2 = sum_d R_d g.dF
         float S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
 #pragma unroll
         for (int d = 0; d < NA; ++d) {
@@ -891,17 +897,19 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
             gf += gz[z] * F[z];
             gdf += gz[z] * dF[z];
           }
-          S0 += gf * R;
-          S1 += gf * R * (-2.0f * p.angular_eta * y);
+          const float gR = gf * R;
+          S0 += gR;
+          S1 += gR * y;
           S2 += gdf * R;
         }
-        const float sw = pref * fa.x * fb.x;
-        const float dEda = pref * fa.y * fb.x * S0 + 0.5f * sw * S1;
-        const float dEdb = pref * fa.x * fb.y * S0 + 0.5f * sw * S1;
-        const float dEdc = sw * S2;
-        const int ea = __float_as_int(fa.w), eb = __float_as_int(fb.w);
-        const float ra = (ea >> 30) ? 0.0f : dEda, rb = (eb >> 30) ? 0.0f : dEdb;
-        const float ia = hw_rcp(fmaxf(ua.w, 1.0e-5f)), ib = hw_rcp(fmaxf(ub.w, 1.0e-5f));
+        // dE/dd_a = pref fb (fa' S0 + fa dS0/dm / 2), with dS0/dm = -2 eta S1; dE/dc = 0.95 zeta pref fa fb S2
+        const float S1e = p.angular_eta * S1;
+        const float dEda = pref * fb.x * (fa.y * S0 - fa.x * S1e);
+        const float dEdb = pref * fa.x * (fb.y * S0 - fb.x * S1e);
+        const float dEdc = dpref * fa.x * fb.x * S2;
+        // 1/d of the slots (af.w), negative for a degenerate slot, whose radial term is dropped
+        const float ra = fa.w < 0.0f ? 0.0f : dEda, rb = fb.w < 0.0f ? 0.0f : dEdb;
+        const float ia = fabsf(fa.w), ib = fabsf(fb.w);
         gax = ra * ua.x + dEdc * (ub.x - c * ua.x) * ia;
         gay = ra * ua.y + dEdc * (ub.y - c * ua.y) * ia;
         gaz = ra * ua.z + dEdc * (ub.z - c * ua.z) * ia;
@@ -933,7 +941,7 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   for (int a = lane; a < na; a += 64) {
     const float4 gg = gs[a];
     const float4 ua = av[a];
-    const int j = __float_as_int(af[a].w) & 0x3fffffff;
+    const int j = __float_as_int(gg.w);
     unsigned long long* gn = reinterpret_cast<unsigned long long*>(grad_nbr + 3 * j);
     __hip_atomic_fetch_add(gn, fnl_to_fixed(gg.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_fetch_add(gn + 1, fnl_to_fixed(gg.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
