@@ -150,12 +150,15 @@ __device__ __forceinline__ void zero_acc(f4 (&acc)[NB][MB]) {
 }
 
 // acc[nb][mb] += sum_k W(n, k) act(k, m) over k in [0, K) for the n-blocks
-// nb0 + WV*nb (< ncb) of this wave.  W prefetched PF steps ahead.
+// nb0 + WV*nb (< ncb) of this wave.  W prefetched PF steps ahead.  With pieces, those indices
+// count a list of 16-column n-blocks instead: index j is n-block 2 pieces[j / 2] + j % 2 (the two
+// halves of 32-column piece pieces[j / 2]).
 // COMP: the weights are the sum of two BF16 matrices (value and rounding remainder, Wlo), each
 // multiplied on the MFMA: the product keeps ~16 significant bits of W instead of 8.
 template <typename T, int NB, int MB, int PF, int WV, bool COMP = false>
 __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ W, int ldw, int nb0, int ncb,
-                                       const T* act, int lda, int K, const T* __restrict__ Wlo = nullptr) {
+                                       const T* act, int lda, int K, const T* __restrict__ Wlo = nullptr,
+                                       const int* pieces = nullptr) {
   typedef Tr<T> tr;
   const int nsteps = K / tr::KS;
   // Every prefetch load is issued unconditionally (k-step and n-block clamped into range), so
@@ -163,7 +166,10 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
   // conditional load makes it wait for all of them.
   int ncol[NB];
 #pragma unroll
-  for (int nb = 0; nb < NB; ++nb) ncol[nb] = min(nb0 + WV * nb, ncb - 1) * 16;
+  for (int nb = 0; nb < NB; ++nb) {
+    const int j = min(nb0 + WV * nb, ncb - 1);
+    ncol[nb] = (pieces ? 2 * pieces[j >> 1] + (j & 1) : j) * 16;
+  }
   typename tr::frag wq[PF][NB];
   typename tr::frag wl[COMP ? PF : 1][NB];
 #pragma unroll
@@ -323,13 +329,14 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   constexpr int SPP = PW / tr::KS;  // k-steps per piece
   static_assert(KC % PW == 0 && PW % tr::KS == 0 && PW % VEC == 0, "pieces tile the chunks");
   __shared__ int live_piece[FNL_MLP_MAX_CHUNKS * 4];
-  __shared__ int n_chunks_sh;
+  __shared__ int n_chunks_sh, n_live_sh;
   if (t == 0) {
     int n = 0, dead = -1;
     for (int q = 0; q < K / PW; ++q) {
       if (q >= 64 || ((args.k_live >> q) & 1ull)) live_piece[n++] = q;
       else if (dead < 0) dead = q;
     }
+    n_live_sh = n;  // the live pieces: the columns whose dE/dAEV is written
     if (n == 0) live_piece[n++] = dead;  // no live column: one zero piece, the bias-only result
     while (n % PPC) live_piece[n++] = dead;  // a dead piece exists unless every piece is live
     n_chunks_sh = n / PPC;
@@ -516,46 +523,44 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     const int m = mb * 16 + (l & 15);
     atom[mb] = m < nrows ? perm[r0 + m] : -1;
   }
-  const int ncb_x = K / 16;
-  // BF16 rows: each pass's 256 columns of the tile go through LDS (the H2/H3 buffers, free now)
-  // so that the rows leave as contiguous 16-byte chunks
+  // the live 16-column n-blocks of dE/dAEV (two per live piece), in passes of 16
+  const int nlb = 2 * n_live_sh;
+  // BF16 rows: each pass's 16 n-blocks of the tile go through LDS (the H2/H3 buffers, free now)
+  // so that the rows leave as 16-byte chunks
   constexpr int GLD = 256 + 8;
   static_assert(M * GLD * 2 <= L::OFFE - L::OFF2, "BF16 gradient staging fits the H2/H3 buffers");
   unsigned short* gst = (unsigned short*)(smem + L::OFF2);
-  for (int base = 0; base < ncb_x; base += 16) {
-    // input columns [16 base, 16 base + 256): skipped when no atom of the batch has a non-zero
-    // there (their dE/dAEV is never read: the adjoint reads the live blocks only)
-    const int q0 = base / 2;  // first 32-column piece of the pass
-    if (q0 + 8 <= 64 && ((args.k_live >> q0) & 0xFFull) == 0ull) continue;
+  for (int base = 0; base < nlb; base += 16) {
     f4 accx[NB1][MB];
     zero_acc(accx);
-    gemm_t<T, NB1, MB, PF, WV, COMP>(accx, (const T*)P.bw[0], sizeof(T) == 2 ? H1 : K, base + wave, ncb_x, B1,
-                                     L::LD1, H1, (const T*)P.bw_lo[0]);
+    gemm_t<T, NB1, MB, PF, WV, COMP>(accx, (const T*)P.bw[0], sizeof(T) == 2 ? H1 : K, base + wave, nlb, B1,
+                                     L::LD1, H1, (const T*)P.bw_lo[0], live_piece);
     if (grad_bf16) {
 #pragma unroll
       for (int nb = 0; nb < NB1; ++nb) {
-        const int cb = base + wave + WV * nb;
-        if (cb >= ncb_x) continue;
+        if (base + wave + WV * nb >= nlb) continue;
         const int nl = (wave + WV * nb) * 16 + 4 * (l >> 4);
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) Tr<unsigned short>::store4(gst + (mb * 16 + (l & 15)) * GLD + nl, accx[nb][mb]);
       }
       __syncthreads();
-      const int cpr = min(256, K - base * 16) / 8;  // 16-byte chunks per row in this pass
-      unsigned short* gout = (unsigned short*)grad + base * 16;
+      const int cpr = 2 * min(16, nlb - base);  // 16-byte chunks per row in this pass: two per n-block
+      unsigned short* gout = (unsigned short*)grad;
       for (int idx = t; idx < M * cpr; idx += 64 * WV) {
         const int m = idx / cpr, q = idx - m * cpr;
+        const int j = base + (q >> 1);
+        const int col = (2 * live_piece[j >> 1] + (j & 1)) * 16 + (q & 1) * 8;
         if (m < nrows)
           __builtin_nontemporal_store(*(const u4v*)(gst + m * GLD + q * 8),
-                                      (u4v*)(gout + (size_t)perm[r0 + m] * grad_stride + q * 8));
+                                      (u4v*)(gout + (size_t)perm[r0 + m] * grad_stride + col));
       }
       __syncthreads();
     } else {
 #pragma unroll
       for (int nb = 0; nb < NB1; ++nb) {
-        const int cb = base + wave + WV * nb;
-        if (cb >= ncb_x) continue;
-        const int n = cb * 16 + 4 * (l >> 4);
+        const int j = base + wave + WV * nb;
+        if (j >= nlb) continue;
+        const int n = (2 * live_piece[j >> 1] + (j & 1)) * 16 + 4 * (l >> 4);
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
           if (atom[mb] >= 0)

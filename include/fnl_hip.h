@@ -359,9 +359,9 @@ int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_mlp_weights* 
  * (fnl_pool_layout) and launches a HIP graph replays with changing tiles. */
 /* k_live: bit q set if input columns [32q, 32q + 32) may be non-zero for some atom
  * of the batch (0 = all).  The other columns must be zero in every AEV row: the
- * kernel skips their products (exact zeros) and writes dE/dAEV only in the
- * 256-column passes that hold a live column (fnl_aev_backward with the matching
- * species_mask never reads the others).
+ * kernel skips their products (exact zeros) and writes dE/dAEV only in the live
+ * 32-column pieces, leaving the others as they are (fnl_aev_backward with the
+ * matching species_mask never reads them).
  * Columns at and beyond 2048 are always treated as live. */
 
 /* ------------------------------------------------------------------------
