@@ -330,16 +330,22 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   static_assert(KC % PW == 0 && PW % tr::KS == 0 && PW % VEC == 0, "pieces tile the chunks");
   __shared__ int live_piece[FNL_MLP_MAX_CHUNKS * 4];
   __shared__ int n_chunks_sh, n_live_sh;
-  if (t == 0) {
-    int n = 0, dead = -1;
-    for (int q = 0; q < K / PW; ++q) {
-      if (q >= 64 || ((args.k_live >> q) & 1ull)) live_piece[n++] = q;
-      else if (dead < 0) dead = q;
+  if (t < 64) {  // wave 0 compacts the list with a ballot (pieces from 64 on are always live)
+    const int nq = K / PW;
+    const bool lv = t < nq && ((args.k_live >> t) & 1ull);
+    const unsigned long long m = __ballot(lv);
+    if (lv) live_piece[fnl_rank_below(m)] = t;
+    int n = __popcll(m);
+    for (int q = 64 + t; q < nq; q += 64) live_piece[n + q - 64] = q;
+    n += max(nq - 64, 0);
+    if (t == 0) {
+      n_live_sh = n;  // the live pieces: the columns whose dE/dAEV is written
+      const unsigned long long dead_m = ~m & (nq >= 64 ? ~0ull : (1ull << nq) - 1ull);
+      const int dead = dead_m ? __builtin_ctzll(dead_m) : -1;
+      if (n == 0) live_piece[n++] = dead;  // no live column: one zero piece, the bias-only result
+      while (n % PPC) live_piece[n++] = dead;  // a dead piece exists unless every piece is live
+      n_chunks_sh = n / PPC;
     }
-    n_live_sh = n;  // the live pieces: the columns whose dE/dAEV is written
-    if (n == 0) live_piece[n++] = dead;  // no live column: one zero piece, the bias-only result
-    while (n % PPC) live_piece[n++] = dead;  // a dead piece exists unless every piece is live
-    n_chunks_sh = n / PPC;
   }
   __syncthreads();
   const int nlive = n_chunks_sh;
