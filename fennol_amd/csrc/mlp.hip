@@ -310,6 +310,9 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   constexpr int RPP = 64 * WV / TPR;           // rows per pass
   constexpr int NPASS = M / RPP;
   static_assert(M % RPP == 0, "staging passes");
+  // the tile's atoms, read once: the energy and dE/dAEV stores index them (no global load there)
+  __shared__ int atom_sh[M];
+  if (t < M) atom_sh[t] = t < nrows ? perm[r0 + t] : -1;
   int srow[NPASS];
   const int row_safe = perm[r0];
   const int scol = (t % TPR) * VEC;
@@ -478,7 +481,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     float e = 0.0f;
 #pragma unroll
     for (int cb = 0; cb < H3 / 16; ++cb) e += Epart[cb * M + t];
-    energy[perm[r0 + t]] = e + P.b[3][0];
+    energy[atom_sh[t]] = e + P.b[3][0];
   }
   if (grad == nullptr) return;
   __syncthreads();  // energy partials read: B3 takes G3
@@ -527,7 +530,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb) {
     const int m = mb * 16 + (l & 15);
-    atom[mb] = m < nrows ? perm[r0 + m] : -1;
+    atom[mb] = atom_sh[m];
   }
   // the live 16-column n-blocks of dE/dAEV (two per live piece), in passes of 16
   const int nlb = 2 * n_live_sh;
@@ -558,7 +561,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
         const int col = (2 * live_piece[j >> 1] + (j & 1)) * 16 + (q & 1) * 8;
         if (m < nrows)
           __builtin_nontemporal_store(*(const u4v*)(gst + m * GLD + q * 8),
-                                      (u4v*)(gout + (size_t)perm[r0 + m] * grad_stride + col));
+                                      (u4v*)(gout + (size_t)atom_sh[m] * grad_stride + col));
       }
       __syncthreads();
     } else {
