@@ -149,36 +149,50 @@ __device__ __forceinline__ void zero_acc(f4 (&acc)[NB][MB]) {
     for (int j = 0; j < MB; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 }
 
-// acc[nb][mb] += sum_k W(n, k) act(k, m) over k in [0, K) for the n-blocks
-// nb0 + WV*nb (< ncb) of this wave.  W prefetched PF steps ahead.  With pieces, those indices
-// count a list of 16-column n-blocks instead: index j is n-block 2 pieces[j / 2] + j % 2 (the two
-// halves of 32-column piece pieces[j / 2]).
-// COMP: the weights are the sum of two BF16 matrices (value and rounding remainder, Wlo), each
-// multiplied on the MFMA: the product keeps ~16 significant bits of W instead of 8.
-template <typename T, int NB, int MB, int PF, int WV, bool COMP = false>
-__device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ W, int ldw, int nb0, int ncb,
-                                       const T* act, int lda, int K, const T* __restrict__ Wlo = nullptr,
-                                       const int* pieces = nullptr) {
+// The weight fragments of a GEMM's first PF k-steps (A operands of this wave's n-blocks), loaded
+// ahead: they do not depend on the activations, so the caller issues them before the previous
+// layer's epilogue and barrier, whose time then covers their latency.
+template <typename T, int NB, int PF, bool COMP>
+struct WRing {
+  typename Tr<T>::frag q[PF][NB];
+  typename Tr<T>::frag lo[COMP ? PF : 1][NB];
+  int ncol[NB];
+};
+
+// n-blocks nb0 + WV*nb (< ncb) of this wave.  With pieces, those indices count a list of
+// 16-column n-blocks instead: index j is n-block 2 pieces[j / 2] + j % 2 (the two halves of
+// 32-column piece pieces[j / 2]).  Every prefetch load is issued unconditionally (k-step and
+// n-block clamped into range), so that the compiler can count the loads in flight and wait only
+// for the one it needs; a conditional load makes it wait for all of them.
+template <typename T, int NB, int PF, int WV, bool COMP>
+__device__ __forceinline__ void ring_load(WRing<T, NB, PF, COMP>& r, const T* __restrict__ W, int ldw, int nb0,
+                                          int ncb, int K, const T* __restrict__ Wlo, const int* pieces = nullptr) {
   typedef Tr<T> tr;
   const int nsteps = K / tr::KS;
-  // Every prefetch load is issued unconditionally (k-step and n-block clamped into range), so
-  // that the compiler can count the loads in flight and wait only for the one it needs; a
-  // conditional load makes it wait for all of them.
-  int ncol[NB];
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
     const int j = min(nb0 + WV * nb, ncb - 1);
-    ncol[nb] = (pieces ? 2 * pieces[j >> 1] + (j & 1) : j) * 16;
+    r.ncol[nb] = (pieces ? 2 * pieces[j >> 1] + (j & 1) : j) * 16;
   }
-  typename tr::frag wq[PF][NB];
-  typename tr::frag wl[COMP ? PF : 1][NB];
 #pragma unroll
   for (int u = 0; u < PF; ++u)
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-      wq[u][nb] = tr::wload(W, ldw, ncol[nb], min(u, nsteps - 1) * tr::KS);
-      if constexpr (COMP) wl[u][nb] = tr::wload(Wlo, ldw, ncol[nb], min(u, nsteps - 1) * tr::KS);
+      r.q[u][nb] = tr::wload(W, ldw, r.ncol[nb], min(u, nsteps - 1) * tr::KS);
+      if constexpr (COMP) r.lo[u][nb] = tr::wload(Wlo, ldw, r.ncol[nb], min(u, nsteps - 1) * tr::KS);
     }
+}
+
+// acc[nb][mb] += sum_k W(n, k) act(k, m) over k in [0, K) for the n-blocks of ring_load, whose
+// first PF k-steps r holds; the later ones are loaded PF steps ahead.
+// COMP: the weights are the sum of two BF16 matrices (value and rounding remainder, Wlo), each
+// multiplied on the MFMA: the product keeps ~16 significant bits of W instead of 8.
+template <typename T, int NB, int MB, int PF, int WV, bool COMP = false>
+__device__ __forceinline__ void gemm_r(f4 (&acc)[NB][MB], WRing<T, NB, PF, COMP>& r, const T* __restrict__ W,
+                                       int ldw, int nb0, int ncb, const T* act, int lda, int K,
+                                       const T* __restrict__ Wlo = nullptr) {
+  typedef Tr<T> tr;
+  const int nsteps = K / tr::KS;
   typename tr::frag b[MB];
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb) b[mb] = tr::aload(act, lda, mb * 16, 0);
@@ -199,14 +213,14 @@ __device__ __forceinline__ void gemm_t(f4 (&acc)[NB][MB], const T* __restrict__ 
         for (int nb = 0; nb < NB; ++nb) {
           if (nb0 + WV * nb < ncb) {
 #pragma unroll
-            for (int mb = 0; mb < MB; ++mb) acc[nb][mb] = tr::mma(wq[u][nb], b[mb], acc[nb][mb]);
+            for (int mb = 0; mb < MB; ++mb) acc[nb][mb] = tr::mma(r.q[u][nb], b[mb], acc[nb][mb]);
             if constexpr (COMP) {
 #pragma unroll
-              for (int mb = 0; mb < MB; ++mb) acc[nb][mb] = tr::mma(wl[u][nb], b[mb], acc[nb][mb]);
+              for (int mb = 0; mb < MB; ++mb) acc[nb][mb] = tr::mma(r.lo[u][nb], b[mb], acc[nb][mb]);
             }
           }
-          wq[u][nb] = tr::wload(W, ldw, ncol[nb], sn * tr::KS);
-          if constexpr (COMP) wl[u][nb] = tr::wload(Wlo, ldw, ncol[nb], sn * tr::KS);
+          r.q[u][nb] = tr::wload(W, ldw, r.ncol[nb], sn * tr::KS);
+          if constexpr (COMP) r.lo[u][nb] = tr::wload(Wlo, ldw, r.ncol[nb], sn * tr::KS);
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -412,7 +426,7 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
           const int kn = min(s0 + u + 1, SPC - 1) * tr::KS;
 #pragma unroll
           for (int mb = 0; mb < MB; ++mb) bn[mb] = tr::aload(X, L::LDX, mb * 16, kn);
-          __builtin_amdgcn_sched_barrier(0);  // next step's LDS reads stay ahead (see gemm_t)
+          __builtin_amdgcn_sched_barrier(0);  // next step's LDS reads stay ahead (see gemm_r)
           const int kpf = kstep(min(s + PF, nsteps - 1));
 #pragma unroll
           for (int nb = 0; nb < NB1; ++nb) {
@@ -438,15 +452,28 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
       if (c + 1 < nchunks) chunk(c + 1, xa, xb);
     }
   }
+  // each GEMM's first weight fragments are requested before the previous layer's epilogue (AHEAD;
+  // the 32-atom BF16 tiles, three workgroups per CU, have no registers to spare for it and load
+  // them at the GEMM)
+  constexpr bool AHEAD = !(sizeof(T) == 2 && M <= 32);
+  const int ld_f2 = sizeof(T) == 2 ? H1 : H2, ld_f3 = sizeof(T) == 2 ? H2 : H3;
+  const int ld_b3 = sizeof(T) == 2 ? H3 : H2, ld_b2 = sizeof(T) == 2 ? H2 : H1, ld_b1 = sizeof(T) == 2 ? H1 : K;
+  WRing<T, NB2, PF, COMP> ring2;
+  auto load2 = [&] { ring_load<T, NB2, PF, WV, COMP>(ring2, (const T*)P.fw[1], ld_f2, wave, H2 / 16, H1, (const T*)P.fw_lo[1]); };
+  if constexpr (AHEAD) load2();
   fwd_epilogue<T, NB1, MB, WV>(acc1, P.b[0], B1, L::LD1, wave, H1 / 16, alpha);
   __syncthreads();
 
   // ---------------- layer 2 ----------------
+  WRing<T, NB3, PF, COMP> ring3;
+  auto load3 = [&] { ring_load<T, NB3, PF, WV, COMP>(ring3, (const T*)P.fw[2], ld_f3, wave, H3 / 16, H2, (const T*)P.fw_lo[2]); };
   {
     f4 acc2[NB2][MB];
     zero_acc(acc2);
-    gemm_t<T, NB2, MB, PF, WV, COMP>(acc2, (const T*)P.fw[1], sizeof(T) == 2 ? H1 : H2, wave, H2 / 16, B1, L::LD1, H1,
+    if constexpr (!AHEAD) load2();
+    gemm_r<T, NB2, MB, PF, WV, COMP>(acc2, ring2, (const T*)P.fw[1], ld_f2, wave, H2 / 16, B1, L::LD1, H1,
                                      (const T*)P.fw_lo[1]);
+    if constexpr (AHEAD) load3();
     fwd_epilogue<T, NB2, MB, WV>(acc2, P.b[1], B2, L::LD2, wave, H2 / 16, alpha);
   }
   __syncthreads();
@@ -454,8 +481,12 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   // ---------------- layer 3 + energy ----------------
   f4 acc3[NB3][MB];
   zero_acc(acc3);
-  gemm_t<T, NB3, MB, PF, WV, COMP>(acc3, (const T*)P.fw[2], sizeof(T) == 2 ? H2 : H3, wave, H3 / 16, B2, L::LD2, H2,
+  if constexpr (!AHEAD) load3();
+  gemm_r<T, NB3, MB, PF, WV, COMP>(acc3, ring3, (const T*)P.fw[2], ld_f3, wave, H3 / 16, B2, L::LD2, H2,
                                    (const T*)P.fw_lo[2]);
+  WRing<T, NB2, PF, COMP> ring_b3;
+  auto load_b3 = [&] { ring_load<T, NB2, PF, WV, COMP>(ring_b3, (const T*)P.bw[2], ld_b3, wave, H2 / 16, H3, (const T*)P.bw_lo[2]); };
+  if constexpr (AHEAD) load_b3();
   // h3 stays in registers (energy and G3 are formed from acc3); B3 holds the energy partials
   fwd_epilogue<T, NB3, MB, WV, false>(acc3, P.b[2], B3, L::LD3, wave, H3 / 16, alpha);
   {
@@ -509,19 +540,30 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     }
   }
   __syncthreads();
+  // the live 16-column n-blocks of dE/dAEV (two per live piece), in passes of 16
+  const int nlb = 2 * n_live_sh;
+  WRing<T, NB1, PF, COMP> ring_b2;
+  auto load_b2 = [&] { ring_load<T, NB1, PF, WV, COMP>(ring_b2, (const T*)P.bw[1], ld_b2, wave, H1 / 16, H2, (const T*)P.bw_lo[1]); };
   {
     f4 acc2[NB2][MB];
     zero_acc(acc2);
-    gemm_t<T, NB2, MB, PF, WV, COMP>(acc2, (const T*)P.bw[2], sizeof(T) == 2 ? H3 : H2, wave, H2 / 16, B3, L::LD3, H3,
+    if constexpr (!AHEAD) load_b3();
+    gemm_r<T, NB2, MB, PF, WV, COMP>(acc2, ring_b3, (const T*)P.bw[2], ld_b3, wave, H2 / 16, B3, L::LD3, H3,
                                      (const T*)P.bw_lo[2]);
+    if constexpr (AHEAD) load_b2();
     bwd_epilogue<T, NB2, MB, WV>(acc2, B2, L::LD2, wave, H2 / 16, alpha);
   }
   __syncthreads();
+  WRing<T, NB1, PF, COMP> ring_b1;
   {
     f4 accg[NB1][MB];
     zero_acc(accg);
-    gemm_t<T, NB1, MB, PF, WV, COMP>(accg, (const T*)P.bw[1], sizeof(T) == 2 ? H2 : H1, wave, H1 / 16, B2, L::LD2, H2,
+    if constexpr (!AHEAD) load_b2();
+    gemm_r<T, NB1, MB, PF, WV, COMP>(accg, ring_b2, (const T*)P.bw[1], ld_b2, wave, H1 / 16, B2, L::LD2, H2,
                                      (const T*)P.bw_lo[1]);
+    if (AHEAD && nlb > 0)
+      ring_load<T, NB1, PF, WV, COMP>(ring_b1, (const T*)P.bw[0], ld_b1, wave, nlb, H1, (const T*)P.bw_lo[0],
+                                      live_piece);
     bwd_epilogue<T, NB1, MB, WV>(accg, B1, L::LD1, wave, H1 / 16, alpha);
   }
   __syncthreads();
@@ -532,8 +574,6 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     const int m = mb * 16 + (l & 15);
     atom[mb] = atom_sh[m];
   }
-  // the live 16-column n-blocks of dE/dAEV (two per live piece), in passes of 16
-  const int nlb = 2 * n_live_sh;
   // BF16 rows: each pass's 16 n-blocks of the tile go through LDS (the H2/H3 buffers, free now)
   // so that the rows leave as 16-byte chunks
   constexpr int GLD = 256 + 8;
@@ -542,8 +582,11 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   for (int base = 0; base < nlb; base += 16) {
     f4 accx[NB1][MB];
     zero_acc(accx);
-    gemm_t<T, NB1, MB, PF, WV, COMP>(accx, (const T*)P.bw[0], sizeof(T) == 2 ? H1 : K, base + wave, nlb, B1,
-                                     L::LD1, H1, (const T*)P.bw_lo[0], live_piece);
+    if (!AHEAD || base > 0)  // (AHEAD) the first pass's fragments were requested before the last epilogue
+      ring_load<T, NB1, PF, WV, COMP>(ring_b1, (const T*)P.bw[0], ld_b1, base + wave, nlb, H1, (const T*)P.bw_lo[0],
+                                      live_piece);
+    gemm_r<T, NB1, MB, PF, WV, COMP>(accx, ring_b1, (const T*)P.bw[0], ld_b1, base + wave, nlb, B1, L::LD1, H1,
+                                     (const T*)P.bw_lo[0]);
     if (grad_bf16) {
 #pragma unroll
       for (int nb = 0; nb < NB1; ++nb) {
