@@ -54,8 +54,9 @@ __device__ __forceinline__ float hw_cospi(float t) { return __builtin_amdgcn_cos
 __device__ __forceinline__ float hw_sinpi(float t) { return __builtin_amdgcn_sinf(0.5f * t); }
 // exp(-eta x^2)
 __device__ __forceinline__ float gauss(float x, float eta) { return hw_exp2(x * x * (-eta * kLog2e)); }
-// (1 + cos(theta - theta_z))^zeta for a base b = 1 + cos(..) >= 0 up to rounding
-__device__ __forceinline__ float ang_pow(float b, float zeta) { return hw_exp2(zeta * hw_log2(fmaxf(b, 1.0e-30f))); }
+// (1 + cos(theta - theta_z))^zeta for a base b = 1 + cos(..) >= 0 up to rounding and zeta > 0: |b| keeps a
+// rounded negative base away from log2's NaN, and b = 0 gives exp2(-inf) = 0
+__device__ __forceinline__ float ang_pow(float b, float zeta) { return hw_exp2(zeta * hw_log2(fabsf(b))); }
 
 // Each wave owns its LDS slice, so the lanes of one wave only exchange data among
 // themselves: the LDS executes a wave's DS instructions in issue order, and wavefront-scope
@@ -126,6 +127,11 @@ struct AevDims {
   uint64_t live;  // adjoint: channel blocks (radial s at bit s, angular pair p at bit S+p) read from g
   int n_groups;   // adjoint: the live blocks as 8-channel groups (first channel / 8), or 0
   uint16_t groups[kMaxGroups];
+  // forward: Gaussians as exp2(-y^2) of scaled distances, y = k (d - shift) with k = sqrt(eta log2 e);
+  // rs / as are the radial / angular shifts times k, ka_half is k_a / 2 (the scaled mean distance of
+  // a triplet is ka_half (d_a + d_b))
+  float kr, ka_half;
+  float rs[16], as[16];
 };
 
 // blocks of the species in species_mask and of the pairs of two such species (all blocks when
@@ -411,7 +417,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           rdf4[pos] = make_float4(d, 0.25f * cos_switch(d, p.cutoff), 0.25f * cos_switch_grad(d, p.cutoff),
                                   dist_dot(nbv[e], nbt[e], d));
         else
-          rdf[pos] = make_float2(d, 0.25f * cos_switch(d, p.cutoff));
+          rdf[pos] = make_float2(d * dm.kr, 0.25f * cos_switch(d, p.cutoff));
       },
       cnt, off, smask);
   wave_sync();
@@ -419,7 +425,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
   const int q = lane >> 4, k = lane & 15;
   uint32_t rad_present = 0;  // species with at least one radial neighbour
   {
-    const float shift = k < dm.nR ? p.radial_shifts[k] : 0.0f;
+    const float shift = k < dm.nR ? (TAN ? p.radial_shifts[k] : dm.rs[k]) : 0.0f;
 #pragma unroll
     for (int s = 0; s < FNL_MAX_SPECIES; ++s) {
       if (s >= S) break;
@@ -434,9 +440,9 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           const float x = df.x - shift;
           run += (df.z - 2.0f * p.radial_eta * x * df.y) * gauss(x, p.radial_eta) * df.w;
         } else {
-          const float2 df = rdf[e];
-          const float x = df.x - shift;
-          run += df.y * gauss(x, p.radial_eta);
+          const float2 df = rdf[e];  // (k_r d, 0.25 fc)
+          const float y = df.x - shift;
+          run += df.y * hw_exp2(-(y * y));
         }
       }
       run += __shfl_xor(run, 16, 64);
@@ -455,7 +461,7 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           const float d = dist(v);
           const float inv = hw_rcp(fmaxf(d, 1.0e-5f));
           const float ux = v.x * inv, uy = v.y * inv, uz = v.z * inv;
-          av[pos] = make_float4(ux, uy, uz, d);
+          av[pos] = make_float4(ux, uy, uz, TAN ? d : d * dm.ka_half);  // forward: the scaled half distance
           af[pos] = make_float2(cos_switch(d, p.cutoff_ang), __int_as_float(sa));
           if constexpr (TAN) {  // du = (t - u (u . t)) / d, dd
             const float4 t = nbt[e];
@@ -636,7 +642,10 @@ k_aev_fwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       } else {
         const float st = hw_sqrt(fmaxf(1.0f - x * x, 0.0f));
 #pragma unroll
-        for (int d = 0; d < NA; ++d) fac[d * kFacLd + lane] = gauss(m - p.ang_shifts[d], p.angular_eta);
+        for (int d = 0; d < NA; ++d) {
+          const float y = (ua.w + ub.w) - dm.as[d];  // k_a (m - R_d)
+          fac[d * kFacLd + lane] = hw_exp2(-(y * y));
+        }
 #pragma unroll
         for (int z = 0; z < NZ; ++z)
           fac[(NA + z) * kFacLd + lane] = sw * ang_pow(1.0f + x * p.ang_cos[z] + st * p.ang_sin[z], p.zeta);
@@ -1003,6 +1012,15 @@ bool check_params(const fnl_aev_params* p, AevDims* dm) {
   dm->aev_len = dm->rad_len + dm->P * p->n_ang_dist * p->n_ang_sect;
   dm->live = ~0ull;
   dm->n_groups = 0;
+  if (!(p->radial_eta >= 0.0f) || !(p->angular_eta >= 0.0f) || !(p->zeta > 0.0f)) return false;
+  const float l2e = 1.44269504088896340736f;
+  dm->kr = sqrtf(p->radial_eta * l2e);
+  const float ka = sqrtf(p->angular_eta * l2e);
+  dm->ka_half = 0.5f * ka;
+  for (int k = 0; k < 16; ++k) {
+    dm->rs[k] = k < p->n_radial ? p->radial_shifts[k] * dm->kr : 0.0f;
+    dm->as[k] = k < p->n_ang_dist ? p->ang_shifts[k] * ka : 0.0f;
+  }
   return true;
 }
 

@@ -211,9 +211,9 @@ typedef struct fnl_aev_params {
   int32_t n_ang_sect;         /* angle_sections; n_ang_dist*n_ang_sect <= 32 */
   float cutoff;               /* radial graph cutoff                       */
   float cutoff_ang;           /* angular graph cutoff                      */
-  float radial_eta;
-  float angular_eta;
-  float zeta;
+  float radial_eta;            /* >= 0                                      */
+  float angular_eta;           /* >= 0                                      */
+  float zeta;                  /* > 0                                       */
   float radial_shifts[FNL_MAX_RADIAL];   /* linspace(radial_start, rc, nR+1)[:-1]   */
   float ang_shifts[FNL_MAX_ANG_DIST];    /* linspace(angular_start, rca, nA+1)[:-1] */
   float ang_cos[FNL_MAX_ANG_SECT];       /* cos(theta_z)                            */
