@@ -127,10 +127,11 @@ struct AevDims {
   uint64_t live;  // adjoint: channel blocks (radial s at bit s, angular pair p at bit S+p) read from g
   int n_groups;   // adjoint: the live blocks as 8-channel groups (first channel / 8), or 0
   uint16_t groups[kMaxGroups];
-  // forward: Gaussians as exp2(-y^2) of scaled distances, y = k (d - shift) with k = sqrt(eta log2 e);
+  // Gaussians as exp2(-y^2) of scaled distances, y = k (d - shift) with k = sqrt(eta log2 e);
   // rs / as are the radial / angular shifts times k, ka_half is k_a / 2 (the scaled mean distance of
-  // a triplet is ka_half (d_a + d_b))
-  float kr, ka_half;
+  // a triplet is ka_half (d_a + d_b)); for the derivatives 2 eta x = br y (radial, br = 2 eta_r / k_r)
+  // and eta_a x = ang_b y (angular, ang_b = eta_a / k_a)
+  float kr, ka_half, br, ang_b;
   float rs[16], as[16];
 };
 
@@ -768,6 +769,8 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
       const float dfc = cos_switch_grad(d, p.cutoff);
       const float* gr = G + sj * dm.nR;
       const GT* gj = g + (size_t)j * g_stride + si * dm.nR;
+      // with y = k_r (d - R_k): exp(-eta x^2) = exp2(-y^2) and fc' - 2 eta x fc = fc' - br fc y
+      const float dk = d * dm.kr, bfc = dm.br * fc;
       float vi = 0.0f, vj = 0.0f;
       if (sizeof(GT) == 2 && dm.nR == 16) {
         // BF16 rows, the usual block: unrolled, rows read as 16-byte vectors, 8 shifts a step
@@ -786,16 +789,16 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           }
 #pragma unroll
           for (int k = 0; k < 8; ++k) {
-            const float x = d - p.radial_shifts[q + k];
-            const float t = gauss(x, p.radial_eta) * (dfc - 2.0f * p.radial_eta * x * fc);
+            const float y = dk - dm.rs[q + k];
+            const float t = hw_exp2(-(y * y)) * fmaf(-bfc, y, dfc);
             vi += gi[k] * t;
             vj += gn[k] * t;
           }
         }
       } else {
         for (int k = 0; k < dm.nR; ++k) {
-          const float x = d - p.radial_shifts[k];
-          const float t = gauss(x, p.radial_eta) * (dfc - 2.0f * p.radial_eta * x * fc);
+          const float y = dk - dm.rs[k];
+          const float t = hw_exp2(-(y * y)) * fmaf(-bfc, y, dfc);
           vi += gr[k] * t;
           vj += g2f(gj[k]) * t;
         }
@@ -889,14 +892,15 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           F[z] = pm1 * bz;
           dF[z] = pm1 * (p.ang_cos[z] - xi * p.ang_sin[z]);
         }
-        const float m = 0.5f * (ua.w + ub.w);
+        const float mk = (ua.w + ub.w) * dm.ka_half;  // k_a m
         const float* gp = Gang + pair_index(__float_as_int(fa.z), __float_as_int(fb.z), dm.S) * (NA * NZ);
-        // S0 = sum_d R_d g.F, S1 = sum_d y_d R_d g.F (dR_d/dm = -2 eta y_d R_d), S2 = sum_d R_d g.dF
+        // with y_d = k_a (m - R_d): R_d = exp2(-y_d^2), S0 = sum_d R_d g.F, S1 = sum_d y_d R_d g.F
+        // (dR_d/dm = -2 (eta_a / k_a) y_d R_d), S2 = sum_d R_d g.dF
         float S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
 #pragma unroll
         for (int d = 0; d < NA; ++d) {
-          const float y = m - p.ang_shifts[d];
-          const float R = gauss(y, p.angular_eta);
+          const float y = mk - dm.as[d];
+          const float R = hw_exp2(-(y * y));
           float gz[NZ];
 #pragma unroll
           for (int q = 0; q < NZ; q += 4) *(float4*)(gz + q) = *(const float4*)(gp + d * NZ + q);
@@ -911,8 +915,9 @@ k_aev_bwd(const float* __restrict__ coords, const int32_t* __restrict__ species_
           S1 += gR * y;
           S2 += gdf * R;
         }
-        // dE/dd_a = pref fb (fa' S0 + fa dS0/dm / 2), with dS0/dm = -2 eta S1; dE/dc = 0.95 zeta pref fa fb S2
-        const float S1e = p.angular_eta * S1;
+        // dE/dd_a = pref fb (fa' S0 + fa dS0/dm / 2), with dS0/dm = -2 (eta_a / k_a) S1; dE/dc = 0.95 zeta
+        // pref fa fb S2
+        const float S1e = dm.ang_b * S1;
         const float dEda = pref * fb.x * (fa.y * S0 - fa.x * S1e);
         const float dEdb = pref * fa.x * (fb.y * S0 - fb.x * S1e);
         const float dEdc = dpref * fa.x * fb.x * S2;
@@ -1017,6 +1022,8 @@ bool check_params(const fnl_aev_params* p, AevDims* dm) {
   dm->kr = sqrtf(p->radial_eta * l2e);
   const float ka = sqrtf(p->angular_eta * l2e);
   dm->ka_half = 0.5f * ka;
+  dm->br = dm->kr > 0.0f ? 2.0f * p->radial_eta / dm->kr : 0.0f;
+  dm->ang_b = ka > 0.0f ? p->angular_eta / ka : 0.0f;
   for (int k = 0; k < 16; ++k) {
     dm->rs[k] = k < p->n_radial ? p->radial_shifts[k] * dm->kr : 0.0f;
     dm->as[k] = k < p->n_ang_dist ? p->ang_shifts[k] * ka : 0.0f;
