@@ -17,7 +17,7 @@ FNL_ERR_HIP = -2
 FNL_ERR_UNSUPPORTED = -3
 FNL_REDUCE_FUSED_MAX_SYSTEMS = 16  # systems fnl_baoab_step_sums sums in its own launch
 FNL_ERR_SCRATCH_TOO_SMALL = -4
-FNL_NBLIST_SCRATCH_BYTES = 16 << 20
+FNL_NBLIST_SCRATCH_BYTES = 64 << 20
 FNL_PBC_NONE = 0
 FNL_PBC_MINIMUM_IMAGE = 1
 FNL_PBC_GENERAL = 2

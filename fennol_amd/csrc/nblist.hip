@@ -4,9 +4,12 @@
 // and the angular GraphFilter (:780-960) with an O(N) cell list that keeps the
 // reference's pair set bit for bit: every candidate pair is tested with the
 // reference's float32 arithmetic (fnl_min_image / fnl_d12), and the candidate
-// set (3x3x3 neighbouring cells, deduplicated for grids narrower than 3) is a
-// superset of the pairs within the cutoff because each cell is at least one
-// cutoff wide.
+// set is a superset of the pairs within the cutoff: cells at least half a
+// cutoff wide searched two cells deep along each axis (5x5x5 neighbourhood),
+// or, where the caller's grid capacity leaves them at least one cutoff wide,
+// one cell deep (3x3x3); deduplicated for grids narrower than the
+// neighbourhood.  Half-cutoff cells halve the candidates per centre (about
+// 240 instead of 470 in liquid water at a 5.2 A cutoff).
 //
 // Pipeline (all on the caller's stream):
 //   k_grid      one workgroup per system: grid dims (+ bounding box if open);
@@ -36,11 +39,12 @@
 namespace {
 
 // per system: nx ny nz first | ox oy oz (open: box origin) | iwx iwy iwz (cells per unit of
-// fractional or Cartesian extent, float bits) | pbc mode | pad
+// fractional or Cartesian extent, float bits) | pbc mode | search depth per axis (1 or 2:
+// bits 0-3 x, 4-7 y, 8-11 z)
 constexpr int kGridWords = 12;
-constexpr int kCellCap = 64;    // atoms per cell staged by k_bin (more: k_cell_sort gathers them)
-constexpr int kCmap = 640;      // candidates per centre mapped through LDS (more: the search below)
-constexpr int kMaxRanges = 18;  // 3 x 3 columns x at most 2 z-runs
+constexpr int kCellCap = 32;    // atoms per cell staged by k_bin (more: k_cell_sort gathers them)
+constexpr int kCmap = 1024;     // candidates per centre mapped through LDS (more: the search below)
+constexpr int kMaxRanges = 50;  // 5 x 5 columns x at most 2 z-runs
 
 struct NblistWs {
   int32_t* grid;       // [n_sys][kGridWords] (floats stored as bits)
@@ -49,7 +53,7 @@ struct NblistWs {
   int32_t* cell_start; // [n_cells + 1]
   int32_t* stage;      // [n_cells][kCellCap] atoms in binning order
   unsigned* done;      // completion count of k_bin (left at zero)
-  int32_t* sorted;     // [N]
+  int32_t* slot_cell;  // [N] cell of each position of the cell-sorted arrays
   float4* pos;         // [N] cell-sorted x, y, z, atom index bits
   int2* ranges;        // [n_cells][kMaxRanges] candidate ranges of each cell's neighbourhood
 };
@@ -67,7 +71,7 @@ size_t ws_layout(int n_atoms, int n_sys, int n_cells, char* base, NblistWs* ws) 
   size_t o_cs = take((size_t)(n_cells + 1) * 4);
   size_t o_st = take((size_t)n_cells * kCellCap * 4);
   size_t o_dn = take(4);
-  size_t o_s = take((size_t)n_atoms * 4);
+  size_t o_sc = take((size_t)n_atoms * 4);
   size_t o_p = take((size_t)n_atoms * 16);
   size_t o_r = take((size_t)n_cells * kMaxRanges * 8);
   if (base && ws) {
@@ -77,7 +81,7 @@ size_t ws_layout(int n_atoms, int n_sys, int n_cells, char* base, NblistWs* ws) 
     ws->cell_start = (int32_t*)(base + o_cs);
     ws->stage = (int32_t*)(base + o_st);
     ws->done = (unsigned*)(base + o_dn);
-    ws->sorted = (int32_t*)(base + o_s);
+    ws->slot_cell = (int32_t*)(base + o_sc);
     ws->pos = (float4*)(base + o_p);
     ws->ranges = (int2*)(base + o_r);
   }
@@ -96,13 +100,17 @@ __global__ void k_fill_f32(float* __restrict__ p, float v, size_t n) {
   for (; i < n; i += stride) p[i] = v;
 }
 
-// Cell widths must exceed the cutoff with a margin so that rounding in the
-// bin coordinates cannot push two atoms within the cutoff two cells apart.
-__device__ __forceinline__ int cells_along(float width, float rc, int cap) {
+// Cells along one axis and the search depth: cells half a cutoff wide (two
+// deep) where the capacity allows, else at least one cutoff wide (one deep).
+// Depth x width exceeds the cutoff with a margin so that rounding in the bin
+// coordinates cannot push two atoms within the cutoff more than depth cells
+// apart.
+__device__ __forceinline__ int cells_along(float width, float rc, int cap, int* depth) {
   const float rbin = rc * 1.0001f + 1.0e-4f;
-  int n = (int)floorf(width / rbin);
-  n = max(n, 1);
-  return min(n, max(cap, 1));
+  const int n1 = max((int)floorf(width / rbin), 1);         // widths >= rbin
+  const int n = min(max((int)floorf(2.0f * width / rbin), 1), max(cap, 1));  // widths >= rbin / 2
+  *depth = n > n1 ? 2 : 1;
+  return n;
 }
 
 __global__ void k_grid(const float* __restrict__ coords, const int32_t* __restrict__ species_idx,
@@ -137,11 +145,11 @@ __global__ void k_grid(const float* __restrict__ coords, const int32_t* __restri
     if (t == 0) {
       const float* R = rec + b * 9;
       // lattice-plane spacing along a = 1 / |column a of R|
-      int n[3];
+      int n[3], dp[3];
       for (int a = 0; a < 3; ++a) {
         const float nrm = sqrtf(R[a] * R[a] + R[3 + a] * R[3 + a] + R[6 + a] * R[6 + a]);
         const int cap = a == 0 ? capx : (a == 1 ? capy : capz);
-        n[a] = cells_along(1.0f / nrm, cutoff, cap);
+        n[a] = cells_along(1.0f / nrm, cutoff, cap, &dp[a]);
       }
       g[0] = n[0]; g[1] = n[1]; g[2] = n[2]; g[3] = first;
       g[4] = g[5] = g[6] = 0;
@@ -149,7 +157,7 @@ __global__ void k_grid(const float* __restrict__ coords, const int32_t* __restri
       g[8] = __float_as_int((float)n[1]);
       g[9] = __float_as_int((float)n[2]);
       g[10] = 1;
-      g[11] = 0;
+      g[11] = dp[0] | (dp[1] << 4) | (dp[2] << 8);
     }
     return;
   }
@@ -181,12 +189,13 @@ __global__ void k_grid(const float* __restrict__ coords, const int32_t* __restri
     __syncthreads();
   }
   if (t == 0) {
-    int n[3];
+    int n[3], dp[3];
     float inv_w[3];
     for (int a = 0; a < 3; ++a) {
       const float ext = smax[a][0] - smin[a][0];
       const int cap = a == 0 ? capx : (a == 1 ? capy : capz);
-      n[a] = (ext > 0.0f) ? cells_along(ext, cutoff, cap) : 1;
+      dp[a] = 1;
+      n[a] = (ext > 0.0f) ? cells_along(ext, cutoff, cap, &dp[a]) : 1;
       inv_w[a] = (n[a] > 1) ? (float)n[a] / ext : 0.0f;
     }
     g[0] = n[0]; g[1] = n[1]; g[2] = n[2]; g[3] = first;
@@ -197,7 +206,7 @@ __global__ void k_grid(const float* __restrict__ coords, const int32_t* __restri
     g[8] = __float_as_int(inv_w[1]);
     g[9] = __float_as_int(inv_w[2]);
     g[10] = 0;
-    g[11] = 0;
+    g[11] = dp[0] | (dp[1] << 4) | (dp[2] << 8);
   }
 }
 
@@ -404,31 +413,60 @@ __global__ void __launch_bounds__(256) k_bin(const float* __restrict__ coords, c
   }
   __syncthreads();
   if (!last) return;
-  // exclusive scan in chunks of 1024 cells, 4 consecutive cells per thread
+  // exclusive scan in chunks of 256 * kScanPer cells: each chunk's populations are loaded
+  // coalesced (thread t holds cells base + 256 k + t) and requested before the previous chunk is
+  // scanned, then transposed through LDS so that thread t scans kScanPer consecutive cells, and
+  // the starts go back through LDS to coalesced stores
+  constexpr int kScanPer = 16;
+  constexpr int kChunk = 256 * kScanPer;
+  __shared__ int chunk[kChunk];
   const int lane = t & 63, w = t >> 6;
+  auto ld = [&](int q) {
+    return q < n_cells ? __hip_atomic_load(cell_count + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+  };
   int cmax = 0;
-  for (int base = 0; base < n_cells; base += 1024) {
-    const int idx = base + 4 * t;
-    int4 v = make_int4(0, 0, 0, 0);
-    auto ld = [&](int q) { return __hip_atomic_load(cell_count + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
-    if (idx < n_cells) v.x = ld(idx);
-    if (idx + 1 < n_cells) v.y = ld(idx + 1);
-    if (idx + 2 < n_cells) v.z = ld(idx + 2);
-    if (idx + 3 < n_cells) v.w = ld(idx + 3);
-    cmax = max(cmax, max(max(v.x, v.y), max(v.z, v.w)));
-    const int tot = v.x + v.y + v.z + v.w;
+  int nxt[kScanPer];
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) nxt[k] = ld(256 * k + t);
+  for (int base = 0; base < n_cells; base += kChunk) {
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      chunk[256 * k + t] = nxt[k];
+      cmax = max(cmax, nxt[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) nxt[k] = ld(base + kChunk + 256 * k + t);
+    __syncthreads();
+    int mine[kScanPer];
+    int tot = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; k += 4) {
+      const int4 q = *reinterpret_cast<const int4*>(&chunk[kScanPer * t + k]);
+      mine[k] = q.x; mine[k + 1] = q.y; mine[k + 2] = q.z; mine[k + 3] = q.w;
+      tot += (q.x + q.y) + (q.z + q.w);
+    }
     const int incl = fnl_wave_incl_scan(tot);
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
     int before = carry;
     for (int q = 0; q < w; ++q) before += wsum[q];
-    const int e0 = before + incl - tot;  // exclusive start of this thread's 4 cells
-    if (idx < n_cells) cell_start[idx] = e0;
-    if (idx + 1 < n_cells) cell_start[idx + 1] = e0 + v.x;
-    if (idx + 2 < n_cells) cell_start[idx + 2] = e0 + v.x + v.y;
-    if (idx + 3 < n_cells) cell_start[idx + 3] = e0 + v.x + v.y + v.z;
+    int e = before + incl - tot;  // exclusive start of this thread's cells
+#pragma unroll
+    for (int k = 0; k < kScanPer; k += 4) {
+      int4 q;
+      q.x = e; e += mine[k];
+      q.y = e; e += mine[k + 1];
+      q.z = e; e += mine[k + 2];
+      q.w = e; e += mine[k + 3];
+      *reinterpret_cast<int4*>(&chunk[kScanPer * t + k]) = q;
+    }
     __syncthreads();
     if (t == 255) carry = before + incl;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      const int idx = base + 256 * k + t;
+      if (idx < n_cells) cell_start[idx] = chunk[256 * k + t];
+    }
     __syncthreads();
   }
   if (t == 0) {
@@ -439,129 +477,145 @@ __global__ void __launch_bounds__(256) k_bin(const float* __restrict__ coords, c
   if (lane == 0 && cmax > 0) atomicMax(&status[FNL_STATUS_MAX_CELL], cmax);
 }
 
-// offsets along one axis of the 3x3x3 neighbourhood, deduplicated
-__device__ __forceinline__ int axis_offsets(int c, int n, bool pbc, int* out) {
-  if (pbc) {
-    if (n == 1) { out[0] = 0; return 1; }
-    if (n == 2) { out[0] = 0; out[1] = 1 - 2 * c; return 2; }  // the two cells {0, 1}
-    out[0] = (c + n - 1) % n - c; out[1] = 0; out[2] = (c + 1) % n - c;
-    return 3;
-  }
-  int k = 0;
-  if (c > 0) out[k++] = -1;
-  out[k++] = 0;
-  if (c < n - 1) out[k++] = 1;
-  return k;
+// Cells along one axis of the neighbourhood c-depth..c+depth, deduplicated: a periodic axis of
+// at most 2 depth + 1 cells contributes each of its cells once.  axis_count gives how many,
+// axis_cell the q-th of them (q < axis_count), in a fixed order.
+__device__ __forceinline__ int axis_count(int c, int n, int depth, bool pbc) {
+  if (pbc) return min(n, 2 * depth + 1);
+  return min(c + depth, n - 1) - max(c - depth, 0) + 1;
 }
 
-// z-runs of the neighbourhood along one (x, y) column of cells: cells
-// cz-1..cz+1 are consecutive in memory, so their atoms form one contiguous
-// range of the cell-sorted arrays (two ranges when the periodic wrap splits it)
-__device__ __forceinline__ int z_runs(int cz, int nz, bool pbc, int* lo, int* hi) {
+__device__ __forceinline__ int axis_cell(int c, int n, int depth, bool pbc, int q) {
+  if (pbc) return n <= 2 * depth + 1 ? q : (c - depth + q + n) % n;
+  return max(c - depth, 0) + q;
+}
+
+// z-runs of the neighbourhood along one (x, y) column of cells: cells cz-depth..cz+depth are
+// consecutive in memory, so their atoms form one contiguous range of the cell-sorted arrays
+// (two ranges when the periodic wrap splits it).  Returns the number of runs; run r (< 2) is
+// the cells lo..hi of the column.
+__device__ __forceinline__ int z_run(int cz, int nz, int depth, bool pbc, int r, int* lo, int* hi) {
   if (!pbc) {
-    lo[0] = max(cz - 1, 0);
-    hi[0] = min(cz + 1, nz - 1);
+    *lo = max(cz - depth, 0);
+    *hi = min(cz + depth, nz - 1);
     return 1;
   }
-  if (nz <= 3) {  // every cell of the column
-    lo[0] = 0;
-    hi[0] = nz - 1;
+  if (nz <= 2 * depth + 1) {  // every cell of the column
+    *lo = 0;
+    *hi = nz - 1;
     return 1;
   }
-  if (cz == 0) {
-    lo[0] = 0; hi[0] = 1;
-    lo[1] = nz - 1; hi[1] = nz - 1;
+  if (cz - depth < 0) {
+    *lo = r == 0 ? 0 : cz - depth + nz;
+    *hi = r == 0 ? cz + depth : nz - 1;
     return 2;
   }
-  if (cz == nz - 1) {
-    lo[0] = nz - 2; hi[0] = nz - 1;
-    lo[1] = 0; hi[1] = 0;
+  if (cz + depth > nz - 1) {
+    *lo = r == 0 ? cz - depth : 0;
+    *hi = r == 0 ? nz - 1 : cz + depth - nz;
     return 2;
   }
-  lo[0] = cz - 1;
-  hi[0] = cz + 1;
+  *lo = cz - depth;
+  *hi = cz + depth;
   return 1;
 }
 
-// One wave per cell: the staged atoms of the cell ranked by index (rows come
-// out in a fixed order), written to the cell's segment of sorted[] with their
-// float4 positions.  A cell holding more than kCellCap atoms gathers its atoms
-// instead by scanning its system's atoms in index order.  The wave also writes
-// the cell's candidate ranges for k_pairs: range r (lane r < kMaxRanges) =
+// One wave per two cells (32 lanes each): the staged atoms of the cell ranked
+// by index (rows come out in a fixed order), written to the cell's segment of
+// the cell-sorted arrays (float4 position with the atom index in w, and the
+// cell).  A cell holding more than kCellCap atoms is gathered instead by the
+// whole wave, scanning its system's atoms in index order.  Each half also
+// writes its cell's candidate ranges for k_pairs: range r < kMaxRanges =
 // (first position in the cell-sorted arrays, length) of one z-run of one of
-// the 3 x 3 neighbouring columns; unused ranges are (0, 0).
+// the (2 depth_x + 1) x (2 depth_y + 1) neighbouring columns; unused ranges
+// are (0, 0).
 __global__ void k_cell_sort(int n_cells, const int32_t* __restrict__ cell_start,
                             const int32_t* __restrict__ stage, const float* __restrict__ coords,
                             const int32_t* __restrict__ atom_cell, const int32_t* __restrict__ grid,
                             const int32_t* __restrict__ sys_offsets, int n_sys,
-                            int32_t* __restrict__ sorted, float4* __restrict__ pos,
-                            int2* __restrict__ ranges) {
-  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+                            int32_t* __restrict__ slot_cell,
+                            float4* __restrict__ pos, int2* __restrict__ ranges) {
+  static_assert(kCellCap <= 32, "one half-wave ranks a staged cell");
   const int lane = fnl_lane();
-  if (c >= n_cells) return;
+  const int h = lane >> 5, hl = lane & 31;
+  const int c = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 2 + h;
+  const bool live = c < n_cells;
+  // the cell's population and staged atoms, requested before the range table is built
+  int s = 0, cnt = 0, v = 0x7fffffff;
+  if (live) {
+    s = cell_start[c];
+    cnt = cell_start[c + 1] - s;
+    if (hl < min(cnt, kCellCap)) v = stage[(size_t)c * kCellCap + hl];
+  }
   // the system whose cell range holds c: largest b with first cell <= c
   int sys = 0;
-  {
+  if (live) {
     int hi = n_sys - 1;
     while (sys < hi) {
       const int mid = (sys + hi + 1) >> 1;
       if (grid[mid * kGridWords + 3] <= c) sys = mid; else hi = mid - 1;
     }
-  }
-  {
     const int32_t* g = grid + sys * kGridWords;
     const int nx = g[0], ny = g[1], nz = g[2], first = g[3];
     const bool pbc = g[10] != 0;
+    const int dx = g[11] & 15, dy = (g[11] >> 4) & 15, dz = (g[11] >> 8) & 15;
     const int cell = c - first;
-    int2 rr = make_int2(0, 0);
+    const int cz = cell % nz, cy = (cell / nz) % ny, cx = cell / (ny * nz);
+    int nox = 0, noy = 0, nzr = 1, zlo, zhi;
     if (cell < nx * ny * nz) {  // cells past the system's grid (capacity slack) have no atoms
-      const int cz = cell % nz, cy = (cell / nz) % ny, cx = cell / (ny * nz);
-      int ox[3], oy[3];
-      const int nox = axis_offsets(cx, nx, pbc, ox);
-      const int noy = axis_offsets(cy, ny, pbc, oy);
-      int zlo[2], zhi[2];
-      const int nzr = z_runs(cz, nz, pbc, zlo, zhi);
-      if (lane < nox * noy * nzr) {
-        const int zr = lane % nzr, xy = lane / nzr;
-        const int a = xy / noy, bb = xy % noy;
-        const int col = first + ((cx + ox[a]) * ny + (cy + oy[bb])) * nz;
-        rr.x = cell_start[col + zlo[zr]];
-        rr.y = cell_start[col + zhi[zr] + 1] - rr.x;
+      nox = axis_count(cx, nx, dx, pbc);
+      noy = axis_count(cy, ny, dy, pbc);
+      nzr = z_run(cz, nz, dz, pbc, 0, &zlo, &zhi);
+    }
+    for (int e = hl; e < kMaxRanges; e += 32) {
+      int2 rr = make_int2(0, 0);
+      if (e < nox * noy * nzr) {
+        const int zr = e % nzr, xy = e / nzr;
+        const int qx = xy / noy, qy = xy % noy;
+        z_run(cz, nz, dz, pbc, zr, &zlo, &zhi);
+        const int col = first + (axis_cell(cx, nx, dx, pbc, qx) * ny + axis_cell(cy, ny, dy, pbc, qy)) * nz;
+        rr.x = cell_start[col + zlo];
+        rr.y = cell_start[col + zhi + 1] - rr.x;
       }
+      ranges[(size_t)c * kMaxRanges + e] = rr;
     }
-    if (lane < kMaxRanges) ranges[(size_t)c * kMaxRanges + lane] = rr;
   }
-  const int s = cell_start[c], cnt = cell_start[c + 1] - s;
-  if (cnt <= kCellCap) {
-    const int v = lane < cnt ? stage[(size_t)c * kCellCap + lane] : 0x7fffffff;
-    int rank = 0;
-    for (int q = 0; q < cnt; ++q) rank += __shfl(v, q, 64) < v ? 1 : 0;
-    if (lane < cnt) {
-      sorted[s + rank] = v;
-      pos[s + rank] = make_float4(coords[3 * v], coords[3 * v + 1], coords[3 * v + 2], __int_as_float(v));
-    }
-    return;
+  // staged cells: rank among the half's atoms (lanes past the population hold INT_MAX)
+  const int qn = cnt <= kCellCap ? cnt : 0;
+  const int qmax = max(__shfl(qn, 0, 64), __shfl(qn, 32, 64));
+  int rank = 0;
+  for (int q = 0; q < qmax; ++q) rank += __shfl(v, (h << 5) + q, 64) < v ? 1 : 0;
+  if (hl < qn) {
+    slot_cell[s + rank] = c;
+    pos[s + rank] = make_float4(coords[3 * v], coords[3 * v + 1], coords[3 * v + 2], __int_as_float(v));
   }
-  int k = 0;
-  for (int a0 = sys_offsets[sys]; a0 < sys_offsets[sys + 1]; a0 += 64) {
-    const int a = a0 + lane;
-    const bool mine = a < sys_offsets[sys + 1] && atom_cell[a] == c;
-    const unsigned long long m = __ballot(mine);
-    if (mine) {
-      const int r = k + fnl_rank_below(m);
-      sorted[s + r] = a;
-      pos[s + r] = make_float4(coords[3 * a], coords[3 * a + 1], coords[3 * a + 2], __int_as_float(a));
+  // crowded cells: the whole wave gathers the cell's atoms from its system's range
+  for (int hh = 0; hh < 2; ++hh) {
+    const int cc = __shfl(c, hh * 32, 64), ccnt = __shfl(cnt, hh * 32, 64);
+    if (cc >= n_cells || ccnt <= kCellCap) continue;  // wave-uniform
+    const int csys = __shfl(sys, hh * 32, 64), cs = __shfl(s, hh * 32, 64);
+    const int a_end = sys_offsets[csys + 1];
+    int k = 0;
+    for (int a0 = sys_offsets[csys]; a0 < a_end; a0 += 64) {
+      const int a = a0 + lane;
+      const bool mine = a < a_end && atom_cell[a] == cc;
+      const unsigned long long m = __ballot(mine);
+      if (mine) {
+        const int r = k + fnl_rank_below(m);
+        slot_cell[cs + r] = cc;
+        pos[cs + r] = make_float4(coords[3 * a], coords[3 * a + 1], coords[3 * a + 2], __int_as_float(a));
+      }
+      k += __popcll(m);
     }
-    k += __popcll(m);
   }
 }
 
 // One wave per centre (centres in cell order).  The candidates are the atoms
-// of up to 18 contiguous ranges (3x3 columns x z-runs) of the cell-sorted
+// of up to 50 contiguous ranges (columns x z-runs) of the cell-sorted
 // arrays, read from the centre cell's range table, concatenated into one list
 // (a wave prefix sum over the range lengths held in lanes; candidate v is
-// mapped through LDS, or beyond kCmap candidates found by a search over the
-// range offsets), so every pass tests 64 candidates, and the next
+// mapped through LDS, or beyond kCmap candidates found by a binary search over
+// the range offsets), so every pass tests 64 candidates, and the next
 // pass's positions are loaded before the current pass is compacted.  Kept
 // pairs are compacted with a ballot and a prefix popcount into the radial
 // row and, for d12 < rca^2, the angular row, in candidate order.  Only rows
@@ -569,11 +623,10 @@ __global__ void k_cell_sort(int n_cells, const int32_t* __restrict__ cell_start,
 // atomic per centre would serialise at the device's coherence point).
 template <int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
-k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_index,
-        int n_binned_cap, const float* __restrict__ cells, const float* __restrict__ rec,
-        const int32_t* __restrict__ grid, const int32_t* __restrict__ atom_cell,
+k_pairs(const int32_t* __restrict__ batch_index, int n_binned_cap, const float* __restrict__ cells,
+        const float* __restrict__ rec, const int32_t* __restrict__ grid,
         const int32_t* __restrict__ cell_start, const float4* __restrict__ pos,
-        const int32_t* __restrict__ sorted, const int2* __restrict__ ranges, float rc2, float rca2,
+        const int32_t* __restrict__ slot_cell, const int2* __restrict__ ranges, float rc2, float rca2,
         int max_nbr, int max_nbr_ang, int32_t* __restrict__ nbr, int32_t* __restrict__ n_nbr,
         int32_t* __restrict__ nbr_ang, int32_t* __restrict__ n_nbr_ang, int32_t* __restrict__ status,
         int n_cells) {
@@ -582,53 +635,69 @@ k_pairs(const float* __restrict__ coords, const int32_t* __restrict__ batch_inde
   const int slot = blockIdx.x * WAVES + w;
   const int n_binned = cell_start[n_cells];
   if (slot >= n_binned || slot >= n_binned_cap) return;
-  const int i = sorted[slot];
-  const int b = batch_index[i];
-  const bool pbc = grid[b * kGridWords + 10] != 0;
+  // the centre (its cell-sorted position and atom index) and its cell, side by side
+  const float4 pi = pos[slot];
+  const int cell = slot_cell[slot];
+  const int i = __float_as_int(pi.w);
   // lane r < kMaxRanges holds range r of the centre's cell: start and length (k_cell_sort)
   int r_lo = 0, r_len = 0;
   if (lane < kMaxRanges) {
-    const int2 rr = ranges[(size_t)atom_cell[i] * kMaxRanges + lane];
+    const int2 rr = ranges[(size_t)cell * kMaxRanges + lane];
     r_lo = rr.x;
     r_len = rr.y;
   }
+  const int b = batch_index[i];
+  const bool pbc = grid[b * kGridWords + 10] != 0;
   const int r_end = fnl_wave_incl_scan(r_len);
   const int r_off = r_end - r_len;
   const int total = __builtin_amdgcn_readlane(r_end, 63);
   const uint64_t nonempty = __ballot(r_len > 0);
-  // position in the cell-sorted arrays of candidate v < total: a per-wave LDS map filled range
-  // by range (one LDS write per candidate instead of a search per candidate and pass); beyond
-  // kCmap candidates, v + shift of the last range that starts at or before v (empty ranges
-  // start where the next one does)
+  // position in the cell-sorted arrays of candidate v < total: a per-wave LDS map (one LDS
+  // write per candidate instead of a search per candidate and pass), filled by the range lanes
+  // in parallel (as many steps as the longest range) or range by range (as many steps as there
+  // are ranges), whichever is shorter; beyond kCmap candidates, v + shift of the last range
+  // that starts at or before v (empty ranges start where the next one does), found by a
+  // binary search over the range offsets in LDS
   __shared__ int cmap_all[WAVES][kCmap];
+  __shared__ int2 rtab_all[WAVES][64];
   int* cmap = cmap_all[threadIdx.x >> 6];
+  int2* rtab = rtab_all[threadIdx.x >> 6];
   const bool use_map = total <= kCmap;
   if (use_map) {
-    for (uint64_t rm = nonempty; rm;) {
-      const int r = __builtin_ctzll(rm);
-      rm &= rm - 1ull;
-      const int o = __builtin_amdgcn_readlane(r_off, r), lo = __builtin_amdgcn_readlane(r_lo, r);
-      const int len = __builtin_amdgcn_readlane(r_len, r);
-      for (int k = lane; k < len; k += 64) cmap[o + k] = lo + k;
+    const int longest = fnl_wave_max_i(r_len);
+    const int n_ranges = __popcll(nonempty);
+    if (longest <= n_ranges) {
+      for (int k = 0; k < longest; ++k)
+        if (k < r_len) cmap[r_off + k] = r_lo + k;
+    } else {
+      for (uint64_t rm = nonempty; rm;) {
+        const int r = __builtin_ctzll(rm);
+        rm &= rm - 1ull;
+        const int o = __builtin_amdgcn_readlane(r_off, r), lo = __builtin_amdgcn_readlane(r_lo, r);
+        const int len = __builtin_amdgcn_readlane(r_len, r);
+        for (int k = lane; k < len; k += 64) cmap[o + k] = lo + k;
+      }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    rtab[lane] = make_int2(r_off, r_lo - r_off);  // lanes past the ranges: (total, 0)
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   auto candidate = [&](int v) {
     if (use_map) return cmap[min(v, kCmap - 1)];
-    int dl = __builtin_amdgcn_readlane(r_lo, 0) - __builtin_amdgcn_readlane(r_off, 0);
-#pragma unroll
-    for (int r = 1; r < kMaxRanges; ++r)
-      if (__builtin_amdgcn_readlane(r_off, r) <= v)
-        dl = __builtin_amdgcn_readlane(r_lo, r) - __builtin_amdgcn_readlane(r_off, r);
-    return v + dl;
+    int lo = 0, hi = kMaxRanges - 1;  // rtab[0].x = 0 <= v
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (rtab[mid].x <= v) lo = mid; else hi = mid - 1;
+    }
+    return v + rtab[lo].y;
   };
 
   FnlCell cl;
   if (pbc) fnl_load_cell(cells, rec, b, cl);
   const bool diag = pbc && fnl_cell_diagonal(cl);  // orthorhombic box: 3 products per candidate
-  const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
+  const float xi = pi.x, yi = pi.y, zi = pi.z;  // the coordinates as given (k_cell_sort copies them)
   int count = 0, count_a = 0;
   int32_t* row = nbr + (size_t)i * max_nbr;
   int32_t* row_a = nbr_ang + (size_t)i * max_nbr_ang;
@@ -937,14 +1006,13 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
                        batch_index, n_atoms, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_count,
                        n_nbr, n_nbr_ang, n_cells, status, ws.stage, ws.done, ws.cell_start);
     FNL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_cell_sort, dim3(fnl_blocks((long long)n_cells * 64, 256)), dim3(256), 0, s, n_cells,
-                       ws.cell_start, ws.stage, coords, ws.atom_cell, ws.grid, sys_offsets, n_sys, ws.sorted,
-                       ws.pos, ws.ranges);
+    hipLaunchKernelGGL(k_cell_sort, dim3(fnl_blocks((long long)(n_cells + 1) / 2 * 64, 256)), dim3(256), 0, s, n_cells,
+                       ws.cell_start, ws.stage, coords, ws.atom_cell, ws.grid, sys_offsets, n_sys,
+                       ws.slot_cell, ws.pos, ws.ranges);
     FNL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_pairs<W>, dim3(fnl_blocks(n_atoms, W)), dim3(64 * W), 0, s, coords, batch_index,
-                       n_atoms, cells, reciprocal_cells, ws.grid, ws.atom_cell, ws.cell_start, ws.pos,
-                       ws.sorted, ws.ranges, rc2, rca2, max_nbr, max_nbr_ang, nbr, n_nbr, nbr_ang, n_nbr_ang,
-                       status, n_cells);
+    hipLaunchKernelGGL(k_pairs<W>, dim3(fnl_blocks(n_atoms, W)), dim3(64 * W), 0, s, batch_index, n_atoms,
+                       cells, reciprocal_cells, ws.grid, ws.cell_start, ws.pos, ws.slot_cell, ws.ranges, rc2,
+                       rca2, max_nbr, max_nbr_ang, nbr, n_nbr, nbr_ang, n_nbr_ang, status, n_cells);
     FNL_LAUNCH_CHECK();
   }
   return FNL_OK;

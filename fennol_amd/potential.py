@@ -133,8 +133,9 @@ def _status_errors(st) -> None:
 
 def grid_caps(cutoff: float, natoms, cells) -> tuple[np.ndarray, int]:
     """Per-system cell-grid capacities (cap_x, cap_y, cap_z, first_cell) for fnl_nblist_build and
-    the total cell count.  Systems without atoms (the padding system of AtomPadding, whose cell is
-    1000 A wide) get one cell."""
+    the total cell count.  Periodic systems get room for cells half a cutoff wide (the build
+    searches them two deep); open systems about one cell per atom.  Systems without atoms (the
+    padding system of AtomPadding, whose cell is 1000 A wide) get one cell."""
     B = len(natoms)
     g = np.zeros((B, 4), np.int32)
     first = 0
@@ -145,7 +146,7 @@ def grid_caps(cutoff: float, natoms, cells) -> tuple[np.ndarray, int]:
         elif cells is not None:
             R = np.linalg.inv(np.asarray(cells[b], np.float64))
             spacing = 1.0 / np.sqrt((R ** 2).sum(axis=0))
-            caps = np.maximum(np.floor(spacing / rbin).astype(int), 1) + 1
+            caps = np.maximum(np.floor(2.0 * spacing / rbin).astype(int), 1) + 1
         else:
             k = max(1, int(math.ceil(max(int(natoms[b]), 1) ** (1.0 / 3.0))))
             caps = np.array([k, k, k])

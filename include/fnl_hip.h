@@ -99,9 +99,11 @@ const char* fnl_strerror(int code);
  * grid: int32[n_sys][4] = (cap_x, cap_y, cap_z, first_cell) -- the caller's
  * upper bound on the cell grid of each system and the offset of its cells in
  * the global cell arrays; n_cells = sum of cap_x*cap_y*cap_z.  The grid
- * actually used has floor(width / cutoff) cells per direction, clamped to
+ * actually used has floor(2 width / cutoff) cells per direction, clamped to
  * [1, cap]; width is the lattice-plane spacing (periodic) or the bounding box
- * extent (open).
+ * extent (open).  Cells narrower than the cutoff are searched two deep
+ * (5x5x5 neighbourhood, half the candidates of 3x3x3 cutoff-wide cells),
+ * cells at least a cutoff wide one deep; any cap gives the same pair set.
  * ---------------------------------------------------------------------- */
 size_t fnl_nblist_workspace(int32_t n_atoms, int32_t n_sys, int32_t n_cells);
 
@@ -141,7 +143,7 @@ int fnl_nblist_build(const float* coords, const int32_t* species_idx,
  * clearing the counters it needs at zero): calls that overlap on one GPU --
  * on different streams, or from several host threads -- race and must not be
  * made. */
-#define FNL_NBLIST_SCRATCH_BYTES ((size_t)16 << 20)
+#define FNL_NBLIST_SCRATCH_BYTES ((size_t)64 << 20)
 #define FNL_ERR_SCRATCH_TOO_SMALL (-4)
 
 int fnl_nblist_build_noalloc(const float* coords, const int32_t* species_idx,

@@ -110,11 +110,14 @@ def test_nblist_batched_clusters(gpu):
 
 
 def test_nblist_crowded_cells(gpu):
-    """Cells with more than 64 atoms (the binning kernel's staging capacity): the cell sort
-    gathers their atoms from the system's range instead; same pair set as the oracle."""
+    """Cells with more atoms than the binning kernel stages (32): the cell sort gathers them from
+    the system's range instead; a collapsed 1,100-atom cluster (0.28 A lattice, one cell) also has
+    more candidates per centre than the pair kernel maps through LDS (1,024), so they are found by
+    the binary search over the range offsets.  Same pair set as the oracle."""
     _flush("[nblist] crowded cells")
     rng = np.random.Generator(np.random.PCG64(9))
     sysl = [systems.lattice_cluster(int(n), rng) for n in (40, 1000, 64)]
+    sysl.insert(2, systems.lattice_cluster(1100, rng, spacing=0.28, jitter=0.01, min_dist=0.25))
     s = systems.batch_systems(sysl)
     cfg = configs()
     _, pot = _potential(cfg)
