@@ -77,10 +77,11 @@ def live_channels(cfg, species_mask: int) -> tuple[int, int]:
     return S * cfg.radial_dist_divisions, S * (S + 1) // 2 * cfg.angular_dist_divisions * cfg.angle_sections
 
 
-def algorithmic_units(sb, cfg, precision: str, compensated: bool = False) -> dict:
+def algorithmic_units(sb, cfg, precision: str, compensated: bool = False, compensated_backward: bool | None = None) -> dict:
     """Per-launch algorithmic bytes / flops of each stage, from the measured neighbour counts and
     the channel blocks the batch's species use (the other blocks are never read or written).
-    ``compensated``: the BF16 weights are hi + lo pairs (two MFMA products each)."""
+    ``compensated`` (``compensated_backward``, default the same): the BF16 weights of the forward
+    (backward) products are hi + lo pairs (two MFMA products each)."""
     nr = sb.n_nbr.cpu().numpy().astype(np.float64)
     na = sb.n_nbr_ang.cpu().numpy().astype(np.float64)
     N = sb.n_atoms
@@ -95,8 +96,10 @@ def algorithmic_units(sb, cfg, precision: str, compensated: bool = False) -> dic
     live = cfg.live_column_blocks(sb.extra.get("aev_mask", 0xFFFFFFFF))
     pieces = sum(1 for q in range(min(64, cfg.k_pad // 32)) if (live >> q) & 1) + max(0, cfg.k_pad // 32 - 64)
     k_exec = cfg.k_pad if not live else kc * -(-pieces // (kc // 32))
+    comp_b = compensated if compensated_backward is None else compensated_backward
     comp = 2.0 if precision == "bf16" and compensated else 1.0
-    exec_mlp = comp * (2.0 * (k_exec * 256 + H + 160) + 2.0 * (H + 256 * k_exec))
+    comp_b = 2.0 if precision == "bf16" and comp_b else 1.0
+    exec_mlp = comp * 2.0 * (k_exec * 256 + H + 160) + comp_b * 2.0 * (H + 256 * k_exec)
     return {
         # xyz + species in, neighbour rows (radial + angular) and counts out
         "nblist": ("hbm", float(N * (12 + 4 + 8) + 4 * nr.sum() + 4 * na.sum())),
@@ -154,7 +157,7 @@ def copy_calibration(mib: int) -> None:
 
 
 def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_steps=20, skin=-1.0,
-           compensated=True, steps_per_graph=1):
+           compensated="forward", steps_per_graph=1):
     """Warm-up, then a per-stage breakdown (eager steps, HIP events around each stage on the
     launch stream), then the timed run: ``steps_per_graph`` MD steps per HIP graph (one with a
     skin list), replayed back to back until ``steps`` steps ran (the host enqueues one graph
@@ -205,12 +208,13 @@ def run_md(system, cfg, params, precision, steps, warmup, tile_rows=0, stage_ste
     return pot, md, (t1 - t0), lat, stages, eager_ms
 
 
-def form_precision(form: str) -> tuple[str, bool]:
-    """(precision, compensated weights) of a benchmarked form: fp32, bf16 (hi + lo BF16 weights)
-    or bf16_plain (BF16 weights only)."""
+def form_precision(form: str) -> tuple[str, bool | str]:
+    """(precision, compensated weights) of a benchmarked form: fp32, bf16 (hi + lo BF16 weights in
+    the forward products, BF16 in the backward ones), bf16_full (hi + lo in both) or bf16_plain
+    (BF16 weights only)."""
     if form == "fp32":
         return "fp32", False
-    return "bf16", form == "bf16"
+    return "bf16", {"bf16": "forward", "bf16_full": True, "bf16_plain": False}[form]
 
 
 def gpu_energy_forces(system, x, cfg, params, form):
@@ -538,8 +542,8 @@ def main() -> int:
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="launch one plain copy kernel of this size first (for counter runs)")
     ap.add_argument("--precisions", default="fp32,bf16,bf16_plain",
-                    help="MD forms to run (bf16 gives value; its weights are hi + lo BF16 pairs; "
-                         "bf16_plain: BF16 weights only)")
+                    help="MD forms to run (bf16 gives value; its weights are hi + lo BF16 pairs in the forward "
+                         "products; bf16_full: pairs in the backward products too; bf16_plain: BF16 weights only)")
     ap.add_argument("--no-reference-box", dest="reference_box", action="store_false",
                     help="skip the 216-water NVE run that the reference's published MD rate is quoted on")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
@@ -570,7 +574,7 @@ def main() -> int:
 
     if args.calibrate_copy_mib > 0:
         copy_calibration(args.calibrate_copy_mib)
-    precisions = [p for p in ("fp32", "bf16", "bf16_plain") if p in args.precisions.split(",")]
+    precisions = [p for p in ("fp32", "bf16", "bf16_full", "bf16_plain") if p in args.precisions.split(",")]
     if "bf16" not in precisions:
         print("the bf16 form gives the headline value; --precisions must include it", file=sys.stderr)
         return 2
@@ -582,7 +586,7 @@ def main() -> int:
         pot, md, wall, lat, stages, eager_ms = run_md(system, cfg, params, precision, args.steps, args.warmup,
                                                       tile_rows=tile, compensated=comp,
                                                       steps_per_graph=args.steps_per_graph)
-        units = algorithmic_units(md.sb, cfg, precision, pot.nets.compensated)
+        units = algorithmic_units(md.sb, cfg, precision, pot.nets.compensated, pot.nets.compensated_backward)
         results[form] = dict(wall=wall, lat=lat, stages=stages, units=units, eager_ms=eager_ms,
                              max_nbr=md.sb.max_nbr, max_nbr_ang=md.sb.max_nbr_ang)
         log(f"[bench]   {form}: {1000 * wall / args.steps:.3f} ms/step (graph replay), eager "
@@ -623,6 +627,7 @@ def main() -> int:
     bf = results["bf16"]
     fp = results.get("fp32")
     cp = results.get("bf16_plain")
+    cf = results.get("bf16_full")
     ms_bf = 1000.0 * bf["wall"] / args.steps
     value = N * args.steps / bf["wall"]
 
@@ -703,8 +708,9 @@ def main() -> int:
         "config": {"workload": f"periodic water box, {N // 3} waters ({N} atoms), min-image, neighbour list "
                                f"rebuilt every step, BAOAB Langevin dt=0.5 fs 300 K gamma=1/ps",
                    "aev_length": cfg.aev_length, "mlp": "256-192-160-1 CELU(0.1), 4 species networks",
-                   "precision": "BF16 MLP GEMMs on MFMA (FP32 accumulate; weights as BF16 hi + lo pairs), "
-                                "FP32 AEV, FP64 reductions and integrator",
+                   "precision": "BF16 MLP GEMMs on MFMA (FP32 accumulate; weights as BF16 hi + lo pairs in the "
+                                "forward products, BF16 in the backward products), FP32 AEV, FP64 reductions and "
+                                "integrator",
                    "steps_per_graph": args.steps_per_graph if args.steps % args.steps_per_graph == 0 else 1},
         "eager_ms_per_step": round(bf["eager_ms"], 4),
         "fp32": None if fp is None else {
@@ -718,6 +724,12 @@ def main() -> int:
             "value": round(N * args.steps / cp["wall"], 1), "ms_per_step": round(1000.0 * cp["wall"] / args.steps, 4),
             "p50_ms": round(float(np.percentile(cp["lat"], 50)), 4),
             "p99_ms": round(float(np.percentile(cp["lat"], 99)), 4)},
+        **({} if cf is None else {"bf16_full": {
+            "weights": "hi + lo BF16 pairs in the forward and the backward products; its FP64 errors are in "
+                       "accuracy_vs_fp64",
+            "value": round(N * args.steps / cf["wall"], 1), "ms_per_step": round(1000.0 * cf["wall"] / args.steps, 4),
+            "p50_ms": round(float(np.percentile(cf["lat"], 50)), 4),
+            "p99_ms": round(float(np.percentile(cf["lat"], 99)), 4)}}),
         "p50_ms": round(float(np.percentile(bf["lat"], 50)), 4),
         "p99_ms": round(float(np.percentile(bf["lat"], 99)), 4),
         "accuracy_vs_fp64": acc,

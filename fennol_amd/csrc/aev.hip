@@ -24,7 +24,8 @@
 // Adjoint: the same terms are recomputed.  Radial: the lane of slot e sums
 // dE_i/dvec_ie from the centre's dE/dAEV row and dE_j/dvec_ji from the
 // neighbour's row, so radial forces are gathered, not scattered.  Angular:
-// triplets in 4x4 blocks, DPP pre-sums, one LDS float add per slot and block;
+// triplets in the rounds of the circle method (every slot in one pair of each
+// round), per-round terms stored to LDS and summed per slot in round order;
 // the angular slot gradients are then scattered to grad_nbr with global int64
 // atomics on 2^-32 fixed point (exact, order-independent sums: the same bits
 // every run).  grad_self[i] is written once, virial_atom[i] = sum vec (x) dE/dvec.

@@ -277,7 +277,8 @@ __device__ __forceinline__ void bwd_epilogue(const f4 (&acc)[NB][MB], T* HG, int
   }
 }
 
-template <typename T, int M_, int PF_, int WV_, bool COMP = false>
+// COMP / COMPB: compensated BF16 weights (hi + lo products) in the forward / backward GEMMs
+template <typename T, int M_, int PF_, int WV_, bool COMP = false, bool COMPB = COMP>
 __global__ void __launch_bounds__(64 * WV_, (Lds<T, M_, WV_>::BLOCKS_PER_CU))
 k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int32_t* __restrict__ perm,
               const int32_t* __restrict__ tile_start, const int32_t* __restrict__ tile_rows,
@@ -484,8 +485,8 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   if constexpr (!AHEAD) load3();
   gemm_r<T, NB3, MB, PF, WV, COMP>(acc3, ring3, (const T*)P.fw[2], ld_f3, wave, H3 / 16, B2, L::LD2, H2,
                                    (const T*)P.fw_lo[2]);
-  WRing<T, NB2, PF, COMP> ring_b3;
-  auto load_b3 = [&] { ring_load<T, NB2, PF, WV, COMP>(ring_b3, (const T*)P.bw[2], ld_b3, wave, H2 / 16, H3, (const T*)P.bw_lo[2]); };
+  WRing<T, NB2, PF, COMPB> ring_b3;
+  auto load_b3 = [&] { ring_load<T, NB2, PF, WV, COMPB>(ring_b3, (const T*)P.bw[2], ld_b3, wave, H2 / 16, H3, (const T*)P.bw_lo[2]); };
   if constexpr (AHEAD) load_b3();
   // h3 stays in registers (energy and G3 are formed from acc3); B3 holds the energy partials
   fwd_epilogue<T, NB3, MB, WV, false>(acc3, P.b[2], B3, L::LD3, wave, H3 / 16, alpha);
@@ -542,27 +543,27 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
   __syncthreads();
   // the live 16-column n-blocks of dE/dAEV (two per live piece), in passes of 16
   const int nlb = 2 * n_live_sh;
-  WRing<T, NB1, PF, COMP> ring_b2;
-  auto load_b2 = [&] { ring_load<T, NB1, PF, WV, COMP>(ring_b2, (const T*)P.bw[1], ld_b2, wave, H1 / 16, H2, (const T*)P.bw_lo[1]); };
+  WRing<T, NB1, PF, COMPB> ring_b2;
+  auto load_b2 = [&] { ring_load<T, NB1, PF, WV, COMPB>(ring_b2, (const T*)P.bw[1], ld_b2, wave, H1 / 16, H2, (const T*)P.bw_lo[1]); };
   {
     f4 acc2[NB2][MB];
     zero_acc(acc2);
     if constexpr (!AHEAD) load_b3();
-    gemm_r<T, NB2, MB, PF, WV, COMP>(acc2, ring_b3, (const T*)P.bw[2], ld_b3, wave, H2 / 16, B3, L::LD3, H3,
+    gemm_r<T, NB2, MB, PF, WV, COMPB>(acc2, ring_b3, (const T*)P.bw[2], ld_b3, wave, H2 / 16, B3, L::LD3, H3,
                                      (const T*)P.bw_lo[2]);
     if constexpr (AHEAD) load_b2();
     bwd_epilogue<T, NB2, MB, WV>(acc2, B2, L::LD2, wave, H2 / 16, alpha);
   }
   __syncthreads();
-  WRing<T, NB1, PF, COMP> ring_b1;
+  WRing<T, NB1, PF, COMPB> ring_b1;
   {
     f4 accg[NB1][MB];
     zero_acc(accg);
     if constexpr (!AHEAD) load_b2();
-    gemm_r<T, NB1, MB, PF, WV, COMP>(accg, ring_b2, (const T*)P.bw[1], ld_b2, wave, H1 / 16, B2, L::LD2, H2,
+    gemm_r<T, NB1, MB, PF, WV, COMPB>(accg, ring_b2, (const T*)P.bw[1], ld_b2, wave, H1 / 16, B2, L::LD2, H2,
                                      (const T*)P.bw_lo[1]);
     if (AHEAD && nlb > 0)
-      ring_load<T, NB1, PF, WV, COMP>(ring_b1, (const T*)P.bw[0], ld_b1, wave, nlb, H1, (const T*)P.bw_lo[0],
+      ring_load<T, NB1, PF, WV, COMPB>(ring_b1, (const T*)P.bw[0], ld_b1, wave, nlb, H1, (const T*)P.bw_lo[0],
                                       live_piece);
     bwd_epilogue<T, NB1, MB, WV>(accg, B1, L::LD1, wave, H1 / 16, alpha);
   }
@@ -583,9 +584,9 @@ k_species_mlp(const T* __restrict__ aev, int aev_stride, MlpArgs args, const int
     f4 accx[NB1][MB];
     zero_acc(accx);
     if (!AHEAD || base > 0)  // (AHEAD) the first pass's fragments were requested before the last epilogue
-      ring_load<T, NB1, PF, WV, COMP>(ring_b1, (const T*)P.bw[0], ld_b1, base + wave, nlb, H1, (const T*)P.bw_lo[0],
+      ring_load<T, NB1, PF, WV, COMPB>(ring_b1, (const T*)P.bw[0], ld_b1, base + wave, nlb, H1, (const T*)P.bw_lo[0],
                                       live_piece);
-    gemm_r<T, NB1, MB, PF, WV, COMP>(accx, ring_b1, (const T*)P.bw[0], ld_b1, base + wave, nlb, B1, L::LD1, H1,
+    gemm_r<T, NB1, MB, PF, WV, COMPB>(accx, ring_b1, (const T*)P.bw[0], ld_b1, base + wave, nlb, B1, L::LD1, H1,
                                      (const T*)P.bw_lo[0]);
     if (grad_bf16) {
 #pragma unroll
@@ -654,7 +655,9 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
   a.alpha = w->celu_alpha;
   a.k_live = k_live ? (unsigned long long)k_live : ~0ull;
   a.n_tiles_dev = n_tiles_dev;
-  bool comp = bf;  // compensated BF16 when every network carries the remainder matrices
+  // compensated BF16 forward (backward) products when every network carries the three forward
+  // (backward) remainder matrices; backward remainders only with the forward ones
+  bool comp = bf, comp_b = bf;
   for (int n = 0; n < FNL_MLP_MAX_NETS; ++n) {
     NetPtrs& p = a.nets[n];
     if (n >= w->n_nets) {
@@ -664,10 +667,13 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
     const fnl_mlp_net& s = w->nets[n];
     if (!s.w1t || !s.w2t || !s.w3t || !s.w1 || !s.w2 || !s.w3 || !s.w4 || !s.b1 || !s.b2 || !s.b3 || !s.b4)
       return FNL_ERR_INVALID;
-    const bool has_lo = s.w1t_lo && s.w2t_lo && s.w3t_lo && s.w1_lo && s.w2_lo && s.w3_lo;
-    const bool any_lo = s.w1t_lo || s.w2t_lo || s.w3t_lo || s.w1_lo || s.w2_lo || s.w3_lo;
-    if (any_lo && (!has_lo || !bf)) return FNL_ERR_INVALID;  // all six remainders, BF16 only
-    comp = comp && has_lo;
+    const bool f_lo = s.w1t_lo && s.w2t_lo && s.w3t_lo, b_lo = s.w1_lo && s.w2_lo && s.w3_lo;
+    const bool any_f = s.w1t_lo || s.w2t_lo || s.w3t_lo, any_b = s.w1_lo || s.w2_lo || s.w3_lo;
+    // a set is all three or none, BF16 only, the backward set only with the forward one
+    if ((any_f && !f_lo) || (any_b && !b_lo) || ((any_f || any_b) && !bf) || (b_lo && !f_lo))
+      return FNL_ERR_INVALID;
+    comp = comp && f_lo;
+    comp_b = comp_b && b_lo;
     p.fw_lo[0] = s.w1t_lo;
     p.fw_lo[1] = s.w2t_lo;
     p.fw_lo[2] = s.w3t_lo;
@@ -693,10 +699,14 @@ extern "C" int fnl_species_mlp(const void* aev, int32_t aev_stride, const fnl_ml
   const int grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
 #define FNL_MLP_BF16(M_, WV_)                                                                                  \
   do {                                                                                                         \
-    if (comp)                                                                                                  \
+    if (comp && comp_b)                                                                                        \
       hipLaunchKernelGGL((k_species_mlp<unsigned short, M_, 2, WV_, true>), dim3(grid), dim3(64 * WV_), 0, st, \
                          (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net,     \
                          n_tiles, energy, grad, grad_stride, gbf);                                             \
+    else if (comp)                                                                                             \
+      hipLaunchKernelGGL((k_species_mlp<unsigned short, M_, 2, WV_, true, false>), dim3(grid), dim3(64 * WV_), \
+                         0, st, (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows,        \
+                         tile_net, n_tiles, energy, grad, grad_stride, gbf);                                   \
     else                                                                                                       \
       hipLaunchKernelGGL((k_species_mlp<unsigned short, M_, 2, WV_, false>), dim3(grid), dim3(64 * WV_), 0,   \
                          st, (const unsigned short*)aev, aev_stride, a, perm, tile_start, tile_rows, tile_net, \

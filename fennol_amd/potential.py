@@ -189,24 +189,31 @@ class SpeciesNetworks:
     ``params``: {symbol: [(W [in, out], b [out]) x 4]} with widths
     A -> 256 -> 192 -> 160 -> 1 (FullyConnectedNet layers, nets.py:81-94).
 
-    BF16 with ``compensated`` (the default): each hidden-layer matrix is stored as
-    hi = bf16(W) and lo = bf16(W - hi), and every product takes both on the MFMA, so the
-    weights keep about 16 significant bits.  Plain BF16 weights (bf16(W) only, half the MFMA
-    work and weight traffic) bias every atom of a similar environment the same way, an energy
-    error that grows with the system's energy scale: -0.08 meV/atom on a 216-water box with
-    random weights against -0.004 compensated, and along a 1,000-step trajectory of a
-    23,328-atom box 0.09 -> 1.8 meV/atom against 0.0006 -> 0.005.
+    BF16 with ``compensated``: each hidden-layer matrix is stored as hi = bf16(W) and
+    lo = bf16(W - hi), and a product that takes both on the MFMA sees weights of about 16
+    significant bits.  Plain BF16 weights (bf16(W) only, half the MFMA work and weight traffic)
+    bias every atom of a similar environment the same way, an energy error that grows with the
+    system's energy scale: -0.08 meV/atom on a 216-water box with random weights against -0.004
+    compensated, and along a 1,000-step trajectory of a 23,328-atom box 0.09 -> 1.7 meV/atom
+    against 0.0006 -> 0.007.  ``compensated="forward"`` (the default): hi + lo in the forward
+    products, hi only in the backward products that form dE/dAEV -- the same energies as
+    ``True`` (the forward is the same arithmetic) and forces within a few per cent of its error
+    (force RMS vs FP64 at step 1,000 of that trajectory 7.3e-3 against 7.1e-3 eV/A; plain
+    9.7e-3), for a quarter less MFMA work.
     """
 
     def __init__(self, params: dict, k_in: int, k_pad: int, dtype: str = "fp32", device="cuda",
-                 celu_alpha: float = CELU_ALPHA, tile_rows: int = 0, compensated: bool = True):
+                 celu_alpha: float = CELU_ALPHA, tile_rows: int = 0, compensated: bool | str = "forward"):
         if dtype not in ("fp32", "bf16"):
             raise ValueError("dtype must be 'fp32' or 'bf16'")
         if len(params) > _abi.FNL_MLP_MAX_NETS:
             raise ValueError("too many networks")
+        if compensated not in (True, False, "forward"):
+            raise ValueError("compensated must be True, False or 'forward'")
         self.species = list(params.keys())
         self.dtype = dtype
-        self.compensated = bool(compensated) and dtype == "bf16"
+        self.compensated = bool(compensated) and dtype == "bf16"  # forward products
+        self.compensated_backward = compensated is True and dtype == "bf16"
         self.k_pad = k_pad
         tdt = torch.float32 if dtype == "fp32" else torch.bfloat16
         self._keep = []
@@ -238,8 +245,9 @@ class SpeciesNetworks:
                         w_lo = torch.tensor(fragment_order(lo), dtype=tdt, device=device).contiguous()
                         wt_lo = torch.tensor(fragment_order(lo.T), dtype=tdt, device=device).contiguous()
                         self._keep += [w_lo, wt_lo]
-                        setattr(net, f"w{l + 1}_lo", w_lo.data_ptr())
-                        setattr(net, f"w{l + 1}t_lo", wt_lo.data_ptr())
+                        setattr(net, f"w{l + 1}t_lo", wt_lo.data_ptr())  # forward: W^T
+                        if self.compensated_backward:
+                            setattr(net, f"w{l + 1}_lo", w_lo.data_ptr())
                 else:
                     w = torch.tensor(Wm, dtype=tdt, device=device).contiguous()
                     wt = torch.tensor(Wm.T, dtype=tdt, device=device).contiguous()
@@ -337,14 +345,15 @@ class ANIPotential:
 
     ``precision``: "fp32" (AEV, GEMMs and activations in FP32) or "bf16" (AEV
     computed in FP32 and stored in BF16, MLP activations in BF16 on MFMA with FP32
-    accumulation, hidden-layer weights as BF16 hi + lo pairs, about 16 significant bits, unless
-    ``bf16_compensated`` is False: plain BF16 weights).  Per-system energies and virials are
+    accumulation; hidden-layer weights as BF16 hi + lo pairs, about 16 significant bits, in the
+    forward products and BF16 in the backward products with ``bf16_compensated="forward"``, the
+    default; pairs in both with True, plain BF16 weights with False).  Per-system energies and virials are
     reduced in FP64.
     """
 
     def __init__(self, config: ANIConfig, params: dict, precision: str = "fp32", device="cuda",
                  max_nbr: int = 64, max_nbr_ang: int = 32, mlp_tile_rows: int = 0,
-                 unhandled_species: str = "raise", bf16_compensated: bool = True):
+                 unhandled_species: str = "raise", bf16_compensated: bool | str = "forward"):
         self.lib = _abi.load()
         self.config = config
         self.precision = precision

@@ -326,13 +326,15 @@ typedef struct fnl_mlp_net {
   const void* w1t; const void* w2t; const void* w3t;
   const void* w1;  const void* w2;  const void* w3;
   const float* w4; const float* b1; const float* b2; const float* b3; const float* b4;
-  /* compensated BF16 (optional; all six or none, BF16 only): the rounding
-   * remainders bf16(W - bf16(W)) of the six matrices above, same layout.  Every
-   * product then uses both parts on the MFMA (twice the matrix work), so the
-   * weights keep ~16 significant bits: rounding W to 8 bits shifts every atom
-   * of a similar environment the same way, a bias that grows with the energy
-   * scale (measured 1.8 meV/atom on the 23,328-atom water box after 1,000
-   * steps with plain BF16 weights, 0.004 with the remainders) */
+  /* compensated BF16 (optional, BF16 only): the rounding remainders
+   * bf16(W - bf16(W)) of the matrices above, same layout; the forward set
+   * (w1t_lo, w2t_lo, w3t_lo) alone, or all six.  A product whose remainder is
+   * given uses both parts on the MFMA (twice the matrix work), so its weights
+   * keep ~16 significant bits: rounding W to 8 bits shifts every atom of a
+   * similar environment the same way, a bias that grows with the energy scale
+   * (measured 1.7 meV/atom on the 23,328-atom water box after 1,000 steps with
+   * plain BF16 weights, 0.007 with the forward remainders; the backward ones
+   * change the force error by a few per cent) */
   const void* w1t_lo; const void* w2t_lo; const void* w3t_lo;
   const void* w1_lo;  const void* w2_lo;  const void* w3_lo;
 } fnl_mlp_net;

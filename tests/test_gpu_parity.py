@@ -7,6 +7,8 @@ within 1 meV/atom and 0.02 eV/A force RMS of FP64.
 
 from __future__ import annotations
 
+import inspect
+
 import numpy as np
 import pytest
 import torch
@@ -470,10 +472,12 @@ def test_energy_forces_bf16(gpu, name):
 
 
 def test_bf16_compensated_weights_remove_the_rounding_bias(gpu):
-    """BF16 weights stored as hi + lo pairs (the default): the energy of the 7-species water216
-    box is within 0.02 meV/atom of the FP64 oracle and at least 5 times closer than with plain
-    BF16 weights (whose rounding shifts every similar atom the same way: measured -0.081 vs
-    -0.004 meV/atom); forces keep the BF16 bar."""
+    """BF16 weights stored as hi + lo pairs: the energy of the 7-species water216 box is within
+    0.02 meV/atom of the FP64 oracle and at least 5 times closer than with plain BF16 weights
+    (whose rounding shifts every similar atom the same way: measured -0.081 vs -0.004
+    meV/atom); forces keep the BF16 bar.  The default form takes the pairs in the forward
+    products only: its energies are those of the pairs in both (the same forward arithmetic,
+    bit for bit) and its force error within 1.5 times theirs."""
     _flush("[bf16 compensated] water216")
     s = _cases()["water216"]
     cfg = configs(("H", "C", "N", "O", "S", "F", "Cl"))
@@ -482,16 +486,23 @@ def test_bf16_compensated_weights_remove_the_rounding_bias(gpu):
     Er, Fr, _, _ = O.energy_forces_virial(model, np.asarray(s["coordinates"], np.float32).astype(np.float64),
                                           s["species"], s["natoms"], s.get("cells"), s.get("reciprocal_cells"))
     n = len(s["species"])
-    err = {}
-    for comp in (True, False):
+    err, ferr, energies = {}, {}, {}
+    for comp in (True, "forward", False):
         pot = ANIPotential(cfg[1], params, precision="bf16", max_nbr=64, max_nbr_ang=32, bf16_compensated=comp)
-        assert pot.nets.compensated == comp
+        assert pot.nets.compensated == bool(comp)
+        assert pot.nets.compensated_backward == (comp is True)
         E, F, _ = pot.energy_forces_virial(s["species"], s["coordinates"], s["natoms"], s.get("cells"))
+        energies[comp] = E
         err[comp] = 1000.0 * abs(E[0] - Er[0]) / n
-        assert np.sqrt(np.mean((F - Fr) ** 2)) <= 0.02
-    _flush(f"  energy error: compensated {err[True]:.5f}, plain {err[False]:.5f} meV/atom")
+        ferr[comp] = float(np.sqrt(np.mean((F - Fr) ** 2)))
+        assert ferr[comp] <= 0.02
+    _flush(f"  energy error: compensated {err[True]:.5f}, forward only {err['forward']:.5f}, plain "
+           f"{err[False]:.5f} meV/atom; force RMS {ferr[True]:.2e}, {ferr['forward']:.2e}, {ferr[False]:.2e} eV/A")
     assert err[True] <= 0.02
     assert 5 * err[True] <= err[False]
+    np.testing.assert_array_equal(energies["forward"], energies[True])
+    assert ferr["forward"] <= 1.5 * ferr[True]
+    assert inspect.signature(ANIPotential).parameters["bf16_compensated"].default == "forward"
 
 
 def test_bf16_aev_rows_are_rounded_fp32_rows(gpu):
