@@ -43,7 +43,6 @@ namespace {
 // bits 0-3 x, 4-7 y, 8-11 z)
 constexpr int kGridWords = 12;
 constexpr int kCellCap = 32;    // atoms per cell staged by k_bin (more: k_cell_sort gathers them)
-constexpr int kCmap = 1024;     // candidates per centre mapped through LDS (more: the search below)
 constexpr int kMaxRanges = 50;  // 5 x 5 columns x at most 2 z-runs
 
 struct NblistWs {
@@ -610,17 +609,36 @@ __global__ void k_cell_sort(int n_cells, const int32_t* __restrict__ cell_start,
   }
 }
 
-// One wave per centre (centres in cell order).  The candidates are the atoms
-// of up to 50 contiguous ranges (columns x z-runs) of the cell-sorted
-// arrays, read from the centre cell's range table, concatenated into one list
-// (a wave prefix sum over the range lengths held in lanes; candidate v is
-// mapped through LDS, or beyond kCmap candidates found by a binary search over
-// the range offsets), so every pass tests 64 candidates, and the next
-// pass's positions are loaded before the current pass is compacted.  Kept
-// pairs are compacted with a ballot and a prefix popcount into the radial
-// row and, for d12 < rca^2, the angular row, in candidate order.  Only rows
-// that overflow update the maxima in the status words (a same-address
-// atomic per centre would serialise at the device's coherence point).
+// inclusive prefix sum and maximum over each half-wave (32 lanes)
+__device__ __forceinline__ int half_incl_scan(int v) {
+  const int hl = fnl_lane() & 31;
+#pragma unroll
+  for (int o = 1; o < 32; o <<= 1) {
+    const int t = __shfl_up(v, o, 32);
+    if (hl >= o) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int half_max(int v) {
+#pragma unroll
+  for (int o = 16; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 32));
+  return v;
+}
+
+constexpr int kCmapHalf = 512;  // candidates per centre mapped through LDS (more: the search below)
+
+// One half-wave (32 lanes) per centre, two centres per wave (centres in cell order), so that a
+// wave has two centres' load chains in flight.  The candidates are the atoms of up to 50
+// contiguous ranges (columns x z-runs) of the cell-sorted arrays, read from the centre cell's
+// range table (ranges hl and 32 + hl in lane hl), concatenated into one list (a half-wave
+// prefix sum over the range lengths; candidate v is mapped through LDS, or beyond kCmapHalf
+// candidates found by a binary search over the range offsets), so every pass tests 32
+// candidates per centre, and the next pass's positions are loaded before the current pass is
+// compacted.  Kept pairs are compacted with a ballot and a prefix popcount over the half's 32
+// bits into the radial row and, for d12 < rca^2, the angular row, in candidate order.  Only
+// rows that overflow update the maxima in the status words (a same-address atomic per centre
+// would serialise at the device's coherence point).
 template <int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
 k_pairs(const int32_t* __restrict__ batch_index, int n_binned_cap, const float* __restrict__ cells,
@@ -630,62 +648,69 @@ k_pairs(const int32_t* __restrict__ batch_index, int n_binned_cap, const float* 
         int max_nbr, int max_nbr_ang, int32_t* __restrict__ nbr, int32_t* __restrict__ n_nbr,
         int32_t* __restrict__ nbr_ang, int32_t* __restrict__ n_nbr_ang, int32_t* __restrict__ status,
         int n_cells) {
+  static_assert(kMaxRanges <= 64, "two ranges per lane of a half-wave");
   const int w = threadIdx.x >> 6;
   const int lane = fnl_lane();
-  const int slot = blockIdx.x * WAVES + w;
-  const int n_binned = cell_start[n_cells];
-  if (slot >= n_binned || slot >= n_binned_cap) return;
+  const int h = lane >> 5, hl = lane & 31;
+  const int slot = (blockIdx.x * WAVES + w) * 2 + h;
+  const int n_binned = min(cell_start[n_cells], n_binned_cap);
+  if (slot - h >= n_binned) return;  // both centres of the wave past the binned atoms
+  const bool live = slot < n_binned;
   // the centre (its cell-sorted position and atom index) and its cell, side by side
-  const float4 pi = pos[slot];
-  const int cell = slot_cell[slot];
-  const int i = __float_as_int(pi.w);
-  // lane r < kMaxRanges holds range r of the centre's cell: start and length (k_cell_sort)
-  int r_lo = 0, r_len = 0;
-  if (lane < kMaxRanges) {
-    const int2 rr = ranges[(size_t)cell * kMaxRanges + lane];
-    r_lo = rr.x;
-    r_len = rr.y;
+  float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  int cell = 0;
+  if (live) {
+    pi = pos[slot];
+    cell = slot_cell[slot];
   }
-  const int b = batch_index[i];
-  const bool pbc = grid[b * kGridWords + 10] != 0;
-  const int r_end = fnl_wave_incl_scan(r_len);
-  const int r_off = r_end - r_len;
-  const int total = __builtin_amdgcn_readlane(r_end, 63);
-  const uint64_t nonempty = __ballot(r_len > 0);
-  // position in the cell-sorted arrays of candidate v < total: a per-wave LDS map (one LDS
+  const int i = __float_as_int(pi.w);
+  // ranges hl (a) and 32 + hl (b) of the centre's cell: start and length (k_cell_sort)
+  int lo_a = 0, len_a = 0, lo_b = 0, len_b = 0;
+  if (live) {
+    const int2 ra = ranges[(size_t)cell * kMaxRanges + hl];
+    lo_a = ra.x;
+    len_a = ra.y;
+    if (32 + hl < kMaxRanges) {
+      const int2 rb = ranges[(size_t)cell * kMaxRanges + 32 + hl];
+      lo_b = rb.x;
+      len_b = rb.y;
+    }
+  }
+  int b = 0;
+  bool pbc = false;
+  if (live) {
+    b = batch_index[i];
+    pbc = grid[b * kGridWords + 10] != 0;
+  }
+  const int inc_a = half_incl_scan(len_a);
+  const int tot_a = __shfl(inc_a, (h << 5) + 31, 64);
+  const int inc_b = half_incl_scan(len_b) + tot_a;
+  const int off_a = inc_a - len_a, off_b = inc_b - len_b;
+  const int total = __shfl(inc_b, (h << 5) + 31, 64);  // candidates of this half's centre
+  // position in the cell-sorted arrays of candidate v < total: a per-centre LDS map (one LDS
   // write per candidate instead of a search per candidate and pass), filled by the range lanes
-  // in parallel (as many steps as the longest range) or range by range (as many steps as there
-  // are ranges), whichever is shorter; beyond kCmap candidates, v + shift of the last range
-  // that starts at or before v (empty ranges start where the next one does), found by a
-  // binary search over the range offsets in LDS
-  __shared__ int cmap_all[WAVES][kCmap];
-  __shared__ int2 rtab_all[WAVES][64];
-  int* cmap = cmap_all[threadIdx.x >> 6];
-  int2* rtab = rtab_all[threadIdx.x >> 6];
-  const bool use_map = total <= kCmap;
+  // in parallel (as many steps as the longest range); beyond kCmapHalf candidates, v + shift of
+  // the last range that starts at or before v (empty ranges start where the next one does),
+  // found by a binary search over the range offsets in the same LDS
+  __shared__ int cmap_all[WAVES][2][kCmapHalf];
+  int* cmap = cmap_all[w][h];
+  int2* rtab = (int2*)cmap;  // [64] when the map is not used
+  const bool use_map = total <= kCmapHalf;
   if (use_map) {
-    const int longest = fnl_wave_max_i(r_len);
-    const int n_ranges = __popcll(nonempty);
-    if (longest <= n_ranges) {
-      for (int k = 0; k < longest; ++k)
-        if (k < r_len) cmap[r_off + k] = r_lo + k;
-    } else {
-      for (uint64_t rm = nonempty; rm;) {
-        const int r = __builtin_ctzll(rm);
-        rm &= rm - 1ull;
-        const int o = __builtin_amdgcn_readlane(r_off, r), lo = __builtin_amdgcn_readlane(r_lo, r);
-        const int len = __builtin_amdgcn_readlane(r_len, r);
-        for (int k = lane; k < len; k += 64) cmap[o + k] = lo + k;
-      }
+    const int longest = half_max(max(len_a, len_b));
+    for (int k = 0; k < longest; ++k) {
+      if (k < len_a) cmap[off_a + k] = lo_a + k;
+      if (k < len_b) cmap[off_b + k] = lo_b + k;
     }
   } else {
-    rtab[lane] = make_int2(r_off, r_lo - r_off);  // lanes past the ranges: (total, 0)
+    rtab[hl] = make_int2(off_a, lo_a - off_a);  // lanes past the ranges: (total, 0)
+    rtab[32 + hl] = make_int2(off_b, lo_b - off_b);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   auto candidate = [&](int v) {
-    if (use_map) return cmap[min(v, kCmap - 1)];
+    if (use_map) return cmap[min(v, kCmapHalf - 1)];
     int lo = 0, hi = kMaxRanges - 1;  // rtab[0].x = 0 <= v
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -701,18 +726,14 @@ k_pairs(const int32_t* __restrict__ batch_index, int n_binned_cap, const float* 
   int count = 0, count_a = 0;
   int32_t* row = nbr + (size_t)i * max_nbr;
   int32_t* row_a = nbr_ang + (size_t)i * max_nbr_ang;
+  const unsigned below = (1u << hl) - 1u;
+  const int passes = max(__shfl(total, 0, 64), __shfl(total, 32, 64));  // wave-uniform
   float4 next = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(i));
-  {
-    const int pc = candidate(lane);
-    if (lane < total) next = pos[pc];
-  }
-  for (int base = 0; base < total; base += 64) {
+  if (hl < total) next = pos[candidate(hl)];
+  for (int base = 0; base < passes; base += 32) {
     const float4 pj = next;
-    const bool valid = base + lane < total;
-    if (base + 64 < total) {
-      const int pc = candidate(base + 64 + lane);
-      if (base + 64 + lane < total) next = pos[pc];
-    }
+    const bool valid = base + hl < total;
+    if (base + 32 + hl < total) next = pos[candidate(base + 32 + hl)];
     bool keep = false, keep_a = false;
     const int j = __float_as_int(pj.w);
     if (valid && j != i) {
@@ -725,20 +746,20 @@ k_pairs(const int32_t* __restrict__ batch_index, int n_binned_cap, const float* 
       keep = d12 < rc2;
       keep_a = d12 < rca2;
     }
-    const unsigned long long m = __ballot(keep);
-    const unsigned long long ma = __ballot(keep_a);
+    const unsigned m = (unsigned)(__ballot(keep) >> (32 * h));
+    const unsigned ma = (unsigned)(__ballot(keep_a) >> (32 * h));
     if (keep) {
-      const int e = count + fnl_rank_below(m);
+      const int e = count + __popc(m & below);
       if (e < max_nbr) row[e] = j;
       if (keep_a) {
-        const int ea = count_a + fnl_rank_below(ma);
+        const int ea = count_a + __popc(ma & below);
         if (ea < max_nbr_ang && e < max_nbr) row_a[ea] = e;
       }
     }
-    count += __popcll(m);
-    count_a += __popcll(ma);
+    count += __popc(m);
+    count_a += __popc(ma);
   }
-  if (lane == 0) {
+  if (live && hl == 0) {
     n_nbr[i] = count;
     if (n_nbr_ang) n_nbr_ang[i] = count_a;
     if (count > max_nbr || count_a > max_nbr_ang) {  // rare: the maxima only matter on overflow
@@ -1010,7 +1031,7 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
                        ws.cell_start, ws.stage, coords, ws.atom_cell, ws.grid, sys_offsets, n_sys,
                        ws.slot_cell, ws.pos, ws.ranges);
     FNL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_pairs<W>, dim3(fnl_blocks(n_atoms, W)), dim3(64 * W), 0, s, batch_index, n_atoms,
+    hipLaunchKernelGGL(k_pairs<W>, dim3(fnl_blocks(n_atoms, 2 * W)), dim3(64 * W), 0, s, batch_index, n_atoms,
                        cells, reciprocal_cells, ws.grid, ws.cell_start, ws.pos, ws.slot_cell, ws.ranges, rc2,
                        rca2, max_nbr, max_nbr_ang, nbr, n_nbr, nbr_ang, n_nbr_ang, status, n_cells);
     FNL_LAUNCH_CHECK();
