@@ -17,20 +17,20 @@
 //   k_bin       one thread per atom: cell id, slot in the cell (population
 //               count), atom staged at its cell's slot; the last workgroup
 //               scans the populations into cell offsets
-//   k_cell_sort one wave per cell: the staged atoms ordered by index
+//   k_cell_sort one half-wave per cell: the staged atoms ordered by index
 //               (deterministic rows) into the cell's segment, with
 //               cell-sorted float4 positions, and the cell's table of
 //               neighbouring-cell candidate ranges
-//   k_pairs     one wave per centre, centres in cell order: candidates of the
-//               neighbouring cells (contiguous z-runs of the cell-sorted
-//               arrays) tested 64 at a time, kept pairs compacted with a wave
-//               ballot + prefix popcount into the centre's row
+//   k_pairs     one half-wave per centre, centres in cell order: candidates
+//               of the neighbouring cells (contiguous z-runs of the
+//               cell-sorted arrays) tested 32 at a time, kept pairs compacted
+//               with a ballot + prefix popcount into the centre's row
 //   (row maxima go to the status words only from rows that overflowed: they
 //   size the regrown rows; a same-address atomic per centre would serialise)
 //
 // Skin update (a list built with cutoff + skin, re-filtered at the current
 // positions on the steps between builds):
-//   k_status_clear, k_skin_filter (one wave per centre)
+//   k_status_clear, k_skin_filter (one half-wave per centre)
 
 #include <vector>
 
@@ -781,18 +781,19 @@ __global__ void k_status_clear(int32_t* __restrict__ status) {
 }
 
 // Skin update (GraphGenerator.update_skin, preprocessing.py:640-696, with the
-// GraphFilter re-filter :958-960): one wave per centre reads the centre's row
-// of the list built with cutoff + skin, 64 slots a pass, recomputes d12 at the
-// current positions with the same float32 arithmetic as the full build, and
-// compacts the slots with d12 < rc^2 (radial row) and d12 < rca^2 (angular row)
-// in row order, as k_pairs does.  For the minimum-image convention the image
-// chosen at the build is the image the current positions select for every
-// pair still inside the cutoff (rc below half the lattice-plane spacing), so
-// the recomputed image reproduces the reference's stored pbc_shifts; for
-// general PBC the stored shift of each slot is applied to the unwrapped
-// positions (the reference's update_skin formula) and carried to the output
-// rows.  The next pass's slot indices are loaded before the current pass is
-// tested.
+// GraphFilter re-filter :958-960): one half-wave per centre (two centres' load
+// chains per wave) reads the centre's row of the list built with cutoff + skin,
+// 32 slots a pass, recomputes d12 at the current positions with the same
+// float32 arithmetic as the full build, and compacts the slots with
+// d12 < rc^2 (radial row) and d12 < rca^2 (angular row) in row order, as
+// k_pairs does.  For the minimum-image convention the image chosen at the
+// build is the image the current positions select for every pair still
+// inside the cutoff (rc below half the lattice-plane spacing), so the
+// recomputed image reproduces the reference's stored pbc_shifts; for general
+// PBC the stored shift of each slot is applied to the unwrapped positions
+// (the reference's update_skin formula) and carried to the output rows.  The
+// slot indices are loaded two passes ahead and the neighbours' positions one
+// pass ahead of the test.
 template <int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
 k_skin_filter(const float* __restrict__ coords, const int32_t* __restrict__ batch_index, int n_atoms,
@@ -803,28 +804,41 @@ k_skin_filter(const float* __restrict__ coords, const int32_t* __restrict__ batc
               int32_t* __restrict__ n_nbr, int32_t* __restrict__ nbr_ang, int32_t* __restrict__ n_nbr_ang,
               int32_t* __restrict__ status) {
   const int lane = fnl_lane();
-  const int i = blockIdx.x * WAVES + (threadIdx.x >> 6);
-  if (i >= n_atoms) return;
-  const int ns = n_skin[i];
+  const int h = lane >> 5, hl = lane & 31;
+  const int i = (blockIdx.x * WAVES + (threadIdx.x >> 6)) * 2 + h;
+  if (i - h >= n_atoms) return;  // both centres of the wave past the atoms
+  const bool live = i < n_atoms;
+  const int ic = live ? i : 0;  // the centre, or atom 0 for a half without one (never written)
+  const int ns = live ? n_skin[i] : 0;
   const int n = min(ns, max_skin);
-  const int32_t* srow = nbr_skin + (size_t)i * max_skin;
+  const int32_t* srow = nbr_skin + (size_t)ic * max_skin;
   FnlCell cl;
   const bool pbc = pbc_mode == FNL_PBC_MINIMUM_IMAGE;
   const bool gen = pbc_mode == FNL_PBC_GENERAL;
-  if (pbc || gen) fnl_load_cell(cells, rec, batch_index[i], cl);
-  const float xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
-  int32_t* row = nbr + (size_t)i * max_nbr;
-  int32_t* row_a = nbr_ang + (size_t)i * max_nbr_ang;
+  if (pbc || gen) fnl_load_cell(cells, rec, batch_index[ic], cl);
+  const float xi = coords[3 * ic], yi = coords[3 * ic + 1], zi = coords[3 * ic + 2];
+  int32_t* row = nbr + (size_t)ic * max_nbr;
+  int32_t* row_a = nbr_ang + (size_t)ic * max_nbr_ang;
+  const unsigned below = (1u << hl) - 1u;
   int count = 0, count_a = 0;
-  int jr = srow[min(lane, max_skin - 1)];
-  for (int base = 0; base < n; base += 64) {
-    const bool valid = base + lane < n;
-    const int j = valid ? jr : i;  // slots past the row end hold stale values: never dereferenced
-    jr = srow[min(base + 64 + lane, max_skin - 1)];
-    float vx = coords[3 * j] - xi, vy = coords[3 * j + 1] - yi, vz = coords[3 * j + 2] - zi;
+  // slot v's neighbour (the centre itself past the row end: a harmless load, never kept)
+  auto slot_atom = [&](int v) { return v < n ? srow[v] : ic; };
+  int j0 = slot_atom(hl);
+  float x0 = coords[3 * j0], y0 = coords[3 * j0 + 1], z0 = coords[3 * j0 + 2];
+  int j1 = slot_atom(32 + hl);
+  const int passes = max(__shfl(n, 0, 64), __shfl(n, 32, 64));  // wave-uniform
+  for (int base = 0; base < passes; base += 32) {
+    const bool valid = base + hl < n;
+    const int j = j0;
+    float vx = x0 - xi, vy = y0 - yi, vz = z0 - zi;
+    j0 = j1;
+    x0 = coords[3 * j1];
+    y0 = coords[3 * j1 + 1];
+    z0 = coords[3 * j1 + 2];
+    j1 = slot_atom(base + 64 + hl);
     int32_t code = 0;
     if (gen) {
-      code = skin_shift[(size_t)i * max_skin + min(base + lane, max_skin - 1)];
+      code = valid ? skin_shift[(size_t)ic * max_skin + base + hl] : 0;
       fnl_apply_shift(vx, vy, vz, cl, code);
     } else if (pbc) {
       fnl_min_image(vx, vy, vz, cl);
@@ -832,23 +846,23 @@ k_skin_filter(const float* __restrict__ coords, const int32_t* __restrict__ batc
     const float d12 = fnl_d12(vx, vy, vz);
     const bool keep = valid && d12 < rc2;
     const bool keep_a = valid && d12 < rca2;
-    const unsigned long long m = __ballot(keep);
-    const unsigned long long ma = __ballot(keep_a);
+    const unsigned m = (unsigned)(__ballot(keep) >> (32 * h));
+    const unsigned ma = (unsigned)(__ballot(keep_a) >> (32 * h));
     if (keep) {
-      const int e = count + fnl_rank_below(m);
+      const int e = count + __popc(m & below);
       if (e < max_nbr) {
         row[e] = j;
-        if (gen) nbr_shift[(size_t)i * max_nbr + e] = code;
+        if (gen) nbr_shift[(size_t)ic * max_nbr + e] = code;
       }
       if (keep_a) {
-        const int ea = count_a + fnl_rank_below(ma);
+        const int ea = count_a + __popc(ma & below);
         if (ea < max_nbr_ang && e < max_nbr) row_a[ea] = e;
       }
     }
-    count += __popcll(m);
-    count_a += __popcll(ma);
+    count += __popc(m);
+    count_a += __popc(ma);
   }
-  if (lane == 0) {
+  if (live && hl == 0) {
     n_nbr[i] = count;
     n_nbr_ang[i] = count_a;
     if (count > max_nbr || count_a > max_nbr_ang || ns > max_skin) {
@@ -1082,7 +1096,7 @@ extern "C" int fnl_nblist_skin_update(const float* coords, const int32_t* batch_
   FNL_LAUNCH_CHECK();
   if (n_atoms == 0) return FNL_OK;
   constexpr int W = 4;
-  hipLaunchKernelGGL(k_skin_filter<W>, dim3(fnl_blocks(n_atoms, W)), dim3(64 * W), 0, s, coords, batch_index,
+  hipLaunchKernelGGL(k_skin_filter<W>, dim3(fnl_blocks(n_atoms, 2 * W)), dim3(64 * W), 0, s, coords, batch_index,
                      n_atoms, pbc_mode, cells, reciprocal_cells, nbr_skin, skin_shift, n_skin, max_skin, rc2,
                      rca2, max_nbr, max_nbr_ang, nbr, nbr_shift, n_nbr, nbr_ang, n_nbr_ang, status);
   FNL_LAUNCH_CHECK();
