@@ -235,7 +235,10 @@ class ServingEngine:
         kp = potential.config.k_pad
         pool = 3 * 24 + 12 + 4 + 16 + 88 + 28 + 36            # x, v, old rows, x32, z, dtm, a2, stage, record
         model = 12 + 4 * (max_nbr + max_nbr_ang + 2) + 2 * kp * e + 4 + 12 + 24 + 36
-        return pool + model + 64                              # + cell-list workspace (about 16 words)
+        # cell-list workspace at the pool's capacity of 4 cells per atom row
+        rows = 1 << 16
+        ws = int(potential.lib.fnl_nblist_workspace(rows, 1, 4 * rows)) // rows + 1
+        return pool + model + ws
 
     @staticmethod
     def estimate_sys_bytes() -> int:
