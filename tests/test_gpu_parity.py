@@ -97,6 +97,46 @@ def test_nblist_pair_set(gpu, name):
         assert np.all(row[slots] == ed[rs[a] + slots])
 
 
+@pytest.mark.parametrize("name", ["water216", "triclinic", "cluster64"])
+def test_nblist_depth_one_and_two_cells_agree(gpu, name):
+    """The build searches half-cutoff cells two deep where the grid capacity allows it and
+    cutoff-wide cells one deep where it does not: capacities that force the cutoff-wide grid give
+    every row the same neighbours (radial and angular) as the default grid, in another order."""
+    _flush(f"[nblist] depth 1 vs 2: {name}")
+    s = _cases()[name]
+    cfg = configs()
+    _, pot = _potential(cfg, max_nbr=96, max_nbr_ang=48)
+    sb, x = _nblist(pot, s)
+
+    def rows():
+        n, na = sb.n_nbr.cpu().numpy(), sb.n_nbr_ang.cpu().numpy()
+        nbr, nbr_ang = sb.nbr.cpu().numpy(), sb.nbr_ang.cpu().numpy()
+        rad = [sorted(nbr[i, : n[i]].tolist()) for i in range(sb.n_atoms)]
+        ang = [sorted(nbr[i, nbr_ang[i, : na[i]]].tolist()) for i in range(sb.n_atoms)]
+        return rad, ang, nbr[:, : n.max()].copy()
+
+    rad2, ang2, raw2 = rows()
+    rbin = cfg[1].cutoff * 1.0001 + 1.0e-4
+    caps = sb.grid.cpu().numpy().copy()
+    if s.get("cells") is not None:
+        R = np.linalg.inv(np.asarray(s["cells"][0], np.float64))
+        width = 1.0 / np.sqrt((R ** 2).sum(axis=0))
+    else:
+        xs = np.asarray(s["coordinates"], np.float64)
+        width = xs.max(axis=0) - xs.min(axis=0)
+    caps[0, :3] = np.maximum(np.floor(width / rbin).astype(int), 1)  # cells at least a cutoff wide
+    assert (caps[0, :3] <= sb.grid.cpu().numpy()[0, :3]).all()
+    sb.grid.copy_(torch.tensor(caps, device=sb.grid.device))
+    pot.build_nblist(sb, x)
+    torch.cuda.synchronize()
+    assert int(sb.status[_abi.FNL_STATUS_OVERFLOW].item()) == 0
+    rad1, ang1, raw1 = rows()
+    assert rad1 == rad2 and ang1 == ang2
+    assert sum(len(r) for r in rad1) > 0 and sum(len(a) for a in ang1) > 0
+    if name == "water216":  # 7 half-cutoff cells per axis against 3: another candidate order
+        assert not np.array_equal(raw1, raw2)
+
+
 def test_nblist_batched_clusters(gpu):
     _flush("[nblist] batched clusters")
     rng = np.random.Generator(np.random.PCG64(5))
