@@ -116,12 +116,14 @@ __global__ void k_grid(const float* __restrict__ coords, const int32_t* __restri
                        const int32_t* __restrict__ sys_offsets, int pbc_mode,
                        const float* __restrict__ rec, const int32_t* __restrict__ grid_cap,
                        float cutoff, int32_t* __restrict__ grid, int32_t* __restrict__ status,
-                       int32_t* __restrict__ cell_count, unsigned* __restrict__ done, int n_cells) {
+                       int32_t* __restrict__ cell_count, unsigned* __restrict__ done, int n_cells, int n_sys) {
   const int b = blockIdx.x;
   const int t = threadIdx.x;
   if (b == 0 && t < FNL_STATUS_STICKY) status[t] = 0;  // the sticky word persists
   if (b == 0 && t == 0) *done = 0u;
+  // every workgroup clears a share of the cell counters; the first n_sys also set up a system
   for (int c = b * blockDim.x + t; c < n_cells; c += gridDim.x * blockDim.x) cell_count[c] = 0;
+  if (b >= n_sys) return;
   const int capx = grid_cap[b * 4 + 0], capy = grid_cap[b * 4 + 1], capz = grid_cap[b * 4 + 2];
   const int first = grid_cap[b * 4 + 3];
   int32_t* g = grid + b * kGridWords;
@@ -1023,8 +1025,10 @@ extern "C" int fnl_nblist_build(const float* coords, const int32_t* species_idx,
   // without angular rows no slot passes the angular test
   const float rca2 = nbr_ang ? (float)(cutoff_ang * cutoff_ang) : -1.0f;
 
-  hipLaunchKernelGGL(k_grid, dim3(n_sys), dim3(256), 0, s, coords, species_idx, sys_offsets, pbc_mode,
-                     reciprocal_cells, grid, (float)cutoff, ws.grid, status, ws.cell_count, ws.done, n_cells);
+  // one workgroup per system, at least one per 2,048 cell counters to clear (up to 64)
+  const int grid_blocks = std::max(n_sys, std::min((n_cells + 2047) / 2048, 64));
+  hipLaunchKernelGGL(k_grid, dim3(grid_blocks), dim3(256), 0, s, coords, species_idx, sys_offsets, pbc_mode,
+                     reciprocal_cells, grid, (float)cutoff, ws.grid, status, ws.cell_count, ws.done, n_cells, n_sys);
   FNL_LAUNCH_CHECK();
   if (n_atoms == 0) return FNL_OK;
   constexpr int W = 4;
