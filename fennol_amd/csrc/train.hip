@@ -350,9 +350,13 @@ __global__ void __launch_bounds__(256) k_train_wgrad(const TrainArgs a, const Tr
 #pragma unroll
   for (int jb = 0; jb < 4; ++jb) acc[jb] = f4{0.f, 0.f, 0.f, 0.f};
   const int sr = t >> 3, sc = (t & 7) * 8;  // staging: row, 8 columns
-  for (int r = rb; r < re; r += RK) {
+  // the staged step's rows (X, X tangent, Y, Y tangent: 8 columns each), loaded one step ahead so
+  // that their latency overlaps the previous step's MFMAs
+  f4 stg[8];
+  auto fetch = [&](int r) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) stg[q] = f4{0.f, 0.f, 0.f, 0.f};
     const int rr = r + sr;
-    f4 x0 = f4{0.f, 0.f, 0.f, 0.f}, x1 = x0, u0 = x0, u1 = x0, y0 = x0, y1 = x0, w0 = x0, w1 = x0;
     if (rr < re) {
       if (i0 + sc < NI) {
         size_t op, ot;
@@ -363,28 +367,32 @@ __global__ void __launch_bounds__(256) k_train_wgrad(const TrainArgs a, const Tr
         } else {
           op = ot = (size_t)rr * ldx + i0 + sc;
         }
-        x0 = *(const f4*)(xp + op);
-        x1 = *(const f4*)(xp + op + 4);
-        u0 = *(const f4*)(xt + ot);
-        u1 = *(const f4*)(xt + ot + 4);
+        stg[0] = *(const f4*)(xp + op);
+        stg[1] = *(const f4*)(xp + op + 4);
+        stg[2] = *(const f4*)(xt + ot);
+        stg[3] = *(const f4*)(xt + ot + 4);
       }
       if (j0 + sc < N) {
         const size_t oy = (size_t)rr * N + j0 + sc;
-        y0 = *(const f4*)(yp + oy);
-        y1 = *(const f4*)(yp + oy + 4);
-        w0 = *(const f4*)(yt + oy);
-        w1 = *(const f4*)(yt + oy + 4);
+        stg[4] = *(const f4*)(yp + oy);
+        stg[5] = *(const f4*)(yp + oy + 4);
+        stg[6] = *(const f4*)(yt + oy);
+        stg[7] = *(const f4*)(yt + oy + 4);
       }
     }
-    *(f4*)&Xs[0][sr][sc] = x0;
-    *(f4*)&Xs[0][sr][sc + 4] = x1;
-    *(f4*)&Xs[1][sr][sc] = u0;
-    *(f4*)&Xs[1][sr][sc + 4] = u1;
-    *(f4*)&Ys[0][sr][sc] = y0;
-    *(f4*)&Ys[0][sr][sc + 4] = y1;
-    *(f4*)&Ys[1][sr][sc] = w0;
-    *(f4*)&Ys[1][sr][sc + 4] = w1;
+  };
+  fetch(rb);
+  for (int r = rb; r < re; r += RK) {
+    *(f4*)&Xs[0][sr][sc] = stg[0];
+    *(f4*)&Xs[0][sr][sc + 4] = stg[1];
+    *(f4*)&Xs[1][sr][sc] = stg[2];
+    *(f4*)&Xs[1][sr][sc + 4] = stg[3];
+    *(f4*)&Ys[0][sr][sc] = stg[4];
+    *(f4*)&Ys[0][sr][sc + 4] = stg[5];
+    *(f4*)&Ys[1][sr][sc] = stg[6];
+    *(f4*)&Ys[1][sr][sc + 4] = stg[7];
     __syncthreads();
+    if (r + RK < re) fetch(r + RK);
 #pragma unroll
     for (int kk = 0; kk < RK; kk += 4) {
 #pragma unroll
