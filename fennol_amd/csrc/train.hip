@@ -156,48 +156,64 @@ __global__ void __launch_bounds__(256) k_train_rows(const TrainArgs a, const Tra
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) acc[c][nb] = f4{0.0f, 0.0f, 0.0f, 0.0f};
 
-  for (int k0 = 0; k0 < K; k0 += KC) {
-    if (MODE == FWD1 && !chunk_live(a.k_live, k0)) continue;
-    {
-      f4 p0 = f4{0.f, 0.f, 0.f, 0.f}, p1 = p0, q0 = p0, q1 = p0;
-      const int k = k0 + xc;
-      if (xrow_ok && k < K) {
-        p0 = *(const f4*)(xp + xoff_p + k);
-        p1 = *(const f4*)(xp + xoff_p + k + 4);
-        q0 = *(const f4*)(xt + xoff_t + k);
-        q1 = *(const f4*)(xt + xoff_t + k + 4);
-      }
+  // the next live K chunk's X rows and W block, loaded into registers one chunk ahead so that
+  // their latency overlaps the current chunk's MFMAs
+  f4 sx[4], sw[2];
+  auto next_live = [&](int k) {
+    if (MODE == FWD1)
+      while (k < K && !chunk_live(a.k_live, k)) k += KC;
+    return k;
+  };
+  auto fetch = [&](int k0) {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        Xs[0][xr][xc + e] = p0[e];
-        Xs[0][xr][xc + 4 + e] = p1[e];
-        Xs[1][xr][xc + e] = q0[e];
-        Xs[1][xr][xc + 4 + e] = q1[e];
-      }
+    for (int q = 0; q < 4; ++q) sx[q] = f4{0.f, 0.f, 0.f, 0.f};
+    sw[0] = sw[1] = f4{0.f, 0.f, 0.f, 0.f};
+    const int k = k0 + xc;
+    if (xrow_ok && k < K) {
+      sx[0] = *(const f4*)(xp + xoff_p + k);
+      sx[1] = *(const f4*)(xp + xoff_p + k + 4);
+      sx[2] = *(const f4*)(xt + xoff_t + k);
+      sx[3] = *(const f4*)(xt + xoff_t + k + 4);
     }
     if (!TRANS) {
       const int kk = t >> 3, nn = (t & 7) * 8;
-      f4 w0 = f4{0.f, 0.f, 0.f, 0.f}, w1 = w0;
       if (k0 + kk < K && n0 + nn < N) {
-        w0 = *(const f4*)(W + (size_t)(k0 + kk) * ldw + n0 + nn);
-        w1 = *(const f4*)(W + (size_t)(k0 + kk) * ldw + n0 + nn + 4);
+        sw[0] = *(const f4*)(W + (size_t)(k0 + kk) * ldw + n0 + nn);
+        sw[1] = *(const f4*)(W + (size_t)(k0 + kk) * ldw + n0 + nn + 4);
       }
-      *(f4*)&Ws[kk][nn] = w0;
-      *(f4*)&Ws[kk][nn + 4] = w1;
     } else {
       const int nn = t >> 2, kk = (t & 3) * 8;
-      f4 w0 = f4{0.f, 0.f, 0.f, 0.f}, w1 = w0;
       if (n0 + nn < N && k0 + kk < K) {
-        w0 = *(const f4*)(W + (size_t)(n0 + nn) * ldw + k0 + kk);
-        w1 = *(const f4*)(W + (size_t)(n0 + nn) * ldw + k0 + kk + 4);
+        sw[0] = *(const f4*)(W + (size_t)(n0 + nn) * ldw + k0 + kk);
+        sw[1] = *(const f4*)(W + (size_t)(n0 + nn) * ldw + k0 + kk + 4);
       }
+    }
+  };
+  int k0 = next_live(0);
+  if (k0 < K) fetch(k0);
+  while (k0 < K) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      Xs[0][xr][xc + e] = sx[0][e];
+      Xs[0][xr][xc + 4 + e] = sx[1][e];
+      Xs[1][xr][xc + e] = sx[2][e];
+      Xs[1][xr][xc + 4 + e] = sx[3][e];
+    }
+    if (!TRANS) {
+      const int kk = t >> 3, nn = (t & 7) * 8;
+      *(f4*)&Ws[kk][nn] = sw[0];
+      *(f4*)&Ws[kk][nn + 4] = sw[1];
+    } else {
+      const int nn = t >> 2, kk = (t & 3) * 8;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        Ws[kk + e][nn] = w0[e];
-        Ws[kk + 4 + e][nn] = w1[e];
+        Ws[kk + e][nn] = sw[0][e];
+        Ws[kk + 4 + e][nn] = sw[1][e];
       }
     }
     __syncthreads();
+    const int kn = next_live(k0 + KC);
+    if (kn < K) fetch(kn);
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       const float bp = Xs[0][wave * 16 + (l & 15)][kk + (l >> 4)];
@@ -210,6 +226,7 @@ __global__ void __launch_bounds__(256) k_train_rows(const TrainArgs a, const Tra
       }
     }
     __syncthreads();
+    k0 = kn;
   }
 
   // epilogue: lane holds column n0 + nb*16 + 4*(l >> 4) + v of tile row m = wave*16 + (l & 15)
