@@ -175,3 +175,26 @@ def test_grown_capacity():
     assert grown_capacity(64, 60) == 64
     assert grown_capacity(64, 70) == 96  # int(1.25 * 70) + 8 = 95 -> multiple of 8
     assert grown_capacity(16, 69) % 8 == 0 and grown_capacity(16, 69) >= 69
+
+
+def test_grid_caps_leave_room_for_half_cutoff_cells():
+    """Periodic grid capacities hold floor(2 w / rc') + 1 cells per lattice direction (w the
+    lattice-plane spacing, rc' the cutoff with the kernel's margin), so the build can use cells
+    half a cutoff wide; open systems get about one cell per atom; empty systems one cell; the
+    first-cell offsets are the running sums of the capacities."""
+    from fennol_amd.potential import grid_caps
+
+    rc = 5.2
+    rbin = rc * 1.0001 + 1.0e-4
+    box = systems.water_config1()
+    cells = np.asarray(box["cells"], np.float64)
+    natoms = np.array([len(box["species"]), 0, 64], np.int32)
+    g, n_cells = grid_caps(rc, natoms, np.stack([cells[0], np.eye(3) * 1000.0, cells[0]]))
+    w = 1.0 / np.sqrt((np.linalg.inv(cells[0]) ** 2).sum(axis=0))
+    np.testing.assert_array_equal(g[0, :3], np.floor(2.0 * w / rbin).astype(int) + 1)
+    assert (g[0, :3] >= 2 * np.floor(w / rbin)).all()
+    np.testing.assert_array_equal(g[1, :3], [1, 1, 1])
+    np.testing.assert_array_equal(g[:, 3], [0, int(np.prod(g[0, :3])), int(np.prod(g[0, :3])) + 1])
+    assert n_cells == int(np.prod(g[0, :3])) + 1 + int(np.prod(g[2, :3]))
+    g_open, _ = grid_caps(rc, np.array([512], np.int32), None)
+    np.testing.assert_array_equal(g_open[0, :3], [8, 8, 8])
